@@ -1,0 +1,216 @@
+"""CPU oracle for the per-keyframe depth pipeline -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper over oracle/libqm_oracle.so (built from oracle/qm_oracle.cpp by
+oracle/Makefile). Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
+of bench.py may import this package; it is the checker, never the product path.
+
+Each function follows a reference file (see qm_oracle.cpp's header):
+  quadtree           src/quadtree.cu:17-77
+  cost_volume        src/depth_extract.cu:135-244
+  bp_extract         src/bpextract.cu:37-395
+  weights/upsample   src/depth_upsample.cu:20-302
+  fuse               src/depth_fusion.cu:13-213
+  keyframe           src/seed_matrix.cu:448-597
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libqm_oracle.so")
+_lib = None
+
+P = C.c_void_p
+I = C.c_int
+F = C.c_float
+D = C.c_double
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-B" if force else "all"], cwd=_HERE)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.qmo_quadtree.argtypes = [P, I, I, I, P]
+        L.qmo_depth_table_f32.argtypes = [I, D, D, P]
+        L.qmo_depth_table_f64.argtypes = [I, D, D, P]
+        L.qmo_cost_f32.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P]
+        L.qmo_cost_f64.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P]
+        L.qmo_bp_extract_f32.argtypes = [P, I, I, P, I, F, F, P, D, D, P, P, P]
+        L.qmo_bp_extract_f64.argtypes = [P, I, I, P, I, F, F, P, D, D, P, P, P]
+        L.qmo_weights_f32.argtypes = [P, I, I, I, P, P]
+        L.qmo_weights_f64.argtypes = [P, I, I, I, P, P]
+        L.qmo_weight_of_delta_f32.argtypes = [I]
+        L.qmo_weight_of_delta_f32.restype = F
+        L.qmo_upsample_f32.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P]
+        L.qmo_upsample_f64.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P]
+        L.qmo_fuse_f32.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
+        L.qmo_fuse_f64.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
+        L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
+        L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
+        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P]
+        L.qmo_set_threads.argtypes = [I]
+        L.qmo_get_threads.restype = I
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _ptr_array(arrs):
+    arr = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return arr
+
+
+def _dt(fp64):
+    return np.float64 if fp64 else np.float32
+
+
+def set_threads(n: int):
+    lib().qmo_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().qmo_get_threads()
+
+
+def quadtree(img: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape
+    out = np.zeros((H, W), np.uint8)
+    lib().qmo_quadtree(_p(img), W, H, W, _p(out))
+    return out
+
+
+def depth_table(N: int, imin: float, imax: float, fp64: bool = False) -> np.ndarray:
+    out = np.zeros(N, _dt(fp64))
+    (lib().qmo_depth_table_f64 if fp64 else lib().qmo_depth_table_f32)(N, imin, imax, _p(out))
+    return out
+
+
+def cost_volume(key, refs, T, cam, N, imin, imax, levels, fp64=False):
+    """refs: 4 uint8 images; T: (4,3,4) float32 T_ref_key. Returns (cost HcxWcxN, count HcxWc)."""
+    key = np.ascontiguousarray(key, dtype=np.uint8)
+    refs = [np.ascontiguousarray(r, dtype=np.uint8) for r in refs]
+    assert len(refs) == 4
+    H, W = key.shape
+    T = np.ascontiguousarray(T, dtype=np.float32).reshape(4, 12)
+    table = depth_table(N, imin, imax, fp64)
+    levels = np.ascontiguousarray(levels, dtype=np.uint8)
+    cost = np.zeros((H // 4, W // 4, N), _dt(fp64))
+    count = np.zeros((H // 4, W // 4), np.int32)
+    fn = lib().qmo_cost_f64 if fp64 else lib().qmo_cost_f32
+    rp = _ptr_array(refs)
+    fn(_p(key), rp, W, H, W, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, _p(table), _p(levels), _p(cost), _p(count))
+    return cost, count
+
+
+def bp_extract(cost, W, H, levels, N, imin, imax, P1=0.003, P2=0.01, iters=(4, 10, 10, 10), fp64=False,
+               want_messages=False):
+    """Returns (depth_coarse HcxWc, label HcxWc int32 (-1: not a node)[, messages 4xHcxWcxN])."""
+    dt = _dt(fp64)
+    cost = np.ascontiguousarray(cost, dtype=dt)
+    levels = np.ascontiguousarray(levels, dtype=np.uint8)
+    Hc, Wc = H // 4, W // 4
+    depth = np.zeros((Hc, Wc), dt)
+    label = np.zeros((Hc, Wc), np.int32)
+    it = np.array(iters, np.int32)
+    msg = np.zeros((4, Hc, Wc, N), dt) if want_messages else None
+    fn = lib().qmo_bp_extract_f64 if fp64 else lib().qmo_bp_extract_f32
+    fn(_p(cost), W, H, _p(levels), N, P1, P2, _p(it), imin, imax, _p(depth), _p(label), _p(msg))
+    return (depth, label, msg) if want_messages else (depth, label)
+
+
+def weights(img, fp64=False):
+    """Reference smoothness weights: (w_row H x (W+1), w_col (H+1) x W)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape
+    dt = _dt(fp64)
+    wr = np.zeros((H, W + 1), dt)
+    wc = np.zeros((H + 1, W), dt)
+    (lib().qmo_weights_f64 if fp64 else lib().qmo_weights_f32)(_p(img), W, H, W, _p(wr), _p(wc))
+    return wr, wc
+
+
+def weight_lut():
+    return np.array([lib().qmo_weight_of_delta_f32(i) for i in range(256)], np.float32)
+
+
+def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None, fp64=False, want_rows=False):
+    dt = _dt(fp64)
+    d = np.ascontiguousarray(d, dtype=dt)
+    H, W = d.shape
+    w_row = np.ascontiguousarray(w_row, dtype=dt)
+    w_col = np.ascontiguousarray(w_col, dtype=dt)
+    conf = None if conf is None else np.ascontiguousarray(conf, dtype=dt)
+    row_seg = W if row_seg is None else row_seg
+    col_seg = H if col_seg is None else col_seg
+    out = np.zeros((H, W), dt)
+    rx = np.zeros((H, W), dt)
+    rn = np.zeros((H, W), dt)
+    fn = lib().qmo_upsample_f64 if fp64 else lib().qmo_upsample_f32
+    fn(W, H, _p(d), _p(conf), _p(w_row), _p(w_col), lam, row_seg, col_seg, _p(out), _p(rx), _p(rn))
+    return (out, rx, rn) if want_rows else out
+
+
+def ref_segments(W, H):
+    """Chain split points of the reference upsample: 16*floor(W/16), 8*floor(H/8)."""
+    return 16 * (W // 16), 8 * (H // 8)
+
+
+def fuse(seeds, depth_in, T_last_to_cur, cam, fp64=False):
+    """Returns (warped seeds (in-place copy), depth_out, seeds_out, table)."""
+    dt = _dt(fp64)
+    H, W = depth_in.shape
+    s = np.array(seeds, dtype=dt, copy=True).reshape(H, W, 4)
+    din = np.ascontiguousarray(depth_in, dtype=dt)
+    T = np.ascontiguousarray(T_last_to_cur, dtype=np.float32).reshape(12)
+    dout = np.zeros((H, W), dt)
+    sout = np.zeros((H, W, 4), dt)
+    table = np.zeros((H, W), np.int32)
+    fn = lib().qmo_fuse_f64 if fp64 else lib().qmo_fuse_f32
+    fn(W, H, cam.fx, cam.fy, cam.cx, cam.cy, _p(T), _p(s), _p(din), _p(dout), _p(sout), _p(table))
+    return s, dout, sout, table
+
+
+def compose_sparse(W, H, depth_coarse, label, prefill=None, fp64=False):
+    dt = _dt(fp64)
+    out = np.zeros((H, W), dt)
+    dc = np.ascontiguousarray(depth_coarse, dtype=dt)
+    lb = np.ascontiguousarray(label, dtype=np.int32)
+    pf = None if prefill is None else np.ascontiguousarray(prefill, dtype=dt)
+    (lib().qmo_compose_sparse_f64 if fp64 else lib().qmo_compose_sparse_f32)(W, H, _p(dc), _p(lb), _p(pf), _p(out))
+    return out
+
+
+def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None):
+    """Full FP32 keyframe. seeds (HxWx4 float32) is updated in place.
+    Returns dict(depth, sparse, levels, label, stage_ms)."""
+    key = np.ascontiguousarray(key, dtype=np.uint8)
+    H, W = key.shape
+    refs = [np.ascontiguousarray(r, dtype=np.uint8) for r in refs]
+    T = np.ascontiguousarray(T, dtype=np.float32).reshape(4, 12)
+    assert seeds.dtype == np.float32 and seeds.flags.c_contiguous
+    Tl = np.ascontiguousarray(T_last_to_cur, dtype=np.float32).reshape(12)
+    pf = None if prefill is None else np.ascontiguousarray(prefill, dtype=np.float32)
+    depth = np.zeros((H, W), np.float32)
+    sparse = np.zeros((H, W), np.float32)
+    levels = np.zeros((H, W), np.uint8)
+    label = np.zeros((H // 4, W // 4), np.int32)
+    ms = np.zeros(5, np.float64)
+    lib().qmo_keyframe_f32(_p(key), _ptr_array(refs), W, H, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, imin, imax,
+                           _p(pf), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label), _p(ms))
+    return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
