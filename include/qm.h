@@ -1,0 +1,143 @@
+/*
+ * qm.h -- C ABI of the MI355X per-keyframe depth pipeline (libqm.so).
+ *
+ * Drop-in for the hot path of QuadtreeMapping (quadmap::Depthmap ->
+ * SeedMatrix::extract_depth + fuse_output_depth). Plain C types only; every
+ * device buffer is a HIP device pointer; `stream` is a hipStream_t passed as
+ * void* (NULL = the null stream). Every call returns 0 on success or a negative
+ * QM_ERR_* code; qm_last_error() gives a thread-local message.
+ *
+ * Layouts (all row-major, no padding unless a pitch argument says otherwise):
+ *   image        uint8  H x W            (pitch in bytes, >= W)
+ *   levels       uint8  H x W            quadtree level per pixel, in {2,3,4}
+ *   cost         float  (H/4) x (W/4) x N
+ *   depth_coarse float  (H/4) x (W/4)    BP winner depth per node cell, 0 elsewhere
+ *   label        int32  (H/4) x (W/4)    BP winner index, -1 for non-node cells
+ *   sparse/dense float  H x W
+ *   seeds        float4 H x W            (a, b, mu, sigma^2) fusion state
+ *   transforms   float[12] 3x4 row-major [R | t]
+ */
+#ifndef QM_H
+#define QM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QM_OK 0
+#define QM_ERR_INVALID (-1) /* bad argument (size, null pointer, unsupported N) */
+#define QM_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define QM_ERR_ALLOC (-3)   /* device allocation failed or was refused */
+#define QM_ERR_STATE (-4)   /* call out of order */
+
+/* Pipeline constants. Defaults are the reference's compile-time values:
+ * include/quadmap/stereo_parameter.cuh:5-7,41-42, include/quadmap/pixel_cost.cuh:4,
+ * src/bpextract.cu:45,222-223, src/seed_matrix.cu:151,182,208. */
+typedef struct qm_params {
+  int n_hyp;            /* inverse-depth samples: 64 (DEPTH_NUM) or 128 */
+  double idepth_min;    /* MIN_INV_DEPTH = 0.01 */
+  double idepth_max;    /* MAX_INV_DEPTH = 2.0; sample step = (max-min)/(n_hyp-1) */
+  float p1;             /* BP penalty for a +-1 label change, 0.003 */
+  float p2;             /* BP penalty for any label change, 0.01 */
+  int bp_iters[4];      /* half-sweeps per level, fine -> coarse: {4,10,10,10} */
+  double lambda;        /* upsample smoothness, upsample_lambda = 10 */
+  int min_refs;         /* references needed before a depth map is produced: 11 (> 10) */
+  int max_refs;         /* reference deque length, KEYFRAME_NUM = 60 */
+  float ref_cos;        /* add a reference if cos(z, z_ref) < 0.95 ... */
+  float ref_baseline;   /* ... or the baseline exceeds 0.03 m */
+} qm_params;
+
+const char* qm_last_error(void);
+void qm_default_params(qm_params* p);
+/* Fault injection for tests: refuse device allocations beyond `bytes` (0 = no cap). */
+void qm_set_alloc_cap(size_t bytes);
+size_t qm_device_bytes_in_use(void);
+
+/* ---------------- stage level (device pointers) ---------------------------- */
+
+/* Quadtree level assignment. Replaces quadtree_image_kernal
+ * (src/quadtree.cu:17-77, launched at src/seed_matrix.cu:479). */
+int qm_quadtree(const uint8_t* img, int W, int H, int pitch, uint8_t* levels, void* stream);
+
+/* Multi-view patch-cost volume over n_hyp inverse-depth samples. refs[0..3] are
+ * the four distinct references read by slots {0,0,1,2,3}; T_ref_key[4][12] is
+ * the host array of key->reference transforms; cam = {fx, fy, cx, cy}.
+ * count (nullable) receives 1 for node cells, 0 otherwise. Replaces
+ * image_to_cost + normalize_the_cost (src/depth_extract.cu:135-244, launched at
+ * src/seed_matrix.cu:508,523). */
+int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int H, int pitch,
+                   const float T_ref_key[48], const float cam[4], const qm_params* p,
+                   const uint8_t* levels, float* cost, int32_t* count, void* stream);
+
+/* Hierarchical BP (cost pyramid, 4 levels, checkerboard min-sum) and winner
+ * extraction. Replaces bp_extract (src/bpextract.cu:37-167) and its kernels
+ * cost_distribute, bp, upsample, depth_extract (:169-395). messages (nullable)
+ * receives the level-0 messages, 4 x (H/4) x (W/4) x N in the order up, down,
+ * left, right (the reference's up/dm/lm/rm maps). */
+int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const qm_params* p,
+                  float* depth_coarse, int32_t* label, float* messages, void* stream);
+
+/* Tridiagonal smoothness upsampling of a sparse depth map, weights from the
+ * keyframe. Replaces global_upsample (src/depth_upsample.cu:20-302). */
+int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch, const qm_params* p,
+                float* dense, void* stream);
+
+/* Generalised upsampling used by the differentiable path: rhs_x = conf*d,
+ * rhs_n = conf (conf == NULL: rhs_x = d, rhs_n = 1{d > 0} as the reference);
+ * w_row is H x (W+1), w_col is (H+1) x W; the chains are split at row_seg and
+ * col_seg (pass W and H for no split). row_x/row_n (nullable) receive the row
+ * pass result. Same arithmetic as qm_upsample. */
+int qm_upsample_general(const float* d, const float* conf, const float* w_row, const float* w_col, int W,
+                        int H, double lambda, int row_seg, int col_seg, float* dense, float* row_x,
+                        float* row_n, void* stream);
+
+/* Gaussian x Beta fusion of a dense depth map into the seed state. seeds
+ * (H x W float4) is warped in place by T_last_to_cur; the update goes to
+ * seeds_out; depth_out gets mu' where the prior is confident, else -1.
+ * table (nullable) receives the hole-filled warp table. Replaces fuse_transform,
+ * hole_filling, fuse_currentmap (src/depth_fusion.cu:48-213) as driven by
+ * SeedMatrix::fuse_output_depth (src/seed_matrix.cu:553-597). */
+int qm_fuse(float* seeds, const float* depth_in, int W, int H, const float cam[4], const float T_last_to_cur[12],
+            float* depth_out, float* seeds_out, int32_t* table, void* stream);
+
+/* ---------------- batched keyframes ---------------------------------------- */
+
+/* One keyframe of one camera stream. Input pointers are device pointers owned by
+ * the caller; NULL outputs are not written. */
+typedef struct qm_frame_desc {
+  int W, H, pitch;
+  float cam[4];                /* fx, fy, cx, cy */
+  const uint8_t* key;          /* keyframe image */
+  const uint8_t* refs[4];      /* references for slots {0,0,1,2,3} */
+  float T_ref_key[4][12];      /* key -> reference */
+  float T_last_to_cur[12];     /* previous fused frame -> this keyframe */
+  const float* prefill;        /* semi-dense pre-fill of the sparse map (H x W) or NULL */
+  float* depth_out;            /* fused depth (H x W), -1 where not confident */
+  float* dense_out;            /* upsampled depth before fusion (H x W) */
+  float* sparse_out;           /* sparse map handed to the upsample (H x W) */
+} qm_frame_desc;
+
+typedef struct qm_batch qm_batch;
+
+/* Plan a batch: allocates all per-stream state (fusion seeds persist across
+ * runs), builds the per-stage work tables. */
+int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_batch** out);
+/* Replace frame i's inputs/transforms (same size); takes effect on the next run. */
+int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* frame);
+/* Run every stage for every frame. mode 0: one launch per stage over all
+ * frames; mode 1: the same captured once as a hipGraph and replayed. */
+int qm_batch_run(qm_batch* b, int mode, void* stream);
+/* Device pointer of an internal buffer of frame i, for inspection:
+ * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state) */
+int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);
+int qm_batch_reset_seeds(qm_batch* b);
+int qm_batch_destroy(qm_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QM_H */

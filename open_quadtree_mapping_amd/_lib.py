@@ -1,0 +1,107 @@
+"""ctypes binding of libqm.so (include/qm.h). No fallback: if the HIP library is
+missing or fails to load, every call raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqm.so")
+_lib = None
+
+P = C.c_void_p
+I = C.c_int
+F = C.c_float
+D = C.c_double
+
+
+class QmError(RuntimeError):
+    pass
+
+
+class qm_params(C.Structure):
+    _fields_ = [
+        ("n_hyp", C.c_int),
+        ("idepth_min", C.c_double),
+        ("idepth_max", C.c_double),
+        ("p1", C.c_float),
+        ("p2", C.c_float),
+        ("bp_iters", C.c_int * 4),
+        ("lambda_", C.c_double),
+        ("min_refs", C.c_int),
+        ("max_refs", C.c_int),
+        ("ref_cos", C.c_float),
+        ("ref_baseline", C.c_float),
+    ]
+
+
+class qm_frame_desc(C.Structure):
+    _fields_ = [
+        ("W", C.c_int),
+        ("H", C.c_int),
+        ("pitch", C.c_int),
+        ("cam", C.c_float * 4),
+        ("key", C.c_void_p),
+        ("refs", C.c_void_p * 4),
+        ("T_ref_key", C.c_float * 48),
+        ("T_last_to_cur", C.c_float * 12),
+        ("prefill", C.c_void_p),
+        ("depth_out", C.c_void_p),
+        ("dense_out", C.c_void_p),
+        ("sparse_out", C.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/qm.h
+SIGNATURES = {
+    "qm_last_error": (C.c_char_p, []),
+    "qm_default_params": (None, [P]),
+    "qm_set_alloc_cap": (None, [C.c_size_t]),
+    "qm_device_bytes_in_use": (C.c_size_t, []),
+    "qm_quadtree": (I, [P, I, I, I, P, P]),
+    "qm_cost_volume": (I, [P, P, I, I, I, P, P, P, P, P, P, P]),
+    "qm_bp_extract": (I, [P, I, I, P, P, P, P, P, P]),
+    "qm_upsample": (I, [P, P, I, I, I, P, P, P]),
+    "qm_upsample_general": (I, [P, P, P, P, I, I, D, I, I, P, P, P, P]),
+    "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
+    "qm_batch_create": (I, [P, I, P, P]),
+    "qm_batch_set_frame": (I, [P, I, P]),
+    "qm_batch_run": (I, [P, I, P]),
+    "qm_batch_buffer": (I, [P, I, I, P]),
+    "qm_batch_reset_seeds": (I, [P]),
+    "qm_batch_destroy": (I, [P]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QmError(f"{LIB_PATH} is not built; run __graft_entry__.build() (the HIP path has no fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = lib().qm_last_error()
+        raise QmError(f"libqm error {rc}: {msg.decode() if msg else ''}")
+
+
+def default_params(**kw) -> qm_params:
+    p = qm_params()
+    lib().qm_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "lambda":
+            k = "lambda_"
+        if k == "bp_iters":
+            for i in range(4):
+                p.bp_iters[i] = int(v[i])
+            continue
+        setattr(p, k, v)
+    return p

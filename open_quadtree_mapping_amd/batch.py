@@ -1,0 +1,97 @@
+"""Batched keyframes: many camera streams of uneven resolution through every
+stage in one launch per stage (optionally one hipGraph replay per run).
+
+Thin owner of a qm_batch (include/qm.h). Inputs are torch uint8 device tensors
+kept alive by this object; per-stream fusion seeds persist across runs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import check, default_params, lib, qm_frame_desc, qm_params
+
+__all__ = ["Batch", "make_desc"]
+
+
+def make_desc(key, refs, T_ref_key, cam, T_last_to_cur=None, prefill=None, depth_out=None, dense_out=None,
+              sparse_out=None) -> qm_frame_desc:
+    H, W = key.shape
+    d = qm_frame_desc()
+    d.W, d.H, d.pitch = W, H, W
+    for i, v in enumerate(cam):
+        d.cam[i] = float(v)
+    d.key = key.data_ptr()
+    for r in range(4):
+        d.refs[r] = refs[r].data_ptr()
+    T = np.ascontiguousarray(T_ref_key, dtype=np.float32).reshape(48)
+    for i in range(48):
+        d.T_ref_key[i] = float(T[i])
+    Tl = np.eye(3, 4, dtype=np.float32) if T_last_to_cur is None else np.asarray(T_last_to_cur, np.float32)
+    for i, v in enumerate(Tl.reshape(12)):
+        d.T_last_to_cur[i] = float(v)
+    d.prefill = None if prefill is None else prefill.data_ptr()
+    d.depth_out = None if depth_out is None else depth_out.data_ptr()
+    d.dense_out = None if dense_out is None else dense_out.data_ptr()
+    d.sparse_out = None if sparse_out is None else sparse_out.data_ptr()
+    return d
+
+
+class Batch:
+    """frames: list of dicts with key (HxW uint8 device), refs (4 of them),
+    T (4,3,4 float32), cam (fx, fy, cx, cy), optional T_last, prefill.
+    Output tensors depth / dense / sparse are allocated per stream when asked."""
+
+    BUF_LEVELS, BUF_COST, BUF_DCOARSE, BUF_LABEL, BUF_DENSE, BUF_SEEDS = range(6)
+
+    def __init__(self, frames, params: qm_params | None = None, outputs=("depth",)):
+        self.params = params or default_params()
+        self.frames = list(frames)
+        self.n = len(self.frames)
+        self.outputs = []
+        descs = (qm_frame_desc * self.n)()
+        self._keep = []
+        for i, fr in enumerate(self.frames):
+            H, W = fr["key"].shape
+            dev = fr["key"].device
+            outs = {k: torch.empty((H, W), dtype=torch.float32, device=dev) for k in outputs}
+            self.outputs.append(outs)
+            descs[i] = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
+                                 outs.get("depth"), outs.get("dense"), outs.get("sparse"))
+            self._keep.append(fr)
+        self._descs = descs
+        self._h = C.c_void_p()
+        check(lib().qm_batch_create(descs, self.n, C.byref(self.params), C.byref(self._h)))
+
+    def set_frame(self, i, fr):
+        self.frames[i] = fr
+        o = self.outputs[i]
+        d = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
+                      o.get("depth"), o.get("dense"), o.get("sparse"))
+        self._descs[i] = d
+        check(lib().qm_batch_set_frame(self._h, i, C.byref(d)))
+
+    def run(self, graph: bool = False, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().qm_batch_run(self._h, 1 if graph else 0, C.c_void_p(s.cuda_stream)))
+
+    def buffer_ptr(self, i, which) -> int:
+        p = C.c_void_p()
+        check(lib().qm_batch_buffer(self._h, i, which, C.byref(p)))
+        return p.value
+
+    def reset_seeds(self):
+        check(lib().qm_batch_reset_seeds(self._h))
+
+    def close(self):
+        if self._h:
+            lib().qm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

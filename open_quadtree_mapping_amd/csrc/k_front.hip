@@ -1,0 +1,280 @@
+// k_front.hip -- quadtree level assignment, multi-view cost volume and the
+// cost pyramid for the hierarchical BP, for a batch of keyframes.
+#include "qm_device.h"
+#include "qm_launch.h"
+
+namespace qm {
+
+// ---------------------------------------------------------------------------
+// Quadtree (reference: src/quadtree.cu:17-77).
+// One lane owns one 4x4 pixel block; a wave covers four 16x16 tiles side by
+// side. Inside a tile the 16 lanes are in Morton order, so the 8x8 parent of a
+// block is the lane quad (xor 1, 2) and the 16x16 tile is the lane row (xor 4, 8).
+// All sums are of integers, so the per-level test |n*I - sum| <= 10*n is exact.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_quadtree(const FrameDev* __restrict__ frames, BlockMap bm) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int W = F.W, H = F.H;
+  const int lane = threadIdx.x & 63;
+  const int wid = (blockIdx.x - bm.start[f]) * 4 + (threadIdx.x >> 6);
+  const int tiles_x = (W + 15) >> 4, tiles_y = (H + 15) >> 4;
+  const int quads_x = (tiles_x + 3) >> 2;
+  const int ty = wid / quads_x, tq = wid - ty * quads_x;
+  if (ty >= tiles_y) return;  // whole wave
+  const int tx = tq * 4 + (lane >> 4);
+  const int m = lane & 15;
+  const int bx = (m & 1) | ((m >> 1) & 2);
+  const int by = ((m >> 1) & 1) | ((m >> 2) & 2);
+  const int X0 = tx * 16 + bx * 4, Y0 = ty * 16 + by * 4;
+  const bool inside = (X0 + 4 <= W) && (Y0 + 4 <= H);
+
+  int px[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) px[i] = 0;
+  if (inside) {
+    const uint8_t* src = F.key + (size_t)Y0 * F.pitch + X0;
+    if ((((uintptr_t)src) & 3) == 0 && (F.pitch & 3) == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t w4 = *reinterpret_cast<const uint32_t*>(src + (size_t)r * F.pitch);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) px[r * 4 + c] = (w4 >> (8 * c)) & 0xFF;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) px[r * 4 + c] = src[(size_t)r * F.pitch + c];
+    }
+  }
+  // level 1 (2x2) and level 2 (4x4) inside the lane
+  int p2 = inside ? 1 : 0;
+#pragma unroll
+  for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx) {
+      const int a = px[(2 * sy) * 4 + 2 * sx], b = px[(2 * sy) * 4 + 2 * sx + 1];
+      const int c = px[(2 * sy + 1) * 4 + 2 * sx], d = px[(2 * sy + 1) * 4 + 2 * sx + 1];
+      const int s4 = a + b + c + d;
+      if (abs(4 * a - s4) > 40 || abs(4 * b - s4) > 40 || abs(4 * c - s4) > 40 || abs(4 * d - s4) > 40) p2 = 0;
+    }
+  int s16 = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s16 += px[i];
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (abs(16 * px[i] - s16) > 160) p2 = 0;
+  // level 3 (8x8): lane quad
+  int s64 = s16 + __shfl_xor(s16, 1, 64);
+  s64 += __shfl_xor(s64, 2, 64);
+  int all2 = p2 & __shfl_xor(p2, 1, 64);
+  all2 &= __shfl_xor(all2, 2, 64);
+  int dev3 = 1;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (abs(64 * px[i] - s64) > 640) dev3 = 0;
+  dev3 &= __shfl_xor(dev3, 1, 64);
+  dev3 &= __shfl_xor(dev3, 2, 64);
+  const int p3 = all2 & dev3;
+  // level 4 (16x16): lane row of 16
+  int s256 = s64 + __shfl_xor(s64, 4, 64);
+  s256 += __shfl_xor(s256, 8, 64);
+  int all3 = p3 & __shfl_xor(p3, 4, 64);
+  all3 &= __shfl_xor(all3, 8, 64);
+  int dev4 = 1;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (abs(256 * px[i] - s256) > 2560) dev4 = 0;
+  // the per-lane deviation test spans all 16 lanes of the tile
+  dev4 &= __shfl_xor(dev4, 1, 64);
+  dev4 &= __shfl_xor(dev4, 2, 64);
+  dev4 &= __shfl_xor(dev4, 4, 64);
+  dev4 &= __shfl_xor(dev4, 8, 64);
+  const int p4 = all3 & dev4;
+  const int lv = p4 ? 4 : (p3 ? 3 : 2);
+
+  uint8_t* dst = F.levels + (size_t)Y0 * W + X0;
+  if (inside) {
+    const uint32_t v4 = 0x01010101u * (uint32_t)lv;
+    if ((((uintptr_t)dst) & 3) == 0 && (W & 3) == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) *reinterpret_cast<uint32_t*>(dst + (size_t)r * W) = v4;
+    } else {
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) dst[(size_t)r * W + c] = (uint8_t)lv;
+    }
+  } else if (X0 < W && Y0 < H) {
+    for (int r = 0; r < 4 && Y0 + r < H; ++r)
+      for (int c = 0; c < 4 && X0 + c < W; ++c) dst[(size_t)r * W + c] = (uint8_t)lv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cost volume (reference: src/depth_extract.cu:135-244).
+// One wave per coarse cell (4x4 pixels), lane = inverse-depth sample (two per
+// lane for N = 128). Non-node cells are written as zeros, so the volume needs
+// no separate clear. Slots {0,0,1,2,3} read reference 0 twice: it is evaluated
+// once and counted twice in the reference's slot tree ((c0+c4)+c2)+(c1+c3).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
+                                              const float* __restrict__ depth_table, int nblocks) {
+  const int lb_global = xcd_logical_block(blockIdx.x, nblocks);
+  const int f = find_frame(bm, lb_global);
+  const FrameDev& F = frames[f];
+  const int W = F.W, H = F.H, Wc = F.Wc;
+  const int lane = threadIdx.x & 63;
+  const int cell = (lb_global - bm.start[f]) * 4 + (threadIdx.x >> 6);
+  if (cell >= Wc * F.Hc) return;
+  const int cyy = cell / Wc, cxx = cell - cyy * Wc;
+  float* out = F.cost[0] + (size_t)cell * N;
+  const int x = 4 * cxx, y = 4 * cyy;
+  const int qs = 1 << F.levels[(size_t)y * W + x];
+  const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
+  if (F.count && lane == 0) F.count[cell] = node ? 1 : 0;
+  if (!node) {
+#pragma unroll
+    for (int k = 0; k < N / 64; ++k) out[lane + 64 * k] = 0.0f;
+    return;
+  }
+  const uint8_t* key = F.key;
+  const int pitch = F.pitch;
+  float patch[3][3];
+#pragma unroll
+  for (int j = -1; j <= 1; ++j)
+#pragma unroll
+    for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = (float)key[(size_t)(y + j) * pitch + (x + i)];
+  const V3 dir = ray_dir(x, y, F.fx, F.fy, F.cx, F.cy);
+#pragma unroll
+  for (int k = 0; k < N / 64; ++k) {
+    const int d = lane + 64 * k;
+    const float depth = depth_table[d];
+    V3 P;
+    P.x = dir.x * depth;
+    P.y = dir.y * depth;
+    P.z = dir.z * depth;
+    float cr[4], nr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float* Tr = F.T[r];
+      const V3 q = se3_apply(Tr, P);
+      const float pxf = (F.fx * q.x) / q.z + F.cx;
+      const float pyf = (F.fy * q.y) / q.z + F.cy;
+      const int pxi = trunc_i32((double)pxf + 0.5);
+      const int pyi = trunc_i32((double)pyf + 0.5);
+      if (pxi >= 2 && pxi < W - 2 && pyi >= 2 && pyi < H - 2) {
+        const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
+        const uint8_t* ref = F.ref[r];
+        float sad = 0.0f;
+#pragma unroll
+        for (int j = -1; j <= 1; ++j) {
+#pragma unroll
+          for (int i = -1; i <= 1; ++i) {
+            int chx = trunc_i32((double)((pxf + u2u * (float)i) + v2u * (float)j) + 0.5);
+            int chy = trunc_i32((double)((pyf + u2v * (float)i) + v2v * (float)j) + 0.5);
+            chx = chx < 0 ? 0 : (chx > W - 1 ? W - 1 : chx);
+            chy = chy < 0 ? 0 : (chy > H - 1 ? H - 1 : chy);
+            sad += fabsf(patch[i + 1][j + 1] - (float)ref[(size_t)chy * pitch + chx]);
+          }
+        }
+        cr[r] = sad;
+        nr[r] = 1.0f;
+      } else {
+        cr[r] = 0.0f;
+        nr[r] = 0.0f;
+      }
+    }
+    const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
+    const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
+    out[d] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cost pyramid (reference: src/bpextract.cu:59-78,169-198). One wave builds
+// one level-3 cell and its whole subtree (4 level-2, 16 level-1 cells), reading
+// level 0 once. Children are summed from 0.0f in the order (2x,2y), (2x,2y+1),
+// (2x+1,2y), (2x+1,2y+1), skipping those outside the level.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ frames, BlockMap bm) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int lane = threadIdx.x & 63;
+  const int c3 = (blockIdx.x - bm.start[f]) * 4 + (threadIdx.x >> 6);
+  if (c3 >= F.lw[3] * F.lh[3]) return;
+  const int y3 = c3 / F.lw[3], x3 = c3 - y3 * F.lw[3];
+  const float* C0 = F.cost[0];
+  float* C1 = F.cost[1];
+  float* C2 = F.cost[2];
+  float* C3 = F.cost[3];
+  const int w0 = F.lw[0], h0 = F.lh[0], w1 = F.lw[1], h1 = F.lh[1], w2 = F.lw[2], h2 = F.lh[2];
+#pragma unroll
+  for (int k = 0; k < N / 64; ++k) {
+    const int d = lane + 64 * k;
+    float s3 = 0.0f;
+    for (int i2 = 0; i2 < 2; ++i2)
+      for (int j2 = 0; j2 < 2; ++j2) {
+        const int x2 = 2 * x3 + i2, y2 = 2 * y3 + j2;
+        if (x2 >= w2 || y2 >= h2) continue;
+        float s2 = 0.0f;
+        for (int i1 = 0; i1 < 2; ++i1)
+          for (int j1 = 0; j1 < 2; ++j1) {
+            const int x1 = 2 * x2 + i1, y1 = 2 * y2 + j1;
+            if (x1 >= w1 || y1 >= h1) continue;
+            float s1 = 0.0f;
+            for (int i0 = 0; i0 < 2; ++i0)
+              for (int j0 = 0; j0 < 2; ++j0) {
+                const int xx = 2 * x1 + i0, yy = 2 * y1 + j0;
+                if (xx < w0 && yy < h0) s1 += C0[((size_t)yy * w0 + xx) * N + d];
+              }
+            C1[((size_t)y1 * w1 + x1) * N + d] = s1;
+            s2 += s1;
+          }
+        C2[((size_t)y2 * w2 + x2) * N + d] = s2;
+        s3 += s2;
+      }
+    C3[((size_t)c3) * N + d] = s3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) {
+    const int tx = (hframes[i].W + 15) / 16, ty = (hframes[i].H + 15) / 16;
+    counts[i] = (((tx + 3) / 4) * ty + 3) / 4;
+  }
+  const int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_quadtree, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].Wc * hframes[i].Hc + 3) / 4;
+  const int nb = make_block_map(bm, counts, n);
+  if (nb <= 0) return;
+  if (N == 64)
+    hipLaunchKernelGGL(k_cost<64>, dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+  else
+    hipLaunchKernelGGL(k_cost<128>, dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+}
+
+void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].lw[3] * hframes[i].lh[3] + 3) / 4;
+  const int nb = make_block_map(bm, counts, n);
+  if (nb <= 0) return;
+  if (N == 64)
+    hipLaunchKernelGGL(k_pyramid<64>, dim3(nb), dim3(256), 0, s, dframes, bm);
+  else
+    hipLaunchKernelGGL(k_pyramid<128>, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+}  // namespace qm

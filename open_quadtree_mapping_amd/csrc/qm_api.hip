@@ -1,0 +1,629 @@
+// qm_api.hip -- C ABI of libqm: stage-level entry points and batched keyframes.
+//
+// Memory: a batch owns one device arena holding every per-stream buffer
+// (quadtree levels, cost pyramid, messages of all four BP levels, Thomas
+// coefficients, row results, dense map, fusion seeds (double-buffered) and the
+// fusion z-buffer); nothing is allocated per keyframe. Stage-level calls are
+// for integration and tests: they allocate their scratch, run on the given
+// stream and synchronise it before returning.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/qm.h"
+#include "qm_device.h"
+#include "qm_launch.h"
+
+using namespace qm;
+
+namespace {
+
+thread_local std::string g_err;
+std::atomic<size_t> g_bytes{0};
+std::atomic<size_t> g_cap{0};
+std::mutex g_mu;
+std::unordered_map<void*, size_t> g_sizes;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define QM_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail(QM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  const size_t cap = g_cap.load();
+  if (cap && g_bytes.load() + bytes > cap)
+    return fail(QM_ERR_ALLOC, "device allocation of " + std::to_string(bytes) + " bytes refused by the cap");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(QM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sizes[*p] = bytes;
+  g_bytes += bytes;
+  return QM_OK;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sizes.find(p);
+    if (it != g_sizes.end()) {
+      g_bytes -= it->second;
+      g_sizes.erase(it);
+    }
+  }
+  (void)hipFree(p);
+}
+
+// RAII scratch list for stage-level calls
+struct Scratch {
+  std::vector<void*> ptrs;
+  int get(void** p, size_t bytes) {
+    int rc = dev_alloc(p, bytes);
+    if (rc == QM_OK) ptrs.push_back(*p);
+    return rc;
+  }
+  ~Scratch() {
+    for (void* p : ptrs) dev_free(p);
+  }
+};
+
+int check_params(const qm_params* p) {
+  if (!p) return fail(QM_ERR_INVALID, "params is NULL");
+  if (p->n_hyp != 64 && p->n_hyp != 128) return fail(QM_ERR_INVALID, "n_hyp must be 64 or 128");
+  if (!(p->idepth_max > p->idepth_min) || !(p->idepth_min > 0.0))
+    return fail(QM_ERR_INVALID, "need 0 < idepth_min < idepth_max");
+  for (int i = 0; i < 4; ++i)
+    if (p->bp_iters[i] < 0 || p->bp_iters[i] > 1000) return fail(QM_ERR_INVALID, "bp_iters out of range");
+  if (!(p->lambda >= 0.0)) return fail(QM_ERR_INVALID, "lambda must be >= 0");
+  return QM_OK;
+}
+
+int check_size(int W, int H, int pitch) {
+  if (W < 16 || H < 16 || W > 16384 || H > 16384) return fail(QM_ERR_INVALID, "image size out of range [16, 16384]");
+  if (pitch < W) return fail(QM_ERR_INVALID, "pitch < width");
+  return QM_OK;
+}
+
+void host_depth_table(int N, double imin, double imax, float* out) {
+  const double step = (imax - imin) / (double)(N - 1);
+  for (int d = 0; d < N; ++d) out[d] = (float)(1.0 / (step * d + imin));
+}
+
+// w(dI) = expf(-(dI)^2 / 400) of src/depth_upsample.cu:72, with the argument
+// rounded to float as the reference does (the division is in double).
+void host_weight_lut(float* out) {
+  for (int d = 0; d < 256; ++d) {
+    const float arg = (float)(-(double)(d * d) / 400.0);
+    out[d] = (float)std::exp((double)arg);
+  }
+}
+
+void init_geometry(FrameDev& F, int W, int H, int pitch) {
+  std::memset(&F, 0, sizeof(F));
+  F.W = W;
+  F.H = H;
+  F.pitch = pitch;
+  F.Wc = W / 4;
+  F.Hc = H / 4;
+  F.lw[0] = F.Wc;
+  F.lh[0] = F.Hc;
+  for (int l = 1; l < 4; ++l) {
+    F.lw[l] = (F.lw[l - 1] + 1) / 2;
+    F.lh[l] = (F.lh[l - 1] + 1) / 2;
+  }
+  F.row_seg = 16 * (W / 16);
+  F.col_seg = 8 * (H / 8);
+  F.edge_r = (W % 16) ? 1 : 0;
+  F.edge_c = (H % 16) ? 1 : 0;
+}
+
+BpLaunchParams bp_params(const qm_params* p) {
+  BpLaunchParams b;
+  b.p1 = p->p1;
+  b.p2 = p->p2;
+  b.idepth_min = p->idepth_min;
+  b.idepth_step = (p->idepth_max - p->idepth_min) / (double)(p->n_hyp - 1);
+  return b;
+}
+
+// The whole hierarchical BP: pyramid, coarse-to-fine half-sweeps, extraction.
+void run_bp(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, hipStream_t s) {
+  const int N = p->n_hyp;
+  const BpLaunchParams bp = bp_params(p);
+  launch_pyramid(df, hf, n, N, s);
+  for (int l = 3; l >= 0; --l) {
+    const int iters = p->bp_iters[l];
+    if (iters == 0) {
+      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, s);
+      continue;
+    }
+    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, s);
+  }
+  launch_extract(df, hf, n, N, bp, s);
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// bytes of the per-frame pipeline state for N hypotheses
+size_t frame_bytes(const FrameDev& F, int N) {
+  const size_t px = (size_t)F.W * F.H;
+  size_t b = align256(px);  // levels
+  for (int l = 0; l < 4; ++l) {
+    const size_t c = (size_t)F.lw[l] * F.lh[l];
+    b += align256(c * N * 4) + align256(c * 4 * N * 4);
+  }
+  const size_t cc = (size_t)F.Wc * F.Hc;
+  b += 2 * align256(cc * 4);               // dcoarse, label
+  b += align256(px * 16);                  // fwd
+  b += align256(px * 8);                   // rowxn
+  b += align256(px * 4);                   // dense
+  b += 2 * align256(px * 16);              // seeds x2
+  b += align256(px * 8);                   // zbuf
+  return b;
+}
+
+struct Carver {
+  char* p;
+  template <typename T> T* take(size_t bytes) {
+    T* r = reinterpret_cast<T*>(p);
+    p += align256(bytes);
+    return r;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct qm_batch {
+  qm_params p;
+  int n = 0;
+  std::vector<FrameDev> hf[2];
+  FrameDev* df[2] = {nullptr, nullptr};
+  void* arena = nullptr;
+  float* table = nullptr;  // depth samples, then the weight LUT (256)
+  float* wlut = nullptr;
+  std::vector<float4*> seeds[2];
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int parity = 0;
+};
+
+namespace {
+
+void run_stages(qm_batch* b, int par, hipStream_t s) {
+  const FrameDev* df = b->df[par];
+  const FrameDev* hf = b->hf[par].data();
+  const int n = b->n, N = b->p.n_hyp;
+  launch_quadtree(df, hf, n, s);
+  launch_cost(df, hf, n, N, b->table, s);
+  run_bp(df, hf, n, &b->p, s);
+  launch_upsample(df, hf, n, 0, b->wlut, b->p.lambda, true, s);
+  launch_fuse(df, hf, n, false, s);
+}
+
+void apply_desc(FrameDev& F, const qm_frame_desc& d) {
+  F.fx = d.cam[0];
+  F.fy = d.cam[1];
+  F.cx = d.cam[2];
+  F.cy = d.cam[3];
+  F.key = d.key;
+  for (int r = 0; r < 4; ++r) {
+    F.ref[r] = d.refs[r];
+    std::memcpy(F.T[r], d.T_ref_key[r], sizeof(float) * 12);
+  }
+  std::memcpy(F.T_fuse, d.T_last_to_cur, sizeof(float) * 12);
+  F.prefill = d.prefill;
+  F.depth_out = d.depth_out;
+  F.dense_out = d.dense_out;
+  F.sparse_out = d.sparse_out;
+}
+
+int check_desc(const qm_frame_desc& d) {
+  int rc = check_size(d.W, d.H, d.pitch);
+  if (rc) return rc;
+  if (!d.key) return fail(QM_ERR_INVALID, "frame key image is NULL");
+  for (int r = 0; r < 4; ++r)
+    if (!d.refs[r]) return fail(QM_ERR_INVALID, "frame reference image is NULL");
+  if (!(d.cam[0] != 0.0f) || !(d.cam[1] != 0.0f)) return fail(QM_ERR_INVALID, "focal length is zero");
+  return QM_OK;
+}
+
+int upload_frames(qm_batch* b) {
+  for (int par = 0; par < 2; ++par)
+    QM_HIP(hipMemcpy(b->df[par], b->hf[par].data(), sizeof(FrameDev) * b->n, hipMemcpyHostToDevice));
+  return QM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qm_last_error(void) { return g_err.c_str(); }
+
+void qm_default_params(qm_params* p) {
+  if (!p) return;
+  p->n_hyp = 64;
+  p->idepth_min = 0.01;
+  p->idepth_max = 2.0;
+  p->p1 = 0.003f;
+  p->p2 = 0.01f;
+  p->bp_iters[0] = 4;
+  p->bp_iters[1] = 10;
+  p->bp_iters[2] = 10;
+  p->bp_iters[3] = 10;
+  p->lambda = 10.0;
+  p->min_refs = 11;
+  p->max_refs = 60;
+  p->ref_cos = 0.95f;
+  p->ref_baseline = 0.03f;
+}
+
+void qm_set_alloc_cap(size_t bytes) { g_cap.store(bytes); }
+size_t qm_device_bytes_in_use(void) { return g_bytes.load(); }
+
+int qm_quadtree(const uint8_t* img, int W, int H, int pitch, uint8_t* levels, void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if (!img || !levels) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, pitch);
+  F.key = img;
+  F.levels = levels;
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_quadtree((FrameDev*)df, &F, 1, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int H, int pitch,
+                   const float T_ref_key[48], const float cam[4], const qm_params* p, const uint8_t* levels,
+                   float* cost, int32_t* count, void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if ((rc = check_params(p))) return rc;
+  if (!key || !refs || !T_ref_key || !cam || !levels || !cost) return fail(QM_ERR_INVALID, "NULL pointer");
+  for (int r = 0; r < 4; ++r)
+    if (!refs[r]) return fail(QM_ERR_INVALID, "NULL reference");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, pitch);
+  F.key = key;
+  for (int r = 0; r < 4; ++r) {
+    F.ref[r] = refs[r];
+    std::memcpy(F.T[r], T_ref_key + 12 * r, sizeof(float) * 12);
+  }
+  F.fx = cam[0];
+  F.fy = cam[1];
+  F.cx = cam[2];
+  F.cy = cam[3];
+  F.levels = const_cast<uint8_t*>(levels);
+  F.cost[0] = cost;
+  F.count = count;
+  std::vector<float> tab(p->n_hyp);
+  host_depth_table(p->n_hyp, p->idepth_min, p->idepth_max, tab.data());
+  Scratch sc;
+  void *df, *dt;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  if ((rc = sc.get(&dt, sizeof(float) * p->n_hyp))) return rc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  QM_HIP(hipMemcpyAsync(dt, tab.data(), sizeof(float) * p->n_hyp, hipMemcpyHostToDevice, s));
+  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const qm_params* p, float* depth_coarse,
+                  int32_t* label, float* messages, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if ((rc = check_params(p))) return rc;
+  if (!cost || !levels || !depth_coarse || !label) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = p->n_hyp;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.levels = const_cast<uint8_t*>(levels);
+  F.cost[0] = const_cast<float*>(cost);
+  F.dcoarse = depth_coarse;
+  F.label = label;
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  for (int l = 0; l < 4; ++l) {
+    const size_t c = (size_t)F.lw[l] * F.lh[l];
+    void* m;
+    if ((rc = sc.get(&m, c * 4 * N * sizeof(float)))) return rc;
+    F.msg[l] = (float*)m;
+    if (l > 0) {
+      void* cl;
+      if ((rc = sc.get(&cl, c * N * sizeof(float)))) return rc;
+      F.cost[l] = (float*)cl;
+    }
+  }
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  run_bp((FrameDev*)df, &F, 1, p, s);
+  if (messages) launch_copy_messages((FrameDev*)df, &F, 1, N, messages, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch, const qm_params* p, float* dense,
+                void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if ((rc = check_params(p))) return rc;
+  if (!sparse || !img || !dense) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, pitch);
+  F.key = img;
+  F.sparse_in = sparse;
+  F.dense = dense;
+  float lut[256];
+  host_weight_lut(lut);
+  Scratch sc;
+  void *df, *fwd, *rxn, *dl;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  if ((rc = sc.get(&fwd, (size_t)W * H * 16))) return rc;
+  if ((rc = sc.get(&rxn, (size_t)W * H * 8))) return rc;
+  if ((rc = sc.get(&dl, sizeof(lut)))) return rc;
+  F.fwd = (float4*)fwd;
+  F.rowxn = (float2*)rxn;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  QM_HIP(hipMemcpyAsync(dl, lut, sizeof(lut), hipMemcpyHostToDevice, s));
+  launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, p->lambda, false, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_upsample_general(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
+                        double lambda, int row_seg, int col_seg, float* dense, float* row_x, float* row_n,
+                        void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!d || !w_row || !w_col || !dense) return fail(QM_ERR_INVALID, "NULL pointer");
+  if ((row_x == nullptr) != (row_n == nullptr)) return fail(QM_ERR_INVALID, "row_x and row_n go together");
+  if (row_seg < 0 || col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.row_seg = row_seg;
+  F.col_seg = col_seg;
+  F.g_d = d;
+  F.g_conf = conf;
+  F.g_wrow = w_row;
+  F.g_wcol = w_col;
+  F.row_x_out = row_x;
+  F.row_n_out = row_n;
+  F.dense = dense;
+  Scratch sc;
+  void *df, *fwd, *rxn;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  if ((rc = sc.get(&fwd, (size_t)W * H * 16))) return rc;
+  if ((rc = sc.get(&rxn, (size_t)W * H * 8))) return rc;
+  F.fwd = (float4*)fwd;
+  F.rowxn = (float2*)rxn;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_upsample((FrameDev*)df, &F, 1, 2, nullptr, lambda, false, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_fuse(float* seeds, const float* depth_in, int W, int H, const float cam[4], const float T_last_to_cur[12],
+            float* depth_out, float* seeds_out, int32_t* table, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!seeds || !depth_in || !cam || !T_last_to_cur || !seeds_out) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.fx = cam[0];
+  F.fy = cam[1];
+  F.cx = cam[2];
+  F.cy = cam[3];
+  std::memcpy(F.T_fuse, T_last_to_cur, sizeof(float) * 12);
+  F.seeds = (float4*)seeds;
+  F.seeds_new = (float4*)seeds_out;
+  F.dense = const_cast<float*>(depth_in);
+  F.depth_out = depth_out;
+  F.table_out = table;
+  Scratch sc;
+  void *df, *zb;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  if ((rc = sc.get(&zb, (size_t)W * H * 8))) return rc;
+  F.zbuf = (unsigned long long*)zb;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_fuse((FrameDev*)df, &F, 1, true, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_batch** out) {
+  if (!out) return fail(QM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!frames || n <= 0 || n > kMaxFrames) return fail(QM_ERR_INVALID, "need 1..128 frames");
+  for (int i = 0; i < n; ++i)
+    if ((rc = check_desc(frames[i]))) return rc;
+  qm_batch* b = new qm_batch();
+  b->p = *p;
+  b->n = n;
+  const int N = p->n_hyp;
+  std::vector<FrameDev> base(n);
+  size_t total = align256(sizeof(float) * (N + 256));
+  for (int i = 0; i < n; ++i) {
+    init_geometry(base[i], frames[i].W, frames[i].H, frames[i].pitch);
+    total += frame_bytes(base[i], N);
+  }
+  void* fd0 = nullptr;
+  void* fd1 = nullptr;
+  if ((rc = dev_alloc(&b->arena, total)) || (rc = dev_alloc(&fd0, sizeof(FrameDev) * n)) ||
+      (rc = dev_alloc(&fd1, sizeof(FrameDev) * n))) {
+    dev_free(b->arena);
+    dev_free(fd0);
+    dev_free(fd1);
+    delete b;
+    return rc;
+  }
+  b->df[0] = (FrameDev*)fd0;
+  b->df[1] = (FrameDev*)fd1;
+  Carver cv{(char*)b->arena};
+  b->table = cv.take<float>(sizeof(float) * (N + 256));
+  b->wlut = b->table + N;
+  b->seeds[0].resize(n);
+  b->seeds[1].resize(n);
+  for (int i = 0; i < n; ++i) {
+    FrameDev& F = base[i];
+    const size_t px = (size_t)F.W * F.H;
+    F.levels = cv.take<uint8_t>(px);
+    for (int l = 0; l < 4; ++l) {
+      const size_t c = (size_t)F.lw[l] * F.lh[l];
+      F.cost[l] = cv.take<float>(c * N * 4);
+      F.msg[l] = cv.take<float>(c * 4 * N * 4);
+    }
+    const size_t cc = (size_t)F.Wc * F.Hc;
+    F.dcoarse = cv.take<float>(cc * 4);
+    F.label = cv.take<int32_t>(cc * 4);
+    F.fwd = cv.take<float4>(px * 16);
+    F.rowxn = cv.take<float2>(px * 8);
+    F.dense = cv.take<float>(px * 4);
+    b->seeds[0][i] = cv.take<float4>(px * 16);
+    b->seeds[1][i] = cv.take<float4>(px * 16);
+    F.zbuf = cv.take<unsigned long long>(px * 8);
+    apply_desc(F, frames[i]);
+  }
+  for (int par = 0; par < 2; ++par) {
+    b->hf[par] = base;
+    for (int i = 0; i < n; ++i) {
+      b->hf[par][i].seeds = b->seeds[par][i];
+      b->hf[par][i].seeds_new = b->seeds[par ^ 1][i];
+    }
+  }
+  std::vector<float> tabs(N + 256);
+  host_depth_table(N, p->idepth_min, p->idepth_max, tabs.data());
+  host_weight_lut(tabs.data() + N);
+  hipError_t e = hipMemcpy(b->table, tabs.data(), sizeof(float) * (N + 256), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+      const size_t px = (size_t)base[i].W * base[i].H;
+      e = hipMemset(b->seeds[0][i], 0, px * 16);
+      if (e == hipSuccess) e = hipMemset(b->seeds[1][i], 0, px * 16);
+    }
+  }
+  if (e != hipSuccess || upload_frames(b) != QM_OK) {
+    const std::string msg = e != hipSuccess ? std::string("setup: ") + hipGetErrorString(e) : g_err;
+    qm_batch_destroy(b);
+    return fail(QM_ERR_HIP, msg);
+  }
+  *out = b;
+  return QM_OK;
+}
+
+int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
+  if (!b || !d) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (i < 0 || i >= b->n) return fail(QM_ERR_INVALID, "frame index out of range");
+  int rc = check_desc(*d);
+  if (rc) return rc;
+  const FrameDev& F0 = b->hf[0][i];
+  if (d->W != F0.W || d->H != F0.H || d->pitch != F0.pitch)
+    return fail(QM_ERR_INVALID, "frame size differs from the planned one");
+  for (int par = 0; par < 2; ++par) apply_desc(b->hf[par][i], *d);
+  return upload_frames(b);
+}
+
+int qm_batch_run(qm_batch* b, int mode, void* stream) {
+  if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  const int par = b->parity;
+  if (mode == 0) {
+    run_stages(b, par, s);
+    QM_HIP(hipGetLastError());
+  } else if (mode == 1) {
+    if (!b->exec[par]) {
+      hipStream_t cs;
+      QM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        run_stages(b, par, cs);
+        e = hipStreamEndCapture(cs, &b->graph[par]);
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[par], b->graph[par], nullptr, nullptr, 0);
+      (void)hipStreamDestroy(cs);
+      if (e != hipSuccess) return fail(QM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    QM_HIP(hipGraphLaunch(b->exec[par], s));
+  } else {
+    return fail(QM_ERR_INVALID, "mode must be 0 or 1");
+  }
+  b->parity ^= 1;
+  return QM_OK;
+}
+
+int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
+  if (!b || !ptr) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (i < 0 || i >= b->n) return fail(QM_ERR_INVALID, "frame index out of range");
+  const FrameDev& F = b->hf[b->parity][i];
+  switch (which) {
+    case 0: *ptr = F.levels; break;
+    case 1: *ptr = F.cost[0]; break;
+    case 2: *ptr = F.dcoarse; break;
+    case 3: *ptr = F.label; break;
+    case 4: *ptr = F.dense; break;
+    case 5: *ptr = F.seeds; break;
+    default: return fail(QM_ERR_INVALID, "unknown buffer id");
+  }
+  return QM_OK;
+}
+
+int qm_batch_reset_seeds(qm_batch* b) {
+  if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  for (int i = 0; i < b->n; ++i) {
+    const size_t px = (size_t)b->hf[0][i].W * b->hf[0][i].H;
+    QM_HIP(hipMemset(b->seeds[0][i], 0, px * 16));
+    QM_HIP(hipMemset(b->seeds[1][i], 0, px * 16));
+  }
+  return QM_OK;
+}
+
+int qm_batch_destroy(qm_batch* b) {
+  if (!b) return QM_OK;
+  for (int par = 0; par < 2; ++par) {
+    if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
+    if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);
+    dev_free(b->df[par]);
+  }
+  dev_free(b->arena);
+  delete b;
+  return QM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+// qm_launch.h -- host-side launchers of the pipeline stages. Each launcher runs
+// its stage for n frames in one launch; hframes mirrors the device array.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "qm_device.h"
+
+namespace qm {
+
+// Fill bm from per-frame block counts; returns the total number of blocks.
+inline int make_block_map(BlockMap& bm, const int* counts, int n) {
+  bm.n = n;
+  int acc = 0;
+  for (int i = 0; i < n; ++i) {
+    bm.start[i] = acc;
+    acc += counts[i];
+  }
+  for (int i = n; i <= kMaxFrames; ++i) bm.start[i] = acc;
+  return acc;
+}
+
+struct BpLaunchParams {
+  float p1, p2;
+  double idepth_min, idepth_step;
+};
+
+void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, hipStream_t s);
+void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s);
+// one half-sweep of level l; first = the level's first sweep (also initialises
+// every cell's messages from the parent level, or zero at the coarsest level);
+// active = 0 turns the first sweep into a pure initialisation.
+void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
+                     bool active, const BpLaunchParams& p, hipStream_t s);
+void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
+                    hipStream_t s);
+void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
+// upsample: mode 0 = compose the sparse map from BP (+ pre-fill), weights from
+// the keyframe; 1 = explicit sparse map, weights from the keyframe;
+// 2 = generalised (d, conf, w_row, w_col given)
+void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                     double lambda, bool clear_zbuf, hipStream_t s);
+void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s);
+
+}  // namespace qm

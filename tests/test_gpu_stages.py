@@ -1,0 +1,194 @@
+"""Parity of every HIP stage with the CPU oracle (FP32), through the C ABI.
+
+Bar: quadtree levels and validity masks bit-exact; BP labels <= 0.1% different;
+depths within 1e-4 relative of the FP32 oracle. The HIP kernels follow the same
+pinned arithmetic as the oracle, so most comparisons are in fact bit-exact and
+are asserted as such where the arithmetic is identical by construction.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from open_quadtree_mapping_amd import default_params
+from open_quadtree_mapping_amd import stages as S
+from tests._data import cam_tuple, keyframe_set
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+REL = 1e-4
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    d = np.abs(a - b) / np.maximum(np.abs(b), 1e-12)
+    d[both_nan] = 0.0
+    return d
+
+
+def constructed_images():
+    rng = np.random.default_rng(11)
+    out = {}
+    out["constant"] = np.full((64, 96), 120, np.uint8)
+    yy, xx = np.mgrid[0:64, 0:96]
+    out["checker20"] = np.where((xx + yy) % 2 == 0, 100, 120).astype(np.uint8)
+    out["random"] = rng.integers(0, 256, size=(80, 112), dtype=np.uint8)
+    img = np.full((64, 96), 50, np.uint8)
+    img[:, 52:] = 200  # step on a 4-pixel boundary, not 8 or 16
+    out["step4"] = img
+    # smooth ramp with flat patches: exercises levels 3 and 4 and near-threshold deviations
+    ramp = (xx * 0.7 + yy * 0.3).astype(np.int32)
+    ramp[8:40, 16:48] = 77
+    ramp[40:56, 60:76] = ramp[40:56, 60:76] // 13 * 13
+    out["ramp"] = np.clip(ramp, 0, 255).astype(np.uint8)
+    # random piecewise-flat blocks with +-10 noise: right at the thresholds
+    base = rng.integers(0, 200, size=(8, 12))
+    blk = np.kron(base, np.ones((16, 16), np.int64))
+    noise = rng.integers(-10, 11, size=blk.shape) * (rng.random(blk.shape) < 0.3)
+    out["blocks"] = np.clip(blk + noise, 0, 255).astype(np.uint8)
+    # partial tiles: width 376-like remainder (W % 16 = 8) and odd sizes
+    out["partial"] = np.clip(np.kron(rng.integers(0, 255, size=(6, 6)), np.ones((13, 17))), 0, 255)[:70, :90].astype(
+        np.uint8)
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(constructed_images().keys()))
+def test_quadtree_constructed(name):
+    img = constructed_images()[name]
+    print(f"quadtree constructed {name} {img.shape}", flush=True)
+    got = host(S.quadtree(dev(img)))
+    ref = O.quadtree(img)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("W,H,seed", [(376, 240, 3), (752, 480, 5), (1280, 720, 9)])
+def test_quadtree_scene(W, H, seed):
+    ks = keyframe_set(W, H, seed)
+    print(f"quadtree scene {W}x{H} seed {seed}", flush=True)
+    got = host(S.quadtree(dev(ks["key"])))
+    assert np.array_equal(got, O.quadtree(ks["key"]))
+
+
+@pytest.mark.parametrize("W,H,seed,N,imin,imax", [(376, 240, 3, 64, 0.02, 2.0), (752, 480, 5, 64, 0.01, 2.0),
+                                                  (376, 240, 4, 128, 0.02, 1.0 / 0.3)])
+def test_cost_volume(W, H, seed, N, imin, imax):
+    ks = keyframe_set(W, H, seed)
+    print(f"cost volume {W}x{H} N={N}", flush=True)
+    levels = O.quadtree(ks["key"])
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    cost, count = S.cost_volume(dev(ks["key"]), [dev(r) for r in ks["refs"]], ks["T"], cam_tuple(ks["cam"]),
+                                dev(levels), p, want_count=True)
+    rc, rn = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], N, imin, imax, levels)
+    assert np.array_equal(host(count), rn)
+    got = host(cost)
+    assert np.array_equal(got, rc), f"max abs diff {np.abs(got - rc).max()}"
+
+
+@pytest.mark.parametrize("W,H,seed,N", [(376, 240, 3, 64), (752, 480, 5, 64), (376, 240, 4, 128)])
+def test_bp_extract(W, H, seed, N):
+    ks = keyframe_set(W, H, seed)
+    print(f"bp {W}x{H} N={N}", flush=True)
+    imin, imax = 0.02, 2.0
+    levels = O.quadtree(ks["key"])
+    rc, _ = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], N, imin, imax, levels)
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    d, lab, msg = S.bp_extract(dev(rc), W, H, dev(levels), p, want_messages=True)
+    rd, rl, rm = O.bp_extract(rc, W, H, levels, N, imin, imax, want_messages=True)
+    gl = host(lab)
+    assert np.array_equal(gl < 0, rl < 0), "node mask differs"
+    frac = np.mean(gl != rl)
+    assert frac <= 1e-3, f"label mismatch fraction {frac}"
+    assert np.array_equal(host(msg), rm), "messages differ"
+    assert np.array_equal(gl, rl)
+    gd = host(d)
+    assert np.nanmax(rel_err(gd[rl >= 0], rd[rl >= 0])) <= REL
+
+
+def test_bp_uniform_cost_gives_label_zero():
+    # uniform data cost: all messages are 0 and the argmin tree keeps index 0
+    W, H, N = 128, 96, 64
+    levels = np.full((H, W), 2, np.uint8)
+    cost = np.full((H // 4, W // 4, N), 0.25, np.float32)
+    d, lab, msg = S.bp_extract(dev(cost), W, H, dev(levels), default_params(), want_messages=True)
+    assert np.all(host(lab) == 0)
+    assert np.all(host(msg) == 0)
+    assert np.allclose(host(d), 100.0)
+
+
+@pytest.mark.parametrize("W,H,seed", [(376, 240, 3), (752, 480, 5)])
+def test_upsample_reference_mode(W, H, seed):
+    ks = keyframe_set(W, H, seed)
+    print(f"upsample {W}x{H}", flush=True)
+    levels = O.quadtree(ks["key"])
+    rc, _ = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], 64, 0.02, 2.0, levels)
+    rd, rl = O.bp_extract(rc, W, H, levels, 64, 0.02, 2.0)
+    sparse = O.compose_sparse(W, H, rd, rl)
+    wr, wc = O.weights(ks["key"])
+    rs, cs = O.ref_segments(W, H)
+    ref = O.upsample(sparse, wr, wc, row_seg=rs, col_seg=cs)
+    got = host(S.upsample(dev(sparse), dev(ks["key"])))
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.nanmax(rel_err(got, ref)) <= REL
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("seg", [False, True])
+def test_upsample_general(seg):
+    rng = np.random.default_rng(5)
+    H, W = 96, 136
+    d = (rng.uniform(1, 10, (H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float32)
+    conf = rng.uniform(1e-3, 1.0, (H, W)).astype(np.float32)
+    wr = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H, W + 1))).astype(np.float32)
+    wc = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H + 1, W))).astype(np.float32)
+    rs, cs = (128, 88) if seg else (W, H)
+    ref, rx, rn = O.upsample(d, wr, wc, conf=conf, row_seg=rs, col_seg=cs, want_rows=True)
+    got, gx, gn = S.upsample_general(dev(d), dev(wr), dev(wc), conf=dev(conf), row_seg=rs, col_seg=cs,
+                                     want_rows=True)
+    assert np.array_equal(host(gx), rx)
+    assert np.array_equal(host(gn), rn)
+    assert np.array_equal(host(got), ref)
+
+
+def fusion_case(W, H, seed):
+    rng = np.random.default_rng(seed)
+    ks = keyframe_set(W, H, 3 if W == 376 else 5)
+    cam = ks["cam"]
+    seeds = np.zeros((H, W, 4), np.float32)
+    seeds[..., 0] = rng.uniform(0.0, 30.0, (H, W))
+    seeds[..., 1] = rng.uniform(0.0, 30.0, (H, W))
+    seeds[..., 2] = ks["gt"] * rng.uniform(0.95, 1.05, (H, W))
+    seeds[..., 3] = rng.uniform(0.001, 5.0, (H, W))
+    seeds[rng.random((H, W)) < 0.05, 0] = 0.0  # bad points
+    depth = (ks["gt"] * rng.uniform(0.9, 1.1, (H, W))).astype(np.float32)
+    depth[rng.random((H, W)) < 0.02] = np.nan
+    depth[rng.random((H, W)) < 0.02] = -1.0
+    from open_quadtree_mapping_amd.se3 import relative_pose
+    T = relative_pose(ks["poses"][-1], ks["poses"][-2])
+    return seeds, depth, cam, T
+
+
+@pytest.mark.parametrize("W,H,seed", [(376, 240, 1), (752, 480, 2)])
+def test_fuse(W, H, seed):
+    seeds, depth, cam, T = fusion_case(W, H, seed)
+    print(f"fuse {W}x{H}", flush=True)
+    rs_w, rd, rs, rt = O.fuse(seeds, depth, T, cam)
+    gs = dev(seeds)
+    gd, gsn, gt = S.fuse(gs, dev(depth), cam_tuple(cam), T, want_table=True)
+    assert np.array_equal(host(gt), rt), "warp table differs"
+    assert np.array_equal(host(gs), rs_w, equal_nan=True), "warped seeds differ"
+    god = host(gd)
+    assert np.array_equal(god == -1.0, rd == -1.0)
+    assert np.nanmax(rel_err(god, rd)) <= REL
+    assert np.array_equal(god, rd, equal_nan=True)
+    assert np.array_equal(host(gsn), rs, equal_nan=True)
