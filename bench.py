@@ -1,0 +1,274 @@
+"""Benchmark: batched per-keyframe depth maps/s at 752x480 with 64 hypotheses.
+
+One step = every stage (quadtree, cost volume, hierarchical BP + extraction,
+tridiagonal upsample, fusion) for a batch of B camera streams of 752x480
+keyframes (B = --streams, default 32), one launch per stage over all streams,
+replayed as one hipGraph. Inputs are resident in HBM before timing starts; the
+fusion state carries over from step to step as for consecutive keyframes.
+
+Also measured, outside the timed region:
+  * the plain per-frame GPU path (one keyframe at a time, per-stage launches,
+    no graph) -> plain_maps_per_s and batched_over_plain;
+  * per-stage kernel time from an event pair around every launch, turned into
+    achieved HBM GB/s from the algorithmic bytes of each stage (roofline);
+  * the FP32 OpenMP CPU oracle on a bounded sample (cpu_baseline).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--streams B]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "depth maps/s @752×480×64 hyps, batched vs plain GPU; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--streams", type=int, default=32)
+    ap.add_argument("--width", type=int, default=752)
+    ap.add_argument("--height", type=int, default=480)
+    ap.add_argument("--hyps", type=int, default=64)
+    ap.add_argument("--scenes", type=int, default=8, help="distinct rendered scenes shared round-robin by streams")
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10)):
+    """Algorithmic bytes of the BP half-sweeps for one keyframe, following the
+    skip rule of the kernels (quadtree lookup at (bx << lev, y << lev)).
+    Active update: data + 4 incoming + 4 outgoing messages = 36N bytes; a cell
+    initialised from its parent at a level's first sweep: 16N read + 16N write
+    (16N write only at the coarsest level)."""
+    import numpy as np
+    Wc, Hc = W // 4, H // 4
+    lw, lh = [Wc], [Hc]
+    for _ in range(3):
+        lw.append((lw[-1] + 1) // 2)
+        lh.append((lh[-1] + 1) // 2)
+    total = 0
+    updates = 0
+    for l in range(4):
+        w, h = lw[l], lh[l]
+        lev = l + 2
+        ys, xs = np.mgrid[0:h, 0:w]
+        bx = xs >> 1
+        if lev <= 4:
+            X = np.minimum(bx << lev, W - 1)
+            Y = np.minimum(ys << lev, H - 1)
+            active = levels[Y, X] <= lev
+        else:
+            active = np.ones((h, w), bool)
+        colA = (xs & 1) == (ys & 1)
+        nA = int((active & colA).sum())
+        nB = int((active & ~colA).sum())
+        cells = w * h
+        for it in range(iters[l]):
+            n_upd = nA if it % 2 == 0 else nB
+            updates += n_upd
+            total += 36 * N * n_upd
+            if it == 0:
+                total += (16 if l == 3 else 32) * N * (cells - n_upd)
+    return total, updates
+
+
+def stage_bytes(levels, W, H, N):
+    import numpy as np
+    C = (W // 4) * (H // 4)
+    lw, lh = [W // 4], [H // 4]
+    for _ in range(3):
+        lw.append((lw[-1] + 1) // 2)
+        lh.append((lh[-1] + 1) // 2)
+    sub = levels[0:4 * (H // 4):4, 0:4 * (W // 4):4].astype(np.int64)
+    ys, xs = np.mgrid[0:H // 4, 0:W // 4]
+    qs = 1 << sub
+    nodes = int((((4 * xs) % qs == 0) & ((4 * ys) % qs == 0)).sum())
+    bp, upd = bp_traffic(levels, W, H, N)
+    px = W * H
+    return {
+        "quadtree": 2 * px,
+        "cost": 5 * px + C + 4 * N * C,
+        "pyramid": 4 * N * sum(a * b for a, b in zip(lw, lh)),
+        "bp_sweep": bp,
+        "extract": nodes * (20 * N + 8) + (C - nodes) * 9,
+        "upsample_row": px + 8 * C + 8 * px + 8 * px,
+        "upsample_col": px + 8 * px + 4 * px,
+        "fuse_warp": 40 * px,
+        "fuse_update": 48 * px,
+    }, upd
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(streams shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from open_quadtree_mapping_amd import default_params
+    from open_quadtree_mapping_amd import stages as S
+    from open_quadtree_mapping_amd.batch import Batch
+    from open_quadtree_mapping_amd.scene import camera_for, forward_lateral_trajectory, make_scene, render
+    from open_quadtree_mapping_amd.se3 import relative_pose
+
+    W, H, N, B = args.width, args.height, args.hyps, args.streams
+    dev = torch.device("cuda:0")
+    cam = camera_for(W, H)
+    imin, imax = 0.02, 2.0  # 0.5 - 50 m
+    params = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+
+    t_setup = time.time()
+    nsc = min(args.scenes, B)
+    scenes = []
+    for s in range(nsc):
+        traj = forward_lateral_trajectory(6, seed=100 + s)
+        sc = make_scene(100 + s)
+        imgs = [render(sc, cam, T)[0] for T in traj]
+        k = 5
+        T = np.stack([relative_pose(traj[k - 1 - i], traj[k]) for i in range(4)])
+        scenes.append((imgs, T, traj))
+    print(f"[bench] rendered {nsc} scenes in {time.time() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    camt = (cam.fx, cam.fy, cam.cx, cam.cy)
+    frames = []
+    for i in range(B):
+        imgs, T, traj = scenes[i % nsc]
+        d = [torch.from_numpy(im).to(dev) for im in imgs[1:]]  # distinct device copies per stream
+        frames.append(dict(key=d[4], refs=[d[3], d[2], d[1], d[0]], T=T, cam=camt))
+
+    # ---- batched path (hipGraph) -------------------------------------------------
+    batch = Batch(frames, params, outputs=("depth",))
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        batch.run(graph=True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        batch.run(graph=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall_s = t1 - t0
+    ev_ms = e0.elapsed_time(e1)
+    maps_per_s = B * args.steps / wall_s
+    ms_per_step = ev_ms / args.steps
+    print(f"[bench] batched: {maps_per_s:.1f} maps/s ({ms_per_step:.3f} ms/step of {B} maps)", file=sys.stderr,
+          flush=True)
+
+    # ---- per-stage kernel time (events around every launch) ----------------------
+    reps = 3
+    acc = {}
+    for _ in range(reps):
+        st = batch.run_timed()
+        for k, (ms, cnt) in st.items():
+            a = acc.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+    levels_per_stream = [S.quadtree(f["key"]).cpu().numpy() for f in frames]
+    stage_alg = {}
+    for lv in levels_per_stream:
+        sb, _ = stage_bytes(lv, W, H, N)
+        for k, v in sb.items():
+            stage_alg[k] = stage_alg.get(k, 0) + v
+    stages = {}
+    for k, (ms, cnt) in acc.items():
+        per_run_ms = ms / reps
+        launches = cnt // reps
+        gbs = stage_alg[k] / (per_run_ms * 1e-3) / 1e9 if per_run_ms > 0 else 0.0
+        stages[k] = {"ms_per_run": round(per_run_ms, 4), "launches": launches,
+                     "avg_launch_us": round(1e3 * per_run_ms / max(launches, 1), 2),
+                     "alg_bytes_per_run": int(stage_alg[k]), "achieved_GBs": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(stages, key=lambda k: stages[k]["ms_per_run"])
+    d = stages[dom]
+    roofline = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["hbm_frac"], "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": int(d["alg_bytes_per_run"] / max(d["launches"], 1)),
+                "avg_launch_us": d["avg_launch_us"]}
+    print(f"[bench] stages: {json.dumps(stages)}", file=sys.stderr, flush=True)
+
+    # ---- plain per-frame path ---------------------------------------------------
+    plain = None
+    if not args.no_plain:
+        singles = [Batch([f], params, outputs=("depth",)) for f in frames]
+        for s_ in singles:
+            s_.run(graph=False)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            for s_ in singles:
+                s_.run(graph=False)
+        torch.cuda.synchronize()
+        tp1 = time.perf_counter()
+        plain = B * args.plain_steps / (tp1 - tp0)
+        for s_ in singles:
+            s_.close()
+        print(f"[bench] plain per-frame: {plain:.1f} maps/s", file=sys.stderr, flush=True)
+    batch.close()
+
+    # ---- CPU baseline: FP32 OpenMP oracle on a bounded sample -------------------
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        O.lib()
+        threads = O.get_threads()
+        imgs, T, traj = scenes[0]
+        seeds = np.zeros((H, W, 4), np.float32)
+        eye = np.eye(3, 4, dtype=np.float32)
+        O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye)  # warm
+        n_done = 0
+        tc0 = time.perf_counter()
+        while True:
+            O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye)
+            n_done += 1
+            if time.perf_counter() - tc0 > args.cpu_seconds or n_done >= 400:
+                break
+        tc = time.perf_counter() - tc0
+        cpu = {"value": round(n_done / tc, 3), "unit": "depth maps/s", "cores": threads, "kind": "port",
+               "sample": f"{n_done} keyframes of {W}x{H}x{N}, full pipeline, FP32 OpenMP oracle, {tc:.1f}s"}
+        print(f"[bench] cpu oracle: {cpu['value']} maps/s on {threads} threads", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(maps_per_s, 2),
+        "unit": "depth maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded planar scenes (6 textured rectangles, fractal value noise) rendered to uint8",
+        "config": {"workload": f"{B} streams x {W}x{H} keyframes, {N} inverse-depth hypotheses, full pipeline "
+                               "(quadtree, cost volume, HBP, upsample, fusion), one hipGraph replay per step",
+                   "streams": B, "width": W, "height": H, "hypotheses": N},
+        "plain_maps_per_s": None if plain is None else round(plain, 2),
+        "batched_over_plain": None if plain is None else round(maps_per_s / plain, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": stages,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -130,6 +130,21 @@ int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* frame);
 /* Run every stage for every frame. mode 0: one launch per stage over all
  * frames; mode 1: the same captured once as a hipGraph and replayed. */
 int qm_batch_run(qm_batch* b, int mode, void* stream);
+/* Stage kinds for qm_batch_run_timed. */
+#define QM_STAGE_QUADTREE 0
+#define QM_STAGE_COST 1
+#define QM_STAGE_PYRAMID 2
+#define QM_STAGE_BP_SWEEP 3
+#define QM_STAGE_EXTRACT 4
+#define QM_STAGE_UPSAMPLE_ROW 5
+#define QM_STAGE_UPSAMPLE_COL 6
+#define QM_STAGE_FUSE_WARP 7
+#define QM_STAGE_FUSE_UPDATE 8
+#define QM_NUM_STAGES 9
+/* One plain run (as mode 0) with a hipEvent pair around every launch; returns
+ * per stage kind the summed kernel time (ms) and the number of launches, after
+ * synchronising the stream. For measurement only. */
+int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
 /* Device pointer of an internal buffer of frame i, for inspection:
  * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state) */
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);

@@ -15,6 +15,10 @@ F = C.c_float
 D = C.c_double
 
 
+STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", "upsample_col", "fuse_warp",
+          "fuse_update")
+
+
 class QmError(RuntimeError):
     pass
 
@@ -67,6 +71,7 @@ SIGNATURES = {
     "qm_batch_create": (I, [P, I, P, P]),
     "qm_batch_set_frame": (I, [P, I, P]),
     "qm_batch_run": (I, [P, I, P]),
+    "qm_batch_run_timed": (I, [P, P, P, P]),
     "qm_batch_buffer": (I, [P, I, I, P]),
     "qm_batch_reset_seeds": (I, [P]),
     "qm_batch_destroy": (I, [P]),

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import check, default_params, lib, qm_frame_desc, qm_params
+from ._lib import STAGES, check, default_params, lib, qm_frame_desc, qm_params
 
 __all__ = ["Batch", "make_desc"]
 
@@ -76,6 +76,15 @@ class Batch:
     def run(self, graph: bool = False, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream()
         check(lib().qm_batch_run(self._h, 1 if graph else 0, C.c_void_p(s.cuda_stream)))
+
+    def run_timed(self, stream=None):
+        """One plain run with an event pair around every launch.
+        Returns {stage: (total_ms, launches)}."""
+        s = stream if stream is not None else torch.cuda.current_stream()
+        ms = (C.c_double * len(STAGES))()
+        cnt = (C.c_int * len(STAGES))()
+        check(lib().qm_batch_run_timed(self._h, C.c_void_p(s.cuda_stream), ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES)}
 
     def buffer_ptr(self, i, which) -> int:
         p = C.c_void_p()

@@ -288,39 +288,74 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                     double lambda, bool clear_zbuf, hipStream_t s) {
-  BlockMap bm;
+static int rows_map(BlockMap& bm, const FrameDev* hframes, int n) {
   int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 63) / 64;
-  int nb = make_block_map(bm, counts, n);
-  if (nb > 0) {
-    if (mode == 0)
-      hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
-    else if (mode == 1)
-      hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
-    else
-      hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
-  }
+  return make_block_map(bm, counts, n);
+}
+static int cols_map(BlockMap& bm, const FrameDev* hframes, int n) {
+  int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;
-  nb = make_block_map(bm, counts, n);
-  if (nb > 0) {
-    if (mode == 2)
-      hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
-    else
-      hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
-  }
+  return make_block_map(bm, counts, n);
+}
+static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W * hframes[i].H + 255) / 256;
+  return make_block_map(bm, counts, n);
+}
+
+void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                         double lambda, bool clear_zbuf, hipStream_t s) {
+  BlockMap bm;
+  const int nb = rows_map(bm, hframes, n);
+  if (nb <= 0) return;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+  else
+    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+}
+
+void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                         double lambda, hipStream_t s) {
+  BlockMap bm;
+  const int nb = cols_map(bm, hframes, n);
+  if (nb <= 0) return;
+  if (mode == 2)
+    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+  else
+    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+}
+
+void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  const int nb = pixels_map(bm, hframes, n);
+  if (nb > 0) hipLaunchKernelGGL(k_fill_zbuf, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  const int nb = pixels_map(bm, hframes, n);
+  if (nb > 0) hipLaunchKernelGGL(k_fuse_warp, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  const int nb = pixels_map(bm, hframes, n);
+  if (nb > 0) hipLaunchKernelGGL(k_fuse_update, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                     double lambda, bool clear_zbuf, hipStream_t s) {
+  launch_upsample_row(dframes, hframes, n, mode, wlut, lambda, clear_zbuf, s);
+  launch_upsample_col(dframes, hframes, n, mode, wlut, lambda, s);
 }
 
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s) {
-  BlockMap bm;
-  int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W * hframes[i].H + 255) / 256;
-  const int nb = make_block_map(bm, counts, n);
-  if (nb <= 0) return;
-  if (clear_zbuf) hipLaunchKernelGGL(k_fill_zbuf, dim3(nb), dim3(256), 0, s, dframes, bm);
-  hipLaunchKernelGGL(k_fuse_warp, dim3(nb), dim3(256), 0, s, dframes, bm);
-  hipLaunchKernelGGL(k_fuse_update, dim3(nb), dim3(256), 0, s, dframes, bm);
+  if (clear_zbuf) launch_fill_zbuf(dframes, hframes, n, s);
+  launch_fuse_warp(dframes, hframes, n, s);
+  launch_fuse_update(dframes, hframes, n, s);
 }
 
 }  // namespace qm

@@ -203,19 +203,69 @@ struct qm_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int parity = 0;
+  std::vector<hipEvent_t> events;
 };
 
 namespace {
 
-void run_stages(qm_batch* b, int par, hipStream_t s) {
+// Optional per-launch timing: an event pair around every launch, attributed
+// to a stage kind (QM_STAGE_*).
+struct Timer {
+  std::vector<hipEvent_t>* pool;
+  std::vector<int> stage;
+  size_t used = 0;
+  hipStream_t s;
+  hipEvent_t next() {
+    if (used == pool->size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      pool->push_back(e);
+    }
+    return (*pool)[used++];
+  }
+  void begin(int st) {
+    stage.push_back(st);
+    (void)hipEventRecord(next(), s);
+  }
+  void end() { (void)hipEventRecord(next(), s); }
+};
+
+struct Mark {
+  Timer* t;
+  Mark(Timer* tt, int st) : t(tt) {
+    if (t) t->begin(st);
+  }
+  ~Mark() {
+    if (t) t->end();
+  }
+};
+
+void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
   const FrameDev* df = b->df[par];
   const FrameDev* hf = b->hf[par].data();
   const int n = b->n, N = b->p.n_hyp;
-  launch_quadtree(df, hf, n, s);
-  launch_cost(df, hf, n, N, b->table, s);
-  run_bp(df, hf, n, &b->p, s);
-  launch_upsample(df, hf, n, 0, b->wlut, b->p.lambda, true, s);
-  launch_fuse(df, hf, n, false, s);
+  const qm_params* p = &b->p;
+  const BpLaunchParams bp = bp_params(p);
+  { Mark m(t, QM_STAGE_QUADTREE); launch_quadtree(df, hf, n, s); }
+  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, s); }
+  { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
+  for (int l = 3; l >= 0; --l) {
+    const int iters = p->bp_iters[l];
+    if (iters == 0) {
+      Mark m(t, QM_STAGE_BP_SWEEP);
+      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, s);
+      continue;
+    }
+    for (int it = 0; it < iters; ++it) {
+      Mark m(t, QM_STAGE_BP_SWEEP);
+      launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, s);
+    }
+  }
+  { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, p->lambda, true, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, p->lambda, s); }
+  { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
+  { Mark m(t, QM_STAGE_FUSE_UPDATE); launch_fuse_update(df, hf, n, s); }
 }
 
 void apply_desc(FrameDev& F, const qm_frame_desc& d) {
@@ -588,6 +638,29 @@ int qm_batch_run(qm_batch* b, int mode, void* stream) {
   return QM_OK;
 }
 
+int qm_batch_run_timed(qm_batch* b, void* stream, double* stage_ms, int* stage_launches) {
+  if (!b || !stage_ms || !stage_launches) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  Timer t;
+  t.pool = &b->events;
+  t.s = s;
+  run_stages(b, b->parity, s, &t);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < QM_NUM_STAGES; ++i) {
+    stage_ms[i] = 0.0;
+    stage_launches[i] = 0;
+  }
+  for (size_t k = 0; k < t.stage.size(); ++k) {
+    float ms = 0.0f;
+    QM_HIP(hipEventElapsedTime(&ms, b->events[2 * k], b->events[2 * k + 1]));
+    stage_ms[t.stage[k]] += ms;
+    stage_launches[t.stage[k]] += 1;
+  }
+  b->parity ^= 1;
+  return QM_OK;
+}
+
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
   if (!b || !ptr) return fail(QM_ERR_INVALID, "NULL pointer");
   if (i < 0 || i >= b->n) return fail(QM_ERR_INVALID, "frame index out of range");
@@ -616,6 +689,7 @@ int qm_batch_reset_seeds(qm_batch* b) {
 
 int qm_batch_destroy(qm_batch* b) {
   if (!b) return QM_OK;
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   for (int par = 0; par < 2; ++par) {
     if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
     if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);

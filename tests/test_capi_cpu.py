@@ -1,0 +1,59 @@
+"""Host-side checks of the C ABI (no GPU needed): the library loads, exports
+every function include/qm.h declares, and rejects bad arguments before it
+touches the device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from open_quadtree_mapping_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "qm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_table_covers_header():
+    assert set(declared_functions()) == set(_lib.SIGNATURES)
+
+
+def test_default_params_are_reference_constants():
+    p = _lib.default_params()
+    assert p.n_hyp == 64 and p.idepth_min == 0.01 and p.idepth_max == 2.0
+    assert abs(p.p1 - 0.003) < 1e-9 and abs(p.p2 - 0.01) < 1e-9
+    assert list(p.bp_iters) == [4, 10, 10, 10] and p.lambda_ == 10.0
+    assert p.min_refs == 11 and p.max_refs == 60
+
+
+def test_invalid_arguments_fail_before_device_work():
+    L = _lib.lib()
+    rc = L.qm_quadtree(None, 8, 8, 8, None, None)
+    assert rc == -1 and b"size" in L.qm_last_error()
+    p = _lib.default_params(n_hyp=48)
+    rc = L.qm_bp_extract(C.c_void_p(1), 64, 64, C.c_void_p(1), C.byref(p), C.c_void_p(1), C.c_void_p(1), None,
+                         None)
+    assert rc == -1 and b"n_hyp" in L.qm_last_error()
+    p = _lib.default_params()
+    rc = L.qm_upsample(None, None, 64, 64, 64, C.byref(p), None, None)
+    assert rc == -1 and b"NULL" in L.qm_last_error()
+    out = C.c_void_p()
+    rc = L.qm_batch_create(None, 0, C.byref(p), C.byref(out))
+    assert rc == -1 and out.value is None
+
+
+def test_check_raises_with_message():
+    with pytest.raises(_lib.QmError, match="size"):
+        _lib.check(_lib.lib().qm_quadtree(None, 4, 4, 4, None, None))

@@ -1,0 +1,455 @@
+"""Pins of the CPU oracle that do not go through the oracle's own formulas.
+
+Each stage is checked against an independent formulation: a simulation of the
+reference kernel's shared-memory algorithm, closed forms, brute force on tiny
+inputs, a dense FP64 linear solve, or invariants the mathematics fixes. The
+reference repository ships no tests, golden vectors or fixtures for this path.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from open_quadtree_mapping_amd.scene import Camera
+
+
+# ---------------------------------------------------------------------------
+# Quadtree: simulate src/quadtree.cu:17-77 thread by thread, one 16x16 tile at
+# a time, with barrier-separated phases (pyramid_num / approve in "shared memory").
+# ---------------------------------------------------------------------------
+def quadtree_simulated(img):
+    H, W = img.shape
+    out = np.zeros((H, W), np.uint8)
+    for ty in range(0, H, 16):
+        for tx in range(0, W, 16):
+            inten = {}
+            num = {}
+            level = {}
+            alive = set()
+            for ly in range(16):
+                for lx in range(16):
+                    x, y = tx + lx, ty + ly
+                    if x < W and y < H:
+                        inten[(lx, ly)] = float(img[y, x])
+                        num[(lx, ly)] = 1
+                        level[(lx, ly)] = 0
+                        alive.add((lx, ly))
+            for i in range(1, 5):
+                s = 1 << i
+                rep = {p: (p[0] - p[0] % s, p[1] - p[1] % s) for p in alive}
+                # phase 1: last nodes of level i-1 add into their level-i representative
+                h = 1 << (i - 1)
+                for p in sorted(alive):
+                    if p[0] % h == 0 and p[1] % h == 0 and rep[p] != p:
+                        inten[rep[p]] += inten[p]
+                        num[rep[p]] += num[p]
+                approve = {rep[p]: True for p in alive}
+                # phase 2: count check, deviation test
+                broke = set()
+                for p in alive:
+                    if num[rep[p]] != s * s:
+                        broke.add(p)
+                still = alive - broke
+                for p in still:
+                    avg = inten[rep[p]] / float(s * s)
+                    if abs(inten_pixel(img, tx, ty, p) - avg) > 10.0:
+                        approve[rep[p]] = False
+                # phase 3
+                nxt = set()
+                for p in still:
+                    if approve[rep[p]]:
+                        level[p] = i
+                        nxt.add(p)
+                    else:
+                        num[rep[p]] = 0
+                alive = nxt
+            for (lx, ly), lv in level.items():
+                out[ty + ly, tx + lx] = max(lv, 2)
+    return out
+
+
+def inten_pixel(img, tx, ty, p):
+    return float(img[ty + p[1], tx + p[0]])
+
+
+def _qt_images():
+    rng = np.random.default_rng(2)
+    imgs = []
+    for k in range(4):
+        base = rng.integers(0, 230, size=(5, 6))
+        blk = np.kron(base, np.ones((8, 8), np.int64))[: 37 + k, : 45 + 2 * k]
+        noise = rng.integers(-12, 13, size=blk.shape) * (rng.random(blk.shape) < 0.15)
+        imgs.append(np.clip(blk + noise, 0, 255).astype(np.uint8))
+    imgs.append(rng.integers(0, 256, size=(20, 33)).astype(np.uint8))
+    yy, xx = np.mgrid[0:32, 0:48]
+    imgs.append(np.clip(xx // 3 + yy // 5 + 60, 0, 255).astype(np.uint8))
+    return imgs
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_quadtree_matches_kernel_simulation(k):
+    img = _qt_images()[k]
+    assert np.array_equal(O.quadtree(img), quadtree_simulated(img))
+
+
+def test_quadtree_constructed_cases():
+    assert np.all(O.quadtree(np.full((48, 64), 77, np.uint8)) == 4)
+    yy, xx = np.mgrid[0:48, 0:64]
+    assert np.all(O.quadtree(np.where((xx + yy) % 2 == 0, 90, 130).astype(np.uint8)) == 2)  # +-20
+    assert np.all(O.quadtree(np.where((xx + yy) % 2 == 0, 100, 120).astype(np.uint8)) == 4)  # +-10: not > 10
+    img = np.full((32, 64), 50, np.uint8)
+    img[:, 52:] = 200
+    lv = O.quadtree(img)
+    assert np.all(lv[:, 48:56] == 2)   # 8x8 block straddles the step
+    assert np.all(lv[:, 56:64] == 3)   # 16x16 block straddles it, 8x8 does not
+    assert np.all(lv[:, 32:48] == 4)
+    # W = 376: the last tile column is 8 wide, so at most level 3 there
+    lv = O.quadtree(np.full((32, 376), 10, np.uint8))
+    assert np.all(lv[:, 368:] == 3) and np.all(lv[:, :368] == 4)
+
+
+# ---------------------------------------------------------------------------
+# Cost volume closed forms.
+# ---------------------------------------------------------------------------
+CAM = Camera(96, 64, 60.0, 60.0, 48.0, 32.0)
+EYE = np.tile(np.eye(3, 4, dtype=np.float32), (4, 1, 1))
+
+
+def _nodes(levels, W, H):
+    Hc, Wc = H // 4, W // 4
+    m = np.zeros((Hc, Wc), bool)
+    for cy in range(Hc):
+        for cx in range(Wc):
+            x, y = 4 * cx, 4 * cy
+            qs = 1 << int(levels[y, x])
+            m[cy, cx] = 0 < x < W - 1 and 0 < y < H - 1 and x % qs == 0 and y % qs == 0
+    return m
+
+
+def test_cost_identical_images_identity_pose_is_zero():
+    rng = np.random.default_rng(1)
+    key = rng.integers(0, 256, (64, 96)).astype(np.uint8)
+    lv = O.quadtree(key)
+    cost, cnt = O.cost_volume(key, [key] * 4, EYE, CAM, 64, 0.02, 2.0, lv)
+    m = _nodes(lv, 96, 64)
+    assert np.array_equal(cnt == 1, m)
+    assert np.all(cost[~m] == 0)
+    # every projection of the same pixel lands inside (2 <= p < W-2) except near the border
+    inner = m.copy()
+    inner[:1, :] = inner[-1:, :] = False
+    inner[:, :1] = inner[:, -1:] = False
+    assert np.all(cost[inner] == 0)
+
+
+def test_cost_slot_multiset_is_0_0_1_2_3():
+    key = np.full((64, 96), 100, np.uint8)
+    vals = [90, 130, 100, 40]
+    refs = [np.full((64, 96), v, np.uint8) for v in vals]
+    lv = O.quadtree(key)
+    cost, _ = O.cost_volume(key, refs, EYE, CAM, 64, 0.02, 2.0, lv)
+    sads = [9.0 * abs(100 - v) for v in vals]
+    want = np.float32((sads[0] + sads[3] + sads[1] + sads[0] + sads[2]) / 5.0 / 255.0)
+    cy, cx = 32 // 4, 48 // 4
+    lvl = int(lv[32, 48])
+    assert (48 % (1 << lvl)) == 0 and (32 % (1 << lvl)) == 0
+    assert np.allclose(cost[cy, cx], want, rtol=1e-6, atol=0)
+
+
+def test_cost_out_of_view_is_100():
+    key = np.full((64, 96), 100, np.uint8)
+    T = EYE.copy()
+    T[:, 0, 3] = 1e4  # everything projects far outside
+    lv = O.quadtree(key)
+    cost, cnt = O.cost_volume(key, [key] * 4, T, CAM, 64, 0.02, 2.0, lv)
+    m = cnt == 1
+    assert m.any() and np.all(cost[m] == 100.0)
+
+
+def test_cost_translation_minimum_at_true_disparity():
+    # pure x translation: the centre pixel (on the optical axis) at depth z shows
+    # in the reference at x + fx*tx/z; a textured reference shifted by that
+    # disparity makes the SAD exactly 0 at the matching sample.
+    W, H = 96, 64
+    rng = np.random.default_rng(4)
+    key = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    N, imin, imax = 64, 0.02, 2.0
+    table = O.depth_table(N, imin, imax)
+    tx = 0.25
+    dstar = 20
+    disp = 60.0 * tx / float(table[dstar])
+    shift = int(disp + 0.5)
+    ref = np.roll(key, shift, axis=1)
+    T = EYE.copy()
+    T[:, 0, 3] = tx
+    lv = np.full((H, W), 2, np.uint8)
+    cost, _ = O.cost_volume(key, [ref] * 4, T, CAM, N, imin, imax, lv)
+    c = cost[32 // 4, 48 // 4]
+    f32 = np.float32
+    proj = [float(f32(f32(60.0) * f32(tx)) / f32(table[d]) + f32(48.0)) for d in range(N)]
+    hits = [d for d in range(N) if int(proj[d] + 0.5) == 48 + shift]
+    assert dstar in hits
+    assert np.all(c[hits] == 0)
+    assert np.all(c[[d for d in range(N) if d not in hits]] > 0)
+
+
+# ---------------------------------------------------------------------------
+# BP: independent FP64 formulation (dense O(N^2) min over the P1/P2 penalty
+# matrix, vectorised sweeps) of the reference schedule, on tiny grids.
+# ---------------------------------------------------------------------------
+def bp_independent(cost, levels, W, H, N, P1=float(np.float32(0.003)), P2=float(np.float32(0.01)),
+                   iters=(4, 10, 10, 10)):
+    # P1, P2 are float constants in the reference (src/bpextract.cu:222-223)
+    d = np.arange(N)
+    V = np.where(d[:, None] == d[None, :], 0.0, np.where(np.abs(d[:, None] - d[None, :]) == 1, P1, P2))
+    costs = [cost.astype(np.float64)]
+    for _ in range(3):
+        c = costs[-1]
+        h0, w0 = c.shape[:2]
+        h1, w1 = (h0 + 1) // 2, (w0 + 1) // 2
+        pad = np.zeros((2 * h1, 2 * w1, N))
+        pad[:h0, :w0] = c
+        costs.append(pad[0::2, 0::2] + pad[1::2, 0::2] + pad[0::2, 1::2] + pad[1::2, 1::2])
+    msg = None
+    for l in range(3, -1, -1):
+        h, w = costs[l].shape[:2]
+        if msg is None:
+            msg = np.zeros((4, h, w, N))
+        else:
+            msg = np.repeat(np.repeat(msg, 2, axis=1), 2, axis=2)[:, :h, :w]
+        lev = l + 2
+        for it in range(iters[l]):
+            A = it % 2 == 0
+            new = msg.copy()
+            for y in range(h):
+                for x in range(w):
+                    if ((x + y) % 2 == 0) != A:
+                        continue
+                    bx = x // 2
+                    if lev <= 4 and levels[min(y << lev, H - 1), min(bx << lev, W - 1)] > lev:
+                        continue
+                    inc = {}
+                    if y > 0:
+                        inc[0] = msg[1, y - 1, x]
+                    if y < h - 1:
+                        inc[1] = msg[0, y + 1, x]
+                    if x > 0:
+                        inc[2] = msg[3, y, x - 1]
+                    if x < w - 1:
+                        inc[3] = msg[2, y, x + 1]
+                    for dr in range(4):
+                        hh = costs[l][y, x] + sum(v for k, v in inc.items() if k != dr)
+                        m = (hh[None, :] + V).min(axis=1)
+                        new[dr, y, x] = m - m.mean()
+            msg = new
+    h, w = costs[0].shape[:2]
+    labels = np.full((h, w), -1)
+    for y in range(h):
+        for x in range(w):
+            qs = 1 << int(levels[4 * y, 4 * x])
+            if (4 * x) % qs or (4 * y) % qs:
+                continue
+            b = costs[0][y, x].copy()
+            if x > 0:
+                b += msg[3, y, x - 1]
+            if x < w - 1:
+                b += msg[2, y, x + 1]
+            if y > 0:
+                b += msg[1, y - 1, x]
+            if y < h - 1:
+                b += msg[0, y + 1, x]
+            labels[y, x] = int(np.argmin(b))
+    return labels, msg
+
+
+@pytest.mark.parametrize("seed,W,H", [(0, 64, 48), (1, 72, 40)])
+def test_bp_matches_independent_fp64_formulation(seed, W, H):
+    rng = np.random.default_rng(seed)
+    N = 16
+    cost = rng.random((H // 4, W // 4, N)) * 0.2
+    cost[..., 5] -= 0.05  # a preferred label with noise around it
+    levels = np.full((H, W), 2, np.uint8)
+    levels[:16, :32] = 3  # some skipped cells at the finest level
+    lab_ref, msg_ref = bp_independent(cost, levels, W, H, N)
+    _, lab, msg = O.bp_extract(cost, W, H, levels, N, 0.01, 2.0, fp64=True, want_messages=True)
+    assert np.allclose(msg, msg_ref, atol=1e-12, rtol=0)
+    assert np.array_equal(lab, lab_ref)
+
+
+def test_bp_messages_are_zero_mean_and_uniform_cost_is_label_zero():
+    rng = np.random.default_rng(3)
+    W, H, N = 96, 64, 64
+    cost = rng.random((H // 4, W // 4, N))
+    lv = np.full((H, W), 2, np.uint8)
+    _, _, msg = O.bp_extract(cost, W, H, lv, N, 0.01, 2.0, fp64=True, want_messages=True)
+    assert np.abs(msg.mean(axis=-1)).max() < 1e-13
+    d, lab = O.bp_extract(np.full((H // 4, W // 4, N), 0.3, np.float32), W, H, lv, N, 0.01, 2.0)
+    assert np.all(lab == 0) and np.allclose(d, 100.0)
+
+
+def test_extract_parabola_vertex():
+    # a cell far from everything: data cost = (d - 20.3)^2 sampled; with uniform
+    # neighbours the messages are constant per cell, so the winner is 20 and the
+    # sub-sample offset is the parabola vertex 0.3
+    W, H, N = 16, 16, 64
+    d = np.arange(N, dtype=np.float64)
+    cost = np.tile(((d - 20.3) ** 2 * 1e-4)[None, None, :], (H // 4, W // 4, 1))
+    lv = np.full((H, W), 4, np.uint8)
+    dep, lab = O.bp_extract(cost, W, H, lv, N, 0.01, 2.0, fp64=True, iters=(0, 0, 0, 0))
+    assert lab[0, 0] == 20
+    step = (2.0 - 0.01) / 63.0
+    assert np.isclose(dep[0, 0], 1.0 / (step * 20.3 + 0.01), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Upsample: dense FP64 solve of (I + lambda L) per line.
+# ---------------------------------------------------------------------------
+def line_matrix(w, lam, n, seg):
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 1 + lam * (w[i] + w[i + 1])
+        if i + 1 < n and i + 1 != seg:
+            A[i, i + 1] = A[i + 1, i] = -lam * w[i + 1]
+    return A
+
+
+def upsample_dense(d, conf, wr, wc, lam, rs, cs):
+    H, W = d.shape
+    rx, rn = conf * d, conf
+    X = np.zeros((H, W))
+    Nn = np.zeros((H, W))
+    for y in range(H):
+        A = line_matrix(wr[y], lam, W, rs)
+        X[y] = np.linalg.solve(A, rx[y])
+        Nn[y] = np.linalg.solve(A, rn[y])
+    OX = np.zeros((H, W))
+    ON = np.zeros((H, W))
+    for x in range(W):
+        A = line_matrix(wc[:, x], lam, H, cs)
+        OX[:, x] = np.linalg.solve(A, X[:, x])
+        ON[:, x] = np.linalg.solve(A, Nn[:, x])
+    return OX / ON
+
+
+@pytest.mark.parametrize("seg", [False, True])
+def test_upsample_equals_dense_solve(seg):
+    rng = np.random.default_rng(7)
+    H, W = 24, 40
+    d = rng.uniform(1, 10, (H, W)) * (rng.random((H, W)) < 0.3)
+    conf = rng.uniform(0.01, 1, (H, W))
+    wr = np.exp(rng.uniform(-7, 2.3, (H, W + 1)))
+    wc = np.exp(rng.uniform(-7, 2.3, (H + 1, W)))
+    rs, cs = (32, 16) if seg else (W, H)
+    want = upsample_dense(d, conf, wr, wc, 10.0, rs, cs)
+    got64 = O.upsample(d, wr, wc, conf=conf, row_seg=rs, col_seg=cs, fp64=True)
+    assert np.allclose(got64, want, rtol=1e-10, atol=0)
+    got32 = O.upsample(d, wr, wc, conf=conf, row_seg=rs, col_seg=cs)
+    assert np.max(np.abs(got32 - want) / np.abs(want)) < 1e-4
+
+
+def test_upsample_constant_and_scale_invariance():
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 256, (32, 48)).astype(np.uint8)
+    wr, wc = O.weights(img, fp64=True)
+    sp = np.where(rng.random((32, 48)) < 0.1, 3.5, 0.0)
+    sp[0, 0] = 3.5
+    out = O.upsample(sp, wr, wc, fp64=True)
+    assert np.allclose(out, 3.5, rtol=1e-12)
+    conf = rng.uniform(0.1, 1, (32, 48))
+    d = rng.uniform(1, 5, (32, 48))
+    a = O.upsample(d, wr, wc, conf=conf, fp64=True)
+    b = O.upsample(d, wr, wc, conf=conf * 7.0, fp64=True)
+    assert np.allclose(a, b, rtol=1e-12)
+
+
+def test_weights_boundaries_follow_launch_grid():
+    img = np.random.default_rng(9).integers(0, 256, (40, 56)).astype(np.uint8)  # 56 % 16 = 8, 40 % 16 = 8
+    wr, wc = O.weights(img)
+    assert np.all(wr[:, 0] == 0) and np.all(wr[:, 56] == 1)
+    assert np.all(wc[0, :] == 0) and np.all(wc[40, :] == 1)
+    img2 = np.zeros((32, 64), np.uint8)
+    wr2, wc2 = O.weights(img2)
+    assert np.all(wr2[:, 64] == 0) and np.all(wc2[32, :] == 0)
+    lut = O.weight_lut()
+    assert lut[0] == 1.0 and np.isclose(lut[20], np.exp(-1.0), rtol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# Fusion: Vogiatzis & Hernandez (2011) Gaussian x Beta update in FP64.
+# ---------------------------------------------------------------------------
+def vh_update(a, b, mu, s2, x, tau2):
+    s = s2 + tau2
+    m = (s2 * x + tau2 * mu) / s
+    v = s2 * tau2 / s
+    c1 = a / (a + b) * np.exp(-(x - mu) ** 2 / (2 * s)) / np.sqrt(2 * np.pi * s)
+    c2 = b / (a + b) / 50.0
+    c1, c2 = c1 / (c1 + c2), c2 / (c1 + c2)
+    f = c1 * (a + 1) / (a + b + 1) + c2 * a / (a + b + 1)
+    e = c1 * (a + 1) * (a + 2) / ((a + b + 1) * (a + b + 2)) + c2 * a * (a + 1) / ((a + b + 1) * (a + b + 2))
+    mu_n = c1 * m + c2 * mu
+    s2_n = c1 * (v + m * m) + c2 * (s2 + mu * mu) - mu_n ** 2
+    a_n = (e - f) / (f - e / f)
+    b_n = a_n * (1 - f) / f
+    return a_n, b_n, mu_n, s2_n
+
+
+CAMF = Camera(64, 48, 50.0, 50.0, 32.0, 24.0)
+
+
+def test_fusion_first_map_is_all_invalid_and_matches_closed_form():
+    rng = np.random.default_rng(0)
+    d = rng.uniform(1, 10, (48, 64))
+    seeds = np.zeros((48, 64, 4))
+    _, out, sn, table = O.fuse(seeds, d, np.eye(3, 4, dtype=np.float32), CAMF, fp64=True)
+    assert np.all(out == -1.0) and np.all(table == 0)
+    tau2 = np.maximum(0.5, 0.2 * d) ** 2
+    want = vh_update(10.0, 10.0, d, 2500.0, d, tau2)
+    for k in range(4):
+        assert np.allclose(sn[..., k], want[k], rtol=1e-12)
+
+
+def test_fusion_static_recurrence_becomes_confident():
+    # static camera, exact measurement every map: the inlier ratio a/(a+b) first
+    # dips (the broad initial prior, sigma^2 = 2500, explains little), then rises;
+    # the output stays -1 until the prior's ratio exceeds 0.6, at map index 15
+    d = np.full((48, 64), 4.0)
+    seeds = np.zeros((48, 64, 4))
+    ratios, first_valid = [], None
+    for k in range(30):
+        _, out, seeds, _ = O.fuse(seeds, d, np.eye(3, 4, dtype=np.float32), CAMF, fp64=True)
+        ratios.append(seeds[24, 32, 0] / (seeds[24, 32, 0] + seeds[24, 32, 1]))
+        if first_valid is None and out[24, 32] > 0:
+            first_valid = k
+            assert np.isclose(out[24, 32], 4.0, rtol=1e-12)
+    assert first_valid == 15
+    assert ratios[13] <= 0.6 < ratios[14]
+    assert np.all(np.diff(ratios[5:]) > 0) and np.all(np.diff(ratios[:4]) < 0)
+
+
+def test_fusion_zbuffer_nearest_wins_and_reset_rule():
+    seeds = np.zeros((48, 64, 4))
+    seeds[..., :2] = (20.0, 5.0)
+    seeds[..., 3] = 0.01
+    seeds[..., 2] = 5.0
+    T = np.eye(3, 4, dtype=np.float32)
+    d = np.full((48, 64), 5.0)
+    ws, out, sn, table = O.fuse(seeds, d, T, CAMF, fp64=True)
+    idx = np.arange(48 * 64).reshape(48, 64)
+    inner = table[2:-2, 2:-2]
+    assert np.array_equal(inner, idx[2:-2, 2:-2])  # identity warp: every pixel is its own source
+    # reset: a measurement far from the prior (outside tau^2 + sigma^2) restarts the seed
+    d2 = d.copy()
+    d2[10, 10] = 9.0
+    _, _, sn2, _ = O.fuse(seeds, d2, T, CAMF, fp64=True)
+    want = vh_update(10.0, 10.0, 9.0, 2500.0, 9.0, max(0.5, 1.8) ** 2)
+    assert np.allclose(sn2[10, 10], want, rtol=1e-12)
+    # two seeds warped onto one pixel: the camera moves +x by 0.1 m, so the seed
+    # at depth 5 on x = 31 shifts one pixel left onto x = 30, where the seed at
+    # depth 500 stays; the nearer one wins
+    s3 = np.zeros((48, 64, 4))
+    s3[..., :2] = (20.0, 5.0)
+    s3[..., 3] = 0.01
+    s3[20, 30] = (20.0, 5.0, 500.0, 0.01)  # far seed: stays on x = 30
+    s3[20, 31] = (20.0, 5.0, 5.0, 0.01)
+    T2 = np.eye(3, 4, dtype=np.float32)
+    T2[0, 3] = -0.1  # camera moves +x: points shift left by fx*0.1/5 = 1 px
+    ws3, _, _, tab3 = O.fuse(s3, np.full((48, 64), 5.0), T2, CAMF, fp64=True)
+    assert tab3[20, 30] == 20 * 64 + 31
