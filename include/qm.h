@@ -94,6 +94,17 @@ int qm_upsample_general(const float* d, const float* conf, const float* w_row, c
                         int H, double lambda, int row_seg, int col_seg, float* dense, float* row_x,
                         float* row_n, void* stream);
 
+/* Adjoint of qm_upsample_general: given g = dL/d(dense), writes dL/dd,
+ * dL/dconf (zeros when conf is NULL), dL/dw_row (H x (W+1)) and dL/dw_col
+ * ((H+1) x W); any gradient output may be NULL. The forward passes are
+ * recomputed, not stored; A = I + lambda L(w) is symmetric, so each adjoint is a
+ * tridiagonal solve with the forward matrices. Pixels where the forward
+ * denominator n is 0 (output NaN) contribute no gradient. No counterpart in the
+ * reference (its upsample is forward-only). */
+int qm_upsample_backward(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
+                         double lambda, int row_seg, int col_seg, const float* g, float* grad_d, float* grad_conf,
+                         float* grad_w_row, float* grad_w_col, void* stream);
+
 /* Gaussian x Beta fusion of a dense depth map into the seed state. seeds
  * (H x W float4) is warped in place by T_last_to_cur; the update goes to
  * seeds_out; depth_out gets mu' where the prior is confident, else -1.
@@ -102,6 +113,14 @@ int qm_upsample_general(const float* d, const float* conf, const float* w_row, c
  * SeedMatrix::fuse_output_depth (src/seed_matrix.cu:553-597). */
 int qm_fuse(float* seeds, const float* depth_in, int W, int H, const float cam[4], const float T_last_to_cur[12],
             float* depth_out, float* seeds_out, int32_t* table, void* stream);
+
+/* Adjoint of the per-pixel fusion update (src/depth_fusion.cu:146-213) with
+ * respect to its dense depth input, for the warped seed state and hole-filled
+ * table qm_fuse produced: grad = g_depth_out * d(depth_out)/d(depth) +
+ * g_seeds_out . d(seeds_out)/d(depth) (either gradient may be NULL, not both).
+ * The prior seed is held fixed. No counterpart in the reference. */
+int qm_fuse_backward(const float* seeds_warped, const int32_t* table, const float* depth_in, int W, int H,
+                     const float* g_depth_out, const float* g_seeds_out, float* grad_depth_in, void* stream);
 
 /* ---------------- batched keyframes ---------------------------------------- */
 
@@ -150,6 +169,37 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);
 int qm_batch_reset_seeds(qm_batch* b);
 int qm_batch_destroy(qm_batch* b);
+
+/* ---------------- Depthmap-level state machine ----------------------------- */
+
+typedef struct qm_ctx qm_ctx;
+
+/* quadmap::Depthmap::Depthmap (src/depthmap.cpp:19-40). The reference
+ * constructor's argument order is fx, cx, fy, cy; this one takes fx, fy, cx, cy
+ * (the C++ wrapper include/quadmap/depthmap.h keeps the reference order).
+ * p == NULL uses qm_default_params. */
+int qm_create(int W, int H, float fx, float fy, float cx, float cy, const qm_params* p, qm_ctx** out);
+int qm_destroy(qm_ctx* c);
+/* SeedMatrix::set_semi2dense_ratio (include/quadmap/seed_matrix.cuh:209). */
+int qm_set_semi2dense_ratio(qm_ctx* c, int ratio);
+/* SeedMatrix::set_remap (src/seed_matrix.cu:75-80) plus the cv::remap(INTER_LINEAR,
+ * BORDER_CONSTANT) of input_raw (:85), done on the GPU: map_x, map_y are host
+ * H x W float source coordinates. NULL maps = identity (no remap). */
+int qm_set_remap(qm_ctx* c, const float* map_x, const float* map_y);
+/* Depthmap::add_frames (src/depthmap.cpp:42-55 -> src/seed_matrix.cu:106-169):
+ * host uint8 image with row pitch in bytes, T_curr_world 3x4 (world->camera).
+ * *has_result = 1 when a new result is available (the first frame always
+ * reports one, with zero maps). */
+int qm_add_frame(qm_ctx* c, const uint8_t* img, int pitch, const float T_curr_world[12], int* has_result);
+/* Host copies (H x W) of the last result: getDepthmap (fused depth, -1 where not
+ * confident), getDebugmap (sparse map handed to the upsample), getReferenceImage
+ * (the remapped uint8 frame), getT_world_ref (include/quadmap/depthmap.h:49-75). */
+int qm_get_depth(qm_ctx* c, float* out);
+int qm_get_debug(qm_ctx* c, float* out);
+int qm_get_reference(qm_ctx* c, uint8_t* out);
+int qm_get_T_world_ref(qm_ctx* c, float out[12]);
+/* Number of reference frames held (the reference's framelist_host.size()). */
+int qm_num_references(qm_ctx* c);
 
 #ifdef __cplusplus
 }

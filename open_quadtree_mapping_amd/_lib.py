@@ -67,7 +67,9 @@ SIGNATURES = {
     "qm_bp_extract": (I, [P, I, I, P, P, P, P, P, P]),
     "qm_upsample": (I, [P, P, I, I, I, P, P, P]),
     "qm_upsample_general": (I, [P, P, P, P, I, I, D, I, I, P, P, P, P]),
+    "qm_upsample_backward": (I, [P, P, P, P, I, I, D, I, I, P, P, P, P, P, P]),
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
+    "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
     "qm_batch_create": (I, [P, I, P, P]),
     "qm_batch_set_frame": (I, [P, I, P]),
     "qm_batch_run": (I, [P, I, P]),
@@ -75,6 +77,16 @@ SIGNATURES = {
     "qm_batch_buffer": (I, [P, I, I, P]),
     "qm_batch_reset_seeds": (I, [P]),
     "qm_batch_destroy": (I, [P]),
+    "qm_create": (I, [I, I, F, F, F, F, P, P]),
+    "qm_destroy": (I, [P]),
+    "qm_set_semi2dense_ratio": (I, [P, I]),
+    "qm_set_remap": (I, [P, P, P]),
+    "qm_add_frame": (I, [P, P, I, P, P]),
+    "qm_get_depth": (I, [P, P]),
+    "qm_get_debug": (I, [P, P]),
+    "qm_get_reference": (I, [P, P]),
+    "qm_get_T_world_ref": (I, [P, P]),
+    "qm_num_references": (I, [P]),
 }
 
 

@@ -1,0 +1,273 @@
+// k_adjoint.hip -- hand-written adjoints of the generalised upsample solve and
+// of the per-pixel fusion update. No counterpart in the reference.
+//
+// Upsample. Per line, A = I + lambda * sum_e w_e M_e is symmetric tridiagonal
+// (M_e = (u_{e-1} - u_e)(u_{e-1} - u_e)^T for an inner edge, u u^T for an edge
+// that only sits on the diagonal: the two image borders and the chain split),
+// so the adjoint of z = A^{-1} r is the same Thomas solve on the incoming
+// gradient: rbar = A^{-1} zbar, and dL/dw_e = -lambda * rbar^T M_e z summed over
+// the two right-hand sides (x and n). With out = x_col / n_col the incoming
+// gradients are g / n_col and -g * out / n_col. The forward passes are
+// recomputed first (the row pass into rowxn, the column pass into colxn); the
+// adjoint runs the column solve, then the row solve.
+#include "qm_device.h"
+#include "qm_launch.h"
+
+namespace qm {
+
+__device__ __forceinline__ void adj_coeffs(float wl, float wr, double lam, float& a, float& b, float& c) {
+  a = (float)(-lam * (double)wl);
+  b = (float)(1.0 + lam * (double)(wl + wr));
+  c = (float)(-lam * (double)wr);
+}
+
+// Column adjoint: lane per column. rhs from g and the recomputed column pass,
+// result (xbar, nbar) into adjxn, dL/dw_col into grad_wcol.
+__global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ frames, BlockMap bm, double lam) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
+  const int W = F.W, H = F.H;
+  if (x >= W) return;
+  const float flam = (float)lam;
+  auto weight = [&](int y) -> float { return F.g_wcol[(size_t)y * W + x]; };
+  const int seg = F.col_seg < H ? F.col_seg : H;
+  const int bounds[3] = {0, seg, H};
+  float carry = 0.0f;
+  for (int sg = 0; sg < 2; ++sg) {
+    const int s0 = bounds[sg], s1 = bounds[sg + 1];
+    if (s1 <= s0) continue;
+    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
+    float wl = weight(s0);
+    for (int i = s0; i < s1; ++i) {
+      const float wr = weight(i + 1);
+      float a, b, c;
+      adj_coeffs(wl, wr, lam, a, b, c);
+      const float2 z = F.colxn[(size_t)i * W + x];
+      const float g = F.g_out[(size_t)i * W + x];
+      float gx = 0.0f, gn = 0.0f;
+      if (z.y != 0.0f) {
+        gx = g / z.y;
+        gn = -gx * (z.x / z.y);
+      }
+      const float den = b - fc * a;
+      const float ch = c / den;
+      const float dh = (gx - fd * a) / den;
+      const float nh = (gn - fn * a) / den;
+      F.fwd[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
+      fc = ch;
+      fd = dh;
+      fn = nh;
+      wl = wr;
+    }
+    float vx_next = 0.0f, vn_next = 0.0f;
+    float2 z_next = make_float2(0.0f, 0.0f);
+    for (int i = s1 - 1; i >= s0; --i) {
+      const float4 t = F.fwd[(size_t)i * W + x];
+      const float vx = t.y - t.x * vx_next;
+      const float vn = t.z - t.x * vn_next;
+      F.adjxn[(size_t)i * W + x] = make_float2(vx, vn);
+      const float2 z = F.colxn[(size_t)i * W + x];
+      if (i + 1 < s1) {
+        F.grad_wcol[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+      } else {
+        const float e = -flam * (vx * z.x + vn * z.y);
+        if (s1 == H) F.grad_wcol[(size_t)H * W + x] = e;
+        else carry = e;
+      }
+      if (i == s0) {
+        const float e = -flam * (vx * z.x + vn * z.y);
+        F.grad_wcol[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
+      }
+      vx_next = vx;
+      vn_next = vn;
+      z_next = z;
+    }
+  }
+}
+
+// Row adjoint: lane per row. rhs = column-adjoint solution; z = recomputed row
+// pass. Gives dL/dd, dL/dconf and dL/dw_row.
+__global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ frames, BlockMap bm, double lam) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
+  const int W = F.W, H = F.H;
+  if (y >= H) return;
+  const float flam = (float)lam;
+  const size_t row = (size_t)y * W;
+  const size_t wrow = (size_t)y * (W + 1);
+  auto weight = [&](int x) -> float { return F.g_wrow[wrow + x]; };
+  const int seg = F.row_seg < W ? F.row_seg : W;
+  const int bounds[3] = {0, seg, W};
+  float carry = 0.0f;
+  for (int sg = 0; sg < 2; ++sg) {
+    const int s0 = bounds[sg], s1 = bounds[sg + 1];
+    if (s1 <= s0) continue;
+    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
+    float wl = weight(s0);
+    for (int i = s0; i < s1; ++i) {
+      const float wr = weight(i + 1);
+      float a, b, c;
+      adj_coeffs(wl, wr, lam, a, b, c);
+      const float2 r = F.adjxn[row + i];
+      const float den = b - fc * a;
+      const float ch = c / den;
+      const float dh = (r.x - fd * a) / den;
+      const float nh = (r.y - fn * a) / den;
+      F.fwd[row + i] = make_float4(ch, dh, nh, 0.0f);
+      fc = ch;
+      fd = dh;
+      fn = nh;
+      wl = wr;
+    }
+    float vx_next = 0.0f, vn_next = 0.0f;
+    float2 z_next = make_float2(0.0f, 0.0f);
+    for (int i = s1 - 1; i >= s0; --i) {
+      const float4 t = F.fwd[row + i];
+      const float vx = t.y - t.x * vx_next;
+      const float vn = t.z - t.x * vn_next;
+      const float2 z = F.rowxn[row + i];
+      if (i + 1 < s1) {
+        if (F.grad_wrow)
+          F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+      } else {
+        const float e = -flam * (vx * z.x + vn * z.y);
+        if (s1 == W) {
+          if (F.grad_wrow) F.grad_wrow[wrow + W] = e;
+        } else {
+          carry = e;
+        }
+      }
+      if (i == s0 && F.grad_wrow) {
+        const float e = -flam * (vx * z.x + vn * z.y);
+        F.grad_wrow[wrow + s0] = (s0 == 0) ? e : carry + e;
+      }
+      // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
+      const float d = F.g_d[row + i];
+      if (F.g_conf) {
+        const float c = F.g_conf[row + i];
+        if (F.grad_d) F.grad_d[row + i] = c * vx;
+        if (F.grad_conf) F.grad_conf[row + i] = d * vx + vn;
+      } else {
+        if (F.grad_d) F.grad_d[row + i] = vx;
+        if (F.grad_conf) F.grad_conf[row + i] = 0.0f;
+      }
+      vx_next = vx;
+      vn_next = vn;
+      z_next = z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fusion update adjoint w.r.t. the dense depth input, by forward-mode dual
+// numbers through the same operation sequence as k_fuse_update: one input, so
+// a single tangent gives every output derivative; the vector-Jacobian product
+// with (dL/d depth_out, dL/d seeds_out) is then a dot product per pixel.
+// ---------------------------------------------------------------------------
+struct Dual {
+  float v, d;
+};
+__device__ __forceinline__ Dual dc(float v) { return {v, 0.0f}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const float q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual dexp(Dual a) {
+  const float e = pinned_expf(a.v);
+  return {e, e * a.d};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a) {
+  const float s = sqrtf(a.v);
+  return {s, a.d / (2.0f * s)};
+}
+
+__global__ __launch_bounds__(256) void k_fuse_backward(const FrameDev* __restrict__ frames, BlockMap bm) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+  const int W = F.W, H = F.H;
+  if (idx >= W * H) return;
+  const int src = F.table_in[idx];
+  const Dual de = {F.dense[idx], 1.0f};
+  // uncertainty: max(0.5, 0.2 de)^2, or 1e9 for non-positive depth
+  const float m02 = (float)((double)de.v * 0.2);
+  Dual unc = (m02 > 0.5f) ? Dual{m02, 0.2f} : dc(0.5f);
+  unc = unc * unc;
+  if (de.v <= 0.0f) unc = dc(1e9f);
+  Dual a, b, mu, s2;
+  if (src > 0) {
+    const float4 s = F.seeds[src];
+    a = dc(s.x);
+    b = dc(s.y);
+    mu = dc(s.z);
+    s2 = dc(s.w);
+  } else {
+    a = dc(10.0f);
+    b = dc(10.0f);
+    mu = de;
+    s2 = dc(2500.0f);
+  }
+  if ((de.v - mu.v) * (de.v - mu.v) > unc.v + s2.v) {
+    a = dc(10.0f);
+    b = dc(10.0f);
+    mu = de;
+    s2 = dc(2500.0f);
+  }
+  const bool good = (double)(a.v / (a.v + b.v)) > 0.60;
+  const Dual one = dc(1.0f), two = dc(2.0f);
+  const Dual new_sq = unc * s2 / (unc + s2);
+  const Dual new_mu = (de * s2 + mu * unc) / (unc + s2);
+  const Dual ss = unc + s2;
+  const Dual e0 = dexp(((-(de - mu)) * (de - mu)) / (two * ss));
+  const Dual r0 = one / dsqrt(dc((float)(2.0 * 3.14159265358979323846)) * ss);
+  Dual c1 = (a / (a + b)) * (e0 * r0);
+  Dual c2 = (b / (a + b)) * one / dc(50.0f);
+  const Dual norm = c1 + c2;
+  c1 = c1 / norm;
+  c2 = c2 / norm;
+  const Dual f1 = c1 * ((a + one) / (a + b + one)) + c2 * (a / (a + b + one));
+  const Dual e1 = c1 * (((a + one) * (a + two)) / ((a + b + one) * (a + b + two))) +
+                  c2 * (a * (a + one) / ((a + b + one) * (a + b + two)));
+  const Dual mu_p = c1 * new_mu + c2 * mu;
+  const Dual s_p = c1 * (new_sq + new_mu * new_mu) + c2 * (s2 + mu * mu) - mu_p * mu_p;
+  const Dual a_p = (e1 - f1) / (f1 - e1 / f1);
+  const Dual b_p = a_p * (one - f1) / f1;
+  float gsum = 0.0f;
+  if (F.g_depth_out && good) gsum += F.g_depth_out[idx] * mu_p.d;
+  if (F.g_seeds_out) {
+    const float4 g = F.g_seeds_out[idx];
+    gsum += g.x * a_p.d + g.y * b_p.d + g.z * mu_p.d + g.w * s_p.d;
+  }
+  F.grad_depth_in[idx] = gsum;
+}
+
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, double lambda,
+                              hipStream_t s) {
+  // recompute the forward passes (row pass -> rowxn, column pass -> colxn)
+  launch_upsample_row(dframes, hframes, n, 2, nullptr, lambda, false, s);
+  launch_upsample_col(dframes, hframes, n, 2, nullptr, lambda, s);
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;
+  int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_adj_col, dim3(nb), dim3(64), 0, s, dframes, bm, lambda);
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 63) / 64;
+  nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_adj_row, dim3(nb), dim3(64), 0, s, dframes, bm, lambda);
+}
+
+void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W * hframes[i].H + 255) / 256;
+  const int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_fuse_backward, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+}  // namespace qm

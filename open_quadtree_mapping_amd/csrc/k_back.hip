@@ -157,6 +157,7 @@ __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ fram
       const float xv = t.y - t.x * fd;
       const float nv = t.z - t.x * fn;
       const float out = xv / nv;
+      if (F.colxn) F.colxn[(size_t)i * W + x] = make_float2(xv, nv);
       F.dense[(size_t)i * W + x] = out;
       if (F.dense_out) F.dense_out[(size_t)i * W + x] = out;
       fd = xv;

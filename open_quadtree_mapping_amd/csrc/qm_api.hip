@@ -19,10 +19,11 @@
 #include "../../include/qm.h"
 #include "qm_device.h"
 #include "qm_launch.h"
+#include "qm_internal.h"
 
 using namespace qm;
 
-namespace {
+namespace qm {
 
 thread_local std::string g_err;
 std::atomic<size_t> g_bytes{0};
@@ -34,12 +35,6 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
-
-#define QM_HIP(call)                                                                         \
-  do {                                                                                       \
-    hipError_t e_ = (call);                                                                  \
-    if (e_ != hipSuccess) return fail(QM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
@@ -70,6 +65,10 @@ void dev_free(void* p) {
   }
   (void)hipFree(p);
 }
+
+}  // namespace qm
+
+namespace {
 
 // RAII scratch list for stage-level calls
 struct Scratch {
@@ -480,6 +479,74 @@ int qm_upsample_general(const float* d, const float* conf, const float* w_row, c
   F.rowxn = (float2*)rxn;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   launch_upsample((FrameDev*)df, &F, 1, 2, nullptr, lambda, false, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_upsample_backward(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
+                         double lambda, int row_seg, int col_seg, const float* g, float* grad_d, float* grad_conf,
+                         float* grad_w_row, float* grad_w_col, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!d || !w_row || !w_col || !g) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (row_seg < 0 || col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t px = (size_t)W * H;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.row_seg = row_seg;
+  F.col_seg = col_seg;
+  F.g_d = d;
+  F.g_conf = conf;
+  F.g_wrow = w_row;
+  F.g_wcol = w_col;
+  F.g_out = g;
+  F.grad_d = grad_d;
+  F.grad_conf = grad_conf;
+  F.grad_wrow = grad_w_row;
+  Scratch sc;
+  void *df, *fwd, *rxn, *cxn, *axn, *dense, *gwc = nullptr;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  if ((rc = sc.get(&fwd, px * 16))) return rc;
+  if ((rc = sc.get(&rxn, px * 8))) return rc;
+  if ((rc = sc.get(&cxn, px * 8))) return rc;
+  if ((rc = sc.get(&axn, px * 8))) return rc;
+  if ((rc = sc.get(&dense, px * 4))) return rc;
+  if (!grad_w_col && (rc = sc.get(&gwc, (size_t)(H + 1) * W * 4))) return rc;
+  F.fwd = (float4*)fwd;
+  F.rowxn = (float2*)rxn;
+  F.colxn = (float2*)cxn;
+  F.adjxn = (float2*)axn;
+  F.dense = (float*)dense;
+  F.grad_wcol = grad_w_col ? grad_w_col : (float*)gwc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_upsample_backward((FrameDev*)df, &F, 1, lambda, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_fuse_backward(const float* seeds_warped, const int32_t* table, const float* depth_in, int W, int H,
+                     const float* g_depth_out, const float* g_seeds_out, float* grad_depth_in, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!seeds_warped || !table || !depth_in || !grad_depth_in) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (!g_depth_out && !g_seeds_out) return fail(QM_ERR_INVALID, "need at least one incoming gradient");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.seeds = (float4*)const_cast<float*>(seeds_warped);
+  F.table_in = table;
+  F.dense = const_cast<float*>(depth_in);
+  F.g_depth_out = g_depth_out;
+  F.g_seeds_out = (const float4*)g_seeds_out;
+  F.grad_depth_in = grad_depth_in;
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_fuse_backward((FrameDev*)df, &F, 1, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   return QM_OK;

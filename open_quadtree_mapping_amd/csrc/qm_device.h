@@ -48,6 +48,19 @@ struct FrameDev {
   const float* g_wcol;
   float* row_x_out;
   float* row_n_out;
+  // adjoint of the upsample (recomputation buffers and gradients)
+  float2* colxn;        // H x W column pass (x, n) or null
+  float2* adjxn;        // H x W column-adjoint solution
+  const float* g_out;   // dL/d(dense)
+  float* grad_d;
+  float* grad_conf;
+  float* grad_wrow;     // H x (W+1)
+  float* grad_wcol;     // (H+1) x W
+  // adjoint of the fusion update
+  const int32_t* table_in;     // hole-filled warp table
+  const float* g_depth_out;    // dL/d(depth_out)
+  const float4* g_seeds_out;   // dL/d(seeds_out) or null
+  float* grad_depth_in;
 };
 
 // blockIdx.x -> frame: frame f owns blocks [start[f], start[f+1]).

@@ -48,5 +48,10 @@ void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+// adjoints: recompute the generalised upsample forward, then the column and row
+// adjoint solves; fusion update adjoint w.r.t. its dense depth input
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, double lambda,
+                              hipStream_t s);
+void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 
 }  // namespace qm

@@ -214,3 +214,86 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     lib().qmo_keyframe_f32(_p(key), _ptr_array(refs), W, H, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, imin, imax,
                            _p(pf), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label), _p(ms))
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
+
+
+# ---------------------------------------------------------------------------
+# Frame-sequence state machine (src/seed_matrix.cu:106-198, src/depthmap.cpp:42-55)
+# without the semi-dense seeds (their pre-fill of the sparse map is zero here).
+# ---------------------------------------------------------------------------
+def _se3_inv(T):
+    d = np.asarray(T, np.float32).reshape(12)
+    r = np.zeros(12, np.float32)
+    r[0], r[1], r[2], r[4], r[5], r[6], r[8], r[9], r[10] = d[0], d[4], d[8], d[1], d[5], d[9], d[2], d[6], d[10]
+    r[3] = ((-d[0]) * d[3] - d[4] * d[7]) - d[8] * d[11]
+    r[7] = ((-d[1]) * d[3] - d[5] * d[7]) - d[9] * d[11]
+    r[11] = ((-d[2]) * d[3] - d[6] * d[7]) - d[10] * d[11]
+    return r.reshape(3, 4)
+
+
+def _se3_mul(A, B):
+    a = np.asarray(A, np.float32).reshape(12)
+    b = np.asarray(B, np.float32).reshape(12)
+    r = np.zeros(12, np.float32)
+    for row in range(3):
+        o = 4 * row
+        for col in range(3):
+            r[o + col] = (a[o] * b[col] + a[o + 1] * b[4 + col]) + a[o + 2] * b[8 + col]
+        r[o + 3] = ((a[o + 3] + a[o] * b[3]) + a[o + 1] * b[7]) + a[o + 2] * b[11]
+    return r.reshape(3, 4)
+
+
+class DepthmapSequence:
+    """Oracle of quadmap::Depthmap over a frame sequence (FP32)."""
+
+    def __init__(self, cam, N=64, imin=0.01, imax=2.0, ratio=1, min_refs=11, max_refs=60, ref_cos=0.95,
+                 ref_baseline=0.03):
+        self.cam, self.N, self.imin, self.imax, self.ratio = cam, N, imin, imax, ratio
+        self.min_refs, self.max_refs = min_refs, max_refs
+        self.ref_cos, self.ref_baseline = np.float32(ref_cos), np.float32(ref_baseline)
+        self.frame_index = 0
+        self.initialized = False
+        self.refs = []  # newest first: (image, T_ref_world)
+        self.fuse_pose = np.eye(3, 4, dtype=np.float32)
+        self.seeds = np.zeros((cam.H, cam.W, 4), np.float32)
+        self.depth = self.debug = self.reference = None
+        self.T_world_ref = np.eye(3, 4, dtype=np.float32)
+
+    def _need_add_reference(self, T_income):
+        if not self.refs:
+            return True
+        A = _se3_inv(self.refs[0][1]).reshape(12)
+        B = _se3_inv(T_income).reshape(12)
+        cosv = (A[2] * B[2] + A[6] * B[6]) + A[10] * B[10]
+        dx, dy, dz = A[3] - B[3], A[7] - B[7], A[11] - B[11]
+        base = np.sqrt((dx * dx + dy * dy) + dz * dz, dtype=np.float32)
+        return bool(cosv < self.ref_cos or base > self.ref_baseline)
+
+    def add_frames(self, img, T_curr_world):
+        T_curr_world = np.asarray(T_curr_world, np.float32)
+        self.frame_index += 1
+        W, H = self.cam.W, self.cam.H
+        if not self.initialized:
+            self.initialized = True
+            self.depth = np.zeros((H, W), np.float32)
+            self.debug = np.zeros((H, W), np.float32)
+            self.reference = img.copy()
+            self.T_world_ref = _se3_inv(T_curr_world)
+            return True
+        if self.frame_index % self.ratio != 0:
+            return False
+        has_output = len(self.refs) >= self.min_refs
+        if has_output:
+            inv_inc = _se3_inv(T_curr_world)
+            T = np.stack([_se3_mul(self.refs[r][1], inv_inc) for r in range(4)])
+            T_last = _se3_mul(T_curr_world, _se3_inv(self.fuse_pose))
+            res = keyframe(img, [self.refs[r][0] for r in range(4)], T, self.cam, self.N, self.imin, self.imax,
+                           self.seeds, T_last)
+        if self._need_add_reference(T_curr_world):
+            self.refs.insert(0, (img.copy(), T_curr_world.copy()))
+            if len(self.refs) > self.max_refs:
+                self.refs.pop()
+        if has_output:
+            self.fuse_pose = T_curr_world.copy()
+            self.depth, self.debug, self.reference = res["depth"], res["sparse"], img.copy()
+            self.T_world_ref = _se3_inv(T_curr_world)
+        return has_output

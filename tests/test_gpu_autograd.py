@@ -1,0 +1,121 @@
+"""Hand-written adjoints (upsample solve, fusion update) against FP64 central
+finite differences of the FP64 oracle (h = 1e-6), through torch.autograd."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from open_quadtree_mapping_amd import autograd as QA
+from open_quadtree_mapping_amd.scene import Camera
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+H_FD = 1e-6
+
+
+def problem(H, W, seed):
+    """Config-3 distributions: depths U(1,10) on 30% of pixels, confidences
+    U(0,1], weights log-U(1e-3, 1e1)."""
+    rng = np.random.default_rng(seed)
+    d = (rng.uniform(1, 10, (H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float64)
+    conf = rng.uniform(0, 1, (H, W))
+    conf = np.where(conf == 0, 1.0, conf)
+    wr = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H, W + 1)))
+    wc = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H + 1, W)))
+    r = rng.standard_normal((H, W))
+    return d, conf, wr, wc, r
+
+
+def fd_upsample(arrays, which, idx, r, lam):
+    def L(arrs):
+        d, conf, wr, wc = arrs
+        out = O.upsample(d, wr, wc, conf=conf, lam=lam, fp64=True)
+        return float(np.sum(r * out))
+    vals = []
+    for sgn in (1, -1):
+        a = [x.copy() for x in arrays]
+        a[which][idx] += sgn * H_FD
+        vals.append(L(a))
+    return (vals[0] - vals[1]) / (2 * H_FD)
+
+
+@pytest.mark.parametrize("H,W,n_entries", [(120, 188, 256), (480, 752, 64)])
+def test_upsample_adjoint_vs_fd(H, W, n_entries):
+    print(f"upsample adjoint {W}x{H}", flush=True)
+    d, conf, wr, wc, r = problem(H, W, 5)
+    lam = 10.0
+    td = torch.tensor(d, dtype=torch.float32, device=DEV, requires_grad=True)
+    tc = torch.tensor(conf, dtype=torch.float32, device=DEV, requires_grad=True)
+    twr = torch.tensor(wr, dtype=torch.float32, device=DEV, requires_grad=True)
+    twc = torch.tensor(wc, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = QA.upsample(td, twr, twc, conf=tc, lam=lam)
+    loss = (out * torch.tensor(r, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    grads = [td.grad.cpu().numpy(), tc.grad.cpu().numpy(), twr.grad.cpu().numpy(), twc.grad.cpu().numpy()]
+    arrays = [d, conf, wr, wc]
+    rng = np.random.default_rng(1)
+    errs = []
+    for k in range(n_entries):
+        which = k % 4
+        shape = arrays[which].shape
+        idx = tuple(int(rng.integers(0, s)) for s in shape)
+        fd = fd_upsample(arrays, which, idx, r, lam)
+        g = float(grads[which][idx])
+        scale = max(abs(fd), 1e-3 * np.abs(grads[which]).max(), 1e-9)
+        errs.append(abs(g - fd) / scale)
+    errs = np.array(errs)
+    assert np.quantile(errs, 0.99) < 2e-3, f"max {errs.max()} p99 {np.quantile(errs, 0.99)}"
+
+
+def test_upsample_lambda_gradient():
+    H, W = 60, 94
+    d, conf, wr, wc, r = problem(H, W, 9)
+    lam = torch.tensor(10.0, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = QA.upsample(torch.tensor(d, dtype=torch.float32, device=DEV),
+                      torch.tensor(wr, dtype=torch.float32, device=DEV),
+                      torch.tensor(wc, dtype=torch.float32, device=DEV),
+                      conf=torch.tensor(conf, dtype=torch.float32, device=DEV), lam=lam)
+    (out * torch.tensor(r, dtype=torch.float32, device=DEV)).sum().backward()
+    f = lambda l: float(np.sum(r * O.upsample(d, wr, wc, conf=conf, lam=l, fp64=True)))  # noqa: E731
+    fd = (f(10.0 + 1e-6) - f(10.0 - 1e-6)) / 2e-6
+    assert abs(lam.grad.item() - fd) <= 2e-3 * abs(fd)
+
+
+def test_fuse_adjoint_vs_fd():
+    H, W = 96, 128
+    cam = Camera(W, H, 100.0, 100.0, 64.0, 48.0)
+    rng = np.random.default_rng(3)
+    seeds = np.zeros((H, W, 4))
+    seeds[..., 0] = rng.uniform(5, 30, (H, W))
+    seeds[..., 1] = rng.uniform(1, 10, (H, W))
+    seeds[..., 2] = rng.uniform(2, 6, (H, W))
+    seeds[..., 3] = rng.uniform(0.05, 2, (H, W))
+    depth = seeds[..., 2] * rng.uniform(0.9, 1.1, (H, W))
+    depth[rng.random((H, W)) < 0.1] = rng.uniform(1, 8, (H, W))[rng.random((H, W)) < 0.1]  # some resets
+    T = np.eye(3, 4, dtype=np.float32)
+    T[0, 3] = 0.01
+    gdep = rng.standard_normal((H, W))
+    gse = rng.standard_normal((H, W, 4))
+    td = torch.tensor(depth, dtype=torch.float32, device=DEV, requires_grad=True)
+    ts = torch.tensor(seeds, dtype=torch.float32, device=DEV)
+    dout, sout = QA.fuse(td, ts, (cam.fx, cam.fy, cam.cx, cam.cy), T)
+    loss = (dout * torch.tensor(gdep, dtype=torch.float32, device=DEV)).sum() + \
+        (sout * torch.tensor(gse, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    g = td.grad.cpu().numpy()
+
+    def L(dep):
+        _, o, s, _ = O.fuse(seeds, dep, T, cam, fp64=True)
+        return float(np.sum(gdep * np.where(o == -1.0, 0.0, o)) + np.sum(gse * s))
+    # the -1 outputs are constants; their gradient is 0 on both sides
+    errs = []
+    for k in range(256):
+        y, x = int(rng.integers(0, H)), int(rng.integers(0, W))
+        dp, dm = depth.copy(), depth.copy()
+        dp[y, x] += H_FD
+        dm[y, x] -= H_FD
+        fd = (L(dp) - L(dm)) / (2 * H_FD)
+        scale = max(abs(fd), 1e-3 * np.abs(g).max())
+        errs.append(abs(g[y, x] - fd) / scale)
+    errs = np.array(errs)
+    assert np.quantile(errs, 0.98) < 2e-3, f"p98 {np.quantile(errs, 0.98)} max {errs.max()}"
