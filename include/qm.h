@@ -50,6 +50,10 @@ typedef struct qm_params {
   float ref_baseline;   /* ... or the baseline exceeds 0.03 m */
 } qm_params;
 
+/* Version of this ABI; bindings check it at load time so that a stale library
+ * fails loudly instead of being called with the wrong argument list. */
+#define QM_ABI_VERSION 3
+int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
 /* Fault injection for tests: refuse device allocations beyond `bytes` (0 = no cap). */
@@ -184,13 +188,16 @@ int qm_destroy(qm_ctx* c);
 int qm_set_semi2dense_ratio(qm_ctx* c, int ratio);
 /* SeedMatrix::set_remap (src/seed_matrix.cu:75-80) plus the cv::remap(INTER_LINEAR,
  * BORDER_CONSTANT) of input_raw (:85), done on the GPU: map_x, map_y are host
- * H x W float source coordinates. NULL maps = identity (no remap). */
+ * H x W float coordinates into the raw camera image (the CV_32FC1 maps of
+ * cv::initUndistortRectifyMap, src/depthmap_node.cpp:84-93, which may also
+ * resize). NULL maps = identity (the raw image must then be W x H). */
 int qm_set_remap(qm_ctx* c, const float* map_x, const float* map_y);
 /* Depthmap::add_frames (src/depthmap.cpp:42-55 -> src/seed_matrix.cu:106-169):
- * host uint8 image with row pitch in bytes, T_curr_world 3x4 (world->camera).
- * *has_result = 1 when a new result is available (the first frame always
- * reports one, with zero maps). */
-int qm_add_frame(qm_ctx* c, const uint8_t* img, int pitch, const float T_curr_world[12], int* has_result);
+ * host uint8 raw image of img_w x img_h with row pitch in bytes, T_curr_world
+ * 3x4 (world->camera). *has_result = 1 when a new result is available (the
+ * first frame always reports one, with zero maps). */
+int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch, const float T_curr_world[12],
+                 int* has_result);
 /* Host copies (H x W) of the last result: getDepthmap (fused depth, -1 where not
  * confident), getDebugmap (sparse map handed to the upsample), getReferenceImage
  * (the remapped uint8 frame), getT_world_ref (include/quadmap/depthmap.h:49-75). */

@@ -42,7 +42,7 @@ class Depthmap {
     qm_throw_if(qm_set_semi2dense_ratio(ctx_, semi2dense_ratio), "qm_set_semi2dense_ratio");
     if (!remap_1.empty() && !remap_2.empty()) {
       if (remap_1.type() != CV_32FC1 || remap_2.type() != CV_32FC1 || remap_1.rows != (int)height ||
-          remap_1.cols != (int)width)
+          remap_1.cols != (int)width || remap_2.rows != (int)height || remap_2.cols != (int)width)
         throw QmException("remap maps must be CV_32FC1 height x width coordinate maps", QM_ERR_INVALID);
       std::vector<float> mx(width * height), my(width * height);
       for (int y = 0; y < (int)height; ++y) {
@@ -57,10 +57,11 @@ class Depthmap {
   Depthmap& operator=(const Depthmap&) = delete;
 
   bool add_frames(const cv::Mat& img_curr, const SE3<float>& T_curr_world) {
-    if (img_curr.type() != CV_8UC1 || img_curr.rows != (int)height_ || img_curr.cols != (int)width_)
-      throw QmException("add_frames expects a CV_8UC1 image of the configured size", QM_ERR_INVALID);
+    if (img_curr.type() != CV_8UC1) throw QmException("add_frames expects a CV_8UC1 image", QM_ERR_INVALID);
     int has = 0;
-    qm_throw_if(qm_add_frame(ctx_, img_curr.data, (int)img_curr.step, T_curr_world.data.data, &has), "qm_add_frame");
+    qm_throw_if(qm_add_frame(ctx_, img_curr.data, img_curr.cols, img_curr.rows, (int)img_curr.step,
+                             T_curr_world.data.data, &has),
+                "qm_add_frame");
     if (has) {
       std::lock_guard<std::mutex> lk(ref_img_mutex_);
       depth_out_.create((int)height_, (int)width_, CV_32FC1);

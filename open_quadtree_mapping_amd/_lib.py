@@ -56,8 +56,11 @@ class qm_frame_desc(C.Structure):
     ]
 
 
+ABI_VERSION = 3  # QM_ABI_VERSION of include/qm.h this binding is written against
+
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {
+    "qm_abi_version": (I, []),
     "qm_last_error": (C.c_char_p, []),
     "qm_default_params": (None, [P]),
     "qm_set_alloc_cap": (None, [C.c_size_t]),
@@ -81,7 +84,7 @@ SIGNATURES = {
     "qm_destroy": (I, [P]),
     "qm_set_semi2dense_ratio": (I, [P, I]),
     "qm_set_remap": (I, [P, P, P]),
-    "qm_add_frame": (I, [P, P, I, P, P]),
+    "qm_add_frame": (I, [P, P, I, I, I, P, P]),
     "qm_get_depth": (I, [P, P]),
     "qm_get_debug": (I, [P, P]),
     "qm_get_reference": (I, [P, P]),
@@ -100,6 +103,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        got = L.qm_abi_version()
+        if got != ABI_VERSION:
+            raise QmError(f"{LIB_PATH} has ABI version {got}, the binding expects {ABI_VERSION}: rebuild it")
         _lib = L
     return _lib
 

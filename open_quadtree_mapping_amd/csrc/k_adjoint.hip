@@ -66,7 +66,7 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
       const float4 t = F.fwd[(size_t)i * W + x];
       const float vx = t.y - t.x * vx_next;
       const float vn = t.z - t.x * vn_next;
-      F.adjxn[(size_t)i * W + x] = make_float2(vx, vn);
+      F.adjxn[(size_t)x * H + i] = make_float2(vx, vn);
       const float2 z = F.colxn[(size_t)i * W + x];
       if (i + 1 < s1) {
         F.grad_wcol[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
@@ -110,12 +110,12 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
       const float wr = weight(i + 1);
       float a, b, c;
       adj_coeffs(wl, wr, lam, a, b, c);
-      const float2 r = F.adjxn[row + i];
+      const float2 r = F.adjxn[(size_t)i * H + y];
       const float den = b - fc * a;
       const float ch = c / den;
       const float dh = (r.x - fd * a) / den;
       const float nh = (r.y - fn * a) / den;
-      F.fwd[row + i] = make_float4(ch, dh, nh, 0.0f);
+      F.fwd[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
       fc = ch;
       fd = dh;
       fn = nh;
@@ -124,10 +124,10 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
     float vx_next = 0.0f, vn_next = 0.0f;
     float2 z_next = make_float2(0.0f, 0.0f);
     for (int i = s1 - 1; i >= s0; --i) {
-      const float4 t = F.fwd[row + i];
+      const float4 t = F.fwd[(size_t)i * H + y];
       const float vx = t.y - t.x * vx_next;
       const float vn = t.z - t.x * vn_next;
-      const float2 z = F.rowxn[row + i];
+      const float2 z = F.rowxn[(size_t)i * H + y];
       if (i + 1 < s1) {
         if (F.grad_wrow)
           F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(256) void k_fuse_backward(const FrameDev* __restric
 void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, double lambda,
                               hipStream_t s) {
   // recompute the forward passes (row pass -> rowxn, column pass -> colxn)
-  launch_upsample_row(dframes, hframes, n, 2, nullptr, lambda, false, s);
-  launch_upsample_col(dframes, hframes, n, 2, nullptr, lambda, s);
+  launch_upsample_row(dframes, hframes, n, 2, nullptr, lambda, s);
+  launch_upsample_col(dframes, hframes, n, 2, nullptr, lambda, false, s);
   BlockMap bm;
   int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;

@@ -15,153 +15,273 @@ __device__ __forceinline__ void thomas_coeffs(float wl, float wr, double lam, fl
   c = (float)(-lam * (double)wr);
 }
 
+constexpr int kChunk = 16;
+
 // ---------------------------------------------------------------------------
-// Row pass: one lane per image row. Mode 0 composes the sparse map on the fly
-// (BP winner depth at node pixels (4x, 4y), else the semi-dense pre-fill, else
-// 0), so the full-resolution sparse map never has to be cleared or written;
+// Row pass: one lane per image row, the serial chain walked in chunks of 16
+// unknowns: the chunk's inputs are loaded together and its Thomas coefficients
+// written together, so each wave pays one memory latency per 16 steps, not per
+// step. Scratch (fwd) and the result (rowxn) are column-major, [x][y], so the
+// 64 lanes of a wave (64 consecutive rows) store one contiguous 1 KB / 512 B
+// row segment per step. Mode 0 composes the sparse map on the fly (BP winner
+// depth at node pixels (4x, 4y), else the semi-dense pre-fill, else 0), so the
+// full-resolution sparse map is never cleared or written unless asked for;
 // mode 1 reads an explicit sparse map; mode 2 is the generalised form with
-// caller-supplied right-hand sides and weights. Also resets the fusion z-buffer
-// row when asked (the fusion stage runs after the column pass).
+// caller-supplied right-hand sides and weights. Same operations, in the same
+// order, as the reference's smooth_row (src/depth_upsample.cu:94-197).
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut, double lam, int clear_zbuf) {
+                                               const float* __restrict__ wlut, double lam) {
+  __shared__ float lut[256];
+  if (MODE != 2) {
+    for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
+    __syncthreads();
+  }
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
-  const int W = F.W, H = F.H;
+  const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
   if (y >= H) return;
   const size_t row = (size_t)y * W;
-  if (clear_zbuf) {
-    for (int x = 0; x < W; ++x) F.zbuf[row + x] = ~0ull;
-  }
-  const uint8_t* img = F.key + (size_t)y * F.pitch;
-  auto weight = [&](int x) -> float {
-    if (MODE == 2) return F.g_wrow[(size_t)y * (W + 1) + x];
-    if (x == 0) return 0.0f;
-    if (x == W) return F.edge_r ? 1.0f : 0.0f;
-    return wlut[abs((int)img[x] - (int)img[x - 1])];
-  };
-  auto rhs = [&](int x, float& rx, float& rn) {
-    if (MODE == 2) {
-      const float d = F.g_d[row + x];
-      if (F.g_conf) {
-        const float c = F.g_conf[row + x];
-        rx = c * d;
-        rn = c;
-      } else {
-        rx = d;
-        rn = d > 0.0f ? 1.0f : 0.0f;
-      }
-      return;
-    }
-    float s;
-    if (MODE == 0) {
-      s = F.prefill ? F.prefill[row + x] : 0.0f;
-      if (((x | y) & 3) == 0 && (x >> 2) < F.Wc && (y >> 2) < F.Hc) {
-        const int c = (y >> 2) * F.Wc + (x >> 2);
-        if (F.label[c] >= 0) s = F.dcoarse[c];
-      }
-      if (F.sparse_out) F.sparse_out[row + x] = s;
-    } else {
-      s = F.sparse_in[row + x];
-    }
-    rx = s;
-    rn = s > 0.0f ? 1.0f : 0.0f;
-  };
+  const gptr<const uint8_t> img = gp(F.key + (size_t)y * F.pitch);
+  const gptr<float4> fwd = gp(F.fwd);
+  const gptr<float2> rowxn = gp(F.rowxn);
+  const gptr<const float> prefill = gp(F.prefill);
+  const gptr<const int32_t> label = gp((const int32_t*)F.label);
+  const gptr<const float> dcoarse = gp((const float*)F.dcoarse);
+  const gptr<float> sparse_out = gp(F.sparse_out);
+  const gptr<const float> sparse_in = gp(F.sparse_in);
+  const gptr<const float> g_d = gp(F.g_d);
+  const gptr<const float> g_conf = gp(F.g_conf);
+  const gptr<const float> g_wrow = gp(F.g_wrow);
+  const gptr<float> row_x_out = gp(F.row_x_out);
+  const gptr<float> row_n_out = gp(F.row_n_out);
+  const float edge = F.edge_r ? 1.0f : 0.0f;
   const int seg = F.row_seg < W ? F.row_seg : W;
   const int bounds[3] = {0, seg, W};
+#pragma unroll
   for (int sg = 0; sg < 2; ++sg) {
     const int s0 = bounds[sg], s1 = bounds[sg + 1];
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    float wl = weight(s0);
-    for (int i = s0; i < s1; ++i) {
-      const float wr = weight(i + 1);
-      float a, b, c;
-      thomas_coeffs(wl, wr, lam, a, b, c);
-      float rx, rn;
-      rhs(i, rx, rn);
-      const float den = b - fc * a;
-      const float ch = c / den;
-      const float dh = (rx - fd * a) / den;
-      const float nh = (rn - fn * a) / den;
-      F.fwd[row + i] = make_float4(ch, dh, nh, 0.0f);
-      fc = ch;
-      fd = dh;
-      fn = nh;
-      wl = wr;
+    for (int c0 = s0; c0 < s1; c0 += kChunk) {
+      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+      float w[kChunk + 1], rx[kChunk], rn[kChunk];
+      if (MODE == 2) {
+#pragma unroll
+        for (int k = 0; k <= kChunk; ++k) w[k] = (k <= n) ? g_wrow[(size_t)y * (W + 1) + c0 + k] : 0.0f;
+      } else {
+        int px[kChunk + 2];  // img[c0 - 1 + k]
+#pragma unroll
+        for (int k = 0; k < kChunk + 2; ++k) {
+          const int x = c0 - 1 + k;
+          px[k] = (x >= 0 && x < W && k <= n + 1) ? (int)img[x] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k <= kChunk; ++k) {
+          const int x = c0 + k;
+          w[k] = (x == 0) ? 0.0f : (x == W ? edge : lut[abs(px[k + 1] - px[k])]);
+        }
+      }
+      // every input of the chunk is loaded before anything is stored, so the
+      // loads of the chunk are in flight together
+      if (MODE == 2) {
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+          const float d = (k < n) ? g_d[row + c0 + k] : 0.0f;
+          if (g_conf) {
+            const float c = (k < n) ? g_conf[row + c0 + k] : 0.0f;
+            rx[k] = c * d;
+            rn[k] = c;
+          } else {
+            rx[k] = d;
+            rn[k] = d > 0.0f ? 1.0f : 0.0f;
+          }
+        }
+      } else {
+        float sv[kChunk];
+        if (MODE == 0) {
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) sv[k] = (prefill && k < n) ? prefill[row + c0 + k] : 0.0f;
+          // node pixels (4x, 4y): chunk starts are multiples of 16, so k = 0, 4, 8, 12
+          if ((y & 3) == 0 && (y >> 2) < Hc) {
+            const size_t cb = (size_t)(y >> 2) * Wc;
+            int lb[kChunk / 4];
+            float dv[kChunk / 4];
+#pragma unroll
+            for (int q = 0; q < kChunk / 4; ++q) {
+              const int cx = (c0 >> 2) + q;
+              const bool ok = (4 * q < n) && cx < Wc;
+              lb[q] = ok ? label[cb + cx] : -1;
+              dv[q] = ok ? dcoarse[cb + cx] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < kChunk / 4; ++q)
+              if (lb[q] >= 0) sv[4 * q] = dv[q];
+          }
+          if (sparse_out) {
+#pragma unroll
+            for (int k = 0; k < kChunk; ++k)
+              if (k < n) sparse_out[row + c0 + k] = sv[k];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) sv[k] = (k < n) ? sparse_in[row + c0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+          rx[k] = sv[k];
+          rn[k] = sv[k] > 0.0f ? 1.0f : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        if (k >= n) continue;
+        float a, b, c;
+        thomas_coeffs(w[k], w[k + 1], lam, a, b, c);
+        const float den = b - fc * a;
+        const float ch = c / den;
+        const float dh = (rx[k] - fd * a) / den;
+        const float nh = (rn[k] - fn * a) / den;
+        fwd[(size_t)(c0 + k) * H + y] = make_float4(ch, dh, nh, 0.0f);
+        fc = ch;
+        fd = dh;
+        fn = nh;
+      }
     }
     fd = 0.0f;
     fn = 0.0f;
-    for (int i = s1 - 1; i >= s0; --i) {
-      const float4 t = F.fwd[row + i];
-      const float xv = t.y - t.x * fd;
-      const float nv = t.z - t.x * fn;
-      F.rowxn[row + i] = make_float2(xv, nv);
-      if (MODE == 2 && F.row_x_out) {
-        F.row_x_out[row + i] = xv;
-        F.row_n_out[row + i] = nv;
+    const int last = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
+    for (int c0 = last; c0 >= s0; c0 -= kChunk) {
+      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+      float4 t[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        t[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k < n) t[k] = fwd[(size_t)(c0 + k) * H + y];
       }
-      fd = xv;
-      fn = nv;
+#pragma unroll
+      for (int k = kChunk - 1; k >= 0; --k) {
+        if (k >= n) continue;
+        const float xv = t[k].y - t[k].x * fd;
+        const float nv = t[k].z - t[k].x * fn;
+        rowxn[(size_t)(c0 + k) * H + y] = make_float2(xv, nv);
+        if (MODE == 2 && row_x_out) {
+          row_x_out[row + c0 + k] = xv;
+          row_n_out[row + c0 + k] = nv;
+        }
+        fd = xv;
+        fn = nv;
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Column pass: one lane per column, so every step reads and writes one
-// coalesced row segment across the wave. Output x / n (NaN where n = 0, as the
-// reference).
+// Column pass: one lane per column, chunks of 16 rows. The row results are
+// read as 16 consecutive float2 of the lane's own column (one 128 B line);
+// coefficients, outputs and the image are row-major and coalesced across the
+// wave. Output x / n (NaN where n = 0, as the reference,
+// src/depth_upsample.cu:198-302). Also resets the fusion z-buffer column when
+// asked (the fusion stage runs next).
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut, double lam) {
+                                               const float* __restrict__ wlut, double lam, int clear_zbuf) {
+  __shared__ float lut[256];
+  if (MODE != 2) {
+    for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
+    __syncthreads();
+  }
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
-  const int W = F.W, H = F.H;
+  const int W = F.W, H = F.H, pitch = F.pitch;
   if (x >= W) return;
-  auto weight = [&](int y) -> float {
-    if (MODE == 2) return F.g_wcol[(size_t)y * W + x];
-    if (y == 0) return 0.0f;
-    if (y == H) return F.edge_c ? 1.0f : 0.0f;
-    return wlut[abs((int)F.key[(size_t)y * F.pitch + x] - (int)F.key[(size_t)(y - 1) * F.pitch + x])];
-  };
+  const gptr<const uint8_t> key = gp(F.key);
+  const gptr<float4> fwd = gp(F.fwd);
+  const gptr<const float2> colin = gp((const float2*)F.rowxn + (size_t)x * H);
+  const gptr<float> dense = gp(F.dense);
+  const gptr<float> dense_out = gp(F.dense_out);
+  const gptr<float2> colxn = gp(F.colxn);
+  const gptr<const float> g_wcol = gp(F.g_wcol);
+  if (clear_zbuf) {
+    const gptr<unsigned long long> zbuf = gp(F.zbuf);
+    for (int i = 0; i < H; ++i) zbuf[(size_t)i * W + x] = ~0ull;
+  }
+  const float edge = F.edge_c ? 1.0f : 0.0f;
   const int seg = F.col_seg < H ? F.col_seg : H;
   const int bounds[3] = {0, seg, H};
+#pragma unroll
   for (int sg = 0; sg < 2; ++sg) {
     const int s0 = bounds[sg], s1 = bounds[sg + 1];
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    float wl = weight(s0);
-    for (int i = s0; i < s1; ++i) {
-      const float wr = weight(i + 1);
-      float a, b, c;
-      thomas_coeffs(wl, wr, lam, a, b, c);
-      const float2 r = F.rowxn[(size_t)i * W + x];
-      const float den = b - fc * a;
-      const float ch = c / den;
-      const float dh = (r.x - fd * a) / den;
-      const float nh = (r.y - fn * a) / den;
-      F.fwd[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
-      fc = ch;
-      fd = dh;
-      fn = nh;
-      wl = wr;
+    for (int c0 = s0; c0 < s1; c0 += kChunk) {
+      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+      float w[kChunk + 1];
+      float2 r[kChunk];
+      if (MODE == 2) {
+#pragma unroll
+        for (int k = 0; k <= kChunk; ++k) w[k] = (k <= n) ? g_wcol[(size_t)(c0 + k) * W + x] : 0.0f;
+      } else {
+        int px[kChunk + 2];  // key[(c0 - 1 + k), x]
+#pragma unroll
+        for (int k = 0; k < kChunk + 2; ++k) {
+          const int yy = c0 - 1 + k;
+          px[k] = (yy >= 0 && yy < H && k <= n + 1) ? (int)key[(size_t)yy * pitch + x] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k <= kChunk; ++k) {
+          const int yy = c0 + k;
+          w[k] = (yy == 0) ? 0.0f : (yy == H ? edge : lut[abs(px[k + 1] - px[k])]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        r[k] = make_float2(0.0f, 0.0f);
+        if (k < n) r[k] = colin[c0 + k];
+      }
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        if (k >= n) continue;
+        float a, b, c;
+        thomas_coeffs(w[k], w[k + 1], lam, a, b, c);
+        const float den = b - fc * a;
+        const float ch = c / den;
+        const float dh = (r[k].x - fd * a) / den;
+        const float nh = (r[k].y - fn * a) / den;
+        fwd[(size_t)(c0 + k) * W + x] = make_float4(ch, dh, nh, 0.0f);
+        fc = ch;
+        fd = dh;
+        fn = nh;
+      }
     }
     fd = 0.0f;
     fn = 0.0f;
-    for (int i = s1 - 1; i >= s0; --i) {
-      const float4 t = F.fwd[(size_t)i * W + x];
-      const float xv = t.y - t.x * fd;
-      const float nv = t.z - t.x * fn;
-      const float out = xv / nv;
-      if (F.colxn) F.colxn[(size_t)i * W + x] = make_float2(xv, nv);
-      F.dense[(size_t)i * W + x] = out;
-      if (F.dense_out) F.dense_out[(size_t)i * W + x] = out;
-      fd = xv;
-      fn = nv;
+    const int last = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
+    for (int c0 = last; c0 >= s0; c0 -= kChunk) {
+      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+      float4 t[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        t[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k < n) t[k] = fwd[(size_t)(c0 + k) * W + x];
+      }
+#pragma unroll
+      for (int k = kChunk - 1; k >= 0; --k) {
+        if (k >= n) continue;
+        const float xv = t[k].y - t[k].x * fd;
+        const float nv = t[k].z - t[k].x * fn;
+        const float out = xv / nv;
+        const size_t o = (size_t)(c0 + k) * W + x;
+        if (colxn) colxn[o] = make_float2(xv, nv);
+        dense[o] = out;
+        if (dense_out) dense_out[o] = out;
+        fd = xv;
+        fn = nv;
+      }
     }
   }
 }
@@ -306,27 +426,27 @@ static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
 }
 
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, bool clear_zbuf, hipStream_t s) {
+                         double lambda, hipStream_t s) {
   BlockMap bm;
   const int nb = rows_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
   else
-    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, hipStream_t s) {
+                         double lambda, bool clear_zbuf, hipStream_t s) {
   BlockMap bm;
   const int nb = cols_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 2)
-    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
   else
-    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
 }
 
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
@@ -349,8 +469,8 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
 
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
                      double lambda, bool clear_zbuf, hipStream_t s) {
-  launch_upsample_row(dframes, hframes, n, mode, wlut, lambda, clear_zbuf, s);
-  launch_upsample_col(dframes, hframes, n, mode, wlut, lambda, s);
+  launch_upsample_row(dframes, hframes, n, mode, wlut, lambda, s);
+  launch_upsample_col(dframes, hframes, n, mode, wlut, lambda, clear_zbuf, s);
 }
 
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s) {

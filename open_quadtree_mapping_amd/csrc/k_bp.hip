@@ -25,18 +25,18 @@ template <int N>
 __device__ __forceinline__ Lanes<N> message_from(const Lanes<N>& h, float p1, float p2, int lane) {
   Lanes<N> out;
   if constexpr (N == 64) {
-    const float hmin = wave_min(h.v[0]);
+    const float hmin = tree_min64(h.v[0]);
     const float hl = __shfl_up(h.v[0], 1, 64);
     const float hr = __shfl_down(h.v[0], 1, 64);
     float m = h.v[0];
     if (lane > 0) m = fminf(m, hl + p1);
     if (lane < 63) m = fminf(m, hr + p1);
     m = fminf(m, hmin + p2);
-    const float s = wave_tree_sum(m);
+    const float s = tree_sum64(m);
     out.v[0] = m - s / 64.0f;
   } else {
     const float h0 = h.v[0], h1 = h.v[1];
-    const float hmin = wave_min(h0 < h1 ? h0 : h1);
+    const float hmin = tree_min64(h0 < h1 ? h0 : h1);
     const float l0 = __shfl_up(h0, 1, 64);
     const float r0 = __shfl_down(h0, 1, 64);
     const float l1 = __shfl_up(h1, 1, 64);
@@ -50,7 +50,7 @@ __device__ __forceinline__ Lanes<N> message_from(const Lanes<N>& h, float p1, fl
     if (lane < 63) m1 = fminf(m1, r1 + p1);
     m0 = fminf(m0, hmin + p2);
     m1 = fminf(m1, hmin + p2);
-    const float s = wave_tree_sum(m0 + m1);  // first tree stage: d + (d + 64)
+    const float s = tree_sum64(m0 + m1);  // first tree stage: d + (d + 64)
     out.v[0] = m0 - s / 128.0f;
     out.v[1] = m1 - s / 128.0f;
   }
@@ -93,9 +93,9 @@ __global__ __launch_bounds__(256) void k_bp_sweep(const FrameDev* __restrict__ f
     if (x >= w) return;
     active = true;
   }
-  float* M = F.msg[l];
-  float* dst = M + ((size_t)y * w + x) * 4 * N;
-  const float* P = (first && l < 3) ? F.msg[l + 1] : nullptr;
+  const gptr<float> M = gp(F.msg[l]);
+  const gptr<float> dst = M + ((size_t)y * w + x) * 4 * N;
+  const gptr<const float> P = gp((const float*)((first && l < 3) ? F.msg[l + 1] : nullptr));
   const int wp = (l < 3) ? F.lw[l + 1] : 0;
   if (!active) {
     // first sweep only: copy the parent's messages (zeros at the coarsest level)
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_bp_sweep(const FrameDev* __restrict__ f
     }
     return M[(((size_t)ny * w + nx) * 4 + dir) * N + d];
   };
-  const float* data = F.cost[l] + ((size_t)y * w + x) * N;
+  const gptr<const float> data = gp((const float*)F.cost[l]) + ((size_t)y * w + x) * N;
   Lanes<N> hd[4];
 #pragma unroll
   for (int k = 0; k < N / 64; ++k) {
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(256) void k_extract(const FrameDev* __restrict__ fr
     }
     return;
   }
-  const float* M = F.msg[0];
-  const float* data = F.cost[0] + (size_t)cell * N;
+  const gptr<const float> M = gp((const float*)F.msg[0]);
+  const gptr<const float> data = gp((const float*)F.cost[0]) + (size_t)cell * N;
   float b[N / 64];
 #pragma unroll
   for (int k = 0; k < N / 64; ++k) {
@@ -195,21 +195,7 @@ __global__ __launch_bounds__(256) void k_extract(const FrameDev* __restrict__ fr
       id = lane;
     }
   }
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float ov = __shfl_xor(v, m, 64);
-    const int oid = __shfl_xor(id, m, 64);
-    const bool lower = (lane & m) == 0;
-    const float lo_v = lower ? v : ov, hi_v = lower ? ov : v;
-    const int lo_i = lower ? id : oid, hi_i = lower ? oid : id;
-    if (hi_v < lo_v) {
-      v = hi_v;
-      id = hi_i;
-    } else {
-      v = lo_v;
-      id = lo_i;
-    }
-  }
+  tree_argmin64(v, id);
   // neighbours of the winner
   const int dm1 = id > 0 ? id - 1 : 0, dp1 = id < N - 1 ? id + 1 : N - 1;
   float pre, post;
@@ -223,6 +209,8 @@ __global__ __launch_bounds__(256) void k_extract(const FrameDev* __restrict__ fr
     post = (dp1 >> 6) ? post1 : post0;
   }
   if (lane == 0) {
+    const gptr<float> dco = gp(F.dcoarse);
+    const gptr<int32_t> lab = gp(F.label);
     float disp = (float)id;
     if (id > 0 && id < N - 1) {
       const float a = (pre - 2.0f * v) + post;
@@ -230,8 +218,8 @@ __global__ __launch_bounds__(256) void k_extract(const FrameDev* __restrict__ fr
       const float ba = bb / a;
       if (isfinite(ba)) disp = (float)id - ba / 2.0f;
     }
-    F.dcoarse[cell] = (float)(1.0 / (istep * (double)disp + imin));
-    F.label[cell] = id;
+    dco[cell] = (float)(1.0 / (istep * (double)disp + imin));
+    lab[cell] = id;
   }
 }
 

@@ -128,24 +128,35 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
   const int cell = (lb_global - bm.start[f]) * 4 + (threadIdx.x >> 6);
   if (cell >= Wc * F.Hc) return;
   const int cyy = cell / Wc, cxx = cell - cyy * Wc;
-  float* out = F.cost[0] + (size_t)cell * N;
+  const gptr<float> out = gp(F.cost[0]) + (size_t)cell * N;
   const int x = 4 * cxx, y = 4 * cyy;
-  const int qs = 1 << F.levels[(size_t)y * W + x];
+  const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
   const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
-  if (F.count && lane == 0) F.count[cell] = node ? 1 : 0;
+  if (F.count && lane == 0) gp(F.count)[cell] = node ? 1 : 0;
   if (!node) {
 #pragma unroll
     for (int k = 0; k < N / 64; ++k) out[lane + 64 * k] = 0.0f;
     return;
   }
-  const uint8_t* key = F.key;
+  // descriptor fields into registers once (uniform: scalar registers)
   const int pitch = F.pitch;
+  const float fx = F.fx, fy = F.fy, cx = F.cx, cy = F.cy;
+  float T[4][12];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T[r][i] = F.T[r][i];
+  gptr<const uint8_t> ref[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ref[r] = gp(F.ref[r]);
+  const gptr<const uint8_t> key = gp(F.key);
   float patch[3][3];
 #pragma unroll
   for (int j = -1; j <= 1; ++j)
 #pragma unroll
-    for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = (float)key[(size_t)(y + j) * pitch + (x + i)];
-  const V3 dir = ray_dir(x, y, F.fx, F.fy, F.cx, F.cy);
+    for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = (float)key[(y + j) * pitch + (x + i)];
+  const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
+  float res[N / 64];
 #pragma unroll
   for (int k = 0; k < N / 64; ++k) {
     const int d = lane + 64 * k;
@@ -157,25 +168,21 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
     float cr[4], nr[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float* Tr = F.T[r];
-      const V3 q = se3_apply(Tr, P);
-      const float pxf = (F.fx * q.x) / q.z + F.cx;
-      const float pyf = (F.fy * q.y) / q.z + F.cy;
-      const int pxi = trunc_i32((double)pxf + 0.5);
-      const int pyi = trunc_i32((double)pyf + 0.5);
+      const V3 q = se3_apply(T[r], P);
+      const float pxf = (fx * q.x) / q.z + cx;
+      const float pyf = (fy * q.y) / q.z + cy;
+      const int pxi = round_half_up(pxf);
+      const int pyi = round_half_up(pyf);
       if (pxi >= 2 && pxi < W - 2 && pyi >= 2 && pyi < H - 2) {
-        const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
-        const uint8_t* ref = F.ref[r];
+        const float u2u = T[r][0], u2v = T[r][4], v2u = T[r][1], v2v = T[r][5];
         float sad = 0.0f;
 #pragma unroll
         for (int j = -1; j <= 1; ++j) {
 #pragma unroll
           for (int i = -1; i <= 1; ++i) {
-            int chx = trunc_i32((double)((pxf + u2u * (float)i) + v2u * (float)j) + 0.5);
-            int chy = trunc_i32((double)((pyf + u2v * (float)i) + v2v * (float)j) + 0.5);
-            chx = chx < 0 ? 0 : (chx > W - 1 ? W - 1 : chx);
-            chy = chy < 0 ? 0 : (chy > H - 1 ? H - 1 : chy);
-            sad += fabsf(patch[i + 1][j + 1] - (float)ref[(size_t)chy * pitch + chx]);
+            const int chx = min(max(round_half_up((pxf + u2u * (float)i) + v2u * (float)j), 0), W - 1);
+            const int chy = min(max(round_half_up((pyf + u2v * (float)i) + v2v * (float)j), 0), H - 1);
+            sad += fabsf(patch[i + 1][j + 1] - (float)ref[r][chy * pitch + chx]);
           }
         }
         cr[r] = sad;
@@ -187,8 +194,10 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
     }
     const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
     const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
-    out[d] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
+    res[k] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
   }
+#pragma unroll
+  for (int k = 0; k < N / 64; ++k) out[lane + 64 * k] = res[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -205,10 +214,10 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
   const int c3 = (blockIdx.x - bm.start[f]) * 4 + (threadIdx.x >> 6);
   if (c3 >= F.lw[3] * F.lh[3]) return;
   const int y3 = c3 / F.lw[3], x3 = c3 - y3 * F.lw[3];
-  const float* C0 = F.cost[0];
-  float* C1 = F.cost[1];
-  float* C2 = F.cost[2];
-  float* C3 = F.cost[3];
+  const gptr<const float> C0 = gp((const float*)F.cost[0]);
+  const gptr<float> C1 = gp(F.cost[1]);
+  const gptr<float> C2 = gp(F.cost[2]);
+  const gptr<float> C3 = gp(F.cost[3]);
   const int w0 = F.lw[0], h0 = F.lh[0], w1 = F.lw[1], h1 = F.lh[1], w2 = F.lw[2], h2 = F.lh[2];
 #pragma unroll
   for (int k = 0; k < N / 64; ++k) {

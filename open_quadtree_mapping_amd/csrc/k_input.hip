@@ -11,9 +11,9 @@
 
 namespace qm {
 
-__global__ __launch_bounds__(256) void k_remap_u8(const uint8_t* __restrict__ src, int W, int H, int spitch,
-                                                  const float* __restrict__ mapx, const float* __restrict__ mapy,
-                                                  uint8_t* __restrict__ dst) {
+__global__ __launch_bounds__(256) void k_remap_u8(const uint8_t* __restrict__ src, int SW, int SH, int spitch,
+                                                  int W, int H, const float* __restrict__ mapx,
+                                                  const float* __restrict__ mapy, uint8_t* __restrict__ dst) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= W * H) return;
   const float mx = mapx[i], my = mapy[i];
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_remap_u8(const uint8_t* __restrict__ sr
   const int x0 = ix >> 5, y0 = iy >> 5;
   const int a = ix & 31, b = iy & 31;
   auto px = [&](int x, int y) -> int {
-    return (x >= 0 && x < W && y >= 0 && y < H) ? (int)src[(size_t)y * spitch + x] : 0;
+    return (x >= 0 && x < SW && y >= 0 && y < SH) ? (int)src[(size_t)y * spitch + x] : 0;
   };
   const int v = (32 - a) * (32 - b) * 32 * px(x0, y0) + a * (32 - b) * 32 * px(x0 + 1, y0) +
                 (32 - a) * b * 32 * px(x0, y0 + 1) + a * b * 32 * px(x0 + 1, y0 + 1);
@@ -40,10 +40,10 @@ __global__ __launch_bounds__(256) void k_remap_u8(const uint8_t* __restrict__ sr
   dst[i] = (uint8_t)r;
 }
 
-void launch_remap_u8(const uint8_t* src, int W, int H, int spitch, const float* mapx, const float* mapy, uint8_t* dst,
-                     hipStream_t s) {
+void launch_remap_u8(const uint8_t* src, int SW, int SH, int spitch, int W, int H, const float* mapx,
+                     const float* mapy, uint8_t* dst, hipStream_t s) {
   const int nb = (W * H + 255) / 256;
-  hipLaunchKernelGGL(k_remap_u8, dim3(nb), dim3(256), 0, s, src, W, H, spitch, mapx, mapy, dst);
+  hipLaunchKernelGGL(k_remap_u8, dim3(nb), dim3(256), 0, s, src, SW, SH, spitch, W, H, mapx, mapy, dst);
 }
 
 }  // namespace qm

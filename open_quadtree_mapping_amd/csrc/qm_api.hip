@@ -261,8 +261,8 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
     }
   }
   { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, p->lambda, true, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, p->lambda, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, p->lambda, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, p->lambda, true, s); }
   { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
   { Mark m(t, QM_STAGE_FUSE_UPDATE); launch_fuse_update(df, hf, n, s); }
 }
@@ -303,6 +303,8 @@ int upload_frames(qm_batch* b) {
 }  // namespace
 
 extern "C" {
+
+int qm_abi_version(void) { return QM_ABI_VERSION; }
 
 const char* qm_last_error(void) { return g_err.c_str(); }
 

@@ -47,6 +47,7 @@ struct qm_ctx {
   float* d_mapx = nullptr;
   float* d_mapy = nullptr;
   uint8_t* d_raw = nullptr;
+  size_t raw_bytes = 0;
   uint8_t* h_stage = nullptr;
   float* h_depth = nullptr;
   float* h_debug = nullptr;
@@ -197,9 +198,7 @@ int qm_set_remap(qm_ctx* c, const float* map_x, const float* map_y) {
   if (!map_x) {
     dev_free(c->d_mapx);
     dev_free(c->d_mapy);
-    dev_free(c->d_raw);
     c->d_mapx = c->d_mapy = nullptr;
-    c->d_raw = nullptr;
     return QM_OK;
   }
   int rc;
@@ -209,17 +208,18 @@ int qm_set_remap(qm_ctx* c, const float* map_x, const float* map_y) {
     c->d_mapx = (float*)v;
     if ((rc = dev_alloc(&v, px * 4))) return rc;
     c->d_mapy = (float*)v;
-    if ((rc = dev_alloc(&v, px))) return rc;
-    c->d_raw = (uint8_t*)v;
   }
   QM_HIP(hipMemcpy(c->d_mapx, map_x, px * 4, hipMemcpyHostToDevice));
   QM_HIP(hipMemcpy(c->d_mapy, map_y, px * 4, hipMemcpyHostToDevice));
   return QM_OK;
 }
 
-int qm_add_frame(qm_ctx* c, const uint8_t* img, int pitch, const float T_curr_world[12], int* has_result) {
+int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch, const float T_curr_world[12],
+                 int* has_result) {
   if (!c || !img || !T_curr_world || !has_result) return fail(QM_ERR_INVALID, "NULL pointer");
-  if (pitch < c->W) return fail(QM_ERR_INVALID, "pitch < width");
+  if (img_w <= 0 || img_h <= 0 || pitch < img_w) return fail(QM_ERR_INVALID, "bad image size or pitch");
+  if (!c->d_mapx && (img_w != c->W || img_h != c->H))
+    return fail(QM_ERR_INVALID, "without a remap the image must be the depth map size");
   for (int i = 0; i < 12; ++i)
     if (!(T_curr_world[i] == T_curr_world[i])) return fail(QM_ERR_INVALID, "pose has NaN entries");
   *has_result = 0;
@@ -229,12 +229,28 @@ int qm_add_frame(qm_ctx* c, const uint8_t* img, int pitch, const float T_curr_wo
   if (c->income_slot < 0) c->income_slot = take_slot(c);
   uint8_t* slot = c->slot_ptr(c->income_slot);
   // SeedMatrix::input_raw / add_income_image: (remap) + upload
-  for (int y = 0; y < H; ++y) std::memcpy(c->h_stage + (size_t)y * W, img + (size_t)y * pitch, W);
   if (c->d_mapx) {
-    QM_HIP(hipMemcpyAsync(c->d_raw, c->h_stage, px, hipMemcpyHostToDevice, c->stream));
-    launch_remap_u8(c->d_raw, W, H, W, c->d_mapx, c->d_mapy, slot, c->stream);
+    const size_t raw = (size_t)img_w * img_h;
+    if (raw > c->raw_bytes) {
+      QM_HIP(hipStreamSynchronize(c->stream));
+      dev_free(c->d_raw);
+      c->d_raw = nullptr;
+      c->raw_bytes = 0;
+      if (c->h_stage) (void)hipHostFree(c->h_stage);
+      c->h_stage = nullptr;
+      void* v;
+      int rc = dev_alloc(&v, raw);
+      if (rc) return rc;
+      c->d_raw = (uint8_t*)v;
+      QM_HIP(hipHostMalloc((void**)&c->h_stage, raw > px ? raw : px, hipHostMallocDefault));
+      c->raw_bytes = raw;
+    }
+    for (int y = 0; y < img_h; ++y) std::memcpy(c->h_stage + (size_t)y * img_w, img + (size_t)y * pitch, img_w);
+    QM_HIP(hipMemcpyAsync(c->d_raw, c->h_stage, raw, hipMemcpyHostToDevice, c->stream));
+    launch_remap_u8(c->d_raw, img_w, img_h, img_w, W, H, c->d_mapx, c->d_mapy, slot, c->stream);
     QM_HIP(hipGetLastError());
   } else {
+    for (int y = 0; y < H; ++y) std::memcpy(c->h_stage + (size_t)y * W, img + (size_t)y * pitch, W);
     QM_HIP(hipMemcpyAsync(slot, c->h_stage, px, hipMemcpyHostToDevice, c->stream));
   }
   std::memcpy(c->income_T, T_curr_world, sizeof(float) * 12);

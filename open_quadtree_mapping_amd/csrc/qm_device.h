@@ -63,6 +63,27 @@ struct FrameDev {
   float* grad_depth_in;
 };
 
+// Pointers into the frame buffers, typed as global-memory pointers in device
+// code so that loads and stores compile to global_* instructions (flat ones
+// also count against the scalar-load counter, so every scalar reload of a
+// descriptor field would wait for all outstanding vector loads). Kernels copy
+// the descriptor fields they use into locals of this type once.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+#endif
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> gp(const T* p) {
+  return (gptr<const T>)p;
+}
+
 // blockIdx.x -> frame: frame f owns blocks [start[f], start[f+1]).
 struct BlockMap {
   int n;
@@ -93,6 +114,17 @@ __device__ __forceinline__ int trunc_i32(double v) {
   if (v >= 2147483647.0) return 2147483647;
   if (v <= -2147483648.0) return (-2147483647 - 1);
   return (int)v;
+}
+
+// The same rounding for the texel selection of the cost volume, exact in float:
+// floor(v) + (v - floor(v) >= 0.5), where v - floor(v) is exact (Sterbenz). It
+// equals trunc((double)v + 0.5) for v >= -0.5 and is <= 0 below, where the
+// callers reject (centre test p >= 2) or clamp (patch taps, to 0) either way;
+// NaN maps far below 0 as well.
+__device__ __forceinline__ int round_half_up(float v) {
+  v = fminf(fmaxf(v, -1.0e9f), 1.0e9f);
+  const float fl = floorf(v);
+  return (int)fl + ((v - fl) >= 0.5f ? 1 : 0);
 }
 
 __device__ __forceinline__ float pinned_expf(float x) { return (float)exp((double)x); }
@@ -139,6 +171,75 @@ __device__ __forceinline__ float wave_tree_sum(float v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, kWave);
   return v;
+}
+
+// ---- lane-0 halving trees on gfx950 cross-lane moves (no LDS traffic) ----
+// Stage m (32, 16, ..., 1): lane d < m combines lane d + m. permlane32_swap
+// hands lanes 0-31 the values of lanes 32-63, permlane16_swap hands rows 0 and
+// 2 the values of rows 1 and 3, DPP row_shl:k hands lane d (row-relative
+// d + k < 16) the value of lane d + k. Lane 0 ends with exactly the
+// shared-memory halving tree of the reference; readfirstlane broadcasts it.
+template <int K>
+__device__ __forceinline__ float dpp_row_shl(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ int dpp_row_shl_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x100 + K, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float upper32(float v) {
+  const unsigned u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ float upper16(float v) {
+  const unsigned u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane16_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ int upper32_i(int v) {
+  return (int)__builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false)[1];
+}
+__device__ __forceinline__ int upper16_i(int v) {
+  return (int)__builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false)[1];
+}
+__device__ __forceinline__ float bcast_lane0(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// sum tree: s[d] += s[d + m] for m = 32 .. 1, result of lane 0 in every lane
+__device__ __forceinline__ float tree_sum64(float v) {
+  v = v + upper32(v);
+  v = v + upper16(v);
+  v = v + dpp_row_shl<8>(v);
+  v = v + dpp_row_shl<4>(v);
+  v = v + dpp_row_shl<2>(v);
+  v = v + dpp_row_shl<1>(v);
+  return bcast_lane0(v);
+}
+
+__device__ __forceinline__ float tree_min64(float v) {
+  float o;
+  o = upper32(v); v = o < v ? o : v;
+  o = upper16(v); v = o < v ? o : v;
+  o = dpp_row_shl<8>(v); v = o < v ? o : v;
+  o = dpp_row_shl<4>(v); v = o < v ? o : v;
+  o = dpp_row_shl<2>(v); v = o < v ? o : v;
+  o = dpp_row_shl<1>(v); v = o < v ? o : v;
+  return bcast_lane0(v);
+}
+
+// argmin tree: lane d < m takes lane d + m only if strictly smaller (ties keep
+// the lower index), as src/bpextract.cu:347-355
+__device__ __forceinline__ void tree_argmin64(float& v, int& id) {
+  float ov;
+  int oi;
+  ov = upper32(v); oi = upper32_i(id); if (ov < v) { v = ov; id = oi; }
+  ov = upper16(v); oi = upper16_i(id); if (ov < v) { v = ov; id = oi; }
+  ov = dpp_row_shl<8>(v); oi = dpp_row_shl_i<8>(id); if (ov < v) { v = ov; id = oi; }
+  ov = dpp_row_shl<4>(v); oi = dpp_row_shl_i<4>(id); if (ov < v) { v = ov; id = oi; }
+  ov = dpp_row_shl<2>(v); oi = dpp_row_shl_i<2>(id); if (ov < v) { v = ov; id = oi; }
+  ov = dpp_row_shl<1>(v); oi = dpp_row_shl_i<1>(id); if (ov < v) { v = ov; id = oi; }
+  v = bcast_lane0(v);
+  id = __builtin_amdgcn_readfirstlane(id);
 }
 
 __device__ __forceinline__ int wave_and(int v) {

@@ -16,6 +16,7 @@ int fail(int code, const std::string& msg);
 // device allocation that honours the test cap and the byte accounting
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
-void launch_remap_u8(const uint8_t* src, int W, int H, int spitch, const float* mapx, const float* mapy, uint8_t* dst,
-                     hipStream_t s);
+// remap a SW x SH source into a W x H destination through per-pixel maps
+void launch_remap_u8(const uint8_t* src, int SW, int SH, int spitch, int W, int H, const float* mapx,
+                     const float* mapy, uint8_t* dst, hipStream_t s);
 }  // namespace qm

@@ -42,9 +42,9 @@ void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, in
                      double lambda, bool clear_zbuf, hipStream_t s);
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s);
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, bool clear_zbuf, hipStream_t s);
-void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
                          double lambda, hipStream_t s);
+void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                         double lambda, bool clear_zbuf, hipStream_t s);
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);

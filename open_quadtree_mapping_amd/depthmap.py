@@ -35,11 +35,12 @@ class Depthmap:
 
     def add_frames(self, img_curr, T_curr_world) -> bool:
         img = np.ascontiguousarray(img_curr, dtype=np.uint8)
-        if img.shape != (self.H, self.W):
-            raise ValueError(f"image must be {self.H}x{self.W} uint8")
+        if img.ndim != 2:
+            raise ValueError("image must be a 2-D uint8 array")
+        ih, iw = img.shape
         T = np.ascontiguousarray(T_curr_world, dtype=np.float32).reshape(12)
         has = C.c_int(0)
-        check(lib().qm_add_frame(self._h, img.ctypes.data_as(C.c_void_p), self.W, T.ctypes.data_as(C.c_void_p),
+        check(lib().qm_add_frame(self._h, img.ctypes.data_as(C.c_void_p), iw, ih, iw, T.ctypes.data_as(C.c_void_p),
                                  C.byref(has)))
         return bool(has.value)
 

@@ -57,3 +57,9 @@ def test_invalid_arguments_fail_before_device_work():
 def test_check_raises_with_message():
     with pytest.raises(_lib.QmError, match="size"):
         _lib.check(_lib.lib().qm_quadtree(None, 4, 4, 4, None, None))
+
+
+def test_abi_version_matches_binding():
+    assert _lib.lib().qm_abi_version() == _lib.ABI_VERSION
+    hdr = open(os.path.join(ROOT, "include", "qm.h")).read()
+    assert f"#define QM_ABI_VERSION {_lib.ABI_VERSION}" in hdr

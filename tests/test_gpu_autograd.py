@@ -91,7 +91,8 @@ def test_fuse_adjoint_vs_fd():
     seeds[..., 2] = rng.uniform(2, 6, (H, W))
     seeds[..., 3] = rng.uniform(0.05, 2, (H, W))
     depth = seeds[..., 2] * rng.uniform(0.9, 1.1, (H, W))
-    depth[rng.random((H, W)) < 0.1] = rng.uniform(1, 8, (H, W))[rng.random((H, W)) < 0.1]  # some resets
+    reset = rng.random((H, W)) < 0.1  # measurements far from the prior: the reset branch
+    depth[reset] = rng.uniform(1, 8, (H, W))[reset]
     T = np.eye(3, 4, dtype=np.float32)
     T[0, 3] = 0.01
     gdep = rng.standard_normal((H, W))

@@ -61,3 +61,36 @@ def test_remap_identity_shift_and_half_pixel():
     half = (a * 16 * 32 * 32 + b * 16 * 32 * 32 + (1 << 14)) >> 15
     assert np.array_equal(outs["half"][:, :-1], half)
     assert np.array_equal(outs["half"][:, -1], (img[:, -1].astype(np.int64) * 16 * 32 * 32 + (1 << 14)) >> 15)
+
+
+def remap_fixed_point(src, mx, my):
+    """OpenCV's 8-bit INTER_LINEAR remap in fixed point: coordinates rounded to
+    1/32 pixel (half to even), weights (32-a)(32-b)*32 ..., (sum + 2^14) >> 15,
+    constant border 0."""
+    SH, SW = src.shape
+    ix = np.rint(mx.astype(np.float32) * np.float32(32)).astype(np.int64)
+    iy = np.rint(my.astype(np.float32) * np.float32(32)).astype(np.int64)
+    x0, y0, a, b = ix >> 5, iy >> 5, ix & 31, iy & 31
+
+    def px(x, y):
+        ok = (x >= 0) & (x < SW) & (y >= 0) & (y < SH)
+        return np.where(ok, src[np.clip(y, 0, SH - 1), np.clip(x, 0, SW - 1)].astype(np.int64), 0)
+    v = ((32 - a) * (32 - b) * 32 * px(x0, y0) + a * (32 - b) * 32 * px(x0 + 1, y0) +
+         (32 - a) * b * 32 * px(x0, y0 + 1) + a * b * 32 * px(x0 + 1, y0 + 1))
+    return np.clip((v + (1 << 14)) >> 15, 0, 255).astype(np.uint8)
+
+
+def test_remap_downsample_from_larger_raw_image():
+    # the ROS node's maps undistort and resize (downsample_factor): a W x H depth
+    # map fed from a larger raw camera image
+    W, H = 64, 48
+    rng = np.random.default_rng(1)
+    raw = rng.integers(0, 256, (2 * H + 3, 2 * W + 5)).astype(np.uint8)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    mx = xx * 2.03 + 0.37
+    my = yy * 1.97 + 0.61
+    dm = Depthmap(W, H, 50.0, 32.0, 50.0, 24.0, mx, my, 1)
+    assert dm.add_frames(raw, np.eye(3, 4, dtype=np.float32))
+    got = dm.getReferenceImage()
+    dm.close()
+    assert np.array_equal(got, remap_fixed_point(raw, mx, my))
