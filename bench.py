@@ -102,8 +102,8 @@ def stage_bytes(levels, W, H, N):
         "pyramid": 4 * N * sum(a * b for a, b in zip(lw, lh)),
         "bp_sweep": bp,
         "extract": nodes * (20 * N + 8) + (C - nodes) * 9,
-        "upsample_row": px + 8 * C + 8 * px + 8 * px,
-        "upsample_col": px + 8 * px + 4 * px,
+        "upsample_row": px + 8 * C + 8 * px,           # image, labels + depths, row results
+        "upsample_col": px + 8 * px + 4 * px + 8 * px,  # image, row results, dense, z-buffer reset
         "fuse_warp": 40 * px,
         "fuse_update": 48 * px,
     }, upd
