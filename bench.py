@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the batched launches without hipGraph replay (for counter collection)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB five times and exit: the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -118,6 +123,16 @@ def main():
     import numpy as np
     import torch
 
+    if args.copy_mb:
+        n = args.copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
+
     from open_quadtree_mapping_amd import default_params
     from open_quadtree_mapping_amd import stages as S
     from open_quadtree_mapping_amd.batch import Batch
@@ -152,8 +167,9 @@ def main():
     # ---- batched path (hipGraph) -------------------------------------------------
     batch = Batch(frames, params, outputs=("depth",))
     stream = torch.cuda.current_stream()
+    use_graph = not args.no_graph
     for _ in range(args.warmup):
-        batch.run(graph=True)
+        batch.run(graph=use_graph)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -161,7 +177,7 @@ def main():
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(args.steps):
-        batch.run(graph=True)
+        batch.run(graph=use_graph)
     e1.record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -259,7 +275,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded planar scenes (6 textured rectangles, fractal value noise) rendered to uint8",
         "config": {"workload": f"{B} streams x {W}x{H} keyframes, {N} inverse-depth hypotheses, full pipeline "
-                               "(quadtree, cost volume, HBP, upsample, fusion), one hipGraph replay per step",
+                               "(quadtree, cost volume, HBP, upsample, fusion), "
+                               + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
                    "streams": B, "width": W, "height": H, "hypotheses": N},
         "plain_maps_per_s": None if plain is None else round(plain, 2),
         "batched_over_plain": None if plain is None else round(maps_per_s / plain, 3),

@@ -8,7 +8,7 @@
  * QM_ERR_* code; qm_last_error() gives a thread-local message.
  *
  * Layouts (all row-major, no padding unless a pitch argument says otherwise):
- *   image        uint8  H x W            (pitch in bytes, >= W)
+ *   image        uint8  H x W            (pitch in bytes, W <= pitch <= 262144)
  *   levels       uint8  H x W            quadtree level per pixel, in {2,3,4}
  *   cost         float  (H/4) x (W/4) x N
  *   depth_coarse float  (H/4) x (W/4)    BP winner depth per node cell, 0 elsewhere

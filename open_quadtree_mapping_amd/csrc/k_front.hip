@@ -112,92 +112,161 @@ __global__ __launch_bounds__(256) void k_quadtree(const FrameDev* __restrict__ f
 
 // ---------------------------------------------------------------------------
 // Cost volume (reference: src/depth_extract.cu:135-244).
-// One wave per coarse cell (4x4 pixels), lane = inverse-depth sample (two per
-// lane for N = 128). Non-node cells are written as zeros, so the volume needs
-// no separate clear. Slots {0,0,1,2,3} read reference 0 twice: it is evaluated
+// One lane per coarse cell (4x4 pixels), 64 consecutive cells per wave, looping
+// over 16 inverse-depth samples: for one sample the taps of neighbouring cells
+// fall on neighbouring texels of the reference images, so a wave's gather
+// touches a few cache lines instead of one per lane. The four waves of a block
+// take four sample ranges of the same cells; their results meet in LDS and
+// leave as contiguous rows. Non-node cells are written as zeros, so the volume
+// needs no separate clear. Slots {0,0,1,2,3} read reference 0 twice: it is evaluated
 // once and counted twice in the reference's slot tree ((c0+c4)+c2)+(c1+c3).
+//
+// Patch taps. Tap (i, j) of reference r samples texel
+// round((px + u2u*i) + v2u*j, (py + u2v*i) + v2v*j), with round(v) =
+// (int)((double)v + 0.5) and the texel index clamped to the image. The SAD of
+// integer grey levels is an integer below 2^24, so it is accumulated exactly
+// in an integer register (v_sad_u16) and converted once.
+// When the centre texel lies in [2, W-3] x [2, H-3] and the rotation
+// terms satisfy |u2u| + |v2u| <= 1.75 and |u2v| + |v2v| <= 1.75 (true for any
+// rotation), every tap lies in [-0.25, W-0.75) (the float error of the two
+// additions is below 2e-3 for W, H <= 16384), so the clamp can never act and
+// the rounding is a single v_cvt_rpi_i32_f32 (floor(v + 0.5), exact on the
+// hardware for every float; the argument is >= -0.5 here, where floor and
+// truncation agree). Other transforms take the general path with the clamp.
 // ---------------------------------------------------------------------------
-template <int N>
+// The tap bound above, evaluated on the host for every reference of a launch
+// (same float arithmetic as the device); false for NaN.
+static bool cost_fast_taps(const float* Tr) {
+  return (fabsf(Tr[0]) + fabsf(Tr[1]) <= 1.75f) && (fabsf(Tr[4]) + fabsf(Tr[5]) <= 1.75f);
+}
+
+__device__ __forceinline__ int cvt_rpi(float v) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+template <bool FAST>
+__device__ __forceinline__ float patch_sad(gptr<const uint8_t> ref, int W, int H, int pitch, float px, float py,
+                                           const float* Tr, const unsigned (&patch)[3][3]) {
+  const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
+  if (FAST) {
+    unsigned sad = 0;
+#pragma unroll
+    for (int i = -1; i <= 1; ++i) {
+      // u2u * i is exact (+-u2u or a signed zero, which the rounding ignores)
+      const float bx = i == 0 ? px : px + (i < 0 ? -u2u : u2u);
+      const float by = i == 0 ? py : py + (i < 0 ? -u2v : u2v);
+#pragma unroll
+      for (int j = -1; j <= 1; ++j) {
+        const float sx = j == 0 ? bx : bx + (j < 0 ? -v2u : v2u);
+        const float sy = j == 0 ? by : by + (j < 0 ? -v2v : v2v);
+        const unsigned off = __umul24((unsigned)cvt_rpi(sy), (unsigned)pitch) + (unsigned)cvt_rpi(sx);
+        sad = __builtin_amdgcn_sad_u16(patch[i + 1][j + 1], (unsigned)ref[off], sad);
+      }
+    }
+    return (float)sad;
+  }
+  float sad = 0.0f;
+#pragma unroll
+  for (int j = -1; j <= 1; ++j) {
+#pragma unroll
+    for (int i = -1; i <= 1; ++i) {
+      const int chx = min(max(round_half_up((px + u2u * (float)i) + v2u * (float)j), 0), W - 1);
+      const int chy = min(max(round_half_up((py + u2v * (float)i) + v2v * (float)j), 0), H - 1);
+      sad += fabsf((float)patch[i + 1][j + 1] - (float)ref[(size_t)chy * pitch + chx]);
+    }
+  }
+  return sad;
+}
+
+template <int N, bool FAST>
 __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
                                               const float* __restrict__ depth_table, int nblocks) {
+  // block = 64 consecutive cells x 64 samples; wave w takes samples
+  // [hb*64 + 16w, hb*64 + 16w + 16) of every cell (hb = half for N = 128)
+  constexpr int kHalves = N / 64;
+  constexpr int kRow = 65;  // LDS row stride in floats (conflict-free column writes)
+  __shared__ float tile[64 * kRow];
   const int lb_global = xcd_logical_block(blockIdx.x, nblocks);
   const int f = find_frame(bm, lb_global);
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H, Wc = F.Wc;
-  const int lane = threadIdx.x & 63;
-  const int cell = (lb_global - bm.start[f]) * 4 + (threadIdx.x >> 6);
-  if (cell >= Wc * F.Hc) return;
-  const int cyy = cell / Wc, cxx = cell - cyy * Wc;
-  const gptr<float> out = gp(F.cost[0]) + (size_t)cell * N;
-  const int x = 4 * cxx, y = 4 * cyy;
-  const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
-  const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
-  if (F.count && lane == 0) gp(F.count)[cell] = node ? 1 : 0;
-  if (!node) {
+  const int ncell = Wc * F.Hc;
+  const int lb = lb_global - bm.start[f];
+  const int cell0 = (lb / kHalves) * 64, hb = lb - (lb / kHalves) * kHalves;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cell = cell0 + lane;
+  const int dbase = hb * 64 + 16 * w;
+  float* trow = tile + lane * kRow + 16 * w;
+  if (cell < ncell) {
+    const int cyy = cell / Wc, cxx = cell - cyy * Wc;
+    const int x = 4 * cxx, y = 4 * cyy;
+    const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
+    const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
+    if (F.count && hb == 0 && w == 0) gp(F.count)[cell] = node ? 1 : 0;
+    if (!node) {
 #pragma unroll
-    for (int k = 0; k < N / 64; ++k) out[lane + 64 * k] = 0.0f;
-    return;
-  }
-  // descriptor fields into registers once (uniform: scalar registers)
-  const int pitch = F.pitch;
-  const float fx = F.fx, fy = F.fy, cx = F.cx, cy = F.cy;
-  float T[4][12];
+      for (int t = 0; t < 16; ++t) trow[t] = 0.0f;
+    } else {
+      // descriptor fields into registers once (uniform: scalar registers)
+      const int pitch = F.pitch;
+      const float fx = F.fx, fy = F.fy, cx = F.cx, cy = F.cy;
+      float T[4][12];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int i = 0; i < 12; ++i) T[r][i] = F.T[r][i];
-  gptr<const uint8_t> ref[4];
+        for (int i = 0; i < 12; ++i) T[r][i] = F.T[r][i];
+      gptr<const uint8_t> ref[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) ref[r] = gp(F.ref[r]);
-  const gptr<const uint8_t> key = gp(F.key);
-  float patch[3][3];
+      for (int r = 0; r < 4; ++r) ref[r] = gp(F.ref[r]);
+      const gptr<const uint8_t> key = gp(F.key);
+      unsigned patch[3][3];
 #pragma unroll
-  for (int j = -1; j <= 1; ++j)
+      for (int j = -1; j <= 1; ++j)
 #pragma unroll
-    for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = (float)key[(y + j) * pitch + (x + i)];
-  const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
-  float res[N / 64];
+        for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = key[(y + j) * pitch + (x + i)];
+      const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
+      const float wlim = (float)W - 2.5f, hlim = (float)H - 2.5f;  // exact for W, H <= 16384
+#pragma unroll 2
+      for (int t = 0; t < 16; ++t) {
+        const float depth = depth_table[dbase + t];
+        V3 P;
+        P.x = dir.x * depth;
+        P.y = dir.y * depth;
+        P.z = dir.z * depth;
+        float cr[4], nr[4];
 #pragma unroll
-  for (int k = 0; k < N / 64; ++k) {
-    const int d = lane + 64 * k;
-    const float depth = depth_table[d];
-    V3 P;
-    P.x = dir.x * depth;
-    P.y = dir.y * depth;
-    P.z = dir.z * depth;
-    float cr[4], nr[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const V3 q = se3_apply(T[r], P);
-      const float pxf = (fx * q.x) / q.z + cx;
-      const float pyf = (fy * q.y) / q.z + cy;
-      const int pxi = round_half_up(pxf);
-      const int pyi = round_half_up(pyf);
-      if (pxi >= 2 && pxi < W - 2 && pyi >= 2 && pyi < H - 2) {
-        const float u2u = T[r][0], u2v = T[r][4], v2u = T[r][1], v2v = T[r][5];
-        float sad = 0.0f;
-#pragma unroll
-        for (int j = -1; j <= 1; ++j) {
-#pragma unroll
-          for (int i = -1; i <= 1; ++i) {
-            const int chx = min(max(round_half_up((pxf + u2u * (float)i) + v2u * (float)j), 0), W - 1);
-            const int chy = min(max(round_half_up((pyf + u2v * (float)i) + v2v * (float)j), 0), H - 1);
-            sad += fabsf(patch[i + 1][j + 1] - (float)ref[r][chy * pitch + chx]);
+        for (int r = 0; r < 4; ++r) {
+          const V3 q = se3_apply(T[r], P);
+          const float pxf = (fx * q.x) / q.z + cx;
+          const float pyf = (fy * q.y) / q.z + cy;
+          // 2 <= (int)((double)p + 0.5) < W - 2  <=>  1.5 <= p < W - 2.5 (false for NaN)
+          cr[r] = 0.0f;
+          nr[r] = 0.0f;
+          if (pxf >= 1.5f && pxf < wlim && pyf >= 1.5f && pyf < hlim) {
+            cr[r] = patch_sad<FAST>(ref[r], W, H, pitch, pxf, pyf, T[r], patch);
+            nr[r] = 1.0f;
           }
         }
-        cr[r] = sad;
-        nr[r] = 1.0f;
-      } else {
-        cr[r] = 0.0f;
-        nr[r] = 0.0f;
+        const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
+        const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
+        trow[t] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
       }
     }
-    const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
-    const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
-    res[k] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
   }
+  __syncthreads();
+  // the block's 64 rows of 64 samples go out as whole 256 B segments
+  const gptr<float4> out = (gptr<float4>)gp(F.cost[0]);
 #pragma unroll
-  for (int k = 0; k < N / 64; ++k) out[lane + 64 * k] = res[k];
+  for (int k = 0; k < 4; ++k) {
+    const int i = threadIdx.x + 256 * k;  // float4 index in the 64 x 16 tile
+    const int c = i >> 4, q = i & 15;
+    if (cell0 + c < ncell) {
+      const float* src = tile + c * kRow + 4 * q;
+      out[((size_t)(cell0 + c) * N + hb * 64) / 4 + q] = make_float4(src[0], src[1], src[2], src[3]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -265,13 +334,24 @@ void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hi
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, hipStream_t s) {
   BlockMap bm;
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].Wc * hframes[i].Hc + 3) / 4;
+  bool fast = true;
+  for (int i = 0; i < n; ++i) {
+    counts[i] = ((hframes[i].Wc * hframes[i].Hc + 63) / 64) * (N / 64);
+    for (int r = 0; r < 4; ++r) fast = fast && cost_fast_taps(hframes[i].T[r]);
+  }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
-  if (N == 64)
-    hipLaunchKernelGGL(k_cost<64>, dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
-  else
-    hipLaunchKernelGGL(k_cost<128>, dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+  if (N == 64) {
+    if (fast)
+      hipLaunchKernelGGL((k_cost<64, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+    else
+      hipLaunchKernelGGL((k_cost<64, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+  } else {
+    if (fast)
+      hipLaunchKernelGGL((k_cost<128, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+    else
+      hipLaunchKernelGGL((k_cost<128, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+  }
 }
 
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s) {

@@ -97,6 +97,8 @@ int check_params(const qm_params* p) {
 int check_size(int W, int H, int pitch) {
   if (W < 16 || H < 16 || W > 16384 || H > 16384) return fail(QM_ERR_INVALID, "image size out of range [16, 16384]");
   if (pitch < W) return fail(QM_ERR_INVALID, "pitch < width");
+  // texel offsets y * pitch + x are formed in 32 bits (24-bit multiply)
+  if (pitch > 262144) return fail(QM_ERR_INVALID, "pitch above 262144 bytes");
   return QM_OK;
 }
 

@@ -95,6 +95,24 @@ def test_cost_volume(W, H, seed, N, imin, imax):
     assert np.array_equal(got, rc), f"max abs diff {np.abs(got - rc).max()}"
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_cost_volume_general_transform(N):
+    # transforms whose rotation block is not a rotation (rows scaled by 2, so
+    # |u2u| + |v2u| exceeds 1.75) take the kernel's clamped general tap path;
+    # one such reference switches the whole launch to it
+    W, H, seed = 376, 240, 6
+    ks = keyframe_set(W, H, seed)
+    T = ks["T"].copy()
+    T[1, :2, :3] *= np.float32(2.0)
+    T[3, :2, :3] *= np.float32(0.7)
+    print(f"cost volume general transform N={N}", flush=True)
+    levels = O.quadtree(ks["key"])
+    p = default_params(n_hyp=N, idepth_min=0.02, idepth_max=2.0)
+    cost = S.cost_volume(dev(ks["key"]), [dev(r) for r in ks["refs"]], T, cam_tuple(ks["cam"]), dev(levels), p)
+    rc, _ = O.cost_volume(ks["key"], ks["refs"], T, ks["cam"], N, 0.02, 2.0, levels)
+    assert np.array_equal(host(cost), rc)
+
+
 @pytest.mark.parametrize("W,H,seed,N", [(376, 240, 3, 64), (752, 480, 5, 64), (376, 240, 4, 128)])
 def test_bp_extract(W, H, seed, N):
     ks = keyframe_set(W, H, seed)
