@@ -169,7 +169,9 @@ int qm_batch_run(qm_batch* b, int mode, void* stream);
  * synchronising the stream. For measurement only. */
 int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
 /* Device pointer of an internal buffer of frame i, for inspection:
- * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state) */
+ * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state).
+ * The internal cost volume keeps each cell's row of N samples in the row layout:
+ * sample d = j + 16 s at position (N/16) * j + s (j < 16). */
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);
 int qm_batch_reset_seeds(qm_batch* b);
 int qm_batch_destroy(qm_batch* b);

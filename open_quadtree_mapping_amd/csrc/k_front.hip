@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_quadtree(const FrameDev* __restrict__ f
 // ---------------------------------------------------------------------------
 // Cost volume (reference: src/depth_extract.cu:135-244).
 // One lane per coarse cell (4x4 pixels), 64 consecutive cells per wave, looping
-// over 16 inverse-depth samples: for one sample the taps of neighbouring cells
+// over N/4 inverse-depth samples: for one sample the taps of neighbouring cells
 // fall on neighbouring texels of the reference images, so a wave's gather
 // touches a few cache lines instead of one per lane. The four waves of a block
 // take four sample ranges of the same cells; their results meet in LDS and
@@ -182,32 +182,31 @@ __device__ __forceinline__ float patch_sad(gptr<const uint8_t> ref, int W, int H
 
 template <int N, bool FAST>
 __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
-                                              const float* __restrict__ depth_table, int nblocks) {
-  // block = 64 consecutive cells x 64 samples; wave w takes samples
-  // [hb*64 + 16w, hb*64 + 16w + 16) of every cell (hb = half for N = 128)
-  constexpr int kHalves = N / 64;
-  constexpr int kRow = 65;  // LDS row stride in floats (conflict-free column writes)
+                                              const float* __restrict__ depth_table, int nblocks, int natural) {
+  // block = 64 consecutive cells x N samples; wave w takes samples
+  // [w*N/4, (w+1)*N/4) of every cell
+  constexpr int kPerWave = N / 4;
+  constexpr int kRow = N + 1;  // LDS row stride in floats (conflict-free column writes)
   __shared__ float tile[64 * kRow];
   const int lb_global = xcd_logical_block(blockIdx.x, nblocks);
   const int f = find_frame(bm, lb_global);
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H, Wc = F.Wc;
   const int ncell = Wc * F.Hc;
-  const int lb = lb_global - bm.start[f];
-  const int cell0 = (lb / kHalves) * 64, hb = lb - (lb / kHalves) * kHalves;
+  const int cell0 = (lb_global - bm.start[f]) * 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cell = cell0 + lane;
-  const int dbase = hb * 64 + 16 * w;
-  float* trow = tile + lane * kRow + 16 * w;
+  const int dbase = kPerWave * w;
+  float* trow = tile + lane * kRow + dbase;
   if (cell < ncell) {
     const int cyy = cell / Wc, cxx = cell - cyy * Wc;
     const int x = 4 * cxx, y = 4 * cyy;
     const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
     const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
-    if (F.count && hb == 0 && w == 0) gp(F.count)[cell] = node ? 1 : 0;
+    if (F.count && w == 0) gp(F.count)[cell] = node ? 1 : 0;
     if (!node) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t) trow[t] = 0.0f;
+      for (int t = 0; t < kPerWave; ++t) trow[t] = 0.0f;
     } else {
       // descriptor fields into registers once (uniform: scalar registers)
       const int pitch = F.pitch;
@@ -229,7 +228,7 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
       const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
       const float wlim = (float)W - 2.5f, hlim = (float)H - 2.5f;  // exact for W, H <= 16384
 #pragma unroll 2
-      for (int t = 0; t < 16; ++t) {
+      for (int t = 0; t < kPerWave; ++t) {
         const float depth = depth_table[dbase + t];
         V3 P;
         P.x = dir.x * depth;
@@ -242,12 +241,16 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
           const float pxf = (fx * q.x) / q.z + cx;
           const float pyf = (fy * q.y) / q.z + cy;
           // 2 <= (int)((double)p + 0.5) < W - 2  <=>  1.5 <= p < W - 2.5 (false for NaN)
-          cr[r] = 0.0f;
-          nr[r] = 0.0f;
-          if (pxf >= 1.5f && pxf < wlim && pyf >= 1.5f && pyf < hlim) {
-            cr[r] = patch_sad<FAST>(ref[r], W, H, pitch, pxf, pyf, T[r], patch);
-            nr[r] = 1.0f;
+          const bool in = pxf >= 1.5f && pxf < wlim && pyf >= 1.5f && pyf < hlim;
+          if (FAST) {
+            // branch-free: lanes out of view sample around (2, 2), always in bounds,
+            // and drop the result, so the taps of all four references issue together
+            const float sad = patch_sad<true>(ref[r], W, H, pitch, in ? pxf : 2.0f, in ? pyf : 2.0f, T[r], patch);
+            cr[r] = in ? sad : 0.0f;
+          } else {
+            cr[r] = in ? patch_sad<false>(ref[r], W, H, pitch, pxf, pyf, T[r], patch) : 0.0f;
           }
+          nr[r] = in ? 1.0f : 0.0f;
         }
         const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
         const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
@@ -256,15 +259,24 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
     }
   }
   __syncthreads();
-  // the block's 64 rows of 64 samples go out as whole 256 B segments
-  const gptr<float4> out = (gptr<float4>)gp(F.cost[0]);
+  // the block's 64 cost rows are contiguous in the volume: written as float4,
+  // in the row layout of the pipeline or, for callers, in sample order
+  const gptr<float4> out = (gptr<float4>)(gp(F.cost[0]) + (size_t)cell0 * N);
+  constexpr int kQ = N / 4;  // float4 per row
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int i = threadIdx.x + 256 * k;  // float4 index in the 64 x 16 tile
-    const int c = i >> 4, q = i & 15;
+  for (int k = 0; k < N / 16; ++k) {
+    const int i = threadIdx.x + 256 * k;  // float4 index in the 64-row tile
+    const int c = i / kQ, q = i - (i / kQ) * kQ;
     if (cell0 + c < ncell) {
-      const float* src = tile + c * kRow + 4 * q;
-      out[((size_t)(cell0 + c) * N + hb * 64) / 4 + q] = make_float4(src[0], src[1], src[2], src[3]);
+      const float* src = tile + c * kRow;
+      float4 v;
+      if (natural) {
+        v = make_float4(src[4 * q], src[4 * q + 1], src[4 * q + 2], src[4 * q + 3]);
+      } else {
+        v = make_float4(src[row_sample<N>(4 * q)], src[row_sample<N>(4 * q + 1)], src[row_sample<N>(4 * q + 2)],
+                        src[row_sample<N>(4 * q + 3)]);
+      }
+      out[i] = v;
     }
   }
 }
@@ -331,26 +343,27 @@ void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hi
   if (nb > 0) hipLaunchKernelGGL(k_quadtree, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
-void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, hipStream_t s) {
+void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
+                 hipStream_t s) {
   BlockMap bm;
   int counts[kMaxFrames];
   bool fast = true;
   for (int i = 0; i < n; ++i) {
-    counts[i] = ((hframes[i].Wc * hframes[i].Hc + 63) / 64) * (N / 64);
+    counts[i] = (hframes[i].Wc * hframes[i].Hc + 63) / 64;
     for (int r = 0; r < 4; ++r) fast = fast && cost_fast_taps(hframes[i].T[r]);
   }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
     if (fast)
-      hipLaunchKernelGGL((k_cost<64, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+      hipLaunchKernelGGL((k_cost<64, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
     else
-      hipLaunchKernelGGL((k_cost<64, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+      hipLaunchKernelGGL((k_cost<64, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
   } else {
     if (fast)
-      hipLaunchKernelGGL((k_cost<128, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+      hipLaunchKernelGGL((k_cost<128, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
     else
-      hipLaunchKernelGGL((k_cost<128, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb);
+      hipLaunchKernelGGL((k_cost<128, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
   }
 }
 

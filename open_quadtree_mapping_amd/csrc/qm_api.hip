@@ -248,7 +248,7 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
   { Mark m(t, QM_STAGE_QUADTREE); launch_quadtree(df, hf, n, s); }
-  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, s); }
+  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, false, s); }
   { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
   for (int l = 3; l >= 0; --l) {
     const int iters = p->bp_iters[l];
@@ -382,7 +382,7 @@ int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int 
   if ((rc = sc.get(&dt, sizeof(float) * p->n_hyp))) return rc;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dt, tab.data(), sizeof(float) * p->n_hyp, hipMemcpyHostToDevice, s));
-  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, s);
+  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   return QM_OK;
@@ -399,7 +399,6 @@ int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const 
   FrameDev F;
   init_geometry(F, W, H, W);
   F.levels = const_cast<uint8_t*>(levels);
-  F.cost[0] = const_cast<float*>(cost);
   F.dcoarse = depth_coarse;
   F.label = label;
   Scratch sc;
@@ -410,12 +409,12 @@ int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const 
     void* m;
     if ((rc = sc.get(&m, c * 4 * N * sizeof(float)))) return rc;
     F.msg[l] = (float*)m;
-    if (l > 0) {
-      void* cl;
-      if ((rc = sc.get(&cl, c * N * sizeof(float)))) return rc;
-      F.cost[l] = (float*)cl;
-    }
+    void* cl;
+    if ((rc = sc.get(&cl, c * N * sizeof(float)))) return rc;
+    F.cost[l] = (float*)cl;
   }
+  // the caller's volume is in sample order; the BP kernels read the row layout
+  launch_cost_to_rows(cost, F.cost[0], (size_t)F.lw[0] * F.lh[0], N, s);
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   run_bp((FrameDev*)df, &F, 1, p, s);
   if (messages) launch_copy_messages((FrameDev*)df, &F, 1, N, messages, s);

@@ -155,97 +155,57 @@ __device__ __forceinline__ V3 se3_apply(const float* T, const V3& p) {
   return q;
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float o = __shfl_xor(v, m, kWave);
-    v = o < v ? o : v;
-  }
-  return v;
+// ---- row layout of cost and message rows ----
+// Inside the pipeline a row of N samples (one cell's costs, or one direction of
+// one cell's messages) keeps sample d = j + 16 s at position S*j + s, with
+// S = N / 16, j < 16, s < S. The 16 lanes of a DPP row then hold one row, lane
+// j owning the S samples j, j + 16, ..., j + 16 (S - 1) as S consecutive floats:
+// one row is one 16-lane vector load, the reference's halving-tree stages
+// N/2 .. 16 pair samples inside a lane, and stages 8 .. 1 pair lanes of a row.
+template <int N>
+__host__ __device__ constexpr int row_pos(int d) {
+  return (N / 16) * (d & 15) + (d >> 4);
+}
+template <int N>
+__host__ __device__ constexpr int row_sample(int p) {
+  return (p % (N / 16)) * 16 + p / (N / 16);
 }
 
-// Pairwise tree sum with offsets 32, 16, ..., 1: lane d < m adds lane d + m.
-// The xor butterfly leaves the same bits in every lane (float + is commutative),
-// equal to a shared-memory halving tree.
-__device__ __forceinline__ float wave_tree_sum(float v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, kWave);
-  return v;
+// ---- DPP moves inside rows of 16 lanes (gfx950) ----
+// update_dpp(old, v, ctrl): each lane reads v from its source lane; a lane whose
+// source lies outside its row keeps `old`. row_shl:k = lane j + k, row_shr:k =
+// lane j - k, row_ror:k = lane (j - k) mod 16, row_newbcast:0 = lane 0 of the row.
+constexpr int kRowShl1 = 0x101, kRowShr1 = 0x111, kRowRor1 = 0x121, kRowRor15 = 0x12F, kRowBcast0 = 0x150;
+template <int CTRL>
+__device__ __forceinline__ float dpp_old(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
-
-// ---- lane-0 halving trees on gfx950 cross-lane moves (no LDS traffic) ----
-// Stage m (32, 16, ..., 1): lane d < m combines lane d + m. permlane32_swap
-// hands lanes 0-31 the values of lanes 32-63, permlane16_swap hands rows 0 and
-// 2 the values of rows 1 and 3, DPP row_shl:k hands lane d (row-relative
-// d + k < 16) the value of lane d + k. Lane 0 ends with exactly the
-// shared-memory halving tree of the reference; readfirstlane broadcasts it.
+template <int CTRL>
+__device__ __forceinline__ int dpp_old_i(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xF, 0xF, false);
+}
+// row_shl:K for halving trees: lanes j + K >= 16 read 0 (bound_ctrl), which only
+// reaches lanes whose partial results the tree never uses
 template <int K>
 __device__ __forceinline__ float dpp_row_shl(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, true));
 }
 template <int K>
 __device__ __forceinline__ int dpp_row_shl_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x100 + K, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(0, v, 0x100 + K, 0xF, 0xF, true);
 }
-__device__ __forceinline__ float upper32(float v) {
-  const unsigned u = __float_as_uint(v);
-  return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+// moves whose source lane always exists (rotations, broadcasts): no old value
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
-__device__ __forceinline__ float upper16(float v) {
-  const unsigned u = __float_as_uint(v);
-  return __uint_as_float(__builtin_amdgcn_permlane16_swap(u, u, false, false)[1]);
-}
-__device__ __forceinline__ int upper32_i(int v) {
-  return (int)__builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false)[1];
-}
-__device__ __forceinline__ int upper16_i(int v) {
-  return (int)__builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false)[1];
-}
-__device__ __forceinline__ float bcast_lane0(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
+__device__ __forceinline__ float row_bcast0(float v) { return dpp_old<kRowBcast0>(v, v); }
+__device__ __forceinline__ int row_bcast0_i(int v) { return dpp_old_i<kRowBcast0>(v, v); }
 
-// sum tree: s[d] += s[d + m] for m = 32 .. 1, result of lane 0 in every lane
-__device__ __forceinline__ float tree_sum64(float v) {
-  v = v + upper32(v);
-  v = v + upper16(v);
-  v = v + dpp_row_shl<8>(v);
-  v = v + dpp_row_shl<4>(v);
-  v = v + dpp_row_shl<2>(v);
-  v = v + dpp_row_shl<1>(v);
-  return bcast_lane0(v);
-}
-
-__device__ __forceinline__ float tree_min64(float v) {
-  float o;
-  o = upper32(v); v = o < v ? o : v;
-  o = upper16(v); v = o < v ? o : v;
-  o = dpp_row_shl<8>(v); v = o < v ? o : v;
-  o = dpp_row_shl<4>(v); v = o < v ? o : v;
-  o = dpp_row_shl<2>(v); v = o < v ? o : v;
-  o = dpp_row_shl<1>(v); v = o < v ? o : v;
-  return bcast_lane0(v);
-}
-
-// argmin tree: lane d < m takes lane d + m only if strictly smaller (ties keep
-// the lower index), as src/bpextract.cu:347-355
-__device__ __forceinline__ void tree_argmin64(float& v, int& id) {
-  float ov;
-  int oi;
-  ov = upper32(v); oi = upper32_i(id); if (ov < v) { v = ov; id = oi; }
-  ov = upper16(v); oi = upper16_i(id); if (ov < v) { v = ov; id = oi; }
-  ov = dpp_row_shl<8>(v); oi = dpp_row_shl_i<8>(id); if (ov < v) { v = ov; id = oi; }
-  ov = dpp_row_shl<4>(v); oi = dpp_row_shl_i<4>(id); if (ov < v) { v = ov; id = oi; }
-  ov = dpp_row_shl<2>(v); oi = dpp_row_shl_i<2>(id); if (ov < v) { v = ov; id = oi; }
-  ov = dpp_row_shl<1>(v); oi = dpp_row_shl_i<1>(id); if (ov < v) { v = ov; id = oi; }
-  v = bcast_lane0(v);
-  id = __builtin_amdgcn_readfirstlane(id);
-}
-
-__device__ __forceinline__ int wave_and(int v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v &= __shfl_xor(v, m, kWave);
-  return v;
-}
+// IEEE minimum (v_minimum3_f32, no canonicalising moves): equal to fminf and to
+// the reference's compare-select min for the finite, NaN-free values of a cost
+// volume and its messages. The sign of a zero minimum never reaches an output:
+// minima are only used as min + P1 and min + P2 (P1, P2 > 0) or compared.
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 
 }  // namespace qm

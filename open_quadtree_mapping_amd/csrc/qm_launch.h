@@ -25,7 +25,9 @@ struct BpLaunchParams {
 };
 
 void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
-void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, hipStream_t s);
+// natural = write the volume in sample order (stage API) instead of the row layout
+void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
+                 hipStream_t s);
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s);
 // one half-sweep of level l; first = the level's first sweep (also initialises
 // every cell's messages from the parent level, or zero at the coarsest level);
@@ -35,6 +37,8 @@ void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, in
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
                     hipStream_t s);
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
+// a cost volume in sample order -> the row layout of the pipeline
+void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);
 // upsample: mode 0 = compose the sparse map from BP (+ pre-fill), weights from
 // the keyframe; 1 = explicit sparse map, weights from the keyframe;
 // 2 = generalised (d, conf, w_row, w_col given)
