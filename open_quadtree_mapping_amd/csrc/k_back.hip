@@ -15,21 +15,79 @@ __device__ __forceinline__ void thomas_coeffs(float wl, float wr, double lam, fl
   c = (float)(-lam * (double)wr);
 }
 
-constexpr int kChunk = 16;
+constexpr int kChunk = 32;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Chunked walk of one serial chain [s0, s1) in steps of kChunk, with the
+// inputs of the next chunk in flight while the current one is solved: two
+// register buffers alternate (A, B). Loads of a chunk past the end re-read the
+// last chunk (clamped indices), so every load is issued unconditionally and the
+// chain only waits for the chunk it is about to use.
+template <typename Buf, typename Load, typename Step>
+__device__ __forceinline__ void chain_forward(int s0, int s1, Load load, Step step) {
+  Buf A, B;
+  int c0 = s0;
+  load(A, c0);
+  while (true) {
+    const int c1 = c0 + kChunk;
+    const bool more1 = c1 < s1;
+    load(B, more1 ? c1 : c0);
+    step(A, c0, min(kChunk, s1 - c0));
+    if (!more1) break;
+    const int c2 = c1 + kChunk;
+    const bool more2 = c2 < s1;
+    load(A, more2 ? c2 : c1);
+    step(B, c1, min(kChunk, s1 - c1));
+    if (!more2) break;
+    c0 = c2;
+  }
+}
+template <typename Buf, typename Load, typename Step>
+__device__ __forceinline__ void chain_backward(int s0, int s1, Load load, Step step) {
+  Buf A, B;
+  int c0 = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
+  load(A, c0);
+  while (true) {
+    const int c1 = c0 - kChunk;
+    const bool more1 = c1 >= s0;
+    load(B, more1 ? c1 : c0);
+    step(A, c0, min(kChunk, s1 - c0));
+    if (!more1) break;
+    const int c2 = c1 - kChunk;
+    const bool more2 = c2 >= s0;
+    load(A, more2 ? c2 : c1);
+    step(B, c1, kChunk);
+    if (!more2) break;
+    c0 = c2;
+  }
+}
+
+struct F4Chunk {
+  float4 t[kChunk];
+};
 
 // ---------------------------------------------------------------------------
-// Row pass: one lane per image row, the serial chain walked in chunks of 16
-// unknowns: the chunk's inputs are loaded together and its Thomas coefficients
-// written together, so each wave pays one memory latency per 16 steps, not per
-// step. Scratch (fwd) and the result (rowxn) are column-major, [x][y], so the
-// 64 lanes of a wave (64 consecutive rows) store one contiguous 1 KB / 512 B
-// row segment per step. Mode 0 composes the sparse map on the fly (BP winner
+// Row pass: one lane per image row, the serial Thomas chain walked in chunks.
+// Scratch (fwd) and the result (rowxn) are column-major, [x][y], so the 64
+// lanes of a wave (64 consecutive rows) store one contiguous 1 KB / 512 B row
+// segment per step. Mode 0 composes the sparse map on the fly (BP winner
 // depth at node pixels (4x, 4y), else the semi-dense pre-fill, else 0), so the
 // full-resolution sparse map is never cleared or written unless asked for;
 // mode 1 reads an explicit sparse map; mode 2 is the generalised form with
 // caller-supplied right-hand sides and weights. Same operations, in the same
 // order, as the reference's smooth_row (src/depth_upsample.cu:94-197).
 // ---------------------------------------------------------------------------
+template <int MODE>
+struct RowIn {
+  int px[kChunk + 2];   // image at c0 - 1 + k (modes 0, 1)
+  float sp[kChunk];     // pre-fill (mode 0), sparse map (mode 1), d (mode 2)
+  float cf[kChunk];     // confidence (mode 2)
+  float wv[kChunk + 1];  // row weights (mode 2)
+  int lb[kChunk / 4];   // BP labels of node cells (mode 0)
+  float dv[kChunk / 4];  // BP depths of node cells (mode 0)
+};
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
                                                const float* __restrict__ wlut, double lam) {
@@ -48,77 +106,85 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
   const gptr<float4> fwd = gp(F.fwd);
   const gptr<float2> rowxn = gp(F.rowxn);
   const gptr<const float> prefill = gp(F.prefill);
-  const gptr<const int32_t> label = gp((const int32_t*)F.label);
-  const gptr<const float> dcoarse = gp((const float*)F.dcoarse);
-  const gptr<float> sparse_out = gp(F.sparse_out);
   const gptr<const float> sparse_in = gp(F.sparse_in);
+  const gptr<float> sparse_out = gp(F.sparse_out);
   const gptr<const float> g_d = gp(F.g_d);
   const gptr<const float> g_conf = gp(F.g_conf);
   const gptr<const float> g_wrow = gp(F.g_wrow);
   const gptr<float> row_x_out = gp(F.row_x_out);
   const gptr<float> row_n_out = gp(F.row_n_out);
+  // node cells of this row (mode 0): rows 4*cy, cy < Hc; other rows read a
+  // valid cell row and ignore it
+  const bool node_row = (y & 3) == 0 && (y >> 2) < Hc;
+  const size_t cb = (size_t)min(y >> 2, Hc - 1) * Wc;
+  const gptr<const int32_t> label = gp((const int32_t*)F.label) + cb;
+  const gptr<const float> dcoarse = gp((const float*)F.dcoarse) + cb;
   const float edge = F.edge_r ? 1.0f : 0.0f;
   const int seg = F.row_seg < W ? F.row_seg : W;
-  const int bounds[3] = {0, seg, W};
+  auto load = [&](RowIn<MODE>& b, int c0) {
+    if (MODE == 2) {
 #pragma unroll
+      for (int k = 0; k <= kChunk; ++k) b.wv[k] = g_wrow[(size_t)y * (W + 1) + min(c0 + k, W)];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) b.sp[k] = g_d[row + min(c0 + k, W - 1)];
+      if (g_conf) {
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) b.cf[k] = g_conf[row + min(c0 + k, W - 1)];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kChunk + 2; ++k) b.px[k] = img[clampi(c0 - 1 + k, 0, W - 1)];
+      if (MODE == 1) {
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) b.sp[k] = sparse_in[row + min(c0 + k, W - 1)];
+      } else {
+        if (prefill) {
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) b.sp[k] = prefill[row + min(c0 + k, W - 1)];
+        }
+        // node pixels (4x, 4y): chunk starts are multiples of 4 (0 or 16 * (W / 16), plus 32 k)
+#pragma unroll
+        for (int q = 0; q < kChunk / 4; ++q) {
+          const int cx = min((c0 >> 2) + q, Wc - 1);
+          b.lb[q] = label[cx];
+          b.dv[q] = dcoarse[cx];
+        }
+      }
+    }
+  };
   for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = bounds[sg], s1 = bounds[sg + 1];
+    const int s0 = sg == 0 ? 0 : seg, s1 = sg == 0 ? seg : W;
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    for (int c0 = s0; c0 < s1; c0 += kChunk) {
-      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+    chain_forward<RowIn<MODE>>(s0, s1, load, [&](RowIn<MODE>& b, int c0, int n) {
       float w[kChunk + 1], rx[kChunk], rn[kChunk];
       if (MODE == 2) {
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) w[k] = (k <= n) ? g_wrow[(size_t)y * (W + 1) + c0 + k] : 0.0f;
-      } else {
-        int px[kChunk + 2];  // img[c0 - 1 + k]
-#pragma unroll
-        for (int k = 0; k < kChunk + 2; ++k) {
-          const int x = c0 - 1 + k;
-          px[k] = (x >= 0 && x < W && k <= n + 1) ? (int)img[x] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k <= kChunk; ++k) {
-          const int x = c0 + k;
-          w[k] = (x == 0) ? 0.0f : (x == W ? edge : lut[abs(px[k + 1] - px[k])]);
-        }
-      }
-      // every input of the chunk is loaded before anything is stored, so the
-      // loads of the chunk are in flight together
-      if (MODE == 2) {
+        for (int k = 0; k <= kChunk; ++k) w[k] = b.wv[k];
 #pragma unroll
         for (int k = 0; k < kChunk; ++k) {
-          const float d = (k < n) ? g_d[row + c0 + k] : 0.0f;
           if (g_conf) {
-            const float c = (k < n) ? g_conf[row + c0 + k] : 0.0f;
-            rx[k] = c * d;
-            rn[k] = c;
+            rx[k] = b.cf[k] * b.sp[k];
+            rn[k] = b.cf[k];
           } else {
-            rx[k] = d;
-            rn[k] = d > 0.0f ? 1.0f : 0.0f;
+            rx[k] = b.sp[k];
+            rn[k] = b.sp[k] > 0.0f ? 1.0f : 0.0f;
           }
         }
       } else {
+#pragma unroll
+        for (int k = 0; k <= kChunk; ++k) {
+          const int x = c0 + k;
+          w[k] = (x == 0) ? 0.0f : (x == W ? edge : lut[abs(b.px[k + 1] - b.px[k])]);
+        }
         float sv[kChunk];
         if (MODE == 0) {
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) sv[k] = (prefill && k < n) ? prefill[row + c0 + k] : 0.0f;
-          // node pixels (4x, 4y): chunk starts are multiples of 16, so k = 0, 4, 8, 12
-          if ((y & 3) == 0 && (y >> 2) < Hc) {
-            const size_t cb = (size_t)(y >> 2) * Wc;
-            int lb[kChunk / 4];
-            float dv[kChunk / 4];
-#pragma unroll
-            for (int q = 0; q < kChunk / 4; ++q) {
-              const int cx = (c0 >> 2) + q;
-              const bool ok = (4 * q < n) && cx < Wc;
-              lb[q] = ok ? label[cb + cx] : -1;
-              dv[q] = ok ? dcoarse[cb + cx] : 0.0f;
-            }
+          for (int k = 0; k < kChunk; ++k) sv[k] = prefill ? b.sp[k] : 0.0f;
+          if (node_row) {
 #pragma unroll
             for (int q = 0; q < kChunk / 4; ++q)
-              if (lb[q] >= 0) sv[4 * q] = dv[q];
+              if (4 * q < n && (c0 >> 2) + q < Wc && b.lb[q] >= 0) sv[4 * q] = b.dv[q];
           }
           if (sparse_out) {
 #pragma unroll
@@ -127,7 +193,7 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
           }
         } else {
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) sv[k] = (k < n) ? sparse_in[row + c0 + k] : 0.0f;
+          for (int k = 0; k < kChunk; ++k) sv[k] = b.sp[k];
         }
 #pragma unroll
         for (int k = 0; k < kChunk; ++k) {
@@ -138,9 +204,9 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
         if (k >= n) continue;
-        float a, b, c;
-        thomas_coeffs(w[k], w[k + 1], lam, a, b, c);
-        const float den = b - fc * a;
+        float a, bb, c;
+        thomas_coeffs(w[k], w[k + 1], lam, a, bb, c);
+        const float den = bb - fc * a;
         const float ch = c / den;
         const float dh = (rx[k] - fd * a) / den;
         const float nh = (rn[k] - fn * a) / den;
@@ -149,43 +215,48 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
         fd = dh;
         fn = nh;
       }
-    }
+    });
     fd = 0.0f;
     fn = 0.0f;
-    const int last = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
-    for (int c0 = last; c0 >= s0; c0 -= kChunk) {
-      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
-      float4 t[kChunk];
+    chain_backward<F4Chunk>(
+        s0, s1,
+        [&](F4Chunk& b, int c0) {
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        t[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (k < n) t[k] = fwd[(size_t)(c0 + k) * H + y];
-      }
+          for (int k = 0; k < kChunk; ++k) b.t[k] = fwd[(size_t)min(c0 + k, W - 1) * H + y];
+        },
+        [&](F4Chunk& b, int c0, int n) {
 #pragma unroll
-      for (int k = kChunk - 1; k >= 0; --k) {
-        if (k >= n) continue;
-        const float xv = t[k].y - t[k].x * fd;
-        const float nv = t[k].z - t[k].x * fn;
-        rowxn[(size_t)(c0 + k) * H + y] = make_float2(xv, nv);
-        if (MODE == 2 && row_x_out) {
-          row_x_out[row + c0 + k] = xv;
-          row_n_out[row + c0 + k] = nv;
-        }
-        fd = xv;
-        fn = nv;
-      }
-    }
+          for (int k = kChunk - 1; k >= 0; --k) {
+            if (k >= n) continue;
+            const float xv = b.t[k].y - b.t[k].x * fd;
+            const float nv = b.t[k].z - b.t[k].x * fn;
+            rowxn[(size_t)(c0 + k) * H + y] = make_float2(xv, nv);
+            if (MODE == 2 && row_x_out) {
+              row_x_out[row + c0 + k] = xv;
+              row_n_out[row + c0 + k] = nv;
+            }
+            fd = xv;
+            fn = nv;
+          }
+        });
   }
 }
 
 // ---------------------------------------------------------------------------
-// Column pass: one lane per column, chunks of 16 rows. The row results are
-// read as 16 consecutive float2 of the lane's own column (one 128 B line);
+// Column pass: one lane per column, the same chunked chain down the rows. The
+// row results are read from the lane's own column (column-major rowxn);
 // coefficients, outputs and the image are row-major and coalesced across the
 // wave. Output x / n (NaN where n = 0, as the reference,
 // src/depth_upsample.cu:198-302). Also resets the fusion z-buffer column when
 // asked (the fusion stage runs next).
 // ---------------------------------------------------------------------------
+template <int MODE>
+struct ColIn {
+  int px[kChunk + 2];   // key image at rows c0 - 1 + k (modes 0, 1)
+  float wv[kChunk + 1];  // column weights (mode 2)
+  float2 r[kChunk];     // row pass (x, n)
+};
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
                                                const float* __restrict__ wlut, double lam, int clear_zbuf) {
@@ -199,7 +270,7 @@ __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ fram
   const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H, pitch = F.pitch;
   if (x >= W) return;
-  const gptr<const uint8_t> key = gp(F.key);
+  const gptr<const uint8_t> key = gp(F.key) + x;
   const gptr<float4> fwd = gp(F.fwd);
   const gptr<const float2> colin = gp((const float2*)F.rowxn + (size_t)x * H);
   const gptr<float> dense = gp(F.dense);
@@ -212,77 +283,71 @@ __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ fram
   }
   const float edge = F.edge_c ? 1.0f : 0.0f;
   const int seg = F.col_seg < H ? F.col_seg : H;
-  const int bounds[3] = {0, seg, H};
+  auto load = [&](ColIn<MODE>& b, int c0) {
+    if (MODE == 2) {
 #pragma unroll
+      for (int k = 0; k <= kChunk; ++k) b.wv[k] = g_wcol[(size_t)min(c0 + k, H) * W + x];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kChunk + 2; ++k) b.px[k] = key[(size_t)clampi(c0 - 1 + k, 0, H - 1) * pitch];
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) b.r[k] = colin[min(c0 + k, H - 1)];
+  };
   for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = bounds[sg], s1 = bounds[sg + 1];
+    const int s0 = sg == 0 ? 0 : seg, s1 = sg == 0 ? seg : H;
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    for (int c0 = s0; c0 < s1; c0 += kChunk) {
-      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
+    chain_forward<ColIn<MODE>>(s0, s1, load, [&](ColIn<MODE>& b, int c0, int n) {
       float w[kChunk + 1];
-      float2 r[kChunk];
       if (MODE == 2) {
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) w[k] = (k <= n) ? g_wcol[(size_t)(c0 + k) * W + x] : 0.0f;
+        for (int k = 0; k <= kChunk; ++k) w[k] = b.wv[k];
       } else {
-        int px[kChunk + 2];  // key[(c0 - 1 + k), x]
-#pragma unroll
-        for (int k = 0; k < kChunk + 2; ++k) {
-          const int yy = c0 - 1 + k;
-          px[k] = (yy >= 0 && yy < H && k <= n + 1) ? (int)key[(size_t)yy * pitch + x] : 0;
-        }
 #pragma unroll
         for (int k = 0; k <= kChunk; ++k) {
           const int yy = c0 + k;
-          w[k] = (yy == 0) ? 0.0f : (yy == H ? edge : lut[abs(px[k + 1] - px[k])]);
+          w[k] = (yy == 0) ? 0.0f : (yy == H ? edge : lut[abs(b.px[k + 1] - b.px[k])]);
         }
-      }
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        r[k] = make_float2(0.0f, 0.0f);
-        if (k < n) r[k] = colin[c0 + k];
       }
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
         if (k >= n) continue;
-        float a, b, c;
-        thomas_coeffs(w[k], w[k + 1], lam, a, b, c);
-        const float den = b - fc * a;
+        float a, bb, c;
+        thomas_coeffs(w[k], w[k + 1], lam, a, bb, c);
+        const float den = bb - fc * a;
         const float ch = c / den;
-        const float dh = (r[k].x - fd * a) / den;
-        const float nh = (r[k].y - fn * a) / den;
+        const float dh = (b.r[k].x - fd * a) / den;
+        const float nh = (b.r[k].y - fn * a) / den;
         fwd[(size_t)(c0 + k) * W + x] = make_float4(ch, dh, nh, 0.0f);
         fc = ch;
         fd = dh;
         fn = nh;
       }
-    }
+    });
     fd = 0.0f;
     fn = 0.0f;
-    const int last = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
-    for (int c0 = last; c0 >= s0; c0 -= kChunk) {
-      const int n = (s1 - c0) < kChunk ? (s1 - c0) : kChunk;
-      float4 t[kChunk];
+    chain_backward<F4Chunk>(
+        s0, s1,
+        [&](F4Chunk& b, int c0) {
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        t[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (k < n) t[k] = fwd[(size_t)(c0 + k) * W + x];
-      }
+          for (int k = 0; k < kChunk; ++k) b.t[k] = fwd[(size_t)min(c0 + k, H - 1) * W + x];
+        },
+        [&](F4Chunk& b, int c0, int n) {
 #pragma unroll
-      for (int k = kChunk - 1; k >= 0; --k) {
-        if (k >= n) continue;
-        const float xv = t[k].y - t[k].x * fd;
-        const float nv = t[k].z - t[k].x * fn;
-        const float out = xv / nv;
-        const size_t o = (size_t)(c0 + k) * W + x;
-        if (colxn) colxn[o] = make_float2(xv, nv);
-        dense[o] = out;
-        if (dense_out) dense_out[o] = out;
-        fd = xv;
-        fn = nv;
-      }
-    }
+          for (int k = kChunk - 1; k >= 0; --k) {
+            if (k >= n) continue;
+            const float xv = b.t[k].y - b.t[k].x * fd;
+            const float nv = b.t[k].z - b.t[k].x * fn;
+            const float out = xv / nv;
+            const size_t o = (size_t)(c0 + k) * W + x;
+            if (colxn) colxn[o] = make_float2(xv, nv);
+            dense[o] = out;
+            if (dense_out) dense_out[o] = out;
+            fd = xv;
+            fn = nv;
+          }
+        });
   }
 }
 
