@@ -52,7 +52,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 3
+#define QM_ABI_VERSION 4
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -109,6 +109,32 @@ int qm_upsample_backward(const float* d, const float* conf, const float* w_row, 
                          double lambda, int row_seg, int col_seg, const float* g, float* grad_d, float* grad_conf,
                          float* grad_w_row, float* grad_w_col, void* stream);
 
+/* Many generalised upsample problems (uneven sizes allowed) in one launch per
+ * pass: forward as qm_upsample_general and adjoint as qm_upsample_backward,
+ * per problem. Each problem names its own lambda, chain splits, inputs and
+ * outputs; the backward fields (g, grad_*) are read only by the backward call,
+ * the forward outputs (dense, row_x, row_n) only by the forward call. At most
+ * 128 problems per call. No counterpart in the reference. */
+typedef struct qm_upsample_problem {
+  int W, H;
+  int row_seg, col_seg;        /* chain splits; the reference's are 16*floor(W/16), 8*floor(H/8) */
+  double lambda;
+  const float* d;              /* H x W */
+  const float* conf;           /* H x W or NULL (then rhs = d, indicator = d > 0) */
+  const float* w_row;          /* H x (W+1) */
+  const float* w_col;          /* (H+1) x W */
+  float* dense;                /* forward output, H x W */
+  float* row_x;                /* optional forward output of the row pass, H x W */
+  float* row_n;
+  const float* g;              /* backward input dL/d(dense), H x W */
+  float* grad_d;               /* backward outputs, each may be NULL */
+  float* grad_conf;
+  float* grad_w_row;
+  float* grad_w_col;
+} qm_upsample_problem;
+int qm_upsample_general_batch(const qm_upsample_problem* problems, int n, void* stream);
+int qm_upsample_backward_batch(const qm_upsample_problem* problems, int n, void* stream);
+
 /* Gaussian x Beta fusion of a dense depth map into the seed state. seeds
  * (H x W float4) is warped in place by T_last_to_cur; the update goes to
  * seeds_out; depth_out gets mu' where the prior is confident, else -1.
@@ -141,6 +167,7 @@ typedef struct qm_frame_desc {
   float* depth_out;            /* fused depth (H x W), -1 where not confident */
   float* dense_out;            /* upsampled depth before fusion (H x W) */
   float* sparse_out;           /* sparse map handed to the upsample (H x W) */
+  double lambda;               /* this stream's upsample smoothness weight; <= 0: qm_params.lambda */
 } qm_frame_desc;
 
 typedef struct qm_batch qm_batch;
@@ -168,11 +195,23 @@ int qm_batch_run(qm_batch* b, int mode, void* stream);
  * per stage kind the summed kernel time (ms) and the number of launches, after
  * synchronising the stream. For measurement only. */
 int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
+/* Backward of the last run with respect to each stream's lambda: given
+ * g_depth_out[i] = dL/d(depth_out) of frame i (device pointer, H x W; NULL
+ * array or entry = zero), writes grad_lambda[i] = dL/d(lambda_i) (host array).
+ * Goes through the fusion update adjoint (prior seeds and warp table held
+ * fixed) and the upsample adjoint (its forward recomputed, not stored), one
+ * launch per stage over all frames, then synchronises the stream. Scratch is
+ * allocated on the first call. No counterpart in the reference. */
+int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad_lambda, void* stream);
 /* Device pointer of an internal buffer of frame i, for inspection:
- * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state).
+ * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state),
+ * 6 hole-filled warp table of the last run (int32), 7 warped prior seeds of the last run.
  * The internal cost volume keeps each cell's row of N samples in the row layout:
  * sample d = j + 16 s at position (N/16) * j + s (j < 16). */
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);
+/* Copy that buffer (its full size, which must equal bytes) to dst, host or
+ * device memory, synchronously. */
+int qm_batch_read_buffer(qm_batch* b, int i, int which, void* dst, size_t bytes);
 int qm_batch_reset_seeds(qm_batch* b);
 int qm_batch_destroy(qm_batch* b);
 

@@ -53,10 +53,33 @@ class qm_frame_desc(C.Structure):
         ("depth_out", C.c_void_p),
         ("dense_out", C.c_void_p),
         ("sparse_out", C.c_void_p),
+        ("lambda_", C.c_double),
     ]
 
 
-ABI_VERSION = 3  # QM_ABI_VERSION of include/qm.h this binding is written against
+class qm_upsample_problem(C.Structure):
+    _fields_ = [
+        ("W", C.c_int),
+        ("H", C.c_int),
+        ("row_seg", C.c_int),
+        ("col_seg", C.c_int),
+        ("lambda_", C.c_double),
+        ("d", C.c_void_p),
+        ("conf", C.c_void_p),
+        ("w_row", C.c_void_p),
+        ("w_col", C.c_void_p),
+        ("dense", C.c_void_p),
+        ("row_x", C.c_void_p),
+        ("row_n", C.c_void_p),
+        ("g", C.c_void_p),
+        ("grad_d", C.c_void_p),
+        ("grad_conf", C.c_void_p),
+        ("grad_w_row", C.c_void_p),
+        ("grad_w_col", C.c_void_p),
+    ]
+
+
+ABI_VERSION = 4  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {
@@ -71,6 +94,8 @@ SIGNATURES = {
     "qm_upsample": (I, [P, P, I, I, I, P, P, P]),
     "qm_upsample_general": (I, [P, P, P, P, I, I, D, I, I, P, P, P, P]),
     "qm_upsample_backward": (I, [P, P, P, P, I, I, D, I, I, P, P, P, P, P, P]),
+    "qm_upsample_general_batch": (I, [P, I, P]),
+    "qm_upsample_backward_batch": (I, [P, I, P]),
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
     "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
     "qm_batch_create": (I, [P, I, P, P]),
@@ -78,7 +103,9 @@ SIGNATURES = {
     "qm_batch_run": (I, [P, I, P]),
     "qm_batch_run_timed": (I, [P, P, P, P]),
     "qm_batch_buffer": (I, [P, I, I, P]),
+    "qm_batch_read_buffer": (I, [P, I, I, P, C.c_size_t]),
     "qm_batch_reset_seeds": (I, [P]),
+    "qm_batch_backward": (I, [P, P, P, P]),
     "qm_batch_destroy": (I, [P]),
     "qm_create": (I, [I, I, F, F, F, F, P, P]),
     "qm_destroy": (I, [P]),

@@ -8,6 +8,10 @@ UpsampleFunction: out = x_col / n_col of the generalised upsample
   (dL/dlambda = sum_e w_e dL/dw_e / lambda, since A = I + lambda L(w)).
 FuseFunction: the Gaussian x Beta update of one keyframe (qm_fuse) with
   backward qm_fuse_backward w.r.t. the dense depth input (prior seeds fixed).
+BatchDepthFunction: one batched pipeline run (every stream through every
+  stage, Batch) as a function of the per-stream smoothness weights; backward
+  = qm_batch_backward (fusion adjoint, then the upsample adjoint with its
+  forward recomputed) gives dL/dlambda per stream.
 """
 from __future__ import annotations
 
@@ -16,10 +20,10 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import check, lib
+from ._lib import check, lib, qm_upsample_problem
 from . import stages as S
 
-__all__ = ["UpsampleFunction", "FuseFunction", "upsample", "fuse"]
+__all__ = ["UpsampleFunction", "FuseFunction", "BatchDepthFunction", "upsample", "fuse", "batch_depth"]
 
 
 def _p(t):
@@ -30,42 +34,89 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _problems(d, conf, w_row, w_col, lams, row_seg, col_seg):
+    """qm_upsample_problem array for a (B, H, W) batch (B = 1 for 2-D inputs)."""
+    B, H, W = d.shape
+    probs = (qm_upsample_problem * B)()
+    for b in range(B):
+        q = probs[b]
+        q.W, q.H = W, H
+        q.row_seg = W if row_seg is None else int(row_seg)
+        q.col_seg = H if col_seg is None else int(col_seg)
+        q.lambda_ = float(lams[b])
+        q.d = d[b].data_ptr()
+        q.conf = None if conf is None else conf[b].data_ptr()
+        q.w_row = w_row[b].data_ptr()
+        q.w_col = w_col[b].data_ptr()
+    return probs
+
+
 class UpsampleFunction(torch.autograd.Function):
+    """d (H, W) or (B, H, W); conf like d or None; w_row (.., H, W+1);
+    w_col (.., H+1, W); lam a number, a 0-d tensor or a (B,) tensor (one
+    smoothness weight per problem). A batch runs as one launch per pass."""
+
     @staticmethod
     def forward(ctx, d, conf, w_row, w_col, lam, row_seg, col_seg):
-        lam_f = float(lam.detach().item()) if torch.is_tensor(lam) else float(lam)
-        d_, w_row_, w_col_ = d.detach().contiguous(), w_row.detach().contiguous(), w_col.detach().contiguous()
+        batched = d.dim() == 3
+        d_ = d.detach().contiguous()
+        w_row_, w_col_ = w_row.detach().contiguous(), w_col.detach().contiguous()
         conf_ = None if conf is None else conf.detach().contiguous()
-        out = S.upsample_general(d_, w_row_, w_col_, conf=conf_, lam=lam_f, row_seg=row_seg, col_seg=col_seg)
-        ctx.save_for_backward(d_, w_row_, w_col_, *( [conf_] if conf_ is not None else []))
+        if not batched:
+            d_, w_row_, w_col_ = d_[None], w_row_[None], w_col_[None]
+            conf_ = None if conf_ is None else conf_[None]
+        B = d_.shape[0]
+        if torch.is_tensor(lam):
+            lv = lam.detach().double().cpu().reshape(-1)
+            lams = [float(lv[0])] * B if lv.numel() == 1 else [float(v) for v in lv]
+        else:
+            lams = [float(lam)] * B
+        if len(lams) != B:
+            raise ValueError("lam must be a scalar or hold one value per problem")
+        out = torch.empty_like(d_)
+        probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg)
+        for b in range(B):
+            probs[b].dense = out[b].data_ptr()
+        check(lib().qm_upsample_general_batch(probs, B, _stream()))
+        ctx.save_for_backward(d_, w_row_, w_col_, *([conf_] if conf_ is not None else []))
         ctx.has_conf = conf_ is not None
-        ctx.lam = lam_f
-        ctx.lam_is_tensor = torch.is_tensor(lam)
+        ctx.lams = lams
+        ctx.lam_shape = tuple(lam.shape) if torch.is_tensor(lam) else None
+        ctx.batched = batched
         ctx.segs = (row_seg, col_seg)
-        return out
+        return out if batched else out[0]
 
     @staticmethod
     def backward(ctx, g):
         saved = ctx.saved_tensors
         d, w_row, w_col = saved[:3]
         conf = saved[3] if ctx.has_conf else None
-        H, W = d.shape
-        g = g.contiguous()
+        B = d.shape[0]
+        g = (g if ctx.batched else g[None]).contiguous()
         gd = torch.empty_like(d) if ctx.needs_input_grad[0] else None
         gc = torch.empty_like(d) if (conf is not None and ctx.needs_input_grad[1]) else None
         need_w = ctx.needs_input_grad[2] or ctx.needs_input_grad[3] or ctx.needs_input_grad[4]
         gwr = torch.empty_like(w_row) if need_w else None
         gwc = torch.empty_like(w_col) if need_w else None
-        rs, cs = ctx.segs
-        check(lib().qm_upsample_backward(_p(d), _p(conf), _p(w_row), _p(w_col), W, H, ctx.lam,
-                                         W if rs is None else int(rs), H if cs is None else int(cs), _p(g),
-                                         _p(gd), _p(gc), _p(gwr), _p(gwc), _stream()))
+        probs = _problems(d, conf, w_row, w_col, ctx.lams, *ctx.segs)
+        for b in range(B):
+            q = probs[b]
+            q.g = g[b].data_ptr()
+            q.grad_d = None if gd is None else gd[b].data_ptr()
+            q.grad_conf = None if gc is None else gc[b].data_ptr()
+            q.grad_w_row = None if gwr is None else gwr[b].data_ptr()
+            q.grad_w_col = None if gwc is None else gwc[b].data_ptr()
+        check(lib().qm_upsample_backward_batch(probs, B, _stream()))
         glam = None
-        if ctx.lam_is_tensor and ctx.needs_input_grad[4]:
-            glam = ((w_row.double() * gwr.double()).sum() + (w_col.double() * gwc.double()).sum()) / ctx.lam
-            glam = glam.to(torch.float32)
-        return (gd, gc, gwr if ctx.needs_input_grad[2] else None, gwc if ctx.needs_input_grad[3] else None, glam,
-                None, None)
+        if ctx.lam_shape is not None and ctx.needs_input_grad[4]:
+            lam_t = torch.tensor(ctx.lams, dtype=torch.float64, device=d.device)
+            per = ((w_row.double() * gwr.double()).sum(dim=(1, 2)) +
+                   (w_col.double() * gwc.double()).sum(dim=(1, 2))) / lam_t
+            one_per_problem = len(ctx.lam_shape) == 1 and ctx.lam_shape[0] == B
+            glam = (per if one_per_problem else per.sum()).to(torch.float32).reshape(ctx.lam_shape)
+        unb = (lambda t: t) if ctx.batched else (lambda t: None if t is None else t[0])
+        return (unb(gd), unb(gc), unb(gwr) if ctx.needs_input_grad[2] else None,
+                unb(gwc) if ctx.needs_input_grad[3] else None, glam, None, None)
 
 
 def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None):
@@ -101,3 +152,26 @@ class FuseFunction(torch.autograd.Function):
 def fuse(depth_in, seeds_prev, cam, T_last_to_cur=None):
     T = np.eye(3, 4, dtype=np.float32) if T_last_to_cur is None else T_last_to_cur
     return FuseFunction.apply(depth_in, seeds_prev, cam, T)
+
+
+class BatchDepthFunction(torch.autograd.Function):
+    """lams: (n,) tensor of per-stream smoothness weights; batch: a Batch with
+    the "depth" output. Runs the batch once (graph replay when graph=True)
+    and returns the n fused depth maps (copies)."""
+
+    @staticmethod
+    def forward(ctx, lams, batch, graph=False):
+        batch.set_lambdas([float(v) for v in lams.detach().double().cpu()])
+        batch.run(graph=graph)
+        ctx.batch = batch
+        ctx.dtype, ctx.device = lams.dtype, lams.device
+        return tuple(o["depth"].clone() for o in batch.outputs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        g = ctx.batch.backward(list(grads))
+        return torch.tensor(g, dtype=ctx.dtype, device=ctx.device), None, None
+
+
+def batch_depth(lams, batch, graph=False):
+    return BatchDepthFunction.apply(lams, batch, graph)

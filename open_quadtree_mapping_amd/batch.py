@@ -17,7 +17,7 @@ __all__ = ["Batch", "make_desc"]
 
 
 def make_desc(key, refs, T_ref_key, cam, T_last_to_cur=None, prefill=None, depth_out=None, dense_out=None,
-              sparse_out=None) -> qm_frame_desc:
+              sparse_out=None, lam=None) -> qm_frame_desc:
     H, W = key.shape
     d = qm_frame_desc()
     d.W, d.H, d.pitch = W, H, W
@@ -36,15 +36,17 @@ def make_desc(key, refs, T_ref_key, cam, T_last_to_cur=None, prefill=None, depth
     d.depth_out = None if depth_out is None else depth_out.data_ptr()
     d.dense_out = None if dense_out is None else dense_out.data_ptr()
     d.sparse_out = None if sparse_out is None else sparse_out.data_ptr()
+    d.lambda_ = 0.0 if lam is None else float(lam)
     return d
 
 
 class Batch:
     """frames: list of dicts with key (HxW uint8 device), refs (4 of them),
-    T (4,3,4 float32), cam (fx, fy, cx, cy), optional T_last, prefill.
+    T (4,3,4 float32), cam (fx, fy, cx, cy), optional T_last, prefill and
+    lambda (this stream's smoothness weight; default the params' lambda).
     Output tensors depth / dense / sparse are allocated per stream when asked."""
 
-    BUF_LEVELS, BUF_COST, BUF_DCOARSE, BUF_LABEL, BUF_DENSE, BUF_SEEDS = range(6)
+    BUF_LEVELS, BUF_COST, BUF_DCOARSE, BUF_LABEL, BUF_DENSE, BUF_SEEDS, BUF_TABLE, BUF_PRIOR = range(8)
 
     def __init__(self, frames, params: qm_params | None = None, outputs=("depth",)):
         self.params = params or default_params()
@@ -59,7 +61,7 @@ class Batch:
             outs = {k: torch.empty((H, W), dtype=torch.float32, device=dev) for k in outputs}
             self.outputs.append(outs)
             descs[i] = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
-                                 outs.get("depth"), outs.get("dense"), outs.get("sparse"))
+                                 outs.get("depth"), outs.get("dense"), outs.get("sparse"), fr.get("lambda"))
             self._keep.append(fr)
         self._descs = descs
         self._h = C.c_void_p()
@@ -69,9 +71,26 @@ class Batch:
         self.frames[i] = fr
         o = self.outputs[i]
         d = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
-                      o.get("depth"), o.get("dense"), o.get("sparse"))
+                      o.get("depth"), o.get("dense"), o.get("sparse"), fr.get("lambda"))
         self._descs[i] = d
         check(lib().qm_batch_set_frame(self._h, i, C.byref(d)))
+
+    def set_lambdas(self, lams):
+        """Per-stream smoothness weights for the next run."""
+        for i, v in enumerate(lams):
+            fr = dict(self.frames[i])
+            fr["lambda"] = float(v)
+            self.set_frame(i, fr)
+
+    def backward(self, g_depth_out, stream=None) -> np.ndarray:
+        """dL/dlambda per stream of the last run, given dL/d(depth_out) per
+        stream (float32 device tensors of the stream's size, or None)."""
+        s = stream if stream is not None else torch.cuda.current_stream()
+        gs = [None if g is None else g.contiguous() for g in g_depth_out]
+        arr = (C.c_void_p * self.n)(*[None if g is None else g.data_ptr() for g in gs])
+        out = (C.c_double * self.n)()
+        check(lib().qm_batch_backward(self._h, arr, out, C.c_void_p(s.cuda_stream)))
+        return np.array(out[:], dtype=np.float64)
 
     def run(self, graph: bool = False, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream()
@@ -90,6 +109,19 @@ class Batch:
         p = C.c_void_p()
         check(lib().qm_batch_buffer(self._h, i, which, C.byref(p)))
         return p.value
+
+    def read_buffer(self, i, which) -> np.ndarray:
+        """Host copy of an internal buffer of stream i (BUF_* ids)."""
+        H, W = self.frames[i]["key"].shape
+        N, Hc, Wc = self.params.n_hyp, H // 4, W // 4
+        shapes = {self.BUF_LEVELS: ((H, W), np.uint8), self.BUF_COST: ((Hc, Wc, N), np.float32),
+                  self.BUF_DCOARSE: ((Hc, Wc), np.float32), self.BUF_LABEL: ((Hc, Wc), np.int32),
+                  self.BUF_DENSE: ((H, W), np.float32), self.BUF_SEEDS: ((H, W, 4), np.float32),
+                  self.BUF_TABLE: ((H, W), np.int32), self.BUF_PRIOR: ((H, W, 4), np.float32)}
+        shape, dt = shapes[which]
+        out = np.empty(shape, dt)
+        check(lib().qm_batch_read_buffer(self._h, i, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
 
     def reset_seeds(self):
         check(lib().qm_batch_reset_seeds(self._h))

@@ -23,12 +23,13 @@ __device__ __forceinline__ void adj_coeffs(float wl, float wr, double lam, float
 
 // Column adjoint: lane per column. rhs from g and the recomputed column pass,
 // result (xbar, nbar) into adjxn, dL/dw_col into grad_wcol.
-__global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ frames, BlockMap bm, double lam) {
+__global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ frames, BlockMap bm) {
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H;
   if (x >= W) return;
+  const double lam = F.lam;
   const float flam = (float)lam;
   auto weight = [&](int y) -> float { return F.g_wcol[(size_t)y * W + x]; };
   const int seg = F.col_seg < H ? F.col_seg : H;
@@ -88,12 +89,13 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
 
 // Row adjoint: lane per row. rhs = column-adjoint solution; z = recomputed row
 // pass. Gives dL/dd, dL/dconf and dL/dw_row.
-__global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ frames, BlockMap bm, double lam) {
+__global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ frames, BlockMap bm) {
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H;
   if (y >= H) return;
+  const double lam = F.lam;
   const float flam = (float)lam;
   const size_t row = (size_t)y * W;
   const size_t wrow = (size_t)y * (W + 1);
@@ -247,19 +249,109 @@ __global__ __launch_bounds__(256) void k_fuse_backward(const FrameDev* __restric
   F.grad_depth_in[idx] = gsum;
 }
 
-void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, double lambda,
-                              hipStream_t s) {
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
   // recompute the forward passes (row pass -> rowxn, column pass -> colxn)
-  launch_upsample_row(dframes, hframes, n, 2, nullptr, lambda, s);
-  launch_upsample_col(dframes, hframes, n, 2, nullptr, lambda, false, s);
+  launch_upsample_row(dframes, hframes, n, 2, nullptr, s);
+  launch_upsample_col(dframes, hframes, n, 2, nullptr, false, s);
   BlockMap bm;
   int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;
   int nb = make_block_map(bm, counts, n);
-  if (nb > 0) hipLaunchKernelGGL(k_adj_col, dim3(nb), dim3(64), 0, s, dframes, bm, lambda);
+  if (nb > 0) hipLaunchKernelGGL(k_adj_col, dim3(nb), dim3(64), 0, s, dframes, bm);
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 63) / 64;
   nb = make_block_map(bm, counts, n);
-  if (nb > 0) hipLaunchKernelGGL(k_adj_row, dim3(nb), dim3(64), 0, s, dframes, bm, lambda);
+  if (nb > 0) hipLaunchKernelGGL(k_adj_row, dim3(nb), dim3(64), 0, s, dframes, bm);
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline backward (per-stream lambda). The keyframe upsample (mode 0) is the
+// generalised solve with d = the composed sparse map, no confidence, and the
+// edge weights of the keyframe, so its adjoint reuses the generalised kernels
+// on those three arrays, materialised here with exactly the values the
+// forward pass used: row edge x of row y is 0 at x = 0, edge_r at x = W and
+// lut(|I(x) - I(x-1)|) inside; column edges likewise.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_compose_inputs(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                        const float* __restrict__ wlut) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int i = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+  const int W = F.W, H = F.H;
+  if (i >= (W + 1) * (H + 1)) return;
+  const int y = i / (W + 1), x = i - y * (W + 1);  // x in [0, W], y in [0, H]
+  const uint8_t* key = F.key;
+  // the generalised inputs of this frame's descriptor are the scratch arrays
+  // being filled here
+  float* wrow = const_cast<float*>(F.g_wrow);
+  float* wcol = const_cast<float*>(F.g_wcol);
+  float* sparse = const_cast<float*>(F.g_d);
+  if (y < H) {
+    float w = 0.0f;
+    if (x == W) w = F.edge_r ? 1.0f : 0.0f;
+    else if (x > 0) w = wlut[abs((int)key[(size_t)y * F.pitch + x] - (int)key[(size_t)y * F.pitch + x - 1])];
+    wrow[(size_t)y * (W + 1) + x] = w;
+  }
+  if (x < W) {
+    float w = 0.0f;
+    if (y == H) w = F.edge_c ? 1.0f : 0.0f;
+    else if (y > 0) w = wlut[abs((int)key[(size_t)y * F.pitch + x] - (int)key[(size_t)(y - 1) * F.pitch + x])];
+    wcol[(size_t)y * W + x] = w;
+  }
+  if (x < W && y < H) {
+    float v = F.prefill ? F.prefill[(size_t)y * W + x] : 0.0f;
+    if ((x & 3) == 0 && (y & 3) == 0 && (x >> 2) < F.Wc && (y >> 2) < F.Hc) {
+      const size_t c = (size_t)(y >> 2) * F.Wc + (x >> 2);
+      if (F.label[c] >= 0) v = F.dcoarse[c];
+    }
+    sparse[(size_t)y * W + x] = v;
+  }
+}
+
+// dL/dlambda = sum over edges of w_e * dL/dw_e / lambda (A depends on lambda
+// and w only through lambda * w). Fixed-order block sums in double, one
+// partial per block; the host adds a frame's partials in block order.
+__global__ __launch_bounds__(256) void k_lambda_partials(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                         double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int W = F.W, H = F.H;
+  const size_t nrow = (size_t)H * (W + 1), ncol = (size_t)(H + 1) * W;
+  const size_t i = (size_t)(blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+  double v = 0.0;
+  if (i < nrow) v = (double)F.g_wrow[i] * (double)F.grad_wrow[i];
+  else if (i < nrow + ncol) v = (double)F.g_wcol[i - nrow] * (double)F.grad_wcol[i - nrow];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int m = 128; m >= 1; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] / F.lam;
+}
+
+int lambda_blocks(const FrameDev& F) {
+  return (int)((((size_t)F.H * (F.W + 1) + (size_t)(F.H + 1) * F.W) + 255) / 256);
+}
+
+void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
+                           hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = ((hframes[i].W + 1) * (hframes[i].H + 1) + 255) / 256;
+  const int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_compose_inputs, dim3(nb), dim3(256), 0, s, dframes, bm, wlut);
+}
+
+int launch_lambda_partials(const FrameDev* dframes, const FrameDev* hframes, int n, double* partial,
+                           int* first_block, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) counts[i] = lambda_blocks(hframes[i]);
+  const int nb = make_block_map(bm, counts, n);
+  for (int i = 0; i <= n; ++i) first_block[i] = bm.start[i];
+  if (nb > 0) hipLaunchKernelGGL(k_lambda_partials, dim3(nb), dim3(256), 0, s, dframes, bm, partial);
+  return nb;
 }
 
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {

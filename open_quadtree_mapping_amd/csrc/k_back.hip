@@ -90,7 +90,7 @@ struct RowIn {
 
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut, double lam) {
+                                               const float* __restrict__ wlut) {
   __shared__ float lut[256];
   if (MODE != 2) {
     for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
@@ -100,6 +100,7 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
   const FrameDev& F = frames[f];
   const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
+  const double lam = F.lam;
   if (y >= H) return;
   const size_t row = (size_t)y * W;
   const gptr<const uint8_t> img = gp(F.key + (size_t)y * F.pitch);
@@ -259,7 +260,7 @@ struct ColIn {
 
 template <int MODE>
 __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut, double lam, int clear_zbuf) {
+                                               const float* __restrict__ wlut, int clear_zbuf) {
   __shared__ float lut[256];
   if (MODE != 2) {
     for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
@@ -269,6 +270,7 @@ __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ fram
   const FrameDev& F = frames[f];
   const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H, pitch = F.pitch;
+  const double lam = F.lam;
   if (x >= W) return;
   const gptr<const uint8_t> key = gp(F.key) + x;
   const gptr<float4> fwd = gp(F.fwd);
@@ -491,27 +493,27 @@ static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
 }
 
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, hipStream_t s) {
+                         hipStream_t s) {
   BlockMap bm;
   const int nb = rows_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
   else
-    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda);
+    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, bool clear_zbuf, hipStream_t s) {
+                         bool clear_zbuf, hipStream_t s) {
   BlockMap bm;
   const int nb = cols_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 2)
-    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, (int)clear_zbuf);
   else
-    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, lambda, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, (int)clear_zbuf);
 }
 
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
@@ -533,9 +535,9 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
 }
 
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                     double lambda, bool clear_zbuf, hipStream_t s) {
-  launch_upsample_row(dframes, hframes, n, mode, wlut, lambda, s);
-  launch_upsample_col(dframes, hframes, n, mode, wlut, lambda, clear_zbuf, s);
+                     bool clear_zbuf, hipStream_t s) {
+  launch_upsample_row(dframes, hframes, n, mode, wlut, s);
+  launch_upsample_col(dframes, hframes, n, mode, wlut, clear_zbuf, s);
 }
 
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s) {

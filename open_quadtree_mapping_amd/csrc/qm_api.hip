@@ -133,6 +133,7 @@ void init_geometry(FrameDev& F, int W, int H, int pitch) {
   F.col_seg = 8 * (H / 8);
   F.edge_r = (W % 16) ? 1 : 0;
   F.edge_c = (H % 16) ? 1 : 0;
+  F.lam = 10.0;
 }
 
 BpLaunchParams bp_params(const qm_params* p) {
@@ -177,6 +178,7 @@ size_t frame_bytes(const FrameDev& F, int N) {
   b += align256(px * 4);                   // dense
   b += 2 * align256(px * 16);              // seeds x2
   b += align256(px * 8);                   // zbuf
+  b += align256(px * 4);                   // hole-filled warp table
   return b;
 }
 
@@ -204,7 +206,18 @@ struct qm_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int parity = 0;
+  int runs = 0;
   std::vector<hipEvent_t> events;
+  // backward (per-stream lambda), allocated on the first qm_batch_backward
+  void* bwd_arena = nullptr;
+  FrameDev* dfb = nullptr;
+  double* partial = nullptr;
+  int partial_n = 0;
+  struct Bwd {
+    float *sparse, *wrow, *wcol, *gdense, *gwrow, *gwcol;
+    float2 *colxn, *adjxn;
+  };
+  std::vector<Bwd> bwd;
 };
 
 namespace {
@@ -263,13 +276,14 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
     }
   }
   { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, p->lambda, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, p->lambda, true, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, s); }
+  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, true, s); }
   { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
   { Mark m(t, QM_STAGE_FUSE_UPDATE); launch_fuse_update(df, hf, n, s); }
 }
 
-void apply_desc(FrameDev& F, const qm_frame_desc& d) {
+void apply_desc(FrameDev& F, const qm_frame_desc& d, const qm_params& p) {
+  F.lam = d.lambda > 0.0 ? d.lambda : p.lambda;
   F.fx = d.cam[0];
   F.fy = d.cam[1];
   F.cx = d.cam[2];
@@ -434,6 +448,7 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
   init_geometry(F, W, H, pitch);
   F.key = img;
   F.sparse_in = sparse;
+  F.lam = p->lambda;
   F.dense = dense;
   float lut[256];
   host_weight_lut(lut);
@@ -447,7 +462,108 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
   F.rowxn = (float2*)rxn;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dl, lut, sizeof(lut), hipMemcpyHostToDevice, s));
-  launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, p->lambda, false, s);
+  launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, false, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_upsample_general_batch(const qm_upsample_problem* probs, int n, void* stream) {
+  if (!probs || n <= 0 || n > kMaxFrames) return fail(QM_ERR_INVALID, "need 1..128 problems");
+  int rc;
+  for (int i = 0; i < n; ++i) {
+    const qm_upsample_problem& q = probs[i];
+    if ((rc = check_size(q.W, q.H, q.W))) return rc;
+    if (!q.d || !q.w_row || !q.w_col || !q.dense) return fail(QM_ERR_INVALID, "NULL pointer");
+    if ((q.row_x == nullptr) != (q.row_n == nullptr)) return fail(QM_ERR_INVALID, "row_x and row_n go together");
+    if (q.row_seg < 0 || q.col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
+    if (!(q.lambda >= 0.0)) return fail(QM_ERR_INVALID, "lambda must be >= 0");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<FrameDev> hf(n);
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev) * n))) return rc;
+  for (int i = 0; i < n; ++i) {
+    const qm_upsample_problem& q = probs[i];
+    FrameDev& F = hf[i];
+    init_geometry(F, q.W, q.H, q.W);
+    F.row_seg = q.row_seg;
+    F.col_seg = q.col_seg;
+    F.lam = q.lambda;
+    F.g_d = q.d;
+    F.g_conf = q.conf;
+    F.g_wrow = q.w_row;
+    F.g_wcol = q.w_col;
+    F.row_x_out = q.row_x;
+    F.row_n_out = q.row_n;
+    F.dense = q.dense;
+    const size_t px = (size_t)q.W * q.H;
+    void *fwd, *rxn;
+    if ((rc = sc.get(&fwd, px * 16))) return rc;
+    if ((rc = sc.get(&rxn, px * 8))) return rc;
+    F.fwd = (float4*)fwd;
+    F.rowxn = (float2*)rxn;
+  }
+  QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+  launch_upsample((FrameDev*)df, hf.data(), n, 2, nullptr, false, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* stream) {
+  if (!probs || n <= 0 || n > kMaxFrames) return fail(QM_ERR_INVALID, "need 1..128 problems");
+  int rc;
+  for (int i = 0; i < n; ++i) {
+    const qm_upsample_problem& q = probs[i];
+    if ((rc = check_size(q.W, q.H, q.W))) return rc;
+    if (!q.d || !q.w_row || !q.w_col || !q.g) return fail(QM_ERR_INVALID, "NULL pointer");
+    if (q.row_seg < 0 || q.col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
+    if (!(q.lambda >= 0.0)) return fail(QM_ERR_INVALID, "lambda must be >= 0");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<FrameDev> hf(n);
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev) * n))) return rc;
+  for (int i = 0; i < n; ++i) {
+    const qm_upsample_problem& q = probs[i];
+    FrameDev& F = hf[i];
+    init_geometry(F, q.W, q.H, q.W);
+    F.row_seg = q.row_seg;
+    F.col_seg = q.col_seg;
+    F.lam = q.lambda;
+    F.g_d = q.d;
+    F.g_conf = q.conf;
+    F.g_wrow = q.w_row;
+    F.g_wcol = q.w_col;
+    F.g_out = q.g;
+    F.grad_d = q.grad_d;
+    F.grad_conf = q.grad_conf;
+    F.grad_wrow = q.grad_w_row;
+    const size_t px = (size_t)q.W * q.H;
+    void *fwd, *rxn, *cxn, *axn, *dense;
+    if ((rc = sc.get(&fwd, px * 16))) return rc;
+    if ((rc = sc.get(&rxn, px * 8))) return rc;
+    if ((rc = sc.get(&cxn, px * 8))) return rc;
+    if ((rc = sc.get(&axn, px * 8))) return rc;
+    if ((rc = sc.get(&dense, px * 4))) return rc;
+    F.fwd = (float4*)fwd;
+    F.rowxn = (float2*)rxn;
+    F.colxn = (float2*)cxn;
+    F.adjxn = (float2*)axn;
+    F.dense = (float*)dense;
+    if (q.grad_w_col) {
+      F.grad_wcol = q.grad_w_col;
+    } else {
+      void* gwc;
+      if ((rc = sc.get(&gwc, (size_t)(q.H + 1) * q.W * 4))) return rc;
+      F.grad_wcol = (float*)gwc;
+    }
+  }
+  QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+  launch_upsample_backward((FrameDev*)df, hf.data(), n, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   return QM_OK;
@@ -456,78 +572,43 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
 int qm_upsample_general(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
                         double lambda, int row_seg, int col_seg, float* dense, float* row_x, float* row_n,
                         void* stream) {
-  int rc = check_size(W, H, W);
-  if (rc) return rc;
-  if (!d || !w_row || !w_col || !dense) return fail(QM_ERR_INVALID, "NULL pointer");
-  if ((row_x == nullptr) != (row_n == nullptr)) return fail(QM_ERR_INVALID, "row_x and row_n go together");
-  if (row_seg < 0 || col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
-  hipStream_t s = (hipStream_t)stream;
-  FrameDev F;
-  init_geometry(F, W, H, W);
-  F.row_seg = row_seg;
-  F.col_seg = col_seg;
-  F.g_d = d;
-  F.g_conf = conf;
-  F.g_wrow = w_row;
-  F.g_wcol = w_col;
-  F.row_x_out = row_x;
-  F.row_n_out = row_n;
-  F.dense = dense;
-  Scratch sc;
-  void *df, *fwd, *rxn;
-  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
-  if ((rc = sc.get(&fwd, (size_t)W * H * 16))) return rc;
-  if ((rc = sc.get(&rxn, (size_t)W * H * 8))) return rc;
-  F.fwd = (float4*)fwd;
-  F.rowxn = (float2*)rxn;
-  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
-  launch_upsample((FrameDev*)df, &F, 1, 2, nullptr, lambda, false, s);
-  QM_HIP(hipGetLastError());
-  QM_HIP(hipStreamSynchronize(s));
-  return QM_OK;
+  qm_upsample_problem q;
+  std::memset(&q, 0, sizeof(q));
+  q.W = W;
+  q.H = H;
+  q.row_seg = row_seg;
+  q.col_seg = col_seg;
+  q.lambda = lambda;
+  q.d = d;
+  q.conf = conf;
+  q.w_row = w_row;
+  q.w_col = w_col;
+  q.dense = dense;
+  q.row_x = row_x;
+  q.row_n = row_n;
+  return qm_upsample_general_batch(&q, 1, stream);
 }
 
 int qm_upsample_backward(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
                          double lambda, int row_seg, int col_seg, const float* g, float* grad_d, float* grad_conf,
                          float* grad_w_row, float* grad_w_col, void* stream) {
-  int rc = check_size(W, H, W);
-  if (rc) return rc;
-  if (!d || !w_row || !w_col || !g) return fail(QM_ERR_INVALID, "NULL pointer");
-  if (row_seg < 0 || col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
-  hipStream_t s = (hipStream_t)stream;
-  const size_t px = (size_t)W * H;
-  FrameDev F;
-  init_geometry(F, W, H, W);
-  F.row_seg = row_seg;
-  F.col_seg = col_seg;
-  F.g_d = d;
-  F.g_conf = conf;
-  F.g_wrow = w_row;
-  F.g_wcol = w_col;
-  F.g_out = g;
-  F.grad_d = grad_d;
-  F.grad_conf = grad_conf;
-  F.grad_wrow = grad_w_row;
-  Scratch sc;
-  void *df, *fwd, *rxn, *cxn, *axn, *dense, *gwc = nullptr;
-  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
-  if ((rc = sc.get(&fwd, px * 16))) return rc;
-  if ((rc = sc.get(&rxn, px * 8))) return rc;
-  if ((rc = sc.get(&cxn, px * 8))) return rc;
-  if ((rc = sc.get(&axn, px * 8))) return rc;
-  if ((rc = sc.get(&dense, px * 4))) return rc;
-  if (!grad_w_col && (rc = sc.get(&gwc, (size_t)(H + 1) * W * 4))) return rc;
-  F.fwd = (float4*)fwd;
-  F.rowxn = (float2*)rxn;
-  F.colxn = (float2*)cxn;
-  F.adjxn = (float2*)axn;
-  F.dense = (float*)dense;
-  F.grad_wcol = grad_w_col ? grad_w_col : (float*)gwc;
-  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
-  launch_upsample_backward((FrameDev*)df, &F, 1, lambda, s);
-  QM_HIP(hipGetLastError());
-  QM_HIP(hipStreamSynchronize(s));
-  return QM_OK;
+  qm_upsample_problem q;
+  std::memset(&q, 0, sizeof(q));
+  q.W = W;
+  q.H = H;
+  q.row_seg = row_seg;
+  q.col_seg = col_seg;
+  q.lambda = lambda;
+  q.d = d;
+  q.conf = conf;
+  q.w_row = w_row;
+  q.w_col = w_col;
+  q.g = g;
+  q.grad_d = grad_d;
+  q.grad_conf = grad_conf;
+  q.grad_w_row = grad_w_row;
+  q.grad_w_col = grad_w_col;
+  return qm_upsample_backward_batch(&q, 1, stream);
 }
 
 int qm_fuse_backward(const float* seeds_warped, const int32_t* table, const float* depth_in, int W, int H,
@@ -639,7 +720,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
     b->seeds[0][i] = cv.take<float4>(px * 16);
     b->seeds[1][i] = cv.take<float4>(px * 16);
     F.zbuf = cv.take<unsigned long long>(px * 8);
-    apply_desc(F, frames[i]);
+    F.table_out = cv.take<int32_t>(px * 4);
+    apply_desc(F, frames[i], *p);
   }
   for (int par = 0; par < 2; ++par) {
     b->hf[par] = base;
@@ -676,7 +758,7 @@ int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
   const FrameDev& F0 = b->hf[0][i];
   if (d->W != F0.W || d->H != F0.H || d->pitch != F0.pitch)
     return fail(QM_ERR_INVALID, "frame size differs from the planned one");
-  for (int par = 0; par < 2; ++par) apply_desc(b->hf[par][i], *d);
+  for (int par = 0; par < 2; ++par) apply_desc(b->hf[par][i], *d, b->p);
   return upload_frames(b);
 }
 
@@ -705,6 +787,7 @@ int qm_batch_run(qm_batch* b, int mode, void* stream) {
     return fail(QM_ERR_INVALID, "mode must be 0 or 1");
   }
   b->parity ^= 1;
+  b->runs += 1;
   return QM_OK;
 }
 
@@ -728,6 +811,7 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double* stage_ms, int* stage_l
     stage_launches[t.stage[k]] += 1;
   }
   b->parity ^= 1;
+  b->runs += 1;
   return QM_OK;
 }
 
@@ -742,8 +826,23 @@ int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
     case 3: *ptr = F.label; break;
     case 4: *ptr = F.dense; break;
     case 5: *ptr = F.seeds; break;
+    case 6: *ptr = F.table_out; break;
+    case 7: *ptr = b->hf[b->parity ^ 1][i].seeds; break;
     default: return fail(QM_ERR_INVALID, "unknown buffer id");
   }
+  return QM_OK;
+}
+
+int qm_batch_read_buffer(qm_batch* b, int i, int which, void* dst, size_t bytes) {
+  void* src = nullptr;
+  int rc = qm_batch_buffer(b, i, which, &src);
+  if (rc) return rc;
+  if (!dst) return fail(QM_ERR_INVALID, "dst is NULL");
+  const FrameDev& F = b->hf[0][i];
+  const size_t px = (size_t)F.W * F.H, cc = (size_t)F.Wc * F.Hc;
+  const size_t sizes[8] = {px, cc * b->p.n_hyp * 4, cc * 4, cc * 4, px * 4, px * 16, px * 4, px * 16};
+  if (bytes != sizes[which]) return fail(QM_ERR_INVALID, "byte count differs from the buffer size");
+  QM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
   return QM_OK;
 }
 
@@ -766,7 +865,99 @@ int qm_batch_destroy(qm_batch* b) {
     dev_free(b->df[par]);
   }
   dev_free(b->arena);
+  dev_free(b->bwd_arena);
+  dev_free(b->dfb);
+  dev_free(b->partial);
   delete b;
+  return QM_OK;
+}
+
+int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad_lambda, void* stream) {
+  if (!b || !grad_lambda) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (b->runs == 0) return fail(QM_ERR_INVALID, "no forward run to differentiate");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = b->n, last = b->parity ^ 1;
+  int rc;
+  if (!b->bwd_arena) {
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+      const FrameDev& F = b->hf[0][i];
+      const size_t px = (size_t)F.W * F.H, er = (size_t)F.H * (F.W + 1), ec = (size_t)(F.H + 1) * F.W;
+      total += 2 * align256(px * 4) + 2 * align256(er * 4) + 2 * align256(ec * 4) + 2 * align256(px * 8);
+    }
+    int nblocks = 0;
+    for (int i = 0; i < n; ++i) nblocks += lambda_blocks(b->hf[0][i]);
+    void *arena = nullptr, *dfb = nullptr, *partial = nullptr;
+    if ((rc = dev_alloc(&arena, total)) || (rc = dev_alloc(&dfb, sizeof(FrameDev) * n)) ||
+        (rc = dev_alloc(&partial, sizeof(double) * nblocks))) {
+      dev_free(arena);
+      dev_free(dfb);
+      dev_free(partial);
+      return rc;
+    }
+    b->bwd_arena = arena;
+    b->dfb = (FrameDev*)dfb;
+    b->partial = (double*)partial;
+    b->partial_n = nblocks;
+    Carver cv{(char*)arena};
+    b->bwd.resize(n);
+    for (int i = 0; i < n; ++i) {
+      const FrameDev& F = b->hf[0][i];
+      const size_t px = (size_t)F.W * F.H, er = (size_t)F.H * (F.W + 1), ec = (size_t)(F.H + 1) * F.W;
+      qm_batch::Bwd& w = b->bwd[i];
+      w.sparse = cv.take<float>(px * 4);
+      w.gdense = cv.take<float>(px * 4);
+      w.wrow = cv.take<float>(er * 4);
+      w.gwrow = cv.take<float>(er * 4);
+      w.wcol = cv.take<float>(ec * 4);
+      w.gwcol = cv.take<float>(ec * 4);
+      w.colxn = cv.take<float2>(px * 8);
+      w.adjxn = cv.take<float2>(px * 8);
+    }
+  }
+  std::vector<FrameDev> hb = b->hf[last];
+  for (int i = 0; i < n; ++i) {
+    FrameDev& F = hb[i];
+    const qm_batch::Bwd& w = b->bwd[i];
+    F.table_in = F.table_out;
+    F.table_out = nullptr;
+    F.depth_out = nullptr;
+    F.dense_out = nullptr;
+    F.sparse_out = nullptr;
+    F.g_depth_out = g_depth_out ? g_depth_out[i] : nullptr;
+    F.g_seeds_out = nullptr;
+    F.grad_depth_in = w.gdense;
+    F.g_d = w.sparse;
+    F.g_conf = nullptr;
+    F.g_wrow = w.wrow;
+    F.g_wcol = w.wcol;
+    F.g_out = w.gdense;
+    F.grad_d = nullptr;
+    F.grad_conf = nullptr;
+    F.grad_wrow = w.gwrow;
+    F.grad_wcol = w.gwcol;
+    F.colxn = w.colxn;
+    F.adjxn = w.adjxn;
+    F.row_x_out = nullptr;
+    F.row_n_out = nullptr;
+  }
+  QM_HIP(hipMemcpyAsync(b->dfb, hb.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+  // fusion adjoint first: it reads the forward's dense depth, which the
+  // upsample recomputation then rewrites with the same values
+  launch_fuse_backward(b->dfb, hb.data(), n, s);
+  launch_compose_inputs(b->dfb, hb.data(), n, b->wlut, s);
+  launch_upsample_backward(b->dfb, hb.data(), n, s);
+  int first[kMaxFrames + 1];
+  const int nb = launch_lambda_partials(b->dfb, hb.data(), n, b->partial, first, s);
+  QM_HIP(hipGetLastError());
+  std::vector<double> part(nb);
+  QM_HIP(hipMemcpyAsync(part.data(), b->partial, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  QM_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    double acc = 0.0;
+    for (int k = first[i]; k < first[i + 1]; ++k) acc += part[k];
+    grad_lambda[i] = acc;
+  }
   return QM_OK;
 }
 

@@ -19,6 +19,7 @@ struct FrameDev {
   float fx, fy, cx, cy;
   int row_seg, col_seg;
   int edge_r, edge_c;  // weight of the x=W / y=H edge (1 when W / H is not a multiple of 16)
+  double lam;          // smoothness weight lambda of the upsample (per frame)
   const uint8_t* key;
   const uint8_t* ref[4];
   float T[4][12];       // key -> reference r

@@ -41,21 +41,28 @@ void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int 
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);
 // upsample: mode 0 = compose the sparse map from BP (+ pre-fill), weights from
 // the keyframe; 1 = explicit sparse map, weights from the keyframe;
-// 2 = generalised (d, conf, w_row, w_col given)
+// 2 = generalised (d, conf, w_row, w_col given); lambda is FrameDev::lam
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                     double lambda, bool clear_zbuf, hipStream_t s);
+                     bool clear_zbuf, hipStream_t s);
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s);
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, hipStream_t s);
+                         hipStream_t s);
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         double lambda, bool clear_zbuf, hipStream_t s);
+                         bool clear_zbuf, hipStream_t s);
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // adjoints: recompute the generalised upsample forward, then the column and row
 // adjoint solves; fusion update adjoint w.r.t. its dense depth input
-void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, double lambda,
-                              hipStream_t s);
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+// pipeline backward: the keyframe upsample's inputs written as explicit arrays
+// into each descriptor's g_d / g_wrow / g_wcol, and the per-block partial sums
+// of dL/dlambda
+void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
+                           hipStream_t s);
+int lambda_blocks(const FrameDev& F);
+int launch_lambda_partials(const FrameDev* dframes, const FrameDev* hframes, int n, double* partial,
+                           int* first_block, hipStream_t s);
 
 }  // namespace qm

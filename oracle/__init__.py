@@ -57,6 +57,8 @@ def lib():
         L.qmo_upsample_f64.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P]
         L.qmo_fuse_f32.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
         L.qmo_fuse_f64.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
+        L.qmo_fuse_update_f32.argtypes = [I, I, P, P, P, P, P]
+        L.qmo_fuse_update_f64.argtypes = [I, I, P, P, P, P, P]
         L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
         L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
         L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P]
@@ -184,6 +186,21 @@ def fuse(seeds, depth_in, T_last_to_cur, cam, fp64=False):
     fn = lib().qmo_fuse_f64 if fp64 else lib().qmo_fuse_f32
     fn(W, H, cam.fx, cam.fy, cam.cx, cam.cy, _p(T), _p(s), _p(din), _p(dout), _p(sout), _p(table))
     return s, dout, sout, table
+
+
+def fuse_update(table, seeds_warped, depth_in, fp64=False):
+    """The per-pixel update alone, given the hole-filled table and the warped
+    seeds. Returns (depth_out, seeds_out)."""
+    dt = _dt(fp64)
+    H, W = depth_in.shape
+    tb = np.ascontiguousarray(table, dtype=np.int32)
+    s = np.ascontiguousarray(seeds_warped, dtype=dt).reshape(H, W, 4)
+    din = np.ascontiguousarray(depth_in, dtype=dt)
+    dout = np.zeros((H, W), dt)
+    sout = np.zeros((H, W, 4), dt)
+    fn = lib().qmo_fuse_update_f64 if fp64 else lib().qmo_fuse_update_f32
+    fn(W, H, _p(tb), _p(s), _p(din), _p(dout), _p(sout))
+    return dout, sout
 
 
 def compose_sparse(W, H, depth_coarse, label, prefill=None, fp64=False):

@@ -545,6 +545,10 @@ template <typename Real> inline Real normpdf(Real x, Real mu, Real s2) {
 }
 
 template <typename Real>
+void fuse_update_impl(int W, int H, const int32_t* filled, const Real* seeds, const Real* depth_in, Real* depth_out,
+                      Real* seeds_out);
+
+template <typename Real>
 void fuse_impl(int W, int H, const Cam& cam, const float* T /*12*/, Real* seeds, const Real* depth_in, Real* depth_out,
                Real* seeds_out, int32_t* table_out) {
   const size_t n = (size_t)W * H;
@@ -609,6 +613,15 @@ void fuse_impl(int W, int H, const Cam& cam, const float* T /*12*/, Real* seeds,
         }
     }
   // 3) per-pixel Gaussian x Beta update
+  fuse_update_impl<Real>(W, H, filled.data(), seeds, depth_in, depth_out, seeds_out);
+  if (table_out) std::memcpy(table_out, filled.data(), sizeof(int32_t) * n);
+}
+
+// The per-pixel update of src/depth_fusion.cu:146-213 given the hole-filled
+// table and the warped seeds (the half the differentiable path varies).
+template <typename Real>
+void fuse_update_impl(int W, int H, const int32_t* filled, const Real* seeds, const Real* depth_in, Real* depth_out,
+                      Real* seeds_out) {
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < H; ++y) {
     for (int x = 0; x < W; ++x) {
@@ -659,7 +672,6 @@ void fuse_impl(int W, int H, const Cam& cam, const float* T /*12*/, Real* seeds,
       so[3] = s_p;
     }
   }
-  if (table_out) std::memcpy(table_out, filled.data(), sizeof(int32_t) * n);
 }
 
 }  // namespace
@@ -764,6 +776,15 @@ void qmo_fuse_f64(int W, int H, float fx, float fy, float cx, float cy, const fl
                   const double* depth_in, double* depth_out, double* seeds_out, int32_t* table_out) {
   Cam c{fx, fy, cx, cy};
   fuse_impl<double>(W, H, c, T, seeds, depth_in, depth_out, seeds_out, table_out);
+}
+
+void qmo_fuse_update_f32(int W, int H, const int32_t* table, const float* seeds_warped, const float* depth_in,
+                         float* depth_out, float* seeds_out) {
+  fuse_update_impl<float>(W, H, table, seeds_warped, depth_in, depth_out, seeds_out);
+}
+void qmo_fuse_update_f64(int W, int H, const int32_t* table, const double* seeds_warped, const double* depth_in,
+                         double* depth_out, double* seeds_out) {
+  fuse_update_impl<double>(W, H, table, seeds_warped, depth_in, depth_out, seeds_out);
 }
 
 // Sparse map handed to the upsample (src/bpextract.cu:370 writes the node

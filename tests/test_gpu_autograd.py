@@ -120,3 +120,45 @@ def test_fuse_adjoint_vs_fd():
         errs.append(abs(g[y, x] - fd) / scale)
     errs = np.array(errs)
     assert np.quantile(errs, 0.98) < 2e-3, f"p98 {np.quantile(errs, 0.98)} max {errs.max()}"
+
+
+def test_upsample_batch16_matches_oracle_and_fd():
+    """Config 3: a batch of 16 upsampling problems at 752x480 through one
+    launch per pass, with one smoothness weight per problem; forward against
+    the FP32 oracle, adjoint against FP64 central differences on 256 entries."""
+    B, H, W = 16, 480, 752
+    probs = [problem(H, W, 100 + b) for b in range(B)]
+    lams = np.linspace(5.0, 20.0, B)
+    print("upsample batch of 16 at 752x480", flush=True)
+    t = lambda k: torch.tensor(np.stack([p[k] for p in probs]), dtype=torch.float32, device=DEV,  # noqa: E731
+                               requires_grad=True)
+    td, tc, twr, twc = t(0), t(1), t(2), t(3)
+    tl = torch.tensor(lams, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = QA.upsample(td, twr, twc, conf=tc, lam=tl)
+    r = np.stack([p[4] for p in probs])
+    (out * torch.tensor(r, dtype=torch.float32, device=DEV)).sum().backward()
+    got = out.detach().cpu().numpy()
+    for b in range(B):
+        d, conf, wr, wc, _ = probs[b]
+        want = O.upsample(d.astype(np.float32), wr.astype(np.float32), wc.astype(np.float32),
+                          conf=conf.astype(np.float32), lam=float(lams[b]))
+        assert np.array_equal(got[b], want, equal_nan=True), f"problem {b}"
+    grads = [td.grad.cpu().numpy(), tc.grad.cpu().numpy(), twr.grad.cpu().numpy(), twc.grad.cpu().numpy()]
+    rng = np.random.default_rng(2)
+    errs = []
+    for k in range(256):
+        b, which = int(rng.integers(0, B)), k % 4
+        arrays = list(probs[b][:4])
+        idx = tuple(int(rng.integers(0, s)) for s in arrays[which].shape)
+        fd = fd_upsample(arrays, which, idx, probs[b][4], float(lams[b]))
+        g = float(grads[which][b][idx])
+        scale = max(abs(fd), 1e-3 * np.abs(grads[which][b]).max(), 1e-9)
+        errs.append(abs(g - fd) / scale)
+    errs = np.array(errs)
+    assert np.quantile(errs, 0.99) < 2e-3, f"max {errs.max()} p99 {np.quantile(errs, 0.99)}"
+    gl = tl.grad.cpu().numpy()
+    for b in (0, 7, 15):
+        d, conf, wr, wc, rr = probs[b]
+        f = lambda l: float(np.sum(rr * O.upsample(d, wr, wc, conf=conf, lam=l, fp64=True)))  # noqa: E731
+        fd = (f(lams[b] + 1e-6) - f(lams[b] - 1e-6)) / 2e-6
+        assert abs(gl[b] - fd) <= 2e-3 * abs(fd), f"problem {b}: {gl[b]} vs {fd}"
