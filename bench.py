@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tune-reps", type=int, default=5)
     ap.add_argument("--no-graph", action="store_true",
                     help="time the batched launches without hipGraph replay (for counter collection)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -167,6 +169,10 @@ def main():
     # ---- batched path (hipGraph) -------------------------------------------------
     batch = Batch(frames, params, outputs=("depth",))
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_autotune:
+        tuned = batch.autotune(reps=args.tune_reps)
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     use_graph = not args.no_graph
     for _ in range(args.warmup):
         batch.run(graph=use_graph)
@@ -237,6 +243,7 @@ def main():
         for s_ in singles:
             s_.close()
         print(f"[bench] plain per-frame: {plain:.1f} maps/s", file=sys.stderr, flush=True)
+    batch_tuning = batch.tuning()
     batch.close()
 
     # ---- CPU baseline: FP32 OpenMP oracle on a bounded sample -------------------
@@ -277,7 +284,8 @@ def main():
         "config": {"workload": f"{B} streams x {W}x{H} keyframes, {N} inverse-depth hypotheses, full pipeline "
                                "(quadtree, cost volume, HBP, upsample, fusion), "
                                + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
-                   "streams": B, "width": W, "height": H, "hypotheses": N},
+                   "streams": B, "width": W, "height": H, "hypotheses": N,
+                   "launch_shapes": batch_tuning},
         "plain_maps_per_s": None if plain is None else round(plain, 2),
         "batched_over_plain": None if plain is None else round(maps_per_s / plain, 3),
         "roofline": roofline,

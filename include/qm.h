@@ -195,6 +195,22 @@ int qm_batch_run(qm_batch* b, int mode, void* stream);
  * per stage kind the summed kernel time (ms) and the number of launches, after
  * synchronising the stream. For measurement only. */
 int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
+/* Launch-shape knobs of a batch; every value gives bit-identical results.
+ * Changing one drops the batch's captured graphs (recaptured on the next
+ * graph run). Defaults: 4, 16, 16. */
+#define QM_TUNE_COST_WAVES 0   /* 2, 4 or 8 waves per cost-volume block of 64 cells */
+#define QM_TUNE_BP_ROWS 1      /* 4, 8 or 16 cells (16-lane rows) per BP half-sweep block */
+#define QM_TUNE_EXTRACT_ROWS 2 /* 4, 8 or 16 cells per winner-extraction block */
+#define QM_NUM_TUNE 3
+int qm_batch_set_tuning(qm_batch* b, int knob, int value);
+int qm_batch_get_tuning(qm_batch* b, int knob, int* value);
+/* Autotuner: on this batch's own frames, time every value of every knob
+ * (reps launches each, after one warm-up; the BP knob is timed over the whole
+ * coarse-to-fine sweep schedule) and keep the fastest. It first recomputes the
+ * quadtree, cost volume and pyramid; the fusion state is not touched. chosen
+ * and best_ms (per knob, nullable) receive the choice and its time per
+ * launch set. Synchronises the stream. */
+int qm_batch_autotune(qm_batch* b, int reps, void* stream, int chosen[QM_NUM_TUNE], double best_ms[QM_NUM_TUNE]);
 /* Backward of the last run with respect to each stream's lambda: given
  * g_depth_out[i] = dL/d(depth_out) of frame i (device pointer, H x W; NULL
  * array or entry = zero), writes grad_lambda[i] = dL/d(lambda_i) (host array).

@@ -17,6 +17,7 @@ D = C.c_double
 
 STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", "upsample_col", "fuse_warp",
           "fuse_update")
+TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows")  # QM_TUNE_* order
 
 
 class QmError(RuntimeError):
@@ -106,6 +107,9 @@ SIGNATURES = {
     "qm_batch_read_buffer": (I, [P, I, I, P, C.c_size_t]),
     "qm_batch_reset_seeds": (I, [P]),
     "qm_batch_backward": (I, [P, P, P, P]),
+    "qm_batch_set_tuning": (I, [P, I, I]),
+    "qm_batch_get_tuning": (I, [P, I, P]),
+    "qm_batch_autotune": (I, [P, I, P, P, P]),
     "qm_batch_destroy": (I, [P]),
     "qm_create": (I, [I, I, F, F, F, F, P, P]),
     "qm_destroy": (I, [P]),

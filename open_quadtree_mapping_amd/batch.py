@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import STAGES, check, default_params, lib, qm_frame_desc, qm_params
+from ._lib import STAGES, TUNE_KNOBS, check, default_params, lib, qm_frame_desc, qm_params
 
 __all__ = ["Batch", "make_desc"]
 
@@ -109,6 +109,27 @@ class Batch:
         p = C.c_void_p()
         check(lib().qm_batch_buffer(self._h, i, which, C.byref(p)))
         return p.value
+
+    def autotune(self, reps: int = 5, stream=None) -> dict:
+        """Pick the fastest launch shapes for this batch (qm_batch_autotune).
+        Returns {knob: (value, ms)}."""
+        s = stream if stream is not None else torch.cuda.current_stream()
+        chosen = (C.c_int * len(TUNE_KNOBS))()
+        ms = (C.c_double * len(TUNE_KNOBS))()
+        check(lib().qm_batch_autotune(self._h, int(reps), C.c_void_p(s.cuda_stream), chosen, ms))
+        return {k: (chosen[i], ms[i]) for i, k in enumerate(TUNE_KNOBS)}
+
+    def set_tuning(self, **knobs):
+        for k, v in knobs.items():
+            check(lib().qm_batch_set_tuning(self._h, TUNE_KNOBS.index(k), int(v)))
+
+    def tuning(self) -> dict:
+        out = {}
+        for i, k in enumerate(TUNE_KNOBS):
+            v = C.c_int()
+            check(lib().qm_batch_get_tuning(self._h, i, C.byref(v)))
+            out[k] = v.value
+        return out
 
     def read_buffer(self, i, which) -> np.ndarray:
         """Host copy of an internal buffer of stream i (BUF_* ids)."""

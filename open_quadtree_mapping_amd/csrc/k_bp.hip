@@ -112,18 +112,18 @@ __device__ __forceinline__ bool cell_skipped(const FrameDev& F, int l, int bx, i
   return (int)gp((const uint8_t*)F.levels)[(size_t)Y * F.W + X] > lev;
 }
 
-// One half-sweep (src/bpextract.cu:200-312) for a batch: 16 work items per
+// One half-sweep (src/bpextract.cu:200-312) for a batch: ROWS work items per
 // block, one per 16-lane row. h for direction dir is the data term plus the
 // incoming messages other than the target's, added in the reference's order
 // (up, down, left, right).
-template <int N>
-__global__ __launch_bounds__(256) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l, int A,
-                                                  int first, int active_ok, float p1, float p2) {
+template <int N, int ROWS>
+__global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l,
+                                                        int A, int first, int active_ok, float p1, float p2) {
   constexpr int S = N / 16;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int j = threadIdx.x & 15;
-  const int item = (blockIdx.x - bm.start[f]) * 16 + (threadIdx.x >> 4);
+  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
   const int w = F.lw[l], h = F.lh[l];
   int x = 0, y = 0;
   bool upd = false, copy = false;
@@ -214,14 +214,14 @@ __global__ __launch_bounds__(256) void k_bp_sweep(const FrameDev* __restrict__ f
 // tree (stage m: sample d < m takes sample d + m only if strictly smaller, so
 // ties keep the lower index at every stage), parabola sub-sample refinement,
 // depth = 1 / (step * d + min) in double.
-template <int N>
-__global__ __launch_bounds__(256) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
-                                                 double istep) {
+template <int N, int ROWS>
+__global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
+                                                       double istep) {
   constexpr int S = N / 16;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int lane = threadIdx.x & 63, j = lane & 15;
-  const int cell = (blockIdx.x - bm.start[f]) * 16 + (threadIdx.x >> 4);
+  const int cell = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
   const int w = F.lw[0], h = F.lh[0];
   if (cell >= w * h) return;
   const int y = cell / w, x = cell - y * w;
@@ -333,36 +333,60 @@ __global__ __launch_bounds__(256) void k_cost_to_rows(const float* __restrict__ 
   out[i - d + row_pos<N>(d)] = in[i];
 }
 
+template <int N, int ROWS>
+static void bp_sweep_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool A,
+                           bool first, bool active, const BpLaunchParams& p) {
+  hipLaunchKernelGGL((k_bp_sweep<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, l, (int)A, (int)first,
+                     (int)active, p.p1, p.p2);
+}
+
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, hipStream_t s) {
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s) {
+  const int rows = (t.bp_rows == 4 || t.bp_rows == 8) ? t.bp_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) {
     const int w = hframes[i].lw[l], h = hframes[i].lh[l];
     const int items = first ? w * h : ((w + 1) / 2) * h;
-    counts[i] = (items + 15) / 16;
+    counts[i] = (items + rows - 1) / rows;
   }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
-  if (N == 64)
-    hipLaunchKernelGGL(k_bp_sweep<64>, dim3(nb), dim3(256), 0, s, dframes, bm, l, (int)A, (int)first, (int)active,
-                       p.p1, p.p2);
-  else
-    hipLaunchKernelGGL(k_bp_sweep<128>, dim3(nb), dim3(256), 0, s, dframes, bm, l, (int)A, (int)first, (int)active,
-                       p.p1, p.p2);
+  if (N == 64) {
+    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, p);
+    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, p);
+    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, p);
+  } else {
+    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, p);
+    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, p);
+    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, p);
+  }
+}
+
+template <int N, int ROWS>
+static void extract_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
+                          const BpLaunchParams& p) {
+  hipLaunchKernelGGL((k_extract<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, p.idepth_min,
+                     p.idepth_step);
 }
 
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    hipStream_t s) {
+                    const Tuning& t, hipStream_t s) {
+  const int rows = (t.extract_rows == 4 || t.extract_rows == 8) ? t.extract_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].lw[0] * hframes[i].lh[0] + 15) / 16;
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].lw[0] * hframes[i].lh[0] + rows - 1) / rows;
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
-  if (N == 64)
-    hipLaunchKernelGGL(k_extract<64>, dim3(nb), dim3(256), 0, s, dframes, bm, p.idepth_min, p.idepth_step);
-  else
-    hipLaunchKernelGGL(k_extract<128>, dim3(nb), dim3(256), 0, s, dframes, bm, p.idepth_min, p.idepth_step);
+  if (N == 64) {
+    if (rows == 4) extract_shape<64, 4>(nb, s, dframes, bm, p);
+    else if (rows == 8) extract_shape<64, 8>(nb, s, dframes, bm, p);
+    else extract_shape<64, 16>(nb, s, dframes, bm, p);
+  } else {
+    if (rows == 4) extract_shape<128, 4>(nb, s, dframes, bm, p);
+    else if (rows == 8) extract_shape<128, 8>(nb, s, dframes, bm, p);
+    else extract_shape<128, 16>(nb, s, dframes, bm, p);
+  }
 }
 
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out,

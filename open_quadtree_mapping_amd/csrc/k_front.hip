@@ -180,12 +180,12 @@ __device__ __forceinline__ float patch_sad(gptr<const uint8_t> ref, int W, int H
   return sad;
 }
 
-template <int N, bool FAST>
-__global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
-                                              const float* __restrict__ depth_table, int nblocks, int natural) {
+template <int N, bool FAST, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                     const float* __restrict__ depth_table, int nblocks, int natural) {
   // block = 64 consecutive cells x N samples; wave w takes samples
-  // [w*N/4, (w+1)*N/4) of every cell
-  constexpr int kPerWave = N / 4;
+  // [w*N/WAVES, (w+1)*N/WAVES) of every cell
+  constexpr int kPerWave = N / WAVES;
   constexpr int kRow = N + 1;  // LDS row stride in floats (conflict-free column writes)
   __shared__ float tile[64 * kRow];
   const int lb_global = xcd_logical_block(blockIdx.x, nblocks);
@@ -264,8 +264,8 @@ __global__ __launch_bounds__(256) void k_cost(const FrameDev* __restrict__ frame
   const gptr<float4> out = (gptr<float4>)(gp(F.cost[0]) + (size_t)cell0 * N);
   constexpr int kQ = N / 4;  // float4 per row
 #pragma unroll
-  for (int k = 0; k < N / 16; ++k) {
-    const int i = threadIdx.x + 256 * k;  // float4 index in the 64-row tile
+  for (int k = 0; k < (16 * N) / (64 * WAVES); ++k) {
+    const int i = threadIdx.x + 64 * WAVES * k;  // float4 index in the 64-row tile
     const int c = i / kQ, q = i - (i / kQ) * kQ;
     if (cell0 + c < ncell) {
       const float* src = tile + c * kRow;
@@ -343,8 +343,19 @@ void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hi
   if (nb > 0) hipLaunchKernelGGL(k_quadtree, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
+template <int N, bool FAST>
+static void launch_cost_shape(int waves, int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
+                              const float* table, int natural) {
+  if (waves == 2)
+    hipLaunchKernelGGL((k_cost<N, FAST, 2>), dim3(nb), dim3(128), 0, s, dframes, bm, table, nb, natural);
+  else if (waves == 8)
+    hipLaunchKernelGGL((k_cost<N, FAST, 8>), dim3(nb), dim3(512), 0, s, dframes, bm, table, nb, natural);
+  else
+    hipLaunchKernelGGL((k_cost<N, FAST, 4>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, natural);
+}
+
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
-                 hipStream_t s) {
+                 const Tuning& t, hipStream_t s) {
   BlockMap bm;
   int counts[kMaxFrames];
   bool fast = true;
@@ -355,15 +366,11 @@ void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N,
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
-    if (fast)
-      hipLaunchKernelGGL((k_cost<64, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
-    else
-      hipLaunchKernelGGL((k_cost<64, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
+    if (fast) launch_cost_shape<64, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
+    else launch_cost_shape<64, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
   } else {
-    if (fast)
-      hipLaunchKernelGGL((k_cost<128, true>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
-    else
-      hipLaunchKernelGGL((k_cost<128, false>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, (int)natural);
+    if (fast) launch_cost_shape<128, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
+    else launch_cost_shape<128, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
   }
 }
 

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -145,20 +146,26 @@ BpLaunchParams bp_params(const qm_params* p) {
   return b;
 }
 
-// The whole hierarchical BP: pyramid, coarse-to-fine half-sweeps, extraction.
-void run_bp(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, hipStream_t s) {
+// The coarse-to-fine half-sweeps of the hierarchical BP.
+void run_sweeps(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, const Tuning& t, hipStream_t s) {
   const int N = p->n_hyp;
   const BpLaunchParams bp = bp_params(p);
-  launch_pyramid(df, hf, n, N, s);
   for (int l = 3; l >= 0; --l) {
     const int iters = p->bp_iters[l];
     if (iters == 0) {
-      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, s);
+      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, t, s);
       continue;
     }
-    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, s);
+    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, t, s);
   }
-  launch_extract(df, hf, n, N, bp, s);
+}
+
+// The whole hierarchical BP: pyramid, coarse-to-fine half-sweeps, extraction.
+void run_bp(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, hipStream_t s) {
+  const Tuning t;
+  launch_pyramid(df, hf, n, p->n_hyp, s);
+  run_sweeps(df, hf, n, p, t, s);
+  launch_extract(df, hf, n, p->n_hyp, bp_params(p), t, s);
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -207,6 +214,7 @@ struct qm_batch {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int parity = 0;
   int runs = 0;
+  Tuning tune;
   std::vector<hipEvent_t> events;
   // backward (per-stream lambda), allocated on the first qm_batch_backward
   void* bwd_arena = nullptr;
@@ -261,21 +269,22 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
   { Mark m(t, QM_STAGE_QUADTREE); launch_quadtree(df, hf, n, s); }
-  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, false, s); }
+  const Tuning& tu = b->tune;
+  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, false, tu, s); }
   { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
   for (int l = 3; l >= 0; --l) {
     const int iters = p->bp_iters[l];
     if (iters == 0) {
       Mark m(t, QM_STAGE_BP_SWEEP);
-      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, s);
+      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
       continue;
     }
     for (int it = 0; it < iters; ++it) {
       Mark m(t, QM_STAGE_BP_SWEEP);
-      launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, s);
+      launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s);
     }
   }
-  { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, s); }
+  { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, tu, s); }
   { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, s); }
   { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, true, s); }
   { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
@@ -396,7 +405,7 @@ int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int 
   if ((rc = sc.get(&dt, sizeof(float) * p->n_hyp))) return rc;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dt, tab.data(), sizeof(float) * p->n_hyp, hipMemcpyHostToDevice, s));
-  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, s);
+  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, Tuning(), s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   return QM_OK;
@@ -829,6 +838,112 @@ int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
     case 6: *ptr = F.table_out; break;
     case 7: *ptr = b->hf[b->parity ^ 1][i].seeds; break;
     default: return fail(QM_ERR_INVALID, "unknown buffer id");
+  }
+  return QM_OK;
+}
+
+static int* tune_slot(Tuning& t, int knob) {
+  switch (knob) {
+    case QM_TUNE_COST_WAVES: return &t.cost_waves;
+    case QM_TUNE_BP_ROWS: return &t.bp_rows;
+    case QM_TUNE_EXTRACT_ROWS: return &t.extract_rows;
+    default: return nullptr;
+  }
+}
+
+static void drop_graphs(qm_batch* b) {
+  if (b->exec[0] || b->exec[1]) (void)hipDeviceSynchronize();  // no replay may still be running
+  for (int par = 0; par < 2; ++par) {
+    if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
+    if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);
+    b->exec[par] = nullptr;
+    b->graph[par] = nullptr;
+  }
+}
+
+int qm_batch_set_tuning(qm_batch* b, int knob, int value) {
+  if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  int* slot = tune_slot(b->tune, knob);
+  if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
+  const bool ok = knob == QM_TUNE_COST_WAVES ? (value == 2 || value == 4 || value == 8)
+                                             : (value == 4 || value == 8 || value == 16);
+  if (!ok) return fail(QM_ERR_INVALID, "value not offered for this knob");
+  if (*slot != value) {
+    *slot = value;
+    drop_graphs(b);
+  }
+  return QM_OK;
+}
+
+int qm_batch_get_tuning(qm_batch* b, int knob, int* value) {
+  if (!b || !value) return fail(QM_ERR_INVALID, "NULL pointer");
+  int* slot = tune_slot(b->tune, knob);
+  if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
+  *value = *slot;
+  return QM_OK;
+}
+
+int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* best_ms) {
+  if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  if (reps < 1 || reps > 1000) return fail(QM_ERR_INVALID, "reps must be in [1, 1000]");
+  hipStream_t s = (hipStream_t)stream;
+  const int par = b->parity;
+  const FrameDev* df = b->df[par];
+  const FrameDev* hf = b->hf[par].data();
+  const int n = b->n, N = b->p.n_hyp;
+  const BpLaunchParams bp = bp_params(&b->p);
+  // inputs of the timed stages: quadtree, cost, pyramid (the seeds are untouched)
+  launch_quadtree(df, hf, n, s);
+  launch_cost(df, hf, n, N, b->table, false, b->tune, s);
+  launch_pyramid(df, hf, n, N, s);
+  hipEvent_t e0, e1;
+  QM_HIP(hipEventCreate(&e0));
+  QM_HIP(hipEventCreate(&e1));
+  auto time = [&](const std::function<void(const Tuning&)>& stage, const Tuning& t, double* ms) -> int {
+    stage(t);  // warm
+    QM_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) stage(t);
+    QM_HIP(hipEventRecord(e1, s));
+    QM_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    QM_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = (double)f / reps;
+    return QM_OK;
+  };
+  struct Knob {
+    int id;
+    std::vector<int> values;
+    std::function<void(const Tuning&)> stage;
+  };
+  const Knob knobs[QM_NUM_TUNE] = {
+      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, t, s); }},
+      {QM_TUNE_BP_ROWS, {4, 8, 16}, [&](const Tuning& t) { run_sweeps(df, hf, n, &b->p, t, s); }},
+      {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
+  };
+  int rc = QM_OK;
+  Tuning best = b->tune;
+  for (const Knob& k : knobs) {
+    double best_t = 1e30;
+    for (int v : k.values) {
+      Tuning t = best;
+      *tune_slot(t, k.id) = v;
+      double ms = 0.0;
+      if ((rc = time(k.stage, t, &ms))) break;
+      if (ms < best_t) {
+        best_t = ms;
+        *tune_slot(best, k.id) = v;
+      }
+    }
+    if (rc) break;
+    if (best_ms) best_ms[k.id] = best_t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  QM_HIP(hipGetLastError());
+  for (int k = 0; k < QM_NUM_TUNE; ++k) {
+    if (chosen) chosen[k] = *tune_slot(best, k);
+    if ((rc = qm_batch_set_tuning(b, k, *tune_slot(best, k)))) return rc;
   }
   return QM_OK;
 }

@@ -24,18 +24,28 @@ struct BpLaunchParams {
   double idepth_min, idepth_step;
 };
 
+// Launch shapes the autotuner chooses among (every choice gives identical
+// results): waves per cost-volume block (each wave takes N / cost_waves
+// samples of the block's 64 cells), and cells (16-lane rows) per block of the
+// BP sweep and of the extraction.
+struct Tuning {
+  int cost_waves = 4;    // 2, 4, 8
+  int bp_rows = 16;      // 4, 8, 16
+  int extract_rows = 16;  // 4, 8, 16
+};
+
 void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // natural = write the volume in sample order (stage API) instead of the row layout
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
-                 hipStream_t s);
+                 const Tuning& t, hipStream_t s);
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s);
 // one half-sweep of level l; first = the level's first sweep (also initialises
 // every cell's messages from the parent level, or zero at the coarsest level);
 // active = 0 turns the first sweep into a pure initialisation.
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, hipStream_t s);
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s);
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    hipStream_t s);
+                    const Tuning& t, hipStream_t s);
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
 // a cost volume in sample order -> the row layout of the pipeline
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);

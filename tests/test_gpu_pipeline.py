@@ -66,3 +66,74 @@ def test_batch_matches_oracle(graph, N, imin, imax):
             assert np.array_equal(dd, ref[i]["depth"], equal_nan=True), f"stream {i} fused depth differs"
             assert dn.shape == (sizes[i][1], sizes[i][0])
     b.close()
+
+
+def test_twelve_uneven_streams_config4():
+    """Config 4: 12 streams of uneven size (4 x 376x240, 4 x 752x480,
+    4 x 1280x720, intrinsics scaled from 752x480), 64 samples, every stage in
+    one launch per stage, the keyframe pipeline replayed as a hipGraph; two
+    consecutive keyframes per stream against the FP32 oracle."""
+    N, imin, imax = 64, 0.02, 2.0
+    kinds = [(376, 240, 11), (376, 240, 12), (752, 480, 13), (752, 480, 14), (1280, 720, 15), (1280, 720, 16)]
+    rendered = {k: stream_frames(k[0], k[1], k[2], n=6) for k in kinds}
+    sizes = [kinds[i % 2 + 2 * (i // 4)] for i in range(12)]  # 4 of each size, two scenes per size
+    streams = [rendered[k] for k in sizes]
+    dev_imgs = {k: [torch.from_numpy(im).to(DEV) for im in rendered[k][2]] for k in kinds}
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    frames = [keyframe_inputs(cam, traj, imgs, 4, dev_imgs[sizes[i]])[0] for i, (cam, traj, imgs) in
+              enumerate(streams)]
+    b = Batch(frames, p, outputs=("depth", "sparse"))
+    seeds = [np.zeros((H, W, 4), np.float32) for (W, H, _) in sizes]
+    for k in (4, 5):
+        print(f"12 uneven streams, keyframe {k}", flush=True)
+        refs = []
+        for i, (cam, traj, imgs) in enumerate(streams):
+            fr, T, T_last = keyframe_inputs(cam, traj, imgs, k, dev_imgs[sizes[i]])
+            if k > 4:
+                b.set_frame(i, fr)
+            refs.append(O.keyframe(imgs[k], [imgs[k - 1 - j] for j in range(4)], T, cam, N, imin, imax, seeds[i],
+                                   T_last))
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        for i in range(12):
+            assert np.array_equal(b.outputs[i]["sparse"].cpu().numpy(), refs[i]["sparse"]), f"stream {i}"
+            assert np.array_equal(b.outputs[i]["depth"].cpu().numpy(), refs[i]["depth"], equal_nan=True), f"stream {i}"
+    b.close()
+
+
+def test_every_launch_shape_gives_identical_results_and_autotune():
+    """The autotuner's knobs change launch shapes only: every value of every
+    knob reproduces the default run bit for bit; the autotuner then picks
+    offered values and the batch still matches."""
+    sizes = [(376, 240, 21), (752, 480, 22)]
+    streams = [stream_frames(W, H, s, n=6) for (W, H, s) in sizes]
+    dev_imgs = [[torch.from_numpy(im).to(DEV) for im in imgs] for (_, _, imgs) in streams]
+    p = default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0)
+    frames = [keyframe_inputs(cam, traj, imgs, 5, dev_imgs[i])[0] for i, (cam, traj, imgs) in enumerate(streams)]
+
+    def run(b):
+        b.reset_seeds()
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        return [(o["sparse"].cpu().numpy(), o["dense"].cpu().numpy()) for o in b.outputs]
+
+    b = Batch(frames, p, outputs=("sparse", "dense"))
+    want = run(b)
+    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16)}
+    for knob, values in offered.items():
+        for v in values:
+            print(f"launch shape {knob}={v}", flush=True)
+            b.set_tuning(**{knob: v})
+            got = run(b)
+            for (gs, gd), (ws, wd) in zip(got, want):
+                assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True), f"{knob}={v}"
+        b.set_tuning(**{knob: values[1] if knob == "cost_waves" else 16})
+    res = b.autotune(reps=3)
+    print(f"autotune: {res}", flush=True)
+    for knob, (v, ms) in res.items():
+        assert v in offered[knob] and ms > 0
+    assert b.tuning() == {k: v for k, (v, _) in res.items()}
+    got = run(b)
+    for (gs, gd), (ws, wd) in zip(got, want):
+        assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True)
+    b.close()
