@@ -18,6 +18,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--streams B]
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -114,6 +115,24 @@ def stage_bytes(levels, W, H, N):
         "fuse_warp": 40 * px,
         "fuse_update": 48 * px,
     }, upd
+
+
+def measured_traffic(stage, W, H, N, B):
+    """HBM bytes per launch of the roofline kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there); (None, None) when no summary covers it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"pipeline_{W}x{H}x{N}_b{B}_hbm_counters.md")
+    if not os.path.exists(path):
+        return None, None
+    text = open(path).read()
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None, None
+    rec = json.loads(m.group(1))
+    if rec.get("kernel") != "k_" + stage:
+        return None, None
+    return int(rec["hbm_bytes_per_launch"]), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
 
 
 def main():
@@ -220,8 +239,9 @@ def main():
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     dom = max(stages, key=lambda k: stages[k]["ms_per_run"])
     d = stages[dom]
+    traffic, traffic_src = measured_traffic(dom, W, H, N, B)
     roofline = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["hbm_frac"], "traffic": None, "kernel": dom,
+                "frac": d["hbm_frac"], "traffic": traffic, "traffic_source": traffic_src, "kernel": dom,
                 "alg_bytes_per_launch": int(d["alg_bytes_per_run"] / max(d["launches"], 1)),
                 "avg_launch_us": d["avg_launch_us"]}
     print(f"[bench] stages: {json.dumps(stages)}", file=sys.stderr, flush=True)
