@@ -135,6 +135,26 @@ typedef struct qm_upsample_problem {
 int qm_upsample_general_batch(const qm_upsample_problem* problems, int n, void* stream);
 int qm_upsample_backward_batch(const qm_upsample_problem* problems, int n, void* stream);
 
+/* Semi-dense keyframe seeds: the LSD-style per-pixel inverse-depth filter that
+ * pre-fills the sparse map every frame (src/lsd_semidense.cu, driven by
+ * SeedMatrix::initial_keyframe / update_keyframe / create_new_keyframe,
+ * src/seed_matrix.cu:211-304,399-446). Device buffers: grad W x H x 2 floats
+ * (central differences, 0 on the border), seeds W x H x 8 floats (valid,
+ * blacklist, validity, 0, idepth, variance, smoothed idepth, smoothed
+ * variance). qm_sd_gradient replaces generate_gradient (src/seed_init.cu:13-42);
+ * qm_sd_init replaces initialize_keyframe_kernel of the first frame
+ * (src/lsd_semidense.cu:64-81); qm_sd_step runs one frame: the epipolar
+ * update of the keyframe seeds from the income frame, hole filling,
+ * regularisation and the projection into prefill (H x W, 0 where no seed
+ * lands), then, when switch_keyframe is set, the propagation to the income
+ * frame as the new keyframe (the caller then makes the income frame the
+ * keyframe). Poses are world -> camera. */
+int qm_sd_gradient(const uint8_t* img, int W, int H, int pitch, float* grad, void* stream);
+int qm_sd_init(const float* grad, int W, int H, float* seeds, void* stream);
+int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const uint8_t* inc, const float* inc_grad,
+               int W, int H, int pitch, const float cam[4], const float T_key_world[12],
+               const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream);
+
 /* Gaussian x Beta fusion of a dense depth map into the seed state. seeds
  * (H x W float4) is warped in place by T_last_to_cur; the update goes to
  * seeds_out; depth_out gets mu' where the prior is confident, else -1.
@@ -243,6 +263,9 @@ int qm_create(int W, int H, float fx, float fy, float cx, float cy, const qm_par
 int qm_destroy(qm_ctx* c);
 /* SeedMatrix::set_semi2dense_ratio (include/quadmap/seed_matrix.cuh:209). */
 int qm_set_semi2dense_ratio(qm_ctx* c, int ratio);
+/* Semi-dense pre-fill of the sparse map (on by default, as the reference);
+ * off leaves the pre-fill zero. */
+int qm_set_semidense(qm_ctx* c, int on);
 /* SeedMatrix::set_remap (src/seed_matrix.cu:75-80) plus the cv::remap(INTER_LINEAR,
  * BORDER_CONSTANT) of input_raw (:85), done on the GPU: map_x, map_y are host
  * H x W float coordinates into the raw camera image (the CV_32FC1 maps of

@@ -99,6 +99,9 @@ SIGNATURES = {
     "qm_upsample_backward_batch": (I, [P, I, P]),
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
     "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
+    "qm_sd_gradient": (I, [P, I, I, I, P, P]),
+    "qm_sd_init": (I, [P, I, I, P, P]),
+    "qm_sd_step": (I, [P, P, P, P, P, I, I, I, P, P, P, I, P, P]),
     "qm_batch_create": (I, [P, I, P, P]),
     "qm_batch_set_frame": (I, [P, I, P]),
     "qm_batch_run": (I, [P, I, P]),
@@ -114,6 +117,7 @@ SIGNATURES = {
     "qm_create": (I, [I, I, F, F, F, F, P, P]),
     "qm_destroy": (I, [P]),
     "qm_set_semi2dense_ratio": (I, [P, I]),
+    "qm_set_semidense": (I, [P, I]),
     "qm_set_remap": (I, [P, P, P]),
     "qm_add_frame": (I, [P, P, I, I, I, P, P]),
     "qm_get_depth": (I, [P, P]),

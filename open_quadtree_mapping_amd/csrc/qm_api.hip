@@ -620,6 +620,51 @@ int qm_upsample_backward(const float* d, const float* conf, const float* w_row, 
   return qm_upsample_backward_batch(&q, 1, stream);
 }
 
+int qm_sd_gradient(const uint8_t* img, int W, int H, int pitch, float* grad, void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if (!img || !grad) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  sd_gradient(img, W, H, pitch, (float2*)grad, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_sd_init(const float* grad, int W, int H, float* seeds, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!grad || !seeds) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  sd_init((const float2*)grad, W, H, (float4*)seeds, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const uint8_t* inc, const float* inc_grad,
+               int W, int H, int pitch, const float cam[4], const float T_key_world[12],
+               const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if (!seeds || !key || !key_grad || !inc || !inc_grad || !cam || !T_key_world || !T_inc_world || !prefill)
+    return fail(QM_ERR_INVALID, "NULL pointer");
+  if (!(cam[0] != 0.0f) || !(cam[1] != 0.0f)) return fail(QM_ERR_INVALID, "focal length is zero");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t px = (size_t)W * H;
+  Scratch sc;
+  void *second, *u64, *info;
+  if ((rc = sc.get(&second, px * 32))) return rc;
+  if ((rc = sc.get(&u64, px * 8))) return rc;
+  if ((rc = sc.get(&info, px * 16))) return rc;
+  float4* bufs[2] = {(float4*)seeds, (float4*)second};
+  sd_step(bufs, key, (const float2*)key_grad, inc, (const float2*)inc_grad, W, H, pitch, cam, T_key_world,
+          T_inc_world, switch_keyframe != 0, prefill, (unsigned long long*)u64, (float4*)info, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
 int qm_fuse_backward(const float* seeds_warped, const int32_t* table, const float* depth_in, int W, int H,
                      const float* g_depth_out, const float* g_seeds_out, float* grad_depth_in, void* stream) {
   int rc = check_size(W, H, W);

@@ -2,8 +2,9 @@
 // (reference: quadmap::Depthmap, src/depthmap.cpp:19-68, over
 // SeedMatrix::add_frames and friends, src/seed_matrix.cu:81-198,448-612).
 //
-// Per frame: upload the uint8 image (optionally remapped on the GPU), run the
-// frame gates on the host, and when a depth map is due run the whole keyframe
+// Per frame: upload the uint8 image (optionally remapped on the GPU), update
+// the semi-dense keyframe seeds (the sparse map's pre-fill), run the frame
+// gates on the host, and when a depth map is due run the whole keyframe
 // pipeline as one hipGraph replay of a one-stream qm_batch. Reference frames
 // live in a ring of device slots (no per-frame allocation); the fusion seeds
 // live in the batch and persist across keyframes.
@@ -55,6 +56,16 @@ struct qm_ctx {
   float T_world_ref[12];
   bool have_result = false;
   hipStream_t stream = nullptr;
+  // semi-dense keyframe seeds (the sparse map's pre-fill)
+  bool semidense = true;
+  uint8_t* sd_key = nullptr;        // keyframe image
+  float2* sd_key_grad = nullptr;
+  float2* sd_inc_grad = nullptr;
+  float4* sd_seeds[2] = {nullptr, nullptr};
+  unsigned long long* sd_u64 = nullptr;
+  float4* sd_info = nullptr;
+  float* sd_prefill = nullptr;
+  float sd_key_T[12];
 
   uint8_t* slot_ptr(int s) const { return slots + (size_t)s * W * H; }
 };
@@ -71,6 +82,14 @@ void ctx_free(qm_ctx* c) {
   dev_free(c->d_mapx);
   dev_free(c->d_mapy);
   dev_free(c->d_raw);
+  dev_free(c->sd_key);
+  dev_free(c->sd_key_grad);
+  dev_free(c->sd_inc_grad);
+  dev_free(c->sd_seeds[0]);
+  dev_free(c->sd_seeds[1]);
+  dev_free(c->sd_u64);
+  dev_free(c->sd_info);
+  dev_free(c->sd_prefill);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->h_depth) (void)hipHostFree(c->h_depth);
   if (c->h_debug) (void)hipHostFree(c->h_debug);
@@ -103,7 +122,7 @@ void make_desc(qm_ctx* c, qm_frame_desc* d) {
   float inv_fuse[12];
   se3_inv(c->fuse_pose, inv_fuse);
   se3_mul(c->income_T, inv_fuse, d->T_last_to_cur);  // src/seed_matrix.cu:574
-  d->prefill = nullptr;
+  d->prefill = c->semidense ? c->sd_prefill : nullptr;
   d->depth_out = c->d_depth;
   d->sparse_out = c->d_sparse;
 }
@@ -145,6 +164,23 @@ int qm_create(int W, int H, float fx, float fy, float cx, float cy, const qm_par
   c->d_depth = (float*)v;
   if ((rc = dev_alloc(&v, px * 4))) { ctx_free(c); return rc; }
   c->d_sparse = (float*)v;
+  if ((rc = dev_alloc(&v, px))) { ctx_free(c); return rc; }
+  c->sd_key = (uint8_t*)v;
+  if ((rc = dev_alloc(&v, px * 8))) { ctx_free(c); return rc; }
+  c->sd_key_grad = (float2*)v;
+  if ((rc = dev_alloc(&v, px * 8))) { ctx_free(c); return rc; }
+  c->sd_inc_grad = (float2*)v;
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = dev_alloc(&v, px * 32))) { ctx_free(c); return rc; }
+    c->sd_seeds[k] = (float4*)v;
+  }
+  if ((rc = dev_alloc(&v, px * 8))) { ctx_free(c); return rc; }
+  c->sd_u64 = (unsigned long long*)v;
+  if ((rc = dev_alloc(&v, px * 16))) { ctx_free(c); return rc; }
+  c->sd_info = (float4*)v;
+  if ((rc = dev_alloc(&v, px * 4))) { ctx_free(c); return rc; }
+  c->sd_prefill = (float*)v;
+  se3_identity(c->sd_key_T);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_stage, px, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_depth, px * 4, hipHostMallocDefault);
@@ -188,6 +224,14 @@ int qm_set_semi2dense_ratio(qm_ctx* c, int ratio) {
   if (!c) return fail(QM_ERR_INVALID, "NULL context");
   if (ratio < 1) return fail(QM_ERR_INVALID, "ratio must be >= 1");
   c->ratio = ratio;
+  return QM_OK;
+}
+
+int qm_set_semidense(qm_ctx* c, int on) {
+  if (!c) return fail(QM_ERR_INVALID, "NULL context");
+  if (c->initialized && (on != 0) != c->semidense)
+    return fail(QM_ERR_STATE, "the semi-dense setting is fixed once frames have been added");
+  c->semidense = on != 0;
   return QM_OK;
 }
 
@@ -254,6 +298,30 @@ int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch,
     QM_HIP(hipMemcpyAsync(slot, c->h_stage, px, hipMemcpyHostToDevice, c->stream));
   }
   std::memcpy(c->income_T, T_curr_world, sizeof(float) * 12);
+  // semi-dense keyframe seeds, every frame (src/seed_matrix.cu:113-139)
+  if (c->semidense) {
+    sd_gradient(slot, W, H, W, c->sd_inc_grad, c->stream);
+    if (!c->initialized) {
+      // set_income_as_keyframe + initial_keyframe (src/seed_matrix.cu:96-104,211-245)
+      QM_HIP(hipMemcpyAsync(c->sd_key, slot, px, hipMemcpyDeviceToDevice, c->stream));
+      QM_HIP(hipMemcpyAsync(c->sd_key_grad, c->sd_inc_grad, px * 8, hipMemcpyDeviceToDevice, c->stream));
+      std::memcpy(c->sd_key_T, c->income_T, sizeof(c->sd_key_T));
+      sd_init(c->sd_key_grad, W, H, c->sd_seeds[0], c->stream);
+    } else {
+      // update_keyframe, then need_switchkeyframe / create_new_keyframe (src/seed_matrix.cu:130-139)
+      float cosv, base;
+      se3_axis_cos_baseline(c->sd_key_T, c->income_T, &cosv, &base);
+      const bool sw = ((double)cosv < 0.86) || ((double)base > 0.5);
+      sd_step(c->sd_seeds, c->sd_key, c->sd_key_grad, slot, c->sd_inc_grad, W, H, W, c->cam, c->sd_key_T, c->income_T,
+              sw, c->sd_prefill, c->sd_u64, c->sd_info, c->stream);
+      if (sw) {
+        QM_HIP(hipMemcpyAsync(c->sd_key, slot, px, hipMemcpyDeviceToDevice, c->stream));
+        QM_HIP(hipMemcpyAsync(c->sd_key_grad, c->sd_inc_grad, px * 8, hipMemcpyDeviceToDevice, c->stream));
+        std::memcpy(c->sd_key_T, c->income_T, sizeof(c->sd_key_T));
+      }
+    }
+    QM_HIP(hipGetLastError());
+  }
   if (!c->initialized) {
     // src/seed_matrix.cu:122-128: the first frame only initialises and reports
     // a result; the reference's map is uninitialised memory there, here zeros

@@ -19,4 +19,11 @@ void dev_free(void* p);
 // remap a SW x SH source into a W x H destination through per-pixel maps
 void launch_remap_u8(const uint8_t* src, int SW, int SH, int spitch, int W, int H, const float* mapx,
                      const float* mapy, uint8_t* dst, hipStream_t s);
+// semi-dense keyframe seeds (k_semidense.hip); seeds are W x H x 2 float4
+void sd_gradient(const uint8_t* img, int W, int H, int pitch, float2* grad, hipStream_t s);
+void sd_init(const float2* grad, int W, int H, float4* seeds, hipStream_t s);
+void sd_step(float4* seeds[2], const uint8_t* key, const float2* key_grad, const uint8_t* inc,
+             const float2* inc_grad, int W, int H, int pitch, const float cam[4], const float T_key_world[12],
+             const float T_inc_world[12], bool switch_keyframe, float* prefill, unsigned long long* u64,
+             float4* info, hipStream_t s);
 }  // namespace qm

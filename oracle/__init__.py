@@ -62,6 +62,9 @@ def lib():
         L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
         L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
         L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P]
+        L.qmo_sd_gradient.argtypes = [P, I, I, I, P]
+        L.qmo_sd_init.argtypes = [P, I, I, P]
+        L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P]
         L.qmo_set_threads.argtypes = [I]
         L.qmo_get_threads.restype = I
         _lib = L
@@ -234,6 +237,57 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
 
 
 # ---------------------------------------------------------------------------
+# Semi-dense keyframe seeds (src/seed_init.cu, src/lsd_semidense.cu,
+# src/seed_matrix.cu:211-304,399-446): the sparse map's pre-fill.
+# ---------------------------------------------------------------------------
+def sd_gradient(img):
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape
+    g = np.zeros((H, W, 2), np.float32)
+    lib().qmo_sd_gradient(_p(img), W, H, W, _p(g))
+    return g
+
+
+def sd_init(grad):
+    H, W = grad.shape[:2]
+    seeds = np.zeros((H, W, 8), np.float32)
+    lib().qmo_sd_init(_p(np.ascontiguousarray(grad, np.float32)), W, H, _p(seeds))
+    return seeds
+
+
+def sd_step(seeds, key, key_grad, inc, inc_grad, cam, T_key_world, T_inc_world, switch_keyframe):
+    """One frame of the semi-dense module; seeds (H, W, 8) float32 is updated
+    in place. Returns the sparse-map pre-fill (H, W)."""
+    assert seeds.dtype == np.float32 and seeds.flags.c_contiguous
+    key = np.ascontiguousarray(key, np.uint8)
+    inc = np.ascontiguousarray(inc, np.uint8)
+    H, W = key.shape
+    prefill = np.zeros((H, W), np.float32)
+    Tk = np.ascontiguousarray(T_key_world, np.float32).reshape(12)
+    Ti = np.ascontiguousarray(T_inc_world, np.float32).reshape(12)
+    lib().qmo_sd_step(_p(seeds), _p(key), _p(np.ascontiguousarray(key_grad, np.float32)), _p(inc),
+                      _p(np.ascontiguousarray(inc_grad, np.float32)), W, H, W, cam.fx, cam.fy, cam.cx, cam.cy,
+                      _p(Tk), _p(Ti), int(bool(switch_keyframe)), _p(prefill))
+    return prefill
+
+
+def axis_cos_baseline(T_a_world, T_b_world):
+    """cos of the angle between the optical axes and the distance between the
+    camera centres of two world->camera poses, in float32 as the reference."""
+    A = _se3_inv(T_a_world).reshape(12)
+    B = _se3_inv(T_b_world).reshape(12)
+    cosv = (A[2] * B[2] + A[6] * B[6]) + A[10] * B[10]
+    dx, dy, dz = A[3] - B[3], A[7] - B[7], A[11] - B[11]
+    return np.float32(cosv), np.sqrt((dx * dx + dy * dy) + dz * dz, dtype=np.float32)
+
+
+def need_switch_keyframe(T_key_world, T_inc_world):
+    """src/seed_matrix.cu:200-209 (thresholds compared in double)."""
+    c, b = axis_cos_baseline(T_key_world, T_inc_world)
+    return bool(float(c) < 0.86 or float(b) > 0.5)
+
+
+# ---------------------------------------------------------------------------
 # Frame-sequence state machine (src/seed_matrix.cu:106-198, src/depthmap.cpp:42-55)
 # without the semi-dense seeds (their pre-fill of the sparse map is zero here).
 # ---------------------------------------------------------------------------
@@ -263,7 +317,7 @@ class DepthmapSequence:
     """Oracle of quadmap::Depthmap over a frame sequence (FP32)."""
 
     def __init__(self, cam, N=64, imin=0.01, imax=2.0, ratio=1, min_refs=11, max_refs=60, ref_cos=0.95,
-                 ref_baseline=0.03):
+                 ref_baseline=0.03, semidense=True):
         self.cam, self.N, self.imin, self.imax, self.ratio = cam, N, imin, imax, ratio
         self.min_refs, self.max_refs = min_refs, max_refs
         self.ref_cos, self.ref_baseline = np.float32(ref_cos), np.float32(ref_baseline)
@@ -274,6 +328,9 @@ class DepthmapSequence:
         self.seeds = np.zeros((cam.H, cam.W, 4), np.float32)
         self.depth = self.debug = self.reference = None
         self.T_world_ref = np.eye(3, 4, dtype=np.float32)
+        self.semidense = semidense
+        self.sd_key = self.sd_key_grad = self.sd_key_T = self.sd_seeds = None
+        self.prefill = None
 
     def _need_add_reference(self, T_income):
         if not self.refs:
@@ -289,6 +346,17 @@ class DepthmapSequence:
         T_curr_world = np.asarray(T_curr_world, np.float32)
         self.frame_index += 1
         W, H = self.cam.W, self.cam.H
+        if self.semidense:
+            grad = sd_gradient(img)
+            if not self.initialized:
+                self.sd_key, self.sd_key_grad, self.sd_key_T = img.copy(), grad, T_curr_world.copy()
+                self.sd_seeds = sd_init(grad)
+            else:
+                sw = need_switch_keyframe(self.sd_key_T, T_curr_world)
+                self.prefill = sd_step(self.sd_seeds, self.sd_key, self.sd_key_grad, img, grad, self.cam,
+                                       self.sd_key_T, T_curr_world, sw)
+                if sw:
+                    self.sd_key, self.sd_key_grad, self.sd_key_T = img.copy(), grad, T_curr_world.copy()
         if not self.initialized:
             self.initialized = True
             self.depth = np.zeros((H, W), np.float32)
@@ -304,7 +372,7 @@ class DepthmapSequence:
             T = np.stack([_se3_mul(self.refs[r][1], inv_inc) for r in range(4)])
             T_last = _se3_mul(T_curr_world, _se3_inv(self.fuse_pose))
             res = keyframe(img, [self.refs[r][0] for r in range(4)], T, self.cam, self.N, self.imin, self.imax,
-                           self.seeds, T_last)
+                           self.seeds, T_last, prefill=self.prefill if self.semidense else None)
         if self._need_add_reference(T_curr_world):
             self.refs.insert(0, (img.copy(), T_curr_world.copy()))
             if len(self.refs) > self.max_refs:

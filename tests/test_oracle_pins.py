@@ -453,3 +453,99 @@ def test_fusion_zbuffer_nearest_wins_and_reset_rule():
     T2[0, 3] = -0.1  # camera moves +x: points shift left by fx*0.1/5 = 1 px
     ws3, _, _, tab3 = O.fuse(s3, np.full((48, 64), 5.0), T2, CAMF, fp64=True)
     assert tab3[20, 30] == 20 * 64 + 31
+
+
+# ---------------------------------------------------------------------------
+# Semi-dense keyframe seeds (oracle/qm_semidense.cpp)
+# ---------------------------------------------------------------------------
+def test_sd_gradient_closed_form():
+    H, W = 24, 32
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = (3 * xx + 5 * yy).astype(np.uint8)
+    g = O.sd_gradient(img)
+    assert np.all(g[1:-1, 1:-1, 0] == 3.0) and np.all(g[1:-1, 1:-1, 1] == 5.0)
+    assert np.all(g[0] == 0) and np.all(g[-1] == 0) and np.all(g[:, 0] == 0) and np.all(g[:, -1] == 0)
+
+
+def _textured(H, W, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 256, (H, W)).astype(np.uint8)
+
+
+def _static_seeds(H, W, idepth, var, counter):
+    s = np.zeros((H, W, 8), np.float32)
+    s[3:H - 3, 3:W - 3, 0] = 1.0
+    s[3:H - 3, 3:W - 3, 2] = counter
+    s[3:H - 3, 3:W - 3, 4] = idepth
+    s[3:H - 3, 3:W - 3, 5] = var
+    s[3:H - 3, 3:W - 3, 6] = 1.0
+    s[3:H - 3, 3:W - 3, 7] = 4.0
+    return s
+
+
+def test_sd_zero_motion_regularisation_closed_form():
+    """Without motion the epipolar segment is empty, so the update leaves every
+    seed alone; the regularisation of a constant field keeps the inverse depth
+    exactly and sets the smoothed variance to 1 / sum 1/(var + 0.075^2 r^2)."""
+    from open_quadtree_mapping_amd.scene import Camera
+    H, W = 40, 48
+    cam = Camera(W, H, 60.0, 60.0, W / 2, H / 2)
+    img = _textured(H, W, 1)
+    g = O.sd_gradient(img)
+    s = _static_seeds(H, W, 0.5, 0.1, 10.0)
+    before = s.copy()
+    T = np.eye(3, 4, dtype=np.float32)
+    O.sd_step(s, img, g, img, g, cam, T, T, False)
+    inner = (slice(5, H - 5), slice(5, W - 5))
+    assert np.array_equal(s[inner][..., [0, 1, 2, 4, 5]], before[inner][..., [0, 1, 2, 4, 5]])
+    assert np.all(s[inner][..., 6] == np.float32(0.5))
+    acc = np.float32(0.0)
+    for j in range(-2, 3):
+        for i in range(-2, 3):
+            spatial = np.float32(np.float32(0.075) * np.float32(0.075)) * np.float32(i * i + j * j)
+            acc = np.float32(acc + np.float32(np.float32(1.0) / np.float32(np.float32(0.1) + spatial)))
+    assert np.all(s[inner][..., 7] == np.float32(np.float32(1.0) / acc))
+
+
+def test_sd_projection_identity_pose():
+    """Confident seeds (validity >= 20, smoothed variance <= 0.02) project into
+    the sparse map at their own pixel under the identity pose, with depth =
+    |K^-1 (x, y, 1)| / idepth, where the image gradient is at least 5."""
+    from open_quadtree_mapping_amd.scene import Camera
+    H, W = 40, 48
+    cam = Camera(W, H, 60.0, 60.0, W / 2, H / 2)
+    img = _textured(H, W, 2)
+    g = O.sd_gradient(img)
+    s = _static_seeds(H, W, 0.25, 0.001, 30.0)
+    T = np.eye(3, 4, dtype=np.float32)
+    pre = O.sd_step(s, img, g, img, g, cam, T, T, False)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    want = np.sqrt(((xx - cam.cx) / cam.fx) ** 2 + ((yy - cam.cy) / cam.fy) ** 2 + 1.0) / 0.25
+    strong = (g[..., 0] ** 2 + g[..., 1] ** 2) >= 25.0
+    inner = np.zeros((H, W), bool)
+    inner[5:H - 5, 5:W - 5] = True
+    m = strong & inner
+    assert m.sum() > 100
+    assert np.allclose(pre[m], want[m], rtol=1e-6)
+    assert np.all(pre[~strong] == 0)
+
+
+def test_sd_converges_to_rendered_depth():
+    """The epipolar filter on a lateral sequence over the textured planes of the
+    scene generator: after a few frames the pre-fill depths approach the
+    rendered ground truth (a sign or index slip in the search geometry would
+    leave them far off)."""
+    from open_quadtree_mapping_amd.scene import camera_for, lateral_trajectory, make_scene, render
+    W, H = 376, 240
+    cam = camera_for(W, H)
+    traj = lateral_trajectory(8, step=0.04, seed=11)
+    sc = make_scene(11)
+    fr = [render(sc, cam, T) for T in traj]
+    seq = O.DepthmapSequence(cam, 64, 0.02, 2.0, ratio=1)
+    med = []
+    for k in range(8):
+        seq.add_frames(fr[k][0], traj[k])
+        if seq.prefill is not None:
+            m = seq.prefill > 0
+            med.append(float(np.median(np.abs(seq.prefill[m] - fr[k][1][m]) / fr[k][1][m])))
+    assert med[-1] < 0.1 and med[-1] < med[0]
