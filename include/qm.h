@@ -155,6 +155,15 @@ int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const ui
                int W, int H, int pitch, const float cam[4], const float T_key_world[12],
                const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream);
 
+/* World-frame point cloud of a depth map, replacing the CPU loop of
+ * Publisher::publishPointCloud (src/publisher.cpp:129-164): in raster order,
+ * every pixel whose depth d is not below 0.1 (NaN kept, as there) gives
+ * (x, y, z, intensity) = (T_world_ref * (normalize(K^-1 (x, y, 1)) * d),
+ * ref_img(x, y)). depth, ref_img, xyzi are device pointers; xyzi must hold
+ * W * H * 4 floats; *count (host) receives the number of points. */
+int qm_point_cloud(const float* depth, const uint8_t* ref_img, int W, int H, const float cam[4],
+                   const float T_world_ref[12], float* xyzi, int* count, void* stream);
+
 /* Gaussian x Beta fusion of a dense depth map into the seed state. seeds
  * (H x W float4) is warped in place by T_last_to_cur; the update goes to
  * seeds_out; depth_out gets mu' where the prior is confident, else -1.
@@ -285,6 +294,10 @@ int qm_get_depth(qm_ctx* c, float* out);
 int qm_get_debug(qm_ctx* c, float* out);
 int qm_get_reference(qm_ctx* c, uint8_t* out);
 int qm_get_T_world_ref(qm_ctx* c, float out[12]);
+/* The point cloud of the last result (as qm_point_cloud with the context's
+ * depth map, reference image, intrinsics and T_world_ref), computed on the
+ * GPU; xyzi is host memory for W * H * 4 floats. */
+int qm_get_point_cloud(qm_ctx* c, float* xyzi, int* count);
 /* Number of reference frames held (the reference's framelist_host.size()). */
 int qm_num_references(qm_ctx* c);
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "qm_upsample_backward_batch": (I, [P, I, P]),
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
     "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
+    "qm_point_cloud": (I, [P, P, I, I, P, P, P, P, P]),
     "qm_sd_gradient": (I, [P, I, I, I, P, P]),
     "qm_sd_init": (I, [P, I, I, P, P]),
     "qm_sd_step": (I, [P, P, P, P, P, I, I, I, P, P, P, I, P, P]),
@@ -124,8 +125,21 @@ SIGNATURES = {
     "qm_get_debug": (I, [P, P]),
     "qm_get_reference": (I, [P, P]),
     "qm_get_T_world_ref": (I, [P, P]),
+    "qm_get_point_cloud": (I, [P, P, P]),
     "qm_num_references": (I, [P]),
 }
+
+
+def _init_torch_first():
+    """torch carries its own HIP runtime. When both runtimes live in one
+    process, torch's must initialise the device first (the other order leaves
+    torch without a GPU), so it is started before libqm is loaded."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
 
 
 def lib():
@@ -133,6 +147,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise QmError(f"{LIB_PATH} is not built; run __graft_entry__.build() (the HIP path has no fallback)")
+        _init_torch_first()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

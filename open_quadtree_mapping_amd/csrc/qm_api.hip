@@ -620,6 +620,14 @@ int qm_upsample_backward(const float* d, const float* conf, const float* w_row, 
   return qm_upsample_backward_batch(&q, 1, stream);
 }
 
+int qm_point_cloud(const float* depth, const uint8_t* ref_img, int W, int H, const float cam[4],
+                   const float T_world_ref[12], float* xyzi, int* count, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!depth || !ref_img || !cam || !T_world_ref || !xyzi || !count) return fail(QM_ERR_INVALID, "NULL pointer");
+  return point_cloud(depth, ref_img, W, H, cam, T_world_ref, xyzi, count, (hipStream_t)stream);
+}
+
 int qm_sd_gradient(const uint8_t* img, int W, int H, int pitch, float* grad, void* stream) {
   int rc = check_size(W, H, pitch);
   if (rc) return rc;

@@ -66,6 +66,7 @@ struct qm_ctx {
   float4* sd_info = nullptr;
   float* sd_prefill = nullptr;
   float sd_key_T[12];
+  uint8_t* d_ref = nullptr;  // reference image of the last result
 
   uint8_t* slot_ptr(int s) const { return slots + (size_t)s * W * H; }
 };
@@ -90,6 +91,7 @@ void ctx_free(qm_ctx* c) {
   dev_free(c->sd_u64);
   dev_free(c->sd_info);
   dev_free(c->sd_prefill);
+  dev_free(c->d_ref);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->h_depth) (void)hipHostFree(c->h_depth);
   if (c->h_debug) (void)hipHostFree(c->h_debug);
@@ -180,6 +182,8 @@ int qm_create(int W, int H, float fx, float fy, float cx, float cy, const qm_par
   c->sd_info = (float4*)v;
   if ((rc = dev_alloc(&v, px * 4))) { ctx_free(c); return rc; }
   c->sd_prefill = (float*)v;
+  if ((rc = dev_alloc(&v, px))) { ctx_free(c); return rc; }
+  c->d_ref = (uint8_t*)v;
   se3_identity(c->sd_key_T);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_stage, px, hipHostMallocDefault);
@@ -328,6 +332,8 @@ int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch,
     c->initialized = true;
     std::memset(c->h_depth, 0, px * 4);
     std::memset(c->h_debug, 0, px * 4);
+    QM_HIP(hipMemsetAsync(c->d_depth, 0, px * 4, c->stream));
+    QM_HIP(hipMemcpyAsync(c->d_ref, slot, px, hipMemcpyDeviceToDevice, c->stream));
     QM_HIP(hipMemcpyAsync(c->h_ref, slot, px, hipMemcpyDeviceToHost, c->stream));
     QM_HIP(hipStreamSynchronize(c->stream));
     se3_inv(T_curr_world, c->T_world_ref);
@@ -370,6 +376,7 @@ int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch,
     std::memcpy(c->fuse_pose, c->income_T, sizeof(float) * 12);  // src/seed_matrix.cu:592
     QM_HIP(hipMemcpyAsync(c->h_depth, c->d_depth, px * 4, hipMemcpyDeviceToHost, c->stream));
     QM_HIP(hipMemcpyAsync(c->h_debug, c->d_sparse, px * 4, hipMemcpyDeviceToHost, c->stream));
+    QM_HIP(hipMemcpyAsync(c->d_ref, c->slot_ptr(used_slot), px, hipMemcpyDeviceToDevice, c->stream));
     QM_HIP(hipMemcpyAsync(c->h_ref, c->slot_ptr(used_slot), px, hipMemcpyDeviceToHost, c->stream));
     QM_HIP(hipStreamSynchronize(c->stream));
     se3_inv(T_curr_world, c->T_world_ref);
@@ -406,6 +413,22 @@ int qm_get_T_world_ref(qm_ctx* c, float out[12]) {
   if (!c || !out) return fail(QM_ERR_INVALID, "NULL pointer");
   std::memcpy(out, c->T_world_ref, sizeof(float) * 12);
   return QM_OK;
+}
+
+int qm_get_point_cloud(qm_ctx* c, float* xyzi, int* count) {
+  if (!c || !xyzi || !count) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (!c->have_result) return fail(QM_ERR_STATE, "no result yet");
+  const size_t px = (size_t)c->W * c->H;
+  void* v;
+  int rc = dev_alloc(&v, px * 16);
+  if (rc) return rc;
+  rc = point_cloud(c->d_depth, c->d_ref, c->W, c->H, c->cam, c->T_world_ref, (float*)v, count, c->stream);
+  if (rc == QM_OK && *count > 0) {
+    const hipError_t e = hipMemcpy(xyzi, v, (size_t)*count * 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("point cloud download: ") + hipGetErrorString(e));
+  }
+  dev_free(v);
+  return rc;
 }
 
 int qm_num_references(qm_ctx* c) {

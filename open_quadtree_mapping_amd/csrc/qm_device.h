@@ -134,6 +134,11 @@ struct V3 {
   float x, y, z;
 };
 
+// a 3x4 row-major rigid transform passed by value to a kernel
+struct Se3 {
+  float m[12];
+};
+
 // normalize(cam2world(x, y)) with the pinned order ((x*x + y*y) + z*z)
 __device__ __forceinline__ V3 ray_dir(int x, int y, float fx, float fy, float cx, float cy) {
   V3 v;

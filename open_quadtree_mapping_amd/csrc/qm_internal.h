@@ -26,4 +26,8 @@ void sd_step(float4* seeds[2], const uint8_t* key, const float2* key_grad, const
              const float2* inc_grad, int W, int H, int pitch, const float cam[4], const float T_key_world[12],
              const float T_inc_world[12], bool switch_keyframe, float* prefill, unsigned long long* u64,
              float4* info, hipStream_t s);
+// world-frame point cloud of a depth map (k_output.hip); xyzi holds W*H
+// float4 (x, y, z, intensity); *count (host) receives the number of points
+int point_cloud(const float* depth, const uint8_t* ref, int W, int H, const float cam[4], const float T[12],
+                float* xyzi, int* count, hipStream_t s);
 }  // namespace qm

@@ -64,6 +64,17 @@ class Depthmap:
         check(lib().qm_get_T_world_ref(self._h, out.ctypes.data_as(C.c_void_p)))
         return out.reshape(3, 4)
 
+    def getPointCloud(self):
+        """(n, 4) float32 rows (x, y, z, intensity) of the last result in the
+        world frame (Publisher::publishPointCloud, src/publisher.cpp:129-164)."""
+        out = np.empty((self.H * self.W, 4), np.float32)
+        n = C.c_int()
+        check(lib().qm_get_point_cloud(self._h, out.ctypes.data_as(C.c_void_p), C.byref(n)))
+        return out[: n.value].copy()
+
+    def set_semidense(self, on: bool):
+        check(lib().qm_set_semidense(self._h, int(bool(on))))
+
     def getFx(self):
         return self._fx
 

@@ -128,3 +128,18 @@ def fuse(seeds, depth_in, cam, T_last_to_cur, want_table=False):
     check(lib().qm_fuse(_ptr(seeds), _ptr(depth_in), W, H, camv.ctypes.data_as(C.c_void_p),
                         T.ctypes.data_as(C.c_void_p), _ptr(dout), _ptr(sout), _ptr(table), _stream()))
     return (dout, sout, table) if want_table else (dout, sout)
+
+
+def point_cloud(depth, ref_img, cam, T_world_ref):
+    """World-frame points (n, 4) = (x, y, z, intensity) of a depth map, in
+    raster order, for pixels whose depth is not below 0.1 (qm_point_cloud)."""
+    _need(depth, torch.float32, name="depth")
+    H, W = depth.shape
+    _need(ref_img, torch.uint8, (H, W), name="ref_img")
+    camv = np.ascontiguousarray(cam, dtype=np.float32).reshape(4)
+    T = np.ascontiguousarray(T_world_ref, dtype=np.float32).reshape(12)
+    out = torch.empty((H * W, 4), dtype=torch.float32, device=depth.device)
+    n = C.c_int()
+    check(lib().qm_point_cloud(_ptr(depth), _ptr(ref_img), W, H, camv.ctypes.data_as(C.c_void_p),
+                               T.ctypes.data_as(C.c_void_p), _ptr(out), C.byref(n), _stream()))
+    return out[: n.value]

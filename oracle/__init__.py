@@ -236,6 +236,28 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
 
 
+def point_cloud(depth, ref_img, cam, T_world_ref):
+    """Publisher::publishPointCloud (src/publisher.cpp:129-164) in float32:
+    raster order, pixels with depth not below 0.1 (NaN kept), rows
+    (x, y, z, intensity) of T_world_ref * (normalize(K^-1 (x, y, 1)) * d)."""
+    depth = np.asarray(depth, np.float32)
+    keep = ~(depth.astype(np.float64) < 0.1)
+    ys, xs = np.nonzero(keep)
+    f32 = np.float32
+    rx = (xs.astype(f32) - f32(cam.cx)) / f32(cam.fx)
+    ry = (ys.astype(f32) - f32(cam.cy)) / f32(cam.fy)
+    rz = np.ones_like(rx)
+    inv = f32(1.0) / np.sqrt((rx * rx + ry * ry) + rz * rz)
+    d = depth[ys, xs]
+    p = [(rx * inv) * d, (ry * inv) * d, (rz * inv) * d]
+    T = np.asarray(T_world_ref, f32).reshape(12)
+    out = np.empty((len(xs), 4), f32)
+    for r in range(3):
+        out[:, r] = ((T[4 * r] * p[0] + T[4 * r + 1] * p[1]) + T[4 * r + 2] * p[2]) + T[4 * r + 3]
+    out[:, 3] = np.asarray(ref_img)[ys, xs].astype(f32)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Semi-dense keyframe seeds (src/seed_init.cu, src/lsd_semidense.cu,
 # src/seed_matrix.cu:211-304,399-446): the sparse map's pre-fill.

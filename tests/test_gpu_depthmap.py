@@ -94,3 +94,41 @@ def test_remap_downsample_from_larger_raw_image():
     got = dm.getReferenceImage()
     dm.close()
     assert np.array_equal(got, remap_fixed_point(raw, mx, my))
+
+
+def test_point_cloud_matches_oracle():
+    """qm_point_cloud on a fused depth map (with -1 and NaN pixels) and the
+    Depthmap-level point cloud of a sequence's last result, against the
+    float32 restatement of Publisher::publishPointCloud, in raster order."""
+    import torch
+    from open_quadtree_mapping_amd import stages as S
+    W, H = 376, 240
+    cam = camera_for(W, H)
+    rng = np.random.default_rng(4)
+    depth = rng.uniform(0.05, 6.0, (H, W)).astype(np.float32)
+    depth[rng.random((H, W)) < 0.2] = -1.0
+    depth[rng.random((H, W)) < 0.01] = np.nan
+    ref = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    T = np.array([[0.99, -0.1, 0.05, 0.3], [0.1, 0.98, 0.02, -0.2], [-0.05, 0.0, 0.99, 1.5]], np.float32)
+    print("point cloud stage", flush=True)
+    got = S.point_cloud(torch.from_numpy(depth).cuda(), torch.from_numpy(ref).cuda(), (cam.fx, cam.fy, cam.cx, cam.cy),
+                        T).cpu().numpy()
+    want = O.point_cloud(depth, ref, cam, T)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want, equal_nan=True)
+    # Depthmap level: 32 frames, so that the fused depth has become confident
+    n = 32
+    traj = lateral_trajectory(n, step=0.04, seed=12)
+    sc = make_scene(12)
+    imgs = [render(sc, cam, Tk)[0] for Tk in traj]
+    dm = Depthmap(W, H, cam.fx, cam.cx, cam.fy, cam.cy, semi2dense_ratio=1,
+                  params=default_params(idepth_min=0.02, idepth_max=2.0))
+    last = None
+    for k in range(n):
+        if dm.add_frames(imgs[k], traj[k]) and k > 0:
+            last = k
+    assert last is not None
+    pc = dm.getPointCloud()
+    want = O.point_cloud(dm.getDepthmap(), dm.getReferenceImage(), cam, dm.getT_world_ref())
+    assert len(pc) > 1000 and np.array_equal(pc, want, equal_nan=True)
+    dm.close()
