@@ -69,7 +69,10 @@ int qm_quadtree(const uint8_t* img, int W, int H, int pitch, uint8_t* levels, vo
 /* Multi-view patch-cost volume over n_hyp inverse-depth samples. refs[0..3] are
  * the four distinct references read by slots {0,0,1,2,3}; T_ref_key[4][12] is
  * the host array of key->reference transforms; cam = {fx, fy, cx, cy}.
- * count (nullable) receives 1 for node cells, 0 otherwise. Replaces
+ * Costs are the reference's normalised ones: (SAD sum / views / 255, or 100
+ * where no reference sees the sample) / 64, the 64 being normalize_the_cost's
+ * add_num, which every sample thread of a node increments once.
+ * count (nullable) receives that add_num: 64 for node cells, 0 otherwise. Replaces
  * image_to_cost + normalize_the_cost (src/depth_extract.cu:135-244, launched at
  * src/seed_matrix.cu:508,523). */
 int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int H, int pitch,
@@ -154,6 +157,13 @@ int qm_sd_init(const float* grad, int W, int H, float* seeds, void* stream);
 int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const uint8_t* inc, const float* inc_grad,
                int W, int H, int pitch, const float cam[4], const float T_key_world[12],
                const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream);
+
+/* The reference's local alternate to the global upsample, local_upsample /
+ * depth_interpolate (src/depth_upsample.cu:304-370; not on its live path):
+ * each pixel averages the positive sparse depths of the 7 x 7 node grid (nodes
+ * every 4 pixels) around it, weighted by exp(-(dx^2 + dy^2)/100 - dI^2/50);
+ * NaN where no node contributes. Device pointers. */
+int qm_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense, void* stream);
 
 /* World-frame point cloud of a depth map, replacing the CPU loop of
  * Publisher::publishPointCloud (src/publisher.cpp:129-164): in raster order,

@@ -100,6 +100,7 @@ SIGNATURES = {
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
     "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
     "qm_point_cloud": (I, [P, P, I, I, P, P, P, P, P]),
+    "qm_upsample_local": (I, [P, P, I, I, I, P, P]),
     "qm_sd_gradient": (I, [P, I, I, I, P, P]),
     "qm_sd_init": (I, [P, I, I, P, P]),
     "qm_sd_step": (I, [P, P, P, P, P, I, I, I, P, P, P, I, P, P]),

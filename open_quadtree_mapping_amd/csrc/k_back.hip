@@ -354,6 +354,45 @@ __global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ fram
 }
 
 // ---------------------------------------------------------------------------
+// Local (bilateral) upsample, the reference's alternate to the global solve
+// (src/depth_upsample.cu:304-370, not called on its live path): each pixel
+// averages the sparse depths of the 7 x 7 node grid around it (nodes every 4
+// pixels) with weights exp(-|dx|^2 / 100 - dI^2 / 50), skipping non-positive
+// depths; 0 / 0 = NaN where no node contributes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_up_local(const float* __restrict__ sparse, const uint8_t* __restrict__ img,
+                                                  int W, int H, int pitch, float* __restrict__ dense) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= W * H) return;
+  const int y = i / W, x = i - y * W;
+  const float me = (float)img[(size_t)y * pitch + x];
+  float acc = 0.0f, wsum = 0.0f;
+  for (int a = 0; a < 7; ++a) {
+    for (int b = 0; b < 7; ++b) {
+      const int cx = (x / 4 + a - 3) * 4, cy = (y / 4 + b - 3) * 4;
+      float w = 0.0f, dep = 0.0f;
+      if (!(cx < 0 || cx >= W || cy < 0 || cy >= H)) {
+        dep = sparse[(size_t)cy * W + cx];
+        if ((double)dep <= 0.0) continue;
+        const float di = me - (float)img[(size_t)cy * pitch + cx];
+        const float sx = (float)(x - cx), sy = (float)(y - cy);
+        const float sw = sx * sx + sy * sy;
+        const float iw = di * di;
+        w = pinned_expf(-sw / 100.0f - iw / 50.0f);
+      }
+      acc += w * dep;
+      wsum += w;
+    }
+  }
+  dense[i] = acc / wsum;
+}
+
+void launch_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_up_local, dim3((W * H + 255) / 256), dim3(256), 0, s, sparse, img, W, H, pitch, dense);
+}
+
+// ---------------------------------------------------------------------------
 // Fusion.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool is_badpoint(float a, float b) {

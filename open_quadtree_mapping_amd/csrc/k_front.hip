@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
     const int x = 4 * cxx, y = 4 * cyy;
     const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
     const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
-    if (F.count && w == 0) gp(F.count)[cell] = node ? 1 : 0;
+    if (F.count && w == 0) gp(F.count)[cell] = node ? 64 : 0;
     if (!node) {
 #pragma unroll
       for (int t = 0; t < kPerWave; ++t) trow[t] = 0.0f;
@@ -254,7 +254,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
         }
         const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
         const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
-        trow[t] = n > 0.0f ? (tot / n) / 255.0f : 100.0f;
+        // normalize_the_cost divides by add_num, which each of the 64 sample
+        // threads of a node's block incremented once: an exact / 64
+        trow[t] = (n > 0.0f ? (tot / n) / 255.0f : 100.0f) / 64.0f;
       }
     }
   }

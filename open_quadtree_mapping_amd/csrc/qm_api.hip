@@ -620,6 +620,17 @@ int qm_upsample_backward(const float* d, const float* conf, const float* w_row, 
   return qm_upsample_backward_batch(&q, 1, stream);
 }
 
+int qm_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense, void* stream) {
+  int rc = check_size(W, H, pitch);
+  if (rc) return rc;
+  if (!sparse || !img || !dense) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  launch_upsample_local(sparse, img, W, H, pitch, dense, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
 int qm_point_cloud(const float* depth, const uint8_t* ref_img, int W, int H, const float cam[4],
                    const float T_world_ref[12], float* xyzi, int* count, void* stream) {
   int rc = check_size(W, H, W);

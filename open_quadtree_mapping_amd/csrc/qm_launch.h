@@ -59,6 +59,9 @@ void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n
                          hipStream_t s);
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
                          bool clear_zbuf, hipStream_t s);
+// the reference's 7x7 bilateral alternate to the global upsample
+void launch_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense,
+                           hipStream_t s);
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);

@@ -143,3 +143,13 @@ def point_cloud(depth, ref_img, cam, T_world_ref):
     check(lib().qm_point_cloud(_ptr(depth), _ptr(ref_img), W, H, camv.ctypes.data_as(C.c_void_p),
                                T.ctypes.data_as(C.c_void_p), _ptr(out), C.byref(n), _stream()))
     return out[: n.value]
+
+
+def upsample_local(sparse, img):
+    """The reference's 7x7 bilateral alternate to the global upsample (qm_upsample_local)."""
+    _need(img, torch.uint8, name="img")
+    H, W = img.shape
+    _need(sparse, torch.float32, (H, W), name="sparse")
+    out = torch.empty((H, W), dtype=torch.float32, device=img.device)
+    check(lib().qm_upsample_local(_ptr(sparse), _ptr(img), W, H, W, _ptr(out), _stream()))
+    return out

@@ -236,6 +236,41 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
 
 
+def upsample_local(sparse, img):
+    """local_upsample / depth_interpolate (src/depth_upsample.cu:304-370) in
+    float32, accumulation in the reference's order (node column a outer, row b
+    inner), exp as (float)exp((double)x)."""
+    sparse = np.asarray(sparse, np.float32)
+    img = np.asarray(img, np.uint8)
+    H, W = img.shape
+    f32 = np.float32
+    yy, xx = np.mgrid[0:H, 0:W]
+    me = img.astype(f32)
+    acc = np.zeros((H, W), f32)
+    wsum = np.zeros((H, W), f32)
+    for a in range(7):
+        for b in range(7):
+            cx = (xx // 4 + a - 3) * 4
+            cy = (yy // 4 + b - 3) * 4
+            inb = (cx >= 0) & (cx < W) & (cy >= 0) & (cy < H)
+            cxc, cyc = np.clip(cx, 0, W - 1), np.clip(cy, 0, H - 1)
+            dep = np.where(inb, sparse[cyc, cxc], f32(0))
+            use = inb & (dep.astype(np.float64) > 0.0)
+            di = me - img[cyc, cxc].astype(f32)
+            sx, sy = (xx - cx).astype(f32), (yy - cy).astype(f32)
+            sw = sx * sx + sy * sy
+            iw = di * di
+            arg = (-sw / f32(100.0)) - iw / f32(50.0)
+            w = np.exp(arg.astype(np.float64)).astype(f32)
+            w = np.where(use, w, f32(0))
+            dep = np.where(use, dep, f32(0))
+            skip = inb & ~use
+            acc = np.where(skip, acc, acc + w * dep)
+            wsum = np.where(skip, wsum, wsum + w)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return acc / wsum
+
+
 def point_cloud(depth, ref_img, cam, T_world_ref):
     """Publisher::publishPointCloud (src/publisher.cpp:129-164) in float32:
     raster order, pixels with depth not below 0.1 (NaN kept), rows

@@ -154,8 +154,12 @@ void quadtree_impl(const uint8_t* img, int W, int H, int pitch, uint8_t* levels)
 }
 
 // ---------------------------------------------------------------------------
-// Multi-view patch cost volume (src/depth_extract.cu:135-229; normalize_the_cost
-// at :231-244 divides by a count that is always 0 or 1, an identity).
+// Multi-view patch cost volume (src/depth_extract.cu:135-229). normalize_the_cost
+// (:231-244) divides by add_num, which the 64 sample threads of a node's
+// image_to_cost block (launched 64 x 5 at src/seed_matrix.cu:503-512) each
+// incremented once inside "if (frame_id == 0)" (:225-228): count 64, an exact
+// division by 64 of every node's costs (including the 100 of a sample no
+// reference sees); cells that are not nodes keep 0.
 // Slots f = 0..4 read reference (int)(0.9 * f) = {0,0,1,2,3}
 // (src/depth_extract.cu:153-166 with the age table that is never written).
 // ---------------------------------------------------------------------------
@@ -214,9 +218,9 @@ void cost_impl(const uint8_t* key, const uint8_t* const* refs, int W, int H, int
         // shared-memory tree over the 5 slots: ((c0 + c4) + c2) + (c1 + c3)
         const Real tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
         const Real n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
-        out[d] = n > Real(0) ? (tot / n) / Real(255) : Real(100);
+        out[d] = (n > Real(0) ? (tot / n) / Real(255) : Real(100)) / Real(64);
       }
-      if (count) count[(size_t)cy * Wc + cxi] = 1;
+      if (count) count[(size_t)cy * Wc + cxi] = 64;
     }
   }
 }

@@ -210,3 +210,16 @@ def test_fuse(W, H, seed):
     assert np.nanmax(rel_err(god, rd)) <= REL
     assert np.array_equal(god, rd, equal_nan=True)
     assert np.array_equal(host(gsn), rs, equal_nan=True)
+
+
+@pytest.mark.parametrize("W,H,seed", [(376, 240, 3), (752, 480, 5)])
+def test_upsample_local_matches_oracle(W, H, seed):
+    ks = keyframe_set(W, H, seed)
+    print(f"local upsample {W}x{H}", flush=True)
+    rng = np.random.default_rng(seed)
+    sparse = np.zeros((H, W), np.float32)
+    m = rng.random((H // 4, W // 4)) < 0.6
+    sparse[0::4, 0::4][: H // 4, : W // 4][m] = rng.uniform(0.5, 10.0, m.sum()).astype(np.float32)
+    got = host(S.upsample_local(dev(sparse), dev(ks["key"])))
+    want = O.upsample_local(sparse, ks["key"])
+    assert np.array_equal(got, want, equal_nan=True)

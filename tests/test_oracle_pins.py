@@ -131,7 +131,7 @@ def test_cost_identical_images_identity_pose_is_zero():
     lv = O.quadtree(key)
     cost, cnt = O.cost_volume(key, [key] * 4, EYE, CAM, 64, 0.02, 2.0, lv)
     m = _nodes(lv, 96, 64)
-    assert np.array_equal(cnt == 1, m)
+    assert np.array_equal(cnt == 64, m) and np.all((cnt == 0) | (cnt == 64))
     assert np.all(cost[~m] == 0)
     # every projection of the same pixel lands inside (2 <= p < W-2) except near the border
     inner = m.copy()
@@ -147,21 +147,22 @@ def test_cost_slot_multiset_is_0_0_1_2_3():
     lv = O.quadtree(key)
     cost, _ = O.cost_volume(key, refs, EYE, CAM, 64, 0.02, 2.0, lv)
     sads = [9.0 * abs(100 - v) for v in vals]
-    want = np.float32((sads[0] + sads[3] + sads[1] + sads[0] + sads[2]) / 5.0 / 255.0)
+    # normalize_the_cost: add_num = 64, one increment per sample thread of the node
+    want = np.float32((sads[0] + sads[3] + sads[1] + sads[0] + sads[2]) / 5.0 / 255.0 / 64.0)
     cy, cx = 32 // 4, 48 // 4
     lvl = int(lv[32, 48])
     assert (48 % (1 << lvl)) == 0 and (32 % (1 << lvl)) == 0
     assert np.allclose(cost[cy, cx], want, rtol=1e-6, atol=0)
 
 
-def test_cost_out_of_view_is_100():
+def test_cost_out_of_view_is_100_over_64():
     key = np.full((64, 96), 100, np.uint8)
     T = EYE.copy()
     T[:, 0, 3] = 1e4  # everything projects far outside
     lv = O.quadtree(key)
     cost, cnt = O.cost_volume(key, [key] * 4, T, CAM, 64, 0.02, 2.0, lv)
-    m = cnt == 1
-    assert m.any() and np.all(cost[m] == 100.0)
+    m = cnt == 64
+    assert m.any() and np.all(cost[m] == 1.5625)
 
 
 def test_cost_translation_minimum_at_true_disparity():
@@ -549,3 +550,85 @@ def test_sd_converges_to_rendered_depth():
             m = seq.prefill > 0
             med.append(float(np.median(np.abs(seq.prefill[m] - fr[k][1][m]) / fr[k][1][m])))
     assert med[-1] < 0.1 and med[-1] < med[0]
+
+
+# ---------------------------------------------------------------------------
+# Local (bilateral) upsample, src/depth_upsample.cu:304-370: window extent from
+# a single node, the weight constants from two nodes in closed form (FP64),
+# constant maps, and a scalar per-pixel loop in the reference's loop order.
+# ---------------------------------------------------------------------------
+def test_local_upsample_single_node_window_and_value():
+    H, W = 48, 64
+    sparse = np.zeros((H, W), np.float32)
+    ni, nj = 7, 5  # node column / row
+    sparse[nj * 4, ni * 4] = 3.25
+    img = np.full((H, W), 90, np.uint8)
+    out = O.upsample_local(sparse, img)
+    yy, xx = np.mgrid[0:H, 0:W]
+    reach = (np.abs(xx // 4 - ni) <= 3) & (np.abs(yy // 4 - nj) <= 3)
+    assert np.all(np.isnan(out[~reach]))
+    np.testing.assert_allclose(out[reach], 3.25, rtol=2e-7)
+
+
+def test_local_upsample_two_node_weights_closed_form():
+    H, W = 32, 32
+    sparse = np.zeros((H, W), np.float32)
+    sparse[8, 8], sparse[8, 16] = 2.0, 5.0
+    img = np.full((H, W), 100, np.uint8)
+    img[8, 16] = 107  # intensity step at the second node
+    out = O.upsample_local(sparse, img)
+    for (y, x) in [(8, 10), (9, 13), (11, 12), (6, 9)]:
+        d1 = (x - 8) ** 2 + (y - 8) ** 2
+        d2 = (x - 16) ** 2 + (y - 8) ** 2
+        w1 = np.exp(-d1 / 100.0)
+        w2 = np.exp(-d2 / 100.0 - 49.0 / 50.0)
+        want = (2.0 * w1 + 5.0 * w2) / (w1 + w2)
+        assert abs(out[y, x] - want) <= 2e-6 * want, (y, x)
+
+
+def test_local_upsample_constant_and_nonpositive_skip():
+    rng = np.random.default_rng(2)
+    H, W = 40, 56
+    img = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    sparse = np.zeros((H, W), np.float32)
+    sparse[0::4, 0::4] = 4.5
+    sparse[0::8, 0::8] = -1.0  # skipped like empty nodes
+    out = O.upsample_local(sparse, img)
+    np.testing.assert_allclose(out, 4.5, rtol=1e-6)
+
+
+def _local_upsample_scalar(sparse, img):
+    H, W = img.shape
+    f32 = np.float32
+    out = np.empty((H, W), f32)
+    for y in range(H):
+        for x in range(W):
+            acc, ws = f32(0), f32(0)
+            for a in range(7):
+                for b in range(7):
+                    cx, cy = (x // 4 + a - 3) * 4, (y // 4 + b - 3) * 4
+                    if cx < 0 or cx >= W or cy < 0 or cy >= H:
+                        w, d = f32(0), f32(0)
+                    else:
+                        d = sparse[cy, cx]
+                        if float(d) <= 0.0:
+                            continue
+                        di = f32(img[y, x]) - f32(img[cy, cx])
+                        sx, sy = f32(x - cx), f32(y - cy)
+                        arg = -(sx * sx + sy * sy) / f32(100) - (di * di) / f32(50)
+                        w = f32(np.exp(np.float64(arg)))
+                    acc = f32(acc + w * d)
+                    ws = f32(ws + w)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                out[y, x] = acc / ws
+    return out
+
+
+def test_local_upsample_matches_scalar_loop():
+    rng = np.random.default_rng(4)
+    H, W = 20, 28
+    img = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    sparse = np.zeros((H, W), np.float32)
+    m = rng.random((H // 4, W // 4)) < 0.5
+    sparse[0::4, 0::4][m] = rng.uniform(0.5, 9.0, m.sum()).astype(np.float32)
+    assert np.array_equal(O.upsample_local(sparse, img), _local_upsample_scalar(sparse, img), equal_nan=True)
