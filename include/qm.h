@@ -52,7 +52,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 4
+#define QM_ABI_VERSION 5
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -156,7 +156,32 @@ int qm_sd_gradient(const uint8_t* img, int W, int H, int pitch, float* grad, voi
 int qm_sd_init(const float* grad, int W, int H, float* seeds, void* stream);
 int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const uint8_t* inc, const float* inc_grad,
                int W, int H, int pitch, const float cam[4], const float T_key_world[12],
-               const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream);
+               const float T_inc_world[12], int switch_keyframe, float* prefill, float* prior, void* stream);
+/* prior (nullable, H x W x 2) of qm_sd_step receives the (inverse depth,
+ * variance) map of depth_project_kernel's float2 overload
+ * (src/lsd_semidense.cu:241-294; seeds with validity >= 10, variance scaled
+ * by (depth / z)^2 and held at >= 0.01, nearest depth per pixel, 0 elsewhere),
+ * the input of the prior-driven alternates. The reference returns when both
+ * values are finite (:290-291) and so never stores one; the map here keeps the
+ * finite values, as that code intends. */
+
+/* The reference's prior-driven alternates (not on its live path).
+ * qm_quadtree_prior replaces quadtree_depth_kernal (src/quadtree.cu:79-154; no
+ * launch in the reference): per 16 x 16 tile whose pixels all have a prior
+ * depth > 0, a pixel's level becomes max(level, i) for the largest 2^i block
+ * (i <= 4) whose four children qualified and whose inverse depths all lie
+ * within 0.01 of the block mean. levels is read and written in place.
+ * qm_prior_to_cost replaces prior_to_cost (src/depth_extract.cu:23-47, launch
+ * commented out at src/seed_matrix.cu:492-499) on a cost volume from
+ * qm_cost_volume (same levels and params): each cell gets the truncated
+ * quadratic cost min(0.1 (s_d - p)^2 / v, 0.5) of every prior pixel (p, v) of
+ * its 4 x 4 block (3 <= x <= W-4, 3 <= y <= H-4, not p <= 0, not v <= 0),
+ * renormalised with the image cost as the reference's add_num does:
+ * (sum + 64 cost) / (64 (P + node)). count (nullable) receives that add_num.
+ * Device pointers; prior is H x W x 2 floats. */
+int qm_quadtree_prior(const float* prior_depth, int W, int H, uint8_t* levels, void* stream);
+int qm_prior_to_cost(const float* prior, int W, int H, const uint8_t* levels, const qm_params* p, float* cost,
+                     int32_t* count, void* stream);
 
 /* The reference's local alternate to the global upsample, local_upsample /
  * depth_interpolate (src/depth_upsample.cu:304-370; not on its live path):
@@ -207,6 +232,10 @@ typedef struct qm_frame_desc {
   float* dense_out;            /* upsampled depth before fusion (H x W) */
   float* sparse_out;           /* sparse map handed to the upsample (H x W) */
   double lambda;               /* this stream's upsample smoothness weight; <= 0: qm_params.lambda */
+  /* the reference's prior-driven alternates (see qm_prior_to_cost and
+   * qm_quadtree_prior), off when NULL: */
+  const float* prior;          /* H x W x 2 (inverse depth, variance) prior added to the cost volume */
+  const float* prior_depth;    /* H x W prior depth that can raise the quadtree level */
 } qm_frame_desc;
 
 typedef struct qm_batch qm_batch;
@@ -308,6 +337,13 @@ int qm_get_T_world_ref(qm_ctx* c, float out[12]);
  * depth map, reference image, intrinsics and T_world_ref), computed on the
  * GPU; xyzi is host memory for W * H * 4 floats. */
 int qm_get_point_cloud(qm_ctx* c, float* xyzi, int* count);
+/* Switch the prior-driven alternates on for a context's keyframes (before the
+ * first frame; they need the semi-dense module): prior_cost adds the
+ * semi-dense (inverse depth, variance) prior of the income frame to the cost
+ * volume (qm_prior_to_cost), prior_quadtree raises quadtree levels from the
+ * semi-dense pre-fill depth (qm_quadtree_prior). No counterpart in the
+ * reference, whose launches of both are absent or commented out. */
+int qm_set_prior_alternates(qm_ctx* c, int prior_cost, int prior_quadtree);
 /* Number of reference frames held (the reference's framelist_host.size()). */
 int qm_num_references(qm_ctx* c);
 

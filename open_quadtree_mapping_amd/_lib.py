@@ -55,6 +55,8 @@ class qm_frame_desc(C.Structure):
         ("dense_out", C.c_void_p),
         ("sparse_out", C.c_void_p),
         ("lambda_", C.c_double),
+        ("prior", C.c_void_p),
+        ("prior_depth", C.c_void_p),
     ]
 
 
@@ -80,7 +82,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 4  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 5  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {
@@ -103,7 +105,9 @@ SIGNATURES = {
     "qm_upsample_local": (I, [P, P, I, I, I, P, P]),
     "qm_sd_gradient": (I, [P, I, I, I, P, P]),
     "qm_sd_init": (I, [P, I, I, P, P]),
-    "qm_sd_step": (I, [P, P, P, P, P, I, I, I, P, P, P, I, P, P]),
+    "qm_sd_step": (I, [P, P, P, P, P, I, I, I, P, P, P, I, P, P, P]),
+    "qm_quadtree_prior": (I, [P, I, I, P, P]),
+    "qm_prior_to_cost": (I, [P, I, I, P, P, P, P, P]),
     "qm_batch_create": (I, [P, I, P, P]),
     "qm_batch_set_frame": (I, [P, I, P]),
     "qm_batch_run": (I, [P, I, P]),
@@ -121,6 +125,7 @@ SIGNATURES = {
     "qm_set_semi2dense_ratio": (I, [P, I]),
     "qm_set_semidense": (I, [P, I]),
     "qm_set_remap": (I, [P, P, P]),
+    "qm_set_prior_alternates": (I, [P, I, I]),
     "qm_add_frame": (I, [P, P, I, I, I, P, P]),
     "qm_get_depth": (I, [P, P]),
     "qm_get_debug": (I, [P, P]),

@@ -17,7 +17,7 @@ __all__ = ["Batch", "make_desc"]
 
 
 def make_desc(key, refs, T_ref_key, cam, T_last_to_cur=None, prefill=None, depth_out=None, dense_out=None,
-              sparse_out=None, lam=None) -> qm_frame_desc:
+              sparse_out=None, lam=None, prior=None, prior_depth=None) -> qm_frame_desc:
     H, W = key.shape
     d = qm_frame_desc()
     d.W, d.H, d.pitch = W, H, W
@@ -37,13 +37,16 @@ def make_desc(key, refs, T_ref_key, cam, T_last_to_cur=None, prefill=None, depth
     d.dense_out = None if dense_out is None else dense_out.data_ptr()
     d.sparse_out = None if sparse_out is None else sparse_out.data_ptr()
     d.lambda_ = 0.0 if lam is None else float(lam)
+    d.prior = None if prior is None else prior.data_ptr()
+    d.prior_depth = None if prior_depth is None else prior_depth.data_ptr()
     return d
 
 
 class Batch:
     """frames: list of dicts with key (HxW uint8 device), refs (4 of them),
-    T (4,3,4 float32), cam (fx, fy, cx, cy), optional T_last, prefill and
-    lambda (this stream's smoothness weight; default the params' lambda).
+    T (4,3,4 float32), cam (fx, fy, cx, cy), optional T_last, prefill,
+    lambda (this stream's smoothness weight; default the params' lambda) and
+    the prior-driven alternates' inputs prior (HxWx2 float32) and prior_depth.
     Output tensors depth / dense / sparse are allocated per stream when asked."""
 
     BUF_LEVELS, BUF_COST, BUF_DCOARSE, BUF_LABEL, BUF_DENSE, BUF_SEEDS, BUF_TABLE, BUF_PRIOR = range(8)
@@ -61,7 +64,8 @@ class Batch:
             outs = {k: torch.empty((H, W), dtype=torch.float32, device=dev) for k in outputs}
             self.outputs.append(outs)
             descs[i] = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
-                                 outs.get("depth"), outs.get("dense"), outs.get("sparse"), fr.get("lambda"))
+                                 outs.get("depth"), outs.get("dense"), outs.get("sparse"), fr.get("lambda"),
+                                 fr.get("prior"), fr.get("prior_depth"))
             self._keep.append(fr)
         self._descs = descs
         self._h = C.c_void_p()
@@ -71,7 +75,8 @@ class Batch:
         self.frames[i] = fr
         o = self.outputs[i]
         d = make_desc(fr["key"], fr["refs"], fr["T"], fr["cam"], fr.get("T_last"), fr.get("prefill"),
-                      o.get("depth"), o.get("dense"), o.get("sparse"), fr.get("lambda"))
+                      o.get("depth"), o.get("dense"), o.get("sparse"), fr.get("lambda"), fr.get("prior"),
+                      fr.get("prior_depth"))
         self._descs[i] = d
         check(lib().qm_batch_set_frame(self._h, i, C.byref(d)))
 

@@ -450,6 +450,51 @@ __global__ __launch_bounds__(256) void k_sd_resolve(const unsigned long long* __
   out[i] = k == ~0ull ? 0.0f : __uint_as_float((uint32_t)(k >> 32));
 }
 
+// src/lsd_semidense.cu:241-294: the (inverse depth, variance) prior of the
+// income frame, read by the prior-driven alternates. The reference returns when
+// the projected depth and variance are both finite (:290-291), so it never
+// stores a finite prior and its prior map stays zero; here the test is the one
+// that code intends: finite values are kept. Nearest depth wins a pixel, as
+// for the depth projection.
+__global__ __launch_bounds__(256) void k_sd_project_prior(Params P, const float4* __restrict__ seeds,
+                                                          unsigned long long* __restrict__ best,
+                                                          float4* __restrict__ info) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int W = P.W, H = P.H;
+  if (i >= W * H) return;
+  const int y = i / W, x = i - y * W;
+  if (!interior(x, y, W, H)) return;
+  const float4 t = seeds[2 * i];
+  if (!(t.x > 0.0f) || t.z < 10.0f) return;
+  const float4 d = seeds[2 * i + 1];
+  float3 np;
+  float nid;
+  int nx, ny;
+  if (!project_seed(P, P.k2i, x, y, d.z, np, nid, nx, ny)) return;
+  const float nd = sqrtf((np.x * np.x + np.y * np.y) + np.z * np.z);
+  const float vs = nd / np.z;
+  float var = (vs * vs) * d.w;
+  if (!(isfinite(nd) && isfinite(var))) return;
+  var = (double)var < 0.01 ? (float)0.01 : var;
+  info[i] = make_float4(1.0f / nd, var, 0.0f, 0.0f);
+  const unsigned long long k = ((unsigned long long)__float_as_uint(nd) << 32) | (unsigned long long)(uint32_t)i;
+  atomicMin(&best[(size_t)ny * W + nx], k);
+}
+
+__global__ __launch_bounds__(256) void k_sd_resolve_prior(const unsigned long long* __restrict__ best,
+                                                          const float4* __restrict__ info, float2* __restrict__ out,
+                                                          int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = best[i];
+  float2 v = make_float2(0.0f, 0.0f);
+  if (k != ~0ull) {
+    const float4 in = info[(uint32_t)(k & 0xFFFFFFFFull)];
+    v = make_float2(in.x, in.y);
+  }
+  out[i] = v;
+}
+
 __device__ __forceinline__ uint32_t fkey(float f) {
   const uint32_t b = __float_as_uint(f);
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
@@ -529,8 +574,8 @@ void sd_init(const float2* grad, int W, int H, float4* seeds, hipStream_t s) {
 // (W x H uint64) and info (W x H float4).
 void sd_step(float4* seeds[2], const uint8_t* key, const float2* key_grad, const uint8_t* inc,
              const float2* inc_grad, int W, int H, int pitch, const float cam[4], const float T_key_world[12],
-             const float T_inc_world[12], bool switch_keyframe, float* prefill, unsigned long long* u64,
-             float4* info, hipStream_t s) {
+             const float T_inc_world[12], bool switch_keyframe, float* prefill, float2* prior,
+             unsigned long long* u64, float4* info, hipStream_t s) {
   Params P;
   P.W = W;
   P.H = H;
@@ -561,6 +606,11 @@ void sd_step(float4* seeds[2], const uint8_t* key, const float2* key_grad, const
   hipLaunchKernelGGL(k_sd_fill_u64, dim3(blocks(n, 256)), dim3(256), 0, s, u64, n, ~0ull);
   hipLaunchKernelGGL(k_sd_project, dim3(blocks(n, 256)), dim3(256), 0, s, P, a, u64);
   hipLaunchKernelGGL(k_sd_resolve, dim3(blocks(n, 256)), dim3(256), 0, s, u64, prefill, (int)n);
+  if (prior) {
+    hipLaunchKernelGGL(k_sd_fill_u64, dim3(blocks(n, 256)), dim3(256), 0, s, u64, n, ~0ull);
+    hipLaunchKernelGGL(k_sd_project_prior, dim3(blocks(n, 256)), dim3(256), 0, s, P, a, u64, info);
+    hipLaunchKernelGGL(k_sd_resolve_prior, dim3(blocks(n, 256)), dim3(256), 0, s, u64, info, prior, (int)n);
+  }
   if (switch_keyframe) {
     hipLaunchKernelGGL(k_sd_fill_u64, dim3(blocks(n, 256)), dim3(256), 0, s, u64, n, 0ull);
     hipLaunchKernelGGL(k_sd_propagate, dim3(blocks(n, 256)), dim3(256), 0, s, P, a, u64, info);

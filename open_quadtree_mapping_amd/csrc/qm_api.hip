@@ -262,15 +262,36 @@ struct Mark {
   }
 };
 
+bool any_prior(const qm_batch* b) {
+  for (const FrameDev& F : b->hf[0])
+    if (F.prior) return true;
+  return false;
+}
+
+bool any_prior_depth(const qm_batch* b) {
+  for (const FrameDev& F : b->hf[0])
+    if (F.prior_depth) return true;
+  return false;
+}
+
 void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
   const FrameDev* df = b->df[par];
   const FrameDev* hf = b->hf[par].data();
   const int n = b->n, N = b->p.n_hyp;
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
-  { Mark m(t, QM_STAGE_QUADTREE); launch_quadtree(df, hf, n, s); }
+  const bool prior_depth = any_prior_depth(b), prior = any_prior(b);
+  {
+    Mark m(t, QM_STAGE_QUADTREE);
+    launch_quadtree(df, hf, n, s);
+    if (prior_depth) launch_quadtree_prior(df, hf, n, s);
+  }
   const Tuning& tu = b->tune;
-  { Mark m(t, QM_STAGE_COST); launch_cost(df, hf, n, N, b->table, false, tu, s); }
+  {
+    Mark m(t, QM_STAGE_COST);
+    launch_cost(df, hf, n, N, b->table, false, tu, s);
+    if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
+  }
   { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
   for (int l = 3; l >= 0; --l) {
     const int iters = p->bp_iters[l];
@@ -304,6 +325,8 @@ void apply_desc(FrameDev& F, const qm_frame_desc& d, const qm_params& p) {
   }
   std::memcpy(F.T_fuse, d.T_last_to_cur, sizeof(float) * 12);
   F.prefill = d.prefill;
+  F.prior = (const float2*)d.prior;
+  F.prior_depth = d.prior_depth;
   F.depth_out = d.depth_out;
   F.dense_out = d.dense_out;
   F.sparse_out = d.sparse_out;
@@ -620,6 +643,49 @@ int qm_upsample_backward(const float* d, const float* conf, const float* w_row, 
   return qm_upsample_backward_batch(&q, 1, stream);
 }
 
+int qm_quadtree_prior(const float* prior_depth, int W, int H, uint8_t* levels, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if (!prior_depth || !levels) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.prior_depth = prior_depth;
+  F.levels = levels;
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  launch_quadtree_prior((FrameDev*)df, &F, 1, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
+int qm_prior_to_cost(const float* prior, int W, int H, const uint8_t* levels, const qm_params* p, float* cost,
+                     int32_t* count, void* stream) {
+  int rc = check_size(W, H, W);
+  if (rc) return rc;
+  if ((rc = check_params(p))) return rc;
+  if (!prior || !levels || !cost) return fail(QM_ERR_INVALID, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  FrameDev F;
+  init_geometry(F, W, H, W);
+  F.prior = (const float2*)prior;
+  F.levels = const_cast<uint8_t*>(levels);
+  F.cost[0] = cost;
+  F.count = count;
+  Scratch sc;
+  void* df;
+  if ((rc = sc.get(&df, sizeof(FrameDev)))) return rc;
+  QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
+  const BpLaunchParams bp = bp_params(p);
+  launch_prior_cost((FrameDev*)df, &F, 1, p->n_hyp, true, bp.idepth_min, bp.idepth_step, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));
+  return QM_OK;
+}
+
 int qm_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense, void* stream) {
   int rc = check_size(W, H, pitch);
   if (rc) return rc;
@@ -663,7 +729,7 @@ int qm_sd_init(const float* grad, int W, int H, float* seeds, void* stream) {
 
 int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const uint8_t* inc, const float* inc_grad,
                int W, int H, int pitch, const float cam[4], const float T_key_world[12],
-               const float T_inc_world[12], int switch_keyframe, float* prefill, void* stream) {
+               const float T_inc_world[12], int switch_keyframe, float* prefill, float* prior, void* stream) {
   int rc = check_size(W, H, pitch);
   if (rc) return rc;
   if (!seeds || !key || !key_grad || !inc || !inc_grad || !cam || !T_key_world || !T_inc_world || !prefill)
@@ -678,7 +744,7 @@ int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const ui
   if ((rc = sc.get(&info, px * 16))) return rc;
   float4* bufs[2] = {(float4*)seeds, (float4*)second};
   sd_step(bufs, key, (const float2*)key_grad, inc, (const float2*)inc_grad, W, H, pitch, cam, T_key_world,
-          T_inc_world, switch_keyframe != 0, prefill, (unsigned long long*)u64, (float4*)info, s);
+          T_inc_world, switch_keyframe != 0, prefill, (float2*)prior, (unsigned long long*)u64, (float4*)info, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   return QM_OK;
@@ -823,6 +889,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
   return QM_OK;
 }
 
+static void drop_graphs(qm_batch* b);
+
 int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
   if (!b || !d) return fail(QM_ERR_INVALID, "NULL pointer");
   if (i < 0 || i >= b->n) return fail(QM_ERR_INVALID, "frame index out of range");
@@ -831,7 +899,10 @@ int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
   const FrameDev& F0 = b->hf[0][i];
   if (d->W != F0.W || d->H != F0.H || d->pitch != F0.pitch)
     return fail(QM_ERR_INVALID, "frame size differs from the planned one");
+  const bool had_prior = any_prior(b), had_prior_depth = any_prior_depth(b);
   for (int par = 0; par < 2; ++par) apply_desc(b->hf[par][i], *d, b->p);
+  // the alternates' launches are part of a captured graph only when some frame has the prior
+  if (had_prior != any_prior(b) || had_prior_depth != any_prior_depth(b)) drop_graphs(b);
   return upload_frames(b);
 }
 

@@ -65,6 +65,8 @@ struct qm_ctx {
   unsigned long long* sd_u64 = nullptr;
   float4* sd_info = nullptr;
   float* sd_prefill = nullptr;
+  float2* sd_prior = nullptr;  // (inverse depth, variance) prior of the income frame
+  bool prior_cost = false, prior_quadtree = false;
   float sd_key_T[12];
   uint8_t* d_ref = nullptr;  // reference image of the last result
 
@@ -91,6 +93,7 @@ void ctx_free(qm_ctx* c) {
   dev_free(c->sd_u64);
   dev_free(c->sd_info);
   dev_free(c->sd_prefill);
+  dev_free(c->sd_prior);
   dev_free(c->d_ref);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->h_depth) (void)hipHostFree(c->h_depth);
@@ -125,6 +128,8 @@ void make_desc(qm_ctx* c, qm_frame_desc* d) {
   se3_inv(c->fuse_pose, inv_fuse);
   se3_mul(c->income_T, inv_fuse, d->T_last_to_cur);  // src/seed_matrix.cu:574
   d->prefill = c->semidense ? c->sd_prefill : nullptr;
+  d->prior = (c->semidense && c->prior_cost) ? (const float*)c->sd_prior : nullptr;
+  d->prior_depth = (c->semidense && c->prior_quadtree) ? c->sd_prefill : nullptr;
   d->depth_out = c->d_depth;
   d->sparse_out = c->d_sparse;
 }
@@ -182,6 +187,8 @@ int qm_create(int W, int H, float fx, float fy, float cx, float cy, const qm_par
   c->sd_info = (float4*)v;
   if ((rc = dev_alloc(&v, px * 4))) { ctx_free(c); return rc; }
   c->sd_prefill = (float*)v;
+  if ((rc = dev_alloc(&v, px * 8))) { ctx_free(c); return rc; }
+  c->sd_prior = (float2*)v;
   if ((rc = dev_alloc(&v, px))) { ctx_free(c); return rc; }
   c->d_ref = (uint8_t*)v;
   se3_identity(c->sd_key_T);
@@ -236,6 +243,14 @@ int qm_set_semidense(qm_ctx* c, int on) {
   if (c->initialized && (on != 0) != c->semidense)
     return fail(QM_ERR_STATE, "the semi-dense setting is fixed once frames have been added");
   c->semidense = on != 0;
+  return QM_OK;
+}
+
+int qm_set_prior_alternates(qm_ctx* c, int prior_cost, int prior_quadtree) {
+  if (!c) return fail(QM_ERR_INVALID, "NULL context");
+  if (c->initialized) return fail(QM_ERR_STATE, "the prior alternates are fixed once frames have been added");
+  c->prior_cost = prior_cost != 0;
+  c->prior_quadtree = prior_quadtree != 0;
   return QM_OK;
 }
 
@@ -317,7 +332,7 @@ int qm_add_frame(qm_ctx* c, const uint8_t* img, int img_w, int img_h, int pitch,
       se3_axis_cos_baseline(c->sd_key_T, c->income_T, &cosv, &base);
       const bool sw = ((double)cosv < 0.86) || ((double)base > 0.5);
       sd_step(c->sd_seeds, c->sd_key, c->sd_key_grad, slot, c->sd_inc_grad, W, H, W, c->cam, c->sd_key_T, c->income_T,
-              sw, c->sd_prefill, c->sd_u64, c->sd_info, c->stream);
+              sw, c->sd_prefill, c->prior_cost ? c->sd_prior : nullptr, c->sd_u64, c->sd_info, c->stream);
       if (sw) {
         QM_HIP(hipMemcpyAsync(c->sd_key, slot, px, hipMemcpyDeviceToDevice, c->stream));
         QM_HIP(hipMemcpyAsync(c->sd_key_grad, c->sd_inc_grad, px * 8, hipMemcpyDeviceToDevice, c->stream));

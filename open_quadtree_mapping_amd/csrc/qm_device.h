@@ -25,6 +25,8 @@ struct FrameDev {
   float T[4][12];       // key -> reference r
   float T_fuse[12];     // last fused frame -> this frame
   const float* prefill;  // H x W or null
+  const float2* prior;   // H x W semi-dense prior (inverse depth, variance) for the prior cost, or null
+  const float* prior_depth;  // H x W prior depth for the depth quadtree, or null
   const float* sparse_in;  // explicit sparse map (stage API) or null
   uint8_t* levels;      // H x W
   float* cost[4];       // lh*lw*N per level

@@ -24,8 +24,8 @@ void sd_gradient(const uint8_t* img, int W, int H, int pitch, float2* grad, hipS
 void sd_init(const float2* grad, int W, int H, float4* seeds, hipStream_t s);
 void sd_step(float4* seeds[2], const uint8_t* key, const float2* key_grad, const uint8_t* inc,
              const float2* inc_grad, int W, int H, int pitch, const float cam[4], const float T_key_world[12],
-             const float T_inc_world[12], bool switch_keyframe, float* prefill, unsigned long long* u64,
-             float4* info, hipStream_t s);
+             const float T_inc_world[12], bool switch_keyframe, float* prefill, float2* prior,
+             unsigned long long* u64, float4* info, hipStream_t s);
 // world-frame point cloud of a depth map (k_output.hip); xyzi holds W*H
 // float4 (x, y, z, intensity); *count (host) receives the number of points
 int point_cloud(const float* depth, const uint8_t* ref, int W, int H, const float cam[4], const float T[12],

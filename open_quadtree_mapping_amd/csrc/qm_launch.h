@@ -59,6 +59,10 @@ void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n
                          hipStream_t s);
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
                          bool clear_zbuf, hipStream_t s);
+// the reference's prior-driven alternates; frames without the prior get no work
+void launch_quadtree_prior(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+void launch_prior_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, bool natural,
+                       double idepth_min, double idepth_step, hipStream_t s);
 // the reference's 7x7 bilateral alternate to the global upsample
 void launch_upsample_local(const float* sparse, const uint8_t* img, int W, int H, int pitch, float* dense,
                            hipStream_t s);

@@ -75,6 +75,11 @@ class Depthmap:
     def set_semidense(self, on: bool):
         check(lib().qm_set_semidense(self._h, int(bool(on))))
 
+    def set_prior_alternates(self, prior_cost: bool = False, prior_quadtree: bool = False):
+        """The reference's prior-driven alternates (prior_to_cost,
+        quadtree_depth_kernal) fed by the semi-dense module; before the first frame."""
+        check(lib().qm_set_prior_alternates(self._h, int(bool(prior_cost)), int(bool(prior_quadtree))))
+
     def getFx(self):
         return self._fx
 

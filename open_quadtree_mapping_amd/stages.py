@@ -153,3 +153,27 @@ def upsample_local(sparse, img):
     out = torch.empty((H, W), dtype=torch.float32, device=img.device)
     check(lib().qm_upsample_local(_ptr(sparse), _ptr(img), W, H, W, _ptr(out), _stream()))
     return out
+
+
+def quadtree_prior(prior_depth, levels):
+    """quadtree_depth_kernal: raises the levels (in place) where the prior
+    inverse depth is flat over a block (qm_quadtree_prior)."""
+    H, W = levels.shape
+    _need(levels, torch.uint8, name="levels")
+    _need(prior_depth, torch.float32, (H, W), name="prior_depth")
+    check(lib().qm_quadtree_prior(_ptr(prior_depth), W, H, _ptr(levels), _stream()))
+    return levels
+
+
+def prior_to_cost(prior, levels, cost, params: qm_params | None = None, count=None):
+    """prior_to_cost on a cost volume of cost_volume (in place): prior is
+    (H, W, 2) float32 (inverse depth, variance) (qm_prior_to_cost)."""
+    p = params or default_params()
+    _need(levels, torch.uint8, name="levels")
+    H, W = levels.shape
+    _need(prior, torch.float32, (H, W, 2), name="prior")
+    _need(cost, torch.float32, (H // 4, W // 4, p.n_hyp), name="cost")
+    if count is not None:
+        _need(count, torch.int32, (H // 4, W // 4), name="count")
+    check(lib().qm_prior_to_cost(_ptr(prior), W, H, _ptr(levels), C.byref(p), _ptr(cost), _ptr(count), _stream()))
+    return cost

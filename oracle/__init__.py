@@ -11,6 +11,9 @@ Each function follows a reference file (see qm_oracle.cpp's header):
   weights/upsample   src/depth_upsample.cu:20-302
   fuse               src/depth_fusion.cu:13-213
   keyframe           src/seed_matrix.cu:448-597
+  semi-dense seeds   src/lsd_semidense.cu (qm_semidense.cpp)
+  prior alternates   src/quadtree.cu:79-154, src/depth_extract.cu:23-47 (qm_prior.cpp)
+  local upsample     src/depth_upsample.cu:304-370
 """
 from __future__ import annotations
 
@@ -61,10 +64,12 @@ def lib():
         L.qmo_fuse_update_f64.argtypes = [I, I, P, P, P, P, P]
         L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
         L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
-        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P]
+        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P, P, P]
         L.qmo_sd_gradient.argtypes = [P, I, I, I, P]
         L.qmo_sd_init.argtypes = [P, I, I, P]
-        L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P]
+        L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P, P]
+        L.qmo_quadtree_prior.argtypes = [P, I, I, P]
+        L.qmo_prior_to_cost.argtypes = [P, I, I, P, I, D, D, P, P]
         L.qmo_set_threads.argtypes = [I]
         L.qmo_get_threads.restype = I
         _lib = L
@@ -216,7 +221,7 @@ def compose_sparse(W, H, depth_coarse, label, prefill=None, fp64=False):
     return out
 
 
-def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None):
+def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None, prior=None, prior_depth=None):
     """Full FP32 keyframe. seeds (HxWx4 float32) is updated in place.
     Returns dict(depth, sparse, levels, label, stage_ms)."""
     key = np.ascontiguousarray(key, dtype=np.uint8)
@@ -226,13 +231,16 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     assert seeds.dtype == np.float32 and seeds.flags.c_contiguous
     Tl = np.ascontiguousarray(T_last_to_cur, dtype=np.float32).reshape(12)
     pf = None if prefill is None else np.ascontiguousarray(prefill, dtype=np.float32)
+    pr = None if prior is None else np.ascontiguousarray(prior, dtype=np.float32)
+    pd = None if prior_depth is None else np.ascontiguousarray(prior_depth, dtype=np.float32)
     depth = np.zeros((H, W), np.float32)
     sparse = np.zeros((H, W), np.float32)
     levels = np.zeros((H, W), np.uint8)
     label = np.zeros((H // 4, W // 4), np.int32)
     ms = np.zeros(5, np.float64)
     lib().qmo_keyframe_f32(_p(key), _ptr_array(refs), W, H, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, imin, imax,
-                           _p(pf), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label), _p(ms))
+                           _p(pf), _p(pr), _p(pd), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label),
+                           _p(ms))
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
 
 
@@ -269,6 +277,27 @@ def upsample_local(sparse, img):
             wsum = np.where(skip, wsum, wsum + w)
     with np.errstate(invalid="ignore", divide="ignore"):
         return acc / wsum
+
+
+def quadtree_prior(prior_depth, levels):
+    """quadtree_depth_kernal (src/quadtree.cu:79-154): returns max(levels,
+    depth level) for an (H, W) prior depth map (qm_prior.cpp)."""
+    out = np.array(levels, np.uint8, copy=True, order="C")
+    H, W = out.shape
+    lib().qmo_quadtree_prior(_p(np.ascontiguousarray(prior_depth, np.float32)), W, H, _p(out))
+    return out
+
+
+def prior_to_cost(prior, levels, cost, count, N, imin, imax):
+    """prior_to_cost (src/depth_extract.cu:23-47) on a cost volume of
+    cost_volume(), renormalised by add_num; returns new (cost, count)."""
+    cost = np.array(cost, np.float32, copy=True, order="C")
+    count = np.array(count, np.int32, copy=True, order="C")
+    levels = np.ascontiguousarray(levels, np.uint8)
+    H, W = levels.shape
+    lib().qmo_prior_to_cost(_p(np.ascontiguousarray(prior, np.float32)), W, H, _p(levels), N, imin, imax,
+                            _p(cost), _p(count))
+    return cost, count
 
 
 def point_cloud(depth, ref_img, cam, T_world_ref):
@@ -312,20 +341,22 @@ def sd_init(grad):
     return seeds
 
 
-def sd_step(seeds, key, key_grad, inc, inc_grad, cam, T_key_world, T_inc_world, switch_keyframe):
+def sd_step(seeds, key, key_grad, inc, inc_grad, cam, T_key_world, T_inc_world, switch_keyframe, want_prior=False):
     """One frame of the semi-dense module; seeds (H, W, 8) float32 is updated
-    in place. Returns the sparse-map pre-fill (H, W)."""
+    in place. Returns the sparse-map pre-fill (H, W) [and the (H, W, 2)
+    (inverse depth, variance) prior of src/lsd_semidense.cu:241-294]."""
     assert seeds.dtype == np.float32 and seeds.flags.c_contiguous
     key = np.ascontiguousarray(key, np.uint8)
     inc = np.ascontiguousarray(inc, np.uint8)
     H, W = key.shape
     prefill = np.zeros((H, W), np.float32)
+    prior = np.zeros((H, W, 2), np.float32) if want_prior else None
     Tk = np.ascontiguousarray(T_key_world, np.float32).reshape(12)
     Ti = np.ascontiguousarray(T_inc_world, np.float32).reshape(12)
     lib().qmo_sd_step(_p(seeds), _p(key), _p(np.ascontiguousarray(key_grad, np.float32)), _p(inc),
                       _p(np.ascontiguousarray(inc_grad, np.float32)), W, H, W, cam.fx, cam.fy, cam.cx, cam.cy,
-                      _p(Tk), _p(Ti), int(bool(switch_keyframe)), _p(prefill))
-    return prefill
+                      _p(Tk), _p(Ti), int(bool(switch_keyframe)), _p(prefill), _p(prior))
+    return (prefill, prior) if want_prior else prefill
 
 
 def axis_cos_baseline(T_a_world, T_b_world):
@@ -374,7 +405,7 @@ class DepthmapSequence:
     """Oracle of quadmap::Depthmap over a frame sequence (FP32)."""
 
     def __init__(self, cam, N=64, imin=0.01, imax=2.0, ratio=1, min_refs=11, max_refs=60, ref_cos=0.95,
-                 ref_baseline=0.03, semidense=True):
+                 ref_baseline=0.03, semidense=True, prior_cost=False, prior_quadtree=False):
         self.cam, self.N, self.imin, self.imax, self.ratio = cam, N, imin, imax, ratio
         self.min_refs, self.max_refs = min_refs, max_refs
         self.ref_cos, self.ref_baseline = np.float32(ref_cos), np.float32(ref_baseline)
@@ -386,6 +417,8 @@ class DepthmapSequence:
         self.depth = self.debug = self.reference = None
         self.T_world_ref = np.eye(3, 4, dtype=np.float32)
         self.semidense = semidense
+        self.prior_cost, self.prior_quadtree = prior_cost, prior_quadtree
+        self.prior = None
         self.sd_key = self.sd_key_grad = self.sd_key_T = self.sd_seeds = None
         self.prefill = None
 
@@ -410,8 +443,9 @@ class DepthmapSequence:
                 self.sd_seeds = sd_init(grad)
             else:
                 sw = need_switch_keyframe(self.sd_key_T, T_curr_world)
-                self.prefill = sd_step(self.sd_seeds, self.sd_key, self.sd_key_grad, img, grad, self.cam,
-                                       self.sd_key_T, T_curr_world, sw)
+                out = sd_step(self.sd_seeds, self.sd_key, self.sd_key_grad, img, grad, self.cam,
+                              self.sd_key_T, T_curr_world, sw, want_prior=self.prior_cost)
+                self.prefill, self.prior = out if self.prior_cost else (out, None)
                 if sw:
                     self.sd_key, self.sd_key_grad, self.sd_key_T = img.copy(), grad, T_curr_world.copy()
         if not self.initialized:
@@ -429,7 +463,9 @@ class DepthmapSequence:
             T = np.stack([_se3_mul(self.refs[r][1], inv_inc) for r in range(4)])
             T_last = _se3_mul(T_curr_world, _se3_inv(self.fuse_pose))
             res = keyframe(img, [self.refs[r][0] for r in range(4)], T, self.cam, self.N, self.imin, self.imax,
-                           self.seeds, T_last, prefill=self.prefill if self.semidense else None)
+                           self.seeds, T_last, prefill=self.prefill if self.semidense else None,
+                           prior=self.prior if self.semidense and self.prior_cost else None,
+                           prior_depth=self.prefill if self.semidense and self.prior_quadtree else None)
         if self._need_add_reference(T_curr_world):
             self.refs.insert(0, (img.copy(), T_curr_world.copy()))
             if len(self.refs) > self.max_refs:

@@ -686,6 +686,13 @@ void fuse_update_impl(int W, int H, const int32_t* filled, const Real* seeds, co
 // Hc x Wc x N; seeds H x W x 4; transforms 3x4 row-major float[12].
 // ---------------------------------------------------------------------------
 extern "C" {
+// qm_prior.cpp
+void qmo_quadtree_prior(const float* prior_depth, int W, int H, uint8_t* levels);
+void qmo_prior_to_cost(const float* prior, int W, int H, const uint8_t* levels, int N, double imin, double imax,
+                       float* cost, int32_t* count);
+}
+
+extern "C" {
 
 int qmo_version() { return 1; }
 
@@ -811,11 +818,12 @@ void qmo_compose_sparse_f64(int W, int H, const double* depth_coarse, const int3
 }
 
 // One keyframe through every stage (src/seed_matrix.cu:448-551 then :553-597),
-// FP32. stage_ms receives quadtree, cost, bp, upsample, fusion wall times.
+// FP32; prior / prior_depth (nullable) switch on the prior-driven alternates
+// of qm_prior.cpp after the quadtree and the cost volume. stage_ms receives quadtree, cost, bp, upsample, fusion wall times.
 // seeds is the persistent fusion state (updated: it receives the new seeds).
 void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int H, const float* T_refs, float fx,
                       float fy, float cx, float cy, int N, double imin, double imax, const float* prefill,
-                      float* seeds, const float* T_last_to_cur, float* depth_out, float* sparse_out,
+                      const float* prior, const float* prior_depth, float* seeds, const float* T_last_to_cur, float* depth_out, float* sparse_out,
                       uint8_t* levels_out, int32_t* label_out, double* stage_ms) {
   using clk = std::chrono::steady_clock;
   auto ms = [](clk::time_point a, clk::time_point b) {
@@ -847,8 +855,10 @@ void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int
   const int iters[4] = {4, 10, 10, 10};
   auto t0 = clk::now();
   quadtree_impl(key, W, H, W, levels);
+  if (prior_depth) qmo_quadtree_prior(prior_depth, W, H, levels);
   auto t1 = clk::now();
   cost_impl<float>(key, refs, W, H, W, T_refs, c, N, table.data(), levels, cost.data(), nullptr);
+  if (prior) qmo_prior_to_cost(prior, W, H, levels, N, imin, imax, cost.data(), nullptr);
   auto t2 = clk::now();
   bp_extract_impl<float>(cost.data(), W, H, levels, make_bp(N, 0.003f, 0.01f, iters, imin, imax), dcoarse.data(),
                          label, nullptr);

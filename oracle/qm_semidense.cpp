@@ -445,6 +445,49 @@ void project_depth(const std::vector<Seed>& seeds, int W, int H, const Cam& c, c
   }
 }
 
+// src/lsd_semidense.cu:241-294: the (inverse depth, variance) prior of the
+// income frame. Seeds with validity >= 10 project as above; the variance is
+// scaled by (depth / z)^2 and held at >= 0.01. The reference returns when both
+// values are finite (:290-291), which stores no finite prior at all; this
+// restatement keeps the finite ones, as that code intends. Nearest depth wins,
+// ties to the lower source index; pixels no seed reaches stay (0, 0).
+void project_prior(const std::vector<Seed>& seeds, int W, int H, const Cam& c, const float* k2i, const Frame& key,
+                   const Frame& inc, float* out /* H x W x 2 */) {
+  std::vector<uint64_t> best((size_t)W * H, ~0ull);
+  std::vector<float> val((size_t)W * H * 2, 0.0f);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      if (!interior(x, y, W, H)) continue;
+      const size_t i = (size_t)y * W + x;
+      const Seed& s = seeds[i];
+      if (!s.valid() || s.v[2] < 10.0f) continue;
+      F3 np;
+      float nid;
+      int nx, ny;
+      if (!project_seed(s, x, y, W, H, c, k2i, key, inc, np, nid, nx, ny)) continue;
+      const float nd = std::sqrt((np.x * np.x + np.y * np.y) + np.z * np.z);
+      const float vs = nd / np.z;
+      float var = (vs * vs) * s.v[7];
+      if (!(std::isfinite(nd) && std::isfinite(var))) continue;
+      if ((double)var < 0.01) var = (float)0.01;
+      val[2 * i] = 1.0f / nd;
+      val[2 * i + 1] = var;
+      uint32_t b;
+      std::memcpy(&b, &nd, 4);
+      const uint64_t k = ((uint64_t)b << 32) | (uint32_t)i;
+      uint64_t& t = best[(size_t)ny * W + nx];
+      if (k < t) t = k;
+    }
+  for (size_t i = 0; i < (size_t)W * H; ++i) {
+    out[2 * i] = 0.0f;
+    out[2 * i + 1] = 0.0f;
+    if (best[i] == ~0ull) continue;
+    const uint32_t src = (uint32_t)(best[i] & 0xFFFFFFFFull);
+    out[2 * i] = val[2 * (size_t)src];
+    out[2 * i + 1] = val[2 * (size_t)src + 1];
+  }
+}
+
 // src/lsd_semidense.cu:64-118 and src/seed_matrix.cu:247-304: propagate the old
 // keyframe's seeds into the income frame, initialise the new keyframe from
 // them, then regularise (removing occlusions), fill holes, regularise again
@@ -526,7 +569,8 @@ void qmo_sd_init(const float* grad, int W, int H, float* seeds) {
 // keyframe). T_key_world / T_inc_world are the frames' world -> camera poses.
 void qmo_sd_step(float* seeds, const uint8_t* key_img, const float* key_grad, const uint8_t* inc_img,
                  const float* inc_grad, int W, int H, int pitch, float fx, float fy, float cx, float cy,
-                 const float* T_key_world, const float* T_inc_world, int switch_keyframe, float* prefill) {
+                 const float* T_key_world, const float* T_inc_world, int switch_keyframe, float* prefill,
+                 float* prior /* H x W x 2 or null */) {
   const Cam c{fx, fy, cx, cy};
   const Frame key{key_img, key_grad, W, H, pitch};
   const Frame inc{inc_img, inc_grad, W, H, pitch};
@@ -552,6 +596,7 @@ void qmo_sd_step(float* seeds, const uint8_t* key_img, const float* key_grad, co
   regularize(s, W, H, false);
   std::memset(prefill, 0, sizeof(float) * n);
   project_depth(s, W, H, c, k2i, key, inc, prefill);
+  if (prior) project_prior(s, W, H, c, k2i, key, inc, prior);
   if (switch_keyframe) {
     std::vector<Seed> nk;
     new_keyframe(s, W, H, c, k2i, key, inc, nk);

@@ -1,7 +1,8 @@
 """Semi-dense keyframe seeds (the sparse map's pre-fill) on the GPU against
 the oracle's restatement of src/seed_init.cu and src/lsd_semidense.cu:
 gradient, the initial keyframe, and frame-by-frame updates with keyframe
-switches (propagation), seeds and pre-fill compared bit for bit."""
+switches (propagation), seeds, pre-fill and (inverse depth, variance) prior
+compared bit for bit."""
 import ctypes as C
 
 import numpy as np
@@ -56,6 +57,7 @@ def test_semidense_sequence_matches_oracle(motion, W, H, seed):
     assert np.array_equal(seeds.cpu().numpy(), ref_seeds)
     key_T, key_img, key_grad_ref = traj[0], imgs[0], kg_ref
     prefill = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    prior = torch.empty((H, W, 2), dtype=torch.float32, device="cuda")
     switched = 0
     for k in range(1, n):
         print(f"semidense {motion} {W}x{H} frame {k}", flush=True)
@@ -67,14 +69,17 @@ def test_semidense_sequence_matches_oracle(motion, W, H, seed):
         Ti = np.ascontiguousarray(traj[k], np.float32).reshape(12)
         check(lib().qm_sd_step(p(seeds), p(key_t), p(kg), p(inc_t), p(ig), W, H, W,
                                camv.ctypes.data_as(C.c_void_p), Tk.ctypes.data_as(C.c_void_p),
-                               Ti.ctypes.data_as(C.c_void_p), int(sw), p(prefill), stream()))
-        want_prefill = O.sd_step(ref_seeds, key_img, key_grad_ref, imgs[k], ig_ref, cam, key_T, traj[k], sw)
+                               Ti.ctypes.data_as(C.c_void_p), int(sw), p(prefill), p(prior), stream()))
+        want_prefill, want_prior = O.sd_step(ref_seeds, key_img, key_grad_ref, imgs[k], ig_ref, cam, key_T, traj[k],
+                                             sw, want_prior=True)
         got_seeds = seeds.cpu().numpy()
         bad = np.argwhere(~((got_seeds == ref_seeds) | (np.isnan(got_seeds) & np.isnan(ref_seeds))).all(axis=2))
         assert bad.size == 0, f"frame {k}: {len(bad)} seeds differ, first at {bad[:3].tolist()}"
         assert np.array_equal(prefill.cpu().numpy(), want_prefill), f"frame {k}: pre-fill differs"
+        assert np.array_equal(prior.cpu().numpy(), want_prior), f"frame {k}: (inverse depth, variance) prior differs"
         if sw:
             switched += 1
             key_t, kg, key_T, key_img, key_grad_ref = inc_t, ig, traj[k], imgs[k], ig_ref
     assert switched >= 2
     assert (prefill.cpu().numpy() > 0).sum() > 100
+    assert (prior.cpu().numpy()[..., 0] > 0).sum() > 100

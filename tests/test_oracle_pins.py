@@ -632,3 +632,216 @@ def test_local_upsample_matches_scalar_loop():
     m = rng.random((H // 4, W // 4)) < 0.5
     sparse[0::4, 0::4][m] = rng.uniform(0.5, 9.0, m.sum()).astype(np.float32)
     assert np.array_equal(O.upsample_local(sparse, img), _local_upsample_scalar(sparse, img), equal_nan=True)
+
+
+# ---------------------------------------------------------------------------
+# Prior-driven alternates (oracle/qm_prior.cpp).
+# quadtree_depth_kernal: simulated thread by thread (src/quadtree.cu:79-154),
+# shared atomics applied in thread raster order, plus closed forms.
+# ---------------------------------------------------------------------------
+def quadtree_prior_simulated(prior_depth, levels):
+    H, W = levels.shape
+    out = levels.copy()
+    f32 = np.float32
+    for ty in range(0, H, 16):
+        for tx in range(0, W, 16):
+            threads = [(lx, ly) for ly in range(16) for lx in range(16) if tx + lx < W and ty + ly < H]
+            inv, pin, num = {}, {}, {}
+            all_valid = True
+            for p in threads:
+                d = f32(prior_depth[ty + p[1], tx + p[0]])
+                valid = bool(d > 0.0)
+                inv[p] = f32(f32(1.0) / d) if valid else f32(0.0)
+                pin[p], num[p] = inv[p], 1
+                all_valid &= valid
+            if not all_valid:
+                continue
+            level = {p: 0 for p in threads}
+            alive = set(threads)
+            for i in range(1, 5):
+                s, h = 1 << i, 1 << (i - 1)
+                rep = {p: (p[0] - p[0] % s, p[1] - p[1] % s) for p in alive}
+                for p in sorted(alive, key=lambda q: (q[1], q[0])):  # thread raster order
+                    if p[0] % h == 0 and p[1] % h == 0 and rep[p] != p:
+                        pin[rep[p]] = f32(pin[rep[p]] + pin[p])
+                        num[rep[p]] = num.get(rep[p], 0) + num[p]
+                approve = {rep[p]: True for p in alive}
+                still = {p for p in alive if num.get(rep[p], 0) == s * s}
+                for p in still:
+                    avg = f32(pin[rep[p]] / f32(s * s))
+                    if abs(f32(inv[p] - avg)) > f32(0.01):
+                        approve[rep[p]] = False
+                nxt = set()
+                for p in still:
+                    if approve[rep[p]]:
+                        level[p] = i
+                        nxt.add(p)
+                    else:
+                        num[rep[p]] = 0
+                alive = nxt
+            for (lx, ly), lv in level.items():
+                out[ty + ly, tx + lx] = max(int(out[ty + ly, tx + lx]), lv)
+    return out
+
+
+def _prior_depth_maps():
+    rng = np.random.default_rng(9)
+    maps = []
+    # piecewise-planar inverse depth with noise, some holes, ragged sizes
+    for k, (H, W) in enumerate([(48, 64), (37, 45), (32, 50)]):
+        yy, xx = np.mgrid[0:H, 0:W]
+        inv = 0.5 + 0.002 * (xx // 8) + 0.0015 * (yy // 8) + (xx >= W // 2) * 0.05
+        inv = inv + rng.normal(0, 0.003, (H, W)) * (rng.random((H, W)) < 0.2)
+        d = (1.0 / inv).astype(np.float32)
+        if k == 1:
+            d[rng.random((H, W)) < 0.002] = 0.0
+        maps.append(d)
+    return maps
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_quadtree_prior_matches_kernel_simulation(k):
+    d = _prior_depth_maps()[k]
+    H, W = d.shape
+    rng = np.random.default_rng(k)
+    levels = rng.integers(2, 5, (H, W)).astype(np.uint8)
+    got = O.quadtree_prior(d, levels)
+    assert np.array_equal(got, quadtree_prior_simulated(d, levels))
+    assert np.all(got >= levels) and np.any(got > levels)
+    # with zero image levels the full depth-level map (0..4) shows
+    zero = np.zeros((H, W), np.uint8)
+    got0 = O.quadtree_prior(d, zero)
+    assert np.array_equal(got0, quadtree_prior_simulated(d, zero))
+    assert len(np.unique(got0)) >= 3
+
+
+def test_quadtree_prior_closed_forms():
+    H, W = 32, 48
+    base = np.full((H, W), 2, np.uint8)
+    # constant prior: every complete 16x16 block is flat -> level 4
+    assert np.all(O.quadtree_prior(np.full((H, W), 3.0, np.float32), base) == 4)
+    # inverse depth ramp of slope 1/512 per pixel: a 2^i block deviates by
+    # (2^i - 1)/1024 from its mean: <= 0.01 up to 8x8, 15/1024 > 0.01 at 16x16
+    yy, xx = np.mgrid[0:H, 0:W]
+    inv = (0.25 + xx / 512.0).astype(np.float32)
+    assert np.all(O.quadtree_prior((1.0 / inv).astype(np.float32), base) == 3)
+    # one pixel without prior (0, negative or NaN) keeps its whole tile unchanged
+    for bad in (0.0, -1.0, np.nan):
+        d = np.full((H, W), 3.0, np.float32)
+        d[5, 20] = bad
+        got = O.quadtree_prior(d, base)
+        assert np.all(got[0:16, 16:32] == 2) and np.all(got[16:32, :] == 4) and np.all(got[0:16, 0:16] == 4)
+    # ragged edge: blocks reaching past the image never complete
+    d = np.full((20, 24), 3.0, np.float32)
+    got = O.quadtree_prior(d, np.full((20, 24), 2, np.uint8))
+    assert np.all(got[:16, :16] == 4) and np.all(got[16:, :] == 2) and np.all(got[:16, 16:] == 3)
+
+
+# prior_to_cost: the reference's atomics replayed in thread order (prior pixels
+# in raster order, then the image cost), add_num counted per sample thread as
+# the 64 x 1 and 64 x 5 launches do, then normalize_the_cost; plus closed forms.
+def _prior_to_cost_replay(prior, levels, cost_in, cnt_in, N, imin, imax):
+    H, W = levels.shape
+    Hc, Wc = H // 4, W // 4
+    f32 = np.float32
+    step = (imax - imin) / (N - 1)
+    acc = np.zeros((Hc, Wc, N), f32)
+    num = np.zeros((Hc, Wc), np.int64)
+    for y in range(H):  # prior_to_cost: one block per pixel, one thread per sample
+        for x in range(W):
+            if x >= W - 3 or y >= H - 3 or x <= 2 or y <= 2:
+                continue
+            p, v = prior[y, x]
+            if p <= 0 or v <= 0:
+                continue
+            for d in range(N):
+                s = f32(step * d + imin)
+                c = f32(0.1 * float(f32(s - p)) * float(f32(s - p)) / float(v))
+                c = c if c < 0.5 else f32(0.5)
+                acc[y // 4, x // 4, d] = f32(acc[y // 4, x // 4, d] + c)
+                num[y // 4, x // 4] += 1
+    out = np.zeros_like(cost_in)
+    for cy in range(Hc):  # image_to_cost adds the raw image cost, then normalize_the_cost
+        for cx in range(Wc):
+            raw = cost_in[cy, cx] * f32(64.0)  # the cost oracle's volume is raw / 64
+            a = acc[cy, cx] + raw if cnt_in[cy, cx] else acc[cy, cx]
+            n = num[cy, cx] + (64 if cnt_in[cy, cx] else 0)
+            out[cy, cx] = a / f32(n) if n > 0 else a
+    return out, (num + cnt_in).astype(np.int32)
+
+
+def _prior_case(H, W, seed):
+    rng = np.random.default_rng(seed)
+    key = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    lv = O.quadtree(key)
+    T = np.tile(np.eye(3, 4, dtype=np.float32), (4, 1, 1))
+    T[:, 0, 3] = [0.05, 0.05, -0.04, 0.03]
+    cost, cnt = O.cost_volume(key, [np.roll(key, s, axis=1) for s in (1, 1, 2, -1)], T, CAM, 64, 0.02, 2.0, lv)
+    prior = np.zeros((H, W, 2), np.float32)
+    m = rng.random((H, W)) < 0.3
+    prior[m, 0] = rng.uniform(0.02, 2.0, m.sum())
+    prior[m, 1] = rng.uniform(0.005, 0.5, m.sum())
+    prior[rng.random((H, W)) < 0.02] = [np.nan, 0.1]
+    prior[rng.random((H, W)) < 0.02] = [0.5, -1.0]
+    return prior, lv, cost, cnt
+
+
+def test_prior_to_cost_matches_atomic_replay():
+    prior, lv, cost, cnt = _prior_case(24, 32, 5)
+    got_c, got_n = O.prior_to_cost(prior, lv, cost, cnt, 64, 0.02, 2.0)
+    want_c, want_n = _prior_to_cost_replay(prior, lv, cost, cnt, 64, 0.02, 2.0)
+    assert np.array_equal(got_n, want_n)
+    assert np.array_equal(got_c, want_c)
+
+
+def test_prior_to_cost_closed_forms():
+    H, W = 32, 48
+    levels = np.full((H, W), 4, np.uint8)  # nodes only at multiples of 16 ...
+    levels[8, 12] = 2                      # ... and at cell (3, 2)
+    Hc, Wc, N = H // 4, W // 4, 64
+    cost = np.zeros((Hc, Wc, N), np.float32)
+    cnt = np.zeros((Hc, Wc), np.int32)
+    cost[2, 3] = np.linspace(0.0, 0.01, N, dtype=np.float32)
+    cnt[2, 3] = 64
+    prior = np.zeros((H, W, 2), np.float32)
+    prior[9, 13] = [0.8, 0.05]      # one pixel in node cell (3, 2)
+    prior[16:20, 20:24] = [0.3, 0.2]  # the full 4 x 4 block of non-node cell (5, 4)
+    prior[1, 1] = [0.5, 0.1]        # border pixel: ignored
+    prior[25, 30] = [np.nan, 0.1]   # NaN passes the "<= 0" tests and truncates to 0.5
+    got, n = O.prior_to_cost(prior, levels, cost, cnt, N, 0.02, 2.0)
+    s = (0.02 + (2.0 - 0.02) / (N - 1) * np.arange(N)).astype(np.float32).astype(np.float64)
+    q = lambda p, v: np.minimum(0.1 * (s - p) ** 2 / v, 0.5)
+    want = (q(np.float32(0.8), np.float32(0.05)) + 64.0 * cost[2, 3]) / 128.0
+    np.testing.assert_allclose(got[2, 3], want, rtol=2e-6, atol=1e-9)
+    np.testing.assert_allclose(got[4, 5], q(np.float32(0.3), np.float32(0.2)) / 64.0, rtol=2e-6)
+    assert np.all(got[6, 7] == np.float32(0.5 / 64.0))
+    assert (n[2, 3], n[4, 5], n[6, 7], n[0, 0], n[4, 4]) == (128, 1024, 64, 0, 0)
+    untouched = np.ones((Hc, Wc), bool)
+    untouched[2, 3] = untouched[4, 5] = untouched[6, 7] = False
+    assert np.array_equal(got[untouched], cost[untouched])
+
+
+def test_sd_prior_projection_identity_pose():
+    """Under the identity pose seeds with validity >= 10 give the prior
+    (s_idepth / |K^-1 (x, y, 1)|, max(|K^-1 (x, y, 1)|^2 s_var, 0.01)) at their
+    own pixel where the gradient is at least 5, and (0, 0) elsewhere."""
+    from open_quadtree_mapping_amd.scene import Camera
+    H, W = 40, 48
+    cam = Camera(W, H, 60.0, 60.0, W / 2, H / 2)
+    img = _textured(H, W, 2)
+    g = O.sd_gradient(img)
+    s = _static_seeds(H, W, 0.25, 0.001, 12.0)
+    s[..., 7] = np.where(np.arange(W)[None, :] < W // 2, 0.004, 0.03).astype(np.float32)
+    T = np.eye(3, 4, dtype=np.float32)
+    pre, prior = O.sd_step(s, img, g, img, g, cam, T, T, False, want_prior=True)
+    assert np.all(pre == 0)  # validity 12 < 20: nothing in the depth pre-fill
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    r2 = ((xx - cam.cx) / cam.fx) ** 2 + ((yy - cam.cy) / cam.fy) ** 2 + 1.0
+    strong = (g[..., 0] ** 2 + g[..., 1] ** 2) >= 25.0
+    inner = np.zeros((H, W), bool)
+    inner[5:H - 5, 5:W - 5] = True
+    m = strong & inner
+    assert m.sum() > 100
+    np.testing.assert_allclose(prior[m, 0], (s[..., 6] / np.sqrt(r2))[m], rtol=1e-6)
+    np.testing.assert_allclose(prior[m, 1], np.maximum(r2 * s[..., 7], 0.01)[m], rtol=2e-6)
+    assert np.all(prior[~strong] == 0)
