@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_quadtree(const FrameDev* __restrict__ f
 // round((px + u2u*i) + v2u*j, (py + u2v*i) + v2v*j), with round(v) =
 // (int)((double)v + 0.5) and the texel index clamped to the image. The SAD of
 // integer grey levels is an integer below 2^24, so it is accumulated exactly
-// in an integer register (v_sad_u16) and converted once.
+// in an integer register (v_sad_u8 on zero-extended bytes) and converted once.
 // When the centre texel lies in [2, W-3] x [2, H-3] and the rotation
 // terms satisfy |u2u| + |v2u| <= 1.75 and |u2v| + |v2v| <= 1.75 (true for any
 // rotation), every tap lies in [-0.25, W-0.75) (the float error of the two
@@ -146,27 +146,39 @@ __device__ __forceinline__ int cvt_rpi(float v) {
   return r;
 }
 
-template <bool FAST>
-__device__ __forceinline__ float patch_sad(gptr<const uint8_t> ref, int W, int H, int pitch, float px, float py,
-                                           const float* Tr, const unsigned (&patch)[3][3]) {
+// Fast path: the three taps of a patch row (fixed j) usually land on three
+// consecutive texels of one image row. Such a row is read with one unaligned
+// dword load (its fourth byte masked off) instead of three byte gathers, which
+// cuts the texture-address work that bounds this kernel; lanes whose row is not
+// of that shape, or whose dword would reach past the image, gather the three
+// bytes instead. Either way the SAD is the same integer.
+typedef unsigned u32_unaligned __attribute__((aligned(1)));
+
+// Offsets y * pitch + x of the nine taps, [j][i] for tap (i - 1, j - 1), on
+// the fast path (no clamp).
+__device__ __forceinline__ void tap_offsets(float px, float py, const float* Tr, int pitch, unsigned (&o)[3][3]) {
   const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
-  if (FAST) {
-    unsigned sad = 0;
+  float bx[3], by[3];
 #pragma unroll
-    for (int i = -1; i <= 1; ++i) {
-      // u2u * i is exact (+-u2u or a signed zero, which the rounding ignores)
-      const float bx = i == 0 ? px : px + (i < 0 ? -u2u : u2u);
-      const float by = i == 0 ? py : py + (i < 0 ? -u2v : u2v);
-#pragma unroll
-      for (int j = -1; j <= 1; ++j) {
-        const float sx = j == 0 ? bx : bx + (j < 0 ? -v2u : v2u);
-        const float sy = j == 0 ? by : by + (j < 0 ? -v2v : v2v);
-        const unsigned off = __umul24((unsigned)cvt_rpi(sy), (unsigned)pitch) + (unsigned)cvt_rpi(sx);
-        sad = __builtin_amdgcn_sad_u16(patch[i + 1][j + 1], (unsigned)ref[off], sad);
-      }
-    }
-    return (float)sad;
+  for (int i = -1; i <= 1; ++i) {
+    // u2u * i is exact (+-u2u or a signed zero, which the rounding ignores)
+    bx[i + 1] = i == 0 ? px : px + (i < 0 ? -u2u : u2u);
+    by[i + 1] = i == 0 ? py : py + (i < 0 ? -u2v : u2v);
   }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float sx = j == 1 ? bx[i] : bx[i] + (j < 1 ? -v2u : v2u);
+      const float sy = j == 1 ? by[i] : by[i] + (j < 1 ? -v2v : v2v);
+      o[j][i] = __umul24((unsigned)cvt_rpi(sy), (unsigned)pitch) + (unsigned)cvt_rpi(sx);
+    }
+}
+
+// General path: clamped taps, float SAD in the reference's tap order.
+__device__ __forceinline__ float patch_sad_clamped(gptr<const uint8_t> ref, int W, int H, int pitch, float px,
+                                                   float py, const float* Tr, const unsigned (&patch)[3][3]) {
+  const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
   float sad = 0.0f;
 #pragma unroll
   for (int j = -1; j <= 1; ++j) {
@@ -220,13 +232,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
 #pragma unroll
       for (int r = 0; r < 4; ++r) ref[r] = gp(F.ref[r]);
       const gptr<const uint8_t> key = gp(F.key);
-      unsigned patch[3][3];
+      unsigned patch[3][3], prow[3];
 #pragma unroll
       for (int j = -1; j <= 1; ++j)
 #pragma unroll
         for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = key[(y + j) * pitch + (x + i)];
+      // patch row j as bytes i = -1, 0, 1 (little-endian, like a texel dword)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) prow[j] = patch[0][j] | (patch[1][j] << 8) | (patch[2][j] << 16);
       const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
       const float wlim = (float)W - 2.5f, hlim = (float)H - 2.5f;  // exact for W, H <= 16384
+      const unsigned last4 = (unsigned)H * (unsigned)pitch - 4u;  // last offset a dword may start at
 #pragma unroll 2
       for (int t = 0; t < kPerWave; ++t) {
         const float depth = depth_table[dbase + t];
@@ -234,7 +250,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
         P.x = dir.x * depth;
         P.y = dir.y * depth;
         P.z = dir.z * depth;
-        float cr[4], nr[4];
+        // all four projections and view tests first, then all taps: the texel
+        // loads and their SADs then sit in one basic block. nr[r] is 1 for a
+        // reference that sees the sample, else 0.
+        float pxs[4], pys[4], nr[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const V3 q = se3_apply(T[r], P);
@@ -242,21 +261,61 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
           const float pyf = (fy * q.y) / q.z + cy;
           // 2 <= (int)((double)p + 0.5) < W - 2  <=>  1.5 <= p < W - 2.5 (false for NaN)
           const bool in = pxf >= 1.5f && pxf < wlim && pyf >= 1.5f && pyf < hlim;
-          if (FAST) {
-            // branch-free: lanes out of view sample around (2, 2), always in bounds,
-            // and drop the result, so the taps of all four references issue together
-            const float sad = patch_sad<true>(ref[r], W, H, pitch, in ? pxf : 2.0f, in ? pyf : 2.0f, T[r], patch);
-            cr[r] = in ? sad : 0.0f;
-          } else {
-            cr[r] = in ? patch_sad<false>(ref[r], W, H, pitch, pxf, pyf, T[r], patch) : 0.0f;
-          }
           nr[r] = in ? 1.0f : 0.0f;
+          // out of view, the fast path samples around (2, 2), always in bounds,
+          // and drops the result, so the taps of all references issue together
+          pxs[r] = (FAST && !in) ? 2.0f : pxf;
+          pys[r] = (FAST && !in) ? 2.0f : pyf;
+        }
+        float cr[4];
+        if (FAST) {
+          // Tap offsets of every reference, then one dword per patch row where
+          // the row's three taps are consecutive bytes (offsets o, o + 1, o + 2,
+          // with o + 4 inside the image), and byte gathers, in a divergent
+          // branch, for the rows of other lanes; t keeps the offsets, replaced
+          // by the texels in such rows. All loads are issued before any SAD.
+          unsigned t[4][3][3], v[4][3];
+          bool rw[4][3];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            tap_offsets(pxs[r], pys[r], T[r], pitch, t[r]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              rw[r][j] = (t[r][j][1] == t[r][j][0] + 1u) & (t[r][j][2] == t[r][j][0] + 2u) & (t[r][j][0] <= last4);
+              v[r][j] = *reinterpret_cast<gptr<const u32_unaligned>>(ref[r] + (rw[r][j] ? t[r][j][0] : 0u));
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              if (!rw[r][j]) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) t[r][j][i] = ref[r][t[r][j][i]];
+              }
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            unsigned sad = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              const unsigned a = __builtin_amdgcn_sad_u8(v[r][j] & 0xFFFFFFu, prow[j], 0u);
+              const unsigned b = __builtin_amdgcn_sad_u8(
+                  t[r][j][2], patch[2][j],
+                  __builtin_amdgcn_sad_u8(t[r][j][1], patch[1][j], __builtin_amdgcn_sad_u8(t[r][j][0], patch[0][j], 0u)));
+              sad += rw[r][j] ? a : b;
+            }
+            cr[r] = nr[r] != 0.0f ? (float)sad : 0.0f;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            cr[r] = nr[r] != 0.0f ? patch_sad_clamped(ref[r], W, H, pitch, pxs[r], pys[r], T[r], patch) : 0.0f;
         }
         const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
         const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
         // normalize_the_cost divides by add_num, which each of the 64 sample
         // threads of a node's block incremented once: an exact / 64
-        trow[t] = (n > 0.0f ? (tot / n) / 255.0f : 100.0f) / 64.0f;
+        // branch-free: the quotient is computed for every lane (n >= 1 where used)
+        const float q = (tot / fmaxf(n, 1.0f)) / 255.0f;
+        trow[t] = (n > 0.0f ? q : 100.0f) / 64.0f;
       }
     }
   }
