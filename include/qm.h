@@ -52,7 +52,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 5
+#define QM_ABI_VERSION 6
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -269,7 +269,10 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
 #define QM_TUNE_COST_WAVES 0   /* 2, 4 or 8 waves per cost-volume block of 64 cells */
 #define QM_TUNE_BP_ROWS 1      /* 4, 8 or 16 cells (16-lane rows) per BP half-sweep block */
 #define QM_TUNE_EXTRACT_ROWS 2 /* 4, 8 or 16 cells per winner-extraction block */
-#define QM_NUM_TUNE 3
+#define QM_TUNE_OVERLAP 3      /* 0: one stream; 1, 2, 3: the batch as two frame groups on two
+                                  streams, the second starting with the first, after its cost
+                                  volume, or after its BP (results identical in every case) */
+#define QM_NUM_TUNE 4
 int qm_batch_set_tuning(qm_batch* b, int knob, int value);
 int qm_batch_get_tuning(qm_batch* b, int knob, int* value);
 /* Autotuner: on this batch's own frames, time every value of every knob

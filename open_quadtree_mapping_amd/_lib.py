@@ -17,7 +17,7 @@ D = C.c_double
 
 STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", "upsample_col", "fuse_warp",
           "fuse_update")
-TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows")  # QM_TUNE_* order
+TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows", "overlap")  # QM_TUNE_* order
 
 
 class QmError(RuntimeError):
@@ -82,7 +82,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 5  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 6  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {

@@ -215,6 +215,9 @@ struct qm_batch {
   int parity = 0;
   int runs = 0;
   Tuning tune;
+  // second stream and fork / mark / join events of the two-group overlap
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_mark = nullptr, ev_join = nullptr;
   std::vector<hipEvent_t> events;
   // backward (per-stream lambda), allocated on the first qm_batch_backward
   void* bwd_arena = nullptr;
@@ -274,25 +277,39 @@ bool any_prior_depth(const qm_batch* b) {
   return false;
 }
 
-void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
-  const FrameDev* df = b->df[par];
-  const FrameDev* hf = b->hf[par].data();
-  const int n = b->n, N = b->p.n_hyp;
+// The stages of frames [f0, f0 + nf) in order; when mark is given it is
+// recorded on s once the stage numbered mark_after (0 quadtree, 1 cost volume,
+// 2 pyramid, 3 BP) has been enqueued.
+void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr, int f0 = 0, int nf = -1,
+                int mark_after = -1, hipEvent_t mark = nullptr) {
+  const FrameDev* df = b->df[par] + f0;
+  const FrameDev* hf = b->hf[par].data() + f0;
+  const int n = nf < 0 ? b->n : nf, N = b->p.n_hyp;
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
-  const bool prior_depth = any_prior_depth(b), prior = any_prior(b);
+  bool prior_depth = false, prior = false;
+  for (int i = 0; i < n; ++i) {
+    prior_depth |= hf[i].prior_depth != nullptr;
+    prior |= hf[i].prior != nullptr;
+  }
+  auto marker = [&](int stage) {
+    if (mark && mark_after == stage) (void)hipEventRecord(mark, s);
+  };
   {
     Mark m(t, QM_STAGE_QUADTREE);
     launch_quadtree(df, hf, n, s);
     if (prior_depth) launch_quadtree_prior(df, hf, n, s);
   }
+  marker(0);
   const Tuning& tu = b->tune;
   {
     Mark m(t, QM_STAGE_COST);
     launch_cost(df, hf, n, N, b->table, false, tu, s);
     if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
   }
+  marker(1);
   { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
+  marker(2);
   for (int l = 3; l >= 0; --l) {
     const int iters = p->bp_iters[l];
     if (iters == 0) {
@@ -305,11 +322,48 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
       launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s);
     }
   }
+  marker(3);
   { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, tu, s); }
   { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, s); }
   { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, true, s); }
   { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
   { Mark m(t, QM_STAGE_FUSE_UPDATE); launch_fuse_update(df, hf, n, s); }
+}
+
+// One pipeline run. With the overlap setting on (and two frames or more) the
+// batch runs as two groups on two streams, the second forked from s and
+// joined back into it: its kernels fill the GPU while the first group's
+// latency-bound ones (coarse BP levels, the serial upsample chains) run, and
+// the reverse. overlap 1 starts both groups together; 2 starts the second
+// group once the first group's cost volume is enqueued, 3 once its BP is.
+// the second stream and the events, created once (outside any capture)
+int ensure_side(qm_batch* b) {
+  if (b->side) return QM_OK;
+  QM_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  QM_HIP(hipEventCreateWithFlags(&b->ev_mark, hipEventDisableTiming));
+  QM_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  QM_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  return QM_OK;
+}
+
+int run_pipeline(qm_batch* b, int par, hipStream_t s) {
+  const int ov = b->tune.overlap;
+  if (ov == 0 || b->n < 2) {
+    run_stages(b, par, s);
+    return QM_OK;
+  }
+  int rc = ensure_side(b);
+  if (rc) return rc;
+  const int na = (b->n + 1) / 2, nb = b->n - na;
+  const int mark_after = ov == 2 ? 1 : (ov == 3 ? 3 : -1);
+  QM_HIP(hipEventRecord(b->ev_fork, s));
+  QM_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+  run_stages(b, par, s, nullptr, 0, na, mark_after, b->ev_mark);
+  if (mark_after >= 0) QM_HIP(hipStreamWaitEvent(b->side, b->ev_mark, 0));
+  run_stages(b, par, b->side, nullptr, na, nb);
+  QM_HIP(hipEventRecord(b->ev_join, b->side));
+  QM_HIP(hipStreamWaitEvent(s, b->ev_join, 0));
+  return QM_OK;
 }
 
 void apply_desc(FrameDev& F, const qm_frame_desc& d, const qm_params& p) {
@@ -911,16 +965,22 @@ int qm_batch_run(qm_batch* b, int mode, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int par = b->parity;
   if (mode == 0) {
-    run_stages(b, par, s);
+    int rc = run_pipeline(b, par, s);
+    if (rc) return rc;
     QM_HIP(hipGetLastError());
   } else if (mode == 1) {
     if (!b->exec[par]) {
+      if (b->tune.overlap != 0) {
+        int rc = ensure_side(b);
+        if (rc) return rc;
+      }
       hipStream_t cs;
       QM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        run_stages(b, par, cs);
-        e = hipStreamEndCapture(cs, &b->graph[par]);
+        const int rc = run_pipeline(b, par, cs);
+        const hipError_t ee = hipStreamEndCapture(cs, &b->graph[par]);
+        e = rc ? hipErrorUnknown : ee;
       }
       if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[par], b->graph[par], nullptr, nullptr, 0);
       (void)hipStreamDestroy(cs);
@@ -982,6 +1042,7 @@ static int* tune_slot(Tuning& t, int knob) {
     case QM_TUNE_COST_WAVES: return &t.cost_waves;
     case QM_TUNE_BP_ROWS: return &t.bp_rows;
     case QM_TUNE_EXTRACT_ROWS: return &t.extract_rows;
+    case QM_TUNE_OVERLAP: return &t.overlap;
     default: return nullptr;
   }
 }
@@ -1001,6 +1062,7 @@ int qm_batch_set_tuning(qm_batch* b, int knob, int value) {
   int* slot = tune_slot(b->tune, knob);
   if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
   const bool ok = knob == QM_TUNE_COST_WAVES ? (value == 2 || value == 4 || value == 8)
+                  : knob == QM_TUNE_OVERLAP  ? (value >= 0 && value <= 3)
                                              : (value == 4 || value == 8 || value == 16);
   if (!ok) return fail(QM_ERR_INVALID, "value not offered for this knob");
   if (*slot != value) {
@@ -1050,7 +1112,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     std::vector<int> values;
     std::function<void(const Tuning&)> stage;
   };
-  const Knob knobs[QM_NUM_TUNE] = {
+  const Knob knobs[3] = {
       {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, t, s); }},
       {QM_TUNE_BP_ROWS, {4, 8, 16}, [&](const Tuning& t) { run_sweeps(df, hf, n, &b->p, t, s); }},
       {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
@@ -1071,6 +1133,48 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     }
     if (rc) break;
     if (best_ms) best_ms[k.id] = best_t;
+  }
+  // the overlap knob: whole pipeline runs (plain launches), which move the
+  // fusion state, so both seed buffers of every frame are saved first and
+  // restored afterwards (the runs' depth / dense / sparse outputs are not)
+  if (!rc) {
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += 2 * (size_t)hf[i].W * hf[i].H * sizeof(float4);
+    void* snap = nullptr;
+    rc = dev_alloc(&snap, total);
+    if (!rc) {
+      auto copy_seeds = [&](bool save) -> int {
+        char* p = (char*)snap;
+        for (int i = 0; i < n; ++i) {
+          const size_t bytes = (size_t)hf[i].W * hf[i].H * sizeof(float4);
+          for (int q = 0; q < 2; ++q, p += bytes)
+            QM_HIP(hipMemcpyAsync(save ? (void*)p : (void*)b->seeds[q][i], save ? (void*)b->seeds[q][i] : (void*)p,
+                                  bytes, hipMemcpyDeviceToDevice, s));
+        }
+        return QM_OK;
+      };
+      rc = copy_seeds(true);
+      const Tuning keep = b->tune;
+      double best_t = 1e30;
+      for (int v = 0; v <= 3 && !rc; ++v) {
+        Tuning t = best;
+        t.overlap = v;
+        double ms = 0.0;
+        rc = time([&](const Tuning& tt) { b->tune = tt; (void)run_pipeline(b, par, s); }, t, &ms);
+        if (!rc && ms < best_t) {
+          best_t = ms;
+          best.overlap = v;
+        }
+      }
+      b->tune = keep;
+      if (!rc) rc = copy_seeds(false);
+      if (!rc) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+      }
+      dev_free(snap);
+      if (best_ms) best_ms[QM_TUNE_OVERLAP] = best_t;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1109,6 +1213,12 @@ int qm_batch_reset_seeds(qm_batch* b) {
 int qm_batch_destroy(qm_batch* b) {
   if (!b) return QM_OK;
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  if (b->side) {
+    (void)hipStreamSynchronize(b->side);
+    (void)hipStreamDestroy(b->side);
+  }
+  for (hipEvent_t e : {b->ev_fork, b->ev_mark, b->ev_join})
+    if (e) (void)hipEventDestroy(e);
   for (int par = 0; par < 2; ++par) {
     if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
     if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);

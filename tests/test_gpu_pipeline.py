@@ -119,7 +119,7 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
 
     b = Batch(frames, p, outputs=("sparse", "dense"))
     want = run(b)
-    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16)}
+    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16), "overlap": (0, 1, 2, 3)}
     for knob, values in offered.items():
         for v in values:
             print(f"launch shape {knob}={v}", flush=True)
@@ -127,7 +127,7 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
             got = run(b)
             for (gs, gd), (ws, wd) in zip(got, want):
                 assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True), f"{knob}={v}"
-        b.set_tuning(**{knob: values[1] if knob == "cost_waves" else 16})
+        b.set_tuning(**{knob: {"cost_waves": 4, "overlap": 0}.get(knob, 16)})
     res = b.autotune(reps=3)
     print(f"autotune: {res}", flush=True)
     for knob, (v, ms) in res.items():
@@ -136,4 +136,49 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
     got = run(b)
     for (gs, gd), (ws, wd) in zip(got, want):
         assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True)
+    b.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_two_group_overlap_matches_one_stream_over_keyframes(graph):
+    """The batch run as two frame groups on two streams (every overlap
+    setting) gives the one-stream results over consecutive keyframes, so the
+    fusion state carried between runs is the same; the autotuner's whole-run
+    timings leave that state untouched."""
+    sizes = [(376, 240, 31), (752, 480, 32), (376, 240, 33)]  # odd count: groups of 2 and 1
+    streams = [stream_frames(W, H, s) for (W, H, s) in sizes]
+    dev_imgs = [[torch.from_numpy(im).to(DEV) for im in imgs] for (_, _, imgs) in streams]
+    p = default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0)
+
+    def frames_at(k):
+        return [keyframe_inputs(cam, traj, imgs, k, dev_imgs[i])[0] for i, (cam, traj, imgs) in enumerate(streams)]
+
+    def sequence(b, autotune_after_first=False):
+        b.reset_seeds()
+        out = []
+        for k in range(4, 7):
+            for i, fr in enumerate(frames_at(k)):
+                b.set_frame(i, fr)
+            b.run(graph=graph)
+            torch.cuda.synchronize()
+            out.append([o["depth"].cpu().numpy() for o in b.outputs])
+            if autotune_after_first and k == 4:
+                b.autotune(reps=1)
+        return out
+
+    b = Batch(frames_at(4), p, outputs=("depth",))
+    b.set_tuning(overlap=0)
+    want = sequence(b)
+    for ov in (1, 2, 3):
+        print(f"overlap={ov} graph={graph}", flush=True)
+        b.set_tuning(overlap=ov)
+        got = sequence(b)
+        for k, (g, w) in enumerate(zip(got, want)):
+            for i in range(len(sizes)):
+                assert np.array_equal(g[i], w[i], equal_nan=True), f"overlap {ov} keyframe {k} stream {i}"
+    print(f"autotune between keyframes graph={graph}", flush=True)
+    got = sequence(b, autotune_after_first=True)
+    for k, (g, w) in enumerate(zip(got, want)):
+        for i in range(len(sizes)):
+            assert np.array_equal(g[i], w[i], equal_nan=True), f"after autotune: keyframe {k} stream {i}"
     b.close()
