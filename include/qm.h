@@ -271,7 +271,10 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
 #define QM_TUNE_EXTRACT_ROWS 2 /* 4, 8 or 16 cells per winner-extraction block */
 #define QM_TUNE_OVERLAP 3      /* 0: one stream; 1, 2, 3: the batch as two frame groups on two
                                   streams, the second starting with the first, after its cost
-                                  volume, or after its BP (results identical in every case) */
+                                  volume, or after its BP; 4, 5, 6: three, four or eight
+                                  groups, each starting after the previous group's cost volume;
+                                  7: four groups, each after the previous group's BP (results
+                                  identical in every case) */
 #define QM_NUM_TUNE 4
 int qm_batch_set_tuning(qm_batch* b, int knob, int value);
 int qm_batch_get_tuning(qm_batch* b, int knob, int* value);

@@ -215,9 +215,11 @@ struct qm_batch {
   int parity = 0;
   int runs = 0;
   Tuning tune;
-  // second stream and fork / mark / join events of the two-group overlap
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_mark = nullptr, ev_join = nullptr;
+  // extra streams and fork / mark / join events of the frame-group overlap
+  static constexpr int kMaxGroups = 8;
+  hipStream_t side[kMaxGroups - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_mark[kMaxGroups - 1] = {}, ev_join[kMaxGroups - 1] = {};
+  bool side_ready = false;
   std::vector<hipEvent_t> events;
   // backward (per-stream lambda), allocated on the first qm_batch_backward
   void* bwd_arena = nullptr;
@@ -331,38 +333,65 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr, int f0 
 }
 
 // One pipeline run. With the overlap setting on (and two frames or more) the
-// batch runs as two groups on two streams, the second forked from s and
-// joined back into it: its kernels fill the GPU while the first group's
-// latency-bound ones (coarse BP levels, the serial upsample chains) run, and
-// the reverse. overlap 1 starts both groups together; 2 starts the second
-// group once the first group's cost volume is enqueued, 3 once its BP is.
-// the second stream and the events, created once (outside any capture)
+// batch runs as G frame groups, group 0 on s and the others on their own
+// streams forked from s and joined back into it: the kernels of one group fill
+// the GPU while another group's latency-bound ones (coarse BP levels, the
+// serial upsample chains) run. Group g + 1 starts when group g has enqueued
+// the stage the setting names. Settings: 1-3 two groups, starting together,
+// after the cost volume, after BP; 4 three and 5 four groups, each after the
+// previous group's cost volume. Every setting computes the same results.
+struct OverlapPlan {
+  int groups, mark_after;
+};
+OverlapPlan overlap_plan(int ov, int n) {
+  OverlapPlan p{1, -1};
+  if (ov == 1) p = {2, -1};
+  if (ov == 2) p = {2, 1};
+  if (ov == 3) p = {2, 3};
+  if (ov == 4) p = {3, 1};
+  if (ov == 5) p = {4, 1};
+  if (ov == 6) p = {8, 1};
+  if (ov == 7) p = {4, 3};
+  if (p.groups > n) p.groups = n;
+  return p;
+}
+
+// the extra streams and the events, created once (outside any capture)
 int ensure_side(qm_batch* b) {
-  if (b->side) return QM_OK;
-  QM_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-  QM_HIP(hipEventCreateWithFlags(&b->ev_mark, hipEventDisableTiming));
-  QM_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-  QM_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  if (b->side_ready) return QM_OK;
+  if (!b->ev_fork) QM_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  for (int g = 0; g < qm_batch::kMaxGroups - 1; ++g) {
+    if (!b->ev_mark[g]) QM_HIP(hipEventCreateWithFlags(&b->ev_mark[g], hipEventDisableTiming));
+    if (!b->ev_join[g]) QM_HIP(hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming));
+    if (!b->side[g]) QM_HIP(hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking));
+  }
+  b->side_ready = true;
   return QM_OK;
 }
 
 int run_pipeline(qm_batch* b, int par, hipStream_t s) {
-  const int ov = b->tune.overlap;
-  if (ov == 0 || b->n < 2) {
+  const OverlapPlan plan = overlap_plan(b->tune.overlap, b->n);
+  if (plan.groups < 2) {
     run_stages(b, par, s);
     return QM_OK;
   }
   int rc = ensure_side(b);
   if (rc) return rc;
-  const int na = (b->n + 1) / 2, nb = b->n - na;
-  const int mark_after = ov == 2 ? 1 : (ov == 3 ? 3 : -1);
+  const int G = plan.groups;
   QM_HIP(hipEventRecord(b->ev_fork, s));
-  QM_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-  run_stages(b, par, s, nullptr, 0, na, mark_after, b->ev_mark);
-  if (mark_after >= 0) QM_HIP(hipStreamWaitEvent(b->side, b->ev_mark, 0));
-  run_stages(b, par, b->side, nullptr, na, nb);
-  QM_HIP(hipEventRecord(b->ev_join, b->side));
-  QM_HIP(hipStreamWaitEvent(s, b->ev_join, 0));
+  int f0 = 0;
+  for (int g = 0; g < G; ++g) {
+    const int nf = (b->n - f0) / (G - g);  // split as evenly as the count allows
+    hipStream_t sg = g == 0 ? s : b->side[g - 1];
+    if (g > 0) QM_HIP(hipStreamWaitEvent(sg, plan.mark_after >= 0 ? b->ev_mark[g - 1] : b->ev_fork, 0));
+    const bool mark = g + 1 < G && plan.mark_after >= 0;
+    run_stages(b, par, sg, nullptr, f0, nf, mark ? plan.mark_after : -1, mark ? b->ev_mark[g] : nullptr);
+    f0 += nf;
+  }
+  for (int g = 1; g < G; ++g) {
+    QM_HIP(hipEventRecord(b->ev_join[g - 1], b->side[g - 1]));
+    QM_HIP(hipStreamWaitEvent(s, b->ev_join[g - 1], 0));
+  }
   return QM_OK;
 }
 
@@ -1062,7 +1091,7 @@ int qm_batch_set_tuning(qm_batch* b, int knob, int value) {
   int* slot = tune_slot(b->tune, knob);
   if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
   const bool ok = knob == QM_TUNE_COST_WAVES ? (value == 2 || value == 4 || value == 8)
-                  : knob == QM_TUNE_OVERLAP  ? (value >= 0 && value <= 3)
+                  : knob == QM_TUNE_OVERLAP  ? (value >= 0 && value <= 7)
                                              : (value == 4 || value == 8 || value == 16);
   if (!ok) return fail(QM_ERR_INVALID, "value not offered for this knob");
   if (*slot != value) {
@@ -1156,7 +1185,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
       rc = copy_seeds(true);
       const Tuning keep = b->tune;
       double best_t = 1e30;
-      for (int v = 0; v <= 3 && !rc; ++v) {
+      for (int v = 0; v <= 7 && !rc; ++v) {
         Tuning t = best;
         t.overlap = v;
         double ms = 0.0;
@@ -1213,12 +1242,15 @@ int qm_batch_reset_seeds(qm_batch* b) {
 int qm_batch_destroy(qm_batch* b) {
   if (!b) return QM_OK;
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
-  if (b->side) {
-    (void)hipStreamSynchronize(b->side);
-    (void)hipStreamDestroy(b->side);
+  for (int g = 0; g < qm_batch::kMaxGroups - 1; ++g) {
+    if (b->side[g]) {
+      (void)hipStreamSynchronize(b->side[g]);
+      (void)hipStreamDestroy(b->side[g]);
+    }
+    if (b->ev_mark[g]) (void)hipEventDestroy(b->ev_mark[g]);
+    if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
   }
-  for (hipEvent_t e : {b->ev_fork, b->ev_mark, b->ev_join})
-    if (e) (void)hipEventDestroy(e);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   for (int par = 0; par < 2; ++par) {
     if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
     if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);

@@ -32,7 +32,7 @@ struct Tuning {
   int cost_waves = 4;    // 2, 4, 8
   int bp_rows = 16;      // 4, 8, 16
   int extract_rows = 16;  // 4, 8, 16
-  int overlap = 0;       // 0 one stream; 1-3 two frame groups on two streams (run_pipeline)
+  int overlap = 0;       // 0 one stream; 1-7 two to eight frame groups on their own streams (run_pipeline)
 };
 
 void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
