@@ -259,13 +259,14 @@ int qm_batch_run(qm_batch* b, int mode, void* stream);
 #define QM_STAGE_FUSE_WARP 7
 #define QM_STAGE_FUSE_UPDATE 8
 #define QM_NUM_STAGES 9
-/* One plain run (as mode 0) with a hipEvent pair around every launch; returns
- * per stage kind the summed kernel time (ms) and the number of launches, after
- * synchronising the stream. For measurement only. */
+/* One plain single-stream run with a hipEvent pair around each stage (around
+ * every launch, except that the BP half-sweeps of the run share one pair);
+ * returns per stage kind the summed time (ms) and the number of launches,
+ * after synchronising the stream. For measurement only. */
 int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
 /* Launch-shape knobs of a batch; every value gives bit-identical results.
  * Changing one drops the batch's captured graphs (recaptured on the next
- * graph run). Defaults: 4, 16, 16. */
+ * graph run). Defaults: 4, 16, 16, 0. */
 #define QM_TUNE_COST_WAVES 0   /* 2, 4 or 8 waves per cost-volume block of 64 cells */
 #define QM_TUNE_BP_ROWS 1      /* 4, 8 or 16 cells (16-lane rows) per BP half-sweep block */
 #define QM_TUNE_EXTRACT_ROWS 2 /* 4, 8 or 16 cells per winner-extraction block */

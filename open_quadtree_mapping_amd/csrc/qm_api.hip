@@ -235,11 +235,13 @@ struct qm_batch {
 
 namespace {
 
-// Optional per-launch timing: an event pair around every launch, attributed
-// to a stage kind (QM_STAGE_*).
+// Optional timing: an event pair around each stage's launches (the BP
+// half-sweeps of a run back to back under one pair, so that event overhead
+// does not inflate their per-launch figure), attributed to a stage kind
+// (QM_STAGE_*) with its launch count.
 struct Timer {
   std::vector<hipEvent_t>* pool;
-  std::vector<int> stage;
+  std::vector<int> stage, launches;
   size_t used = 0;
   hipStream_t s;
   hipEvent_t next() {
@@ -250,8 +252,9 @@ struct Timer {
     }
     return (*pool)[used++];
   }
-  void begin(int st) {
+  void begin(int st, int n) {
     stage.push_back(st);
+    launches.push_back(n);
     (void)hipEventRecord(next(), s);
   }
   void end() { (void)hipEventRecord(next(), s); }
@@ -259,8 +262,8 @@ struct Timer {
 
 struct Mark {
   Timer* t;
-  Mark(Timer* tt, int st) : t(tt) {
-    if (t) t->begin(st);
+  Mark(Timer* tt, int st, int n = 1) : t(tt) {
+    if (t) t->begin(st, n);
   }
   ~Mark() {
     if (t) t->end();
@@ -312,16 +315,17 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr, int f0 
   marker(1);
   { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
   marker(2);
-  for (int l = 3; l >= 0; --l) {
-    const int iters = p->bp_iters[l];
-    if (iters == 0) {
-      Mark m(t, QM_STAGE_BP_SWEEP);
-      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
-      continue;
-    }
-    for (int it = 0; it < iters; ++it) {
-      Mark m(t, QM_STAGE_BP_SWEEP);
-      launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s);
+  {
+    int sweeps = 0;
+    for (int l = 0; l < 4; ++l) sweeps += p->bp_iters[l] == 0 ? 1 : p->bp_iters[l];
+    Mark m(t, QM_STAGE_BP_SWEEP, sweeps);
+    for (int l = 3; l >= 0; --l) {
+      const int iters = p->bp_iters[l];
+      if (iters == 0) {
+        launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
+        continue;
+      }
+      for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s);
     }
   }
   marker(3);
@@ -1041,7 +1045,7 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double* stage_ms, int* stage_l
     float ms = 0.0f;
     QM_HIP(hipEventElapsedTime(&ms, b->events[2 * k], b->events[2 * k + 1]));
     stage_ms[t.stage[k]] += ms;
-    stage_launches[t.stage[k]] += 1;
+    stage_launches[t.stage[k]] += t.launches[k];
   }
   b->parity ^= 1;
   b->runs += 1;
