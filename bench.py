@@ -2,15 +2,19 @@
 
 One step = every stage (quadtree, cost volume, hierarchical BP + extraction,
 tridiagonal upsample, fusion) for a batch of B camera streams of 752x480
-keyframes (B = --streams, default 32), one launch per stage over all streams,
-replayed as one hipGraph. Inputs are resident in HBM before timing starts; the
-fusion state carries over from step to step as for consecutive keyframes.
+keyframes (B = --streams, default 32), one launch per stage over all streams
+of a frame group (the autotuner picks how many groups run side by side on
+their own streams), replayed as one hipGraph. Inputs are resident in HBM
+before timing starts; the fusion state carries over from step to step as for
+consecutive keyframes.
 
 Also measured, outside the timed region:
   * the plain per-frame GPU path (one keyframe at a time, per-stage launches,
     no graph) -> plain_maps_per_s and batched_over_plain;
-  * per-stage kernel time from an event pair around every launch, turned into
-    achieved HBM GB/s from the algorithmic bytes of each stage (roofline);
+  * per-stage kernel time from a single-stream run with whole-batch launches
+    and an event pair around each stage (the BP half-sweeps of the run share
+    one pair), turned into achieved HBM GB/s from the algorithmic bytes of
+    each stage (roofline);
   * the FP32 OpenMP CPU oracle on a bounded sample (cpu_baseline).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--streams B]
@@ -57,8 +61,9 @@ def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10)):
     """Algorithmic bytes of the BP half-sweeps for one keyframe, following the
     skip rule of the kernels (quadtree lookup at (bx << lev, y << lev)).
     Active update: data + 4 incoming + 4 outgoing messages = 36N bytes; a cell
-    initialised from its parent at a level's first sweep: 16N read + 16N write
-    (16N write only at the coarsest level)."""
+    initialised from its parent at a level's first sweep (the cells the
+    quadtree skips, or every cell not updated when the level has fewer than two
+    half-sweeps): 16N read + 16N write (16N write only at the coarsest level)."""
     import numpy as np
     Wc, Hc = W // 4, H // 4
     lw, lh = [Wc], [Hc]
@@ -87,7 +92,8 @@ def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10)):
             updates += n_upd
             total += 36 * N * n_upd
             if it == 0:
-                total += (16 if l == 3 else 32) * N * (cells - n_upd)
+                copied = (cells - n_upd) if iters[l] < 2 else int((~active).sum())
+                total += (16 if l == 3 else 32) * N * copied
     return total, updates
 
 

@@ -9,7 +9,9 @@
 // (N = 64) per cell. The message upsample between levels is folded into the
 // first half-sweep of each level: that sweep visits every cell, reads
 // neighbour messages straight from the parent level, and copies the parent's
-// messages into cells it does not update.
+// messages into the cells it does not update that will need them: cells the
+// quadtree skips at this level, and every cell when the level has fewer than
+// two half-sweeps.
 #include "qm_device.h"
 #include "qm_launch.h"
 
@@ -118,7 +120,8 @@ __device__ __forceinline__ bool cell_skipped(const FrameDev& F, int l, int bx, i
 // (up, down, left, right).
 template <int N, int ROWS>
 __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l,
-                                                        int A, int first, int active_ok, float p1, float p2) {
+                                                        int A, int first, int active_ok, int copy_all, float p1,
+                                                        float p2) {
   constexpr int S = N / 16;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
@@ -132,8 +135,12 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
       y = item / w;
       x = item - y * w;
       const bool mine = A ? ((x & 1) == (y & 1)) : ((x & 1) != (y & 1));
-      upd = active_ok && mine && !cell_skipped(F, l, x >> 1, y);
-      copy = !upd;
+      const bool skipped = cell_skipped(F, l, x >> 1, y);
+      upd = active_ok && mine && !skipped;
+      // A cell of the other colour that the level's second half-sweep will
+      // update needs no copy: only this sweep reads its messages, and it reads
+      // them from the parent. Skipped cells keep the parent's messages.
+      copy = !upd && (copy_all || skipped);
     }
   } else {
     const int hw = (w + 1) >> 1;
@@ -335,13 +342,13 @@ __global__ __launch_bounds__(256) void k_cost_to_rows(const float* __restrict__ 
 
 template <int N, int ROWS>
 static void bp_sweep_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool A,
-                           bool first, bool active, const BpLaunchParams& p) {
+                           bool first, bool active, bool copy_all, const BpLaunchParams& p) {
   hipLaunchKernelGGL((k_bp_sweep<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, l, (int)A, (int)first,
-                     (int)active, p.p1, p.p2);
+                     (int)active, (int)copy_all, p.p1, p.p2);
 }
 
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s) {
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all) {
   const int rows = (t.bp_rows == 4 || t.bp_rows == 8) ? t.bp_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
@@ -353,13 +360,13 @@ void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, in
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
-    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, p);
-    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, p);
-    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, p);
+    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
   } else {
-    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, p);
-    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, p);
-    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, p);
+    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
   }
 }
 

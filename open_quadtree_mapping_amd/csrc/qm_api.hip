@@ -156,7 +156,7 @@ void run_sweeps(const FrameDev* df, const FrameDev* hf, int n, const qm_params* 
       launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, t, s);
       continue;
     }
-    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, t, s);
+    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, t, s, iters < 2);
   }
 }
 
@@ -325,7 +325,7 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr, int f0 
         launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
         continue;
       }
-      for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s);
+      for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2);
     }
   }
   marker(3);

@@ -41,10 +41,12 @@ void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N,
                  const Tuning& t, hipStream_t s);
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s);
 // one half-sweep of level l; first = the level's first sweep (also initialises
-// every cell's messages from the parent level, or zero at the coarsest level);
+// cells' messages from the parent level, or zero at the coarsest level: every
+// cell it does not update when copy_all, else only the cells the quadtree
+// skips, which suffices when the level has a second half-sweep);
 // active = 0 turns the first sweep into a pure initialisation.
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s);
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all = true);
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
                     const Tuning& t, hipStream_t s);
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
