@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tune-reps", type=int, default=5)
+    ap.add_argument("--overlap", type=int, default=None,
+                    help="force the frame-group overlap setting (0-7) after autotuning, e.g. for a profiled run")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the batched launches without hipGraph replay (for counter collection)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -198,6 +200,8 @@ def main():
     if not args.no_autotune:
         tuned = batch.autotune(reps=args.tune_reps)
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
+    if args.overlap is not None:
+        batch.set_tuning(overlap=args.overlap)
     use_graph = not args.no_graph
     for _ in range(args.warmup):
         batch.run(graph=use_graph)

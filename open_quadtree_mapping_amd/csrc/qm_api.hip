@@ -282,58 +282,84 @@ bool any_prior_depth(const qm_batch* b) {
   return false;
 }
 
-// The stages of frames [f0, f0 + nf) in order; when mark is given it is
-// recorded on s once the stage numbered mark_after (0 quadtree, 1 cost volume,
-// 2 pyramid, 3 BP) has been enqueued.
-void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr, int f0 = 0, int nf = -1,
-                int mark_after = -1, hipEvent_t mark = nullptr) {
+// One stage (QM_STAGE_* numbering: 0 quadtree, 1 cost volume, 2 pyramid,
+// 3 BP half-sweeps, 4 extraction, 5 / 6 upsample rows / columns, 7 / 8 fusion
+// warp / update) for frames [f0, f0 + n) on stream s.
+void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n, int stage) {
   const FrameDev* df = b->df[par] + f0;
   const FrameDev* hf = b->hf[par].data() + f0;
-  const int n = nf < 0 ? b->n : nf, N = b->p.n_hyp;
+  const int N = b->p.n_hyp;
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
-  bool prior_depth = false, prior = false;
-  for (int i = 0; i < n; ++i) {
-    prior_depth |= hf[i].prior_depth != nullptr;
-    prior |= hf[i].prior != nullptr;
-  }
-  auto marker = [&](int stage) {
-    if (mark && mark_after == stage) (void)hipEventRecord(mark, s);
-  };
-  {
-    Mark m(t, QM_STAGE_QUADTREE);
-    launch_quadtree(df, hf, n, s);
-    if (prior_depth) launch_quadtree_prior(df, hf, n, s);
-  }
-  marker(0);
   const Tuning& tu = b->tune;
-  {
-    Mark m(t, QM_STAGE_COST);
-    launch_cost(df, hf, n, N, b->table, false, tu, s);
-    if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
-  }
-  marker(1);
-  { Mark m(t, QM_STAGE_PYRAMID); launch_pyramid(df, hf, n, N, s); }
-  marker(2);
-  {
-    int sweeps = 0;
-    for (int l = 0; l < 4; ++l) sweeps += p->bp_iters[l] == 0 ? 1 : p->bp_iters[l];
-    Mark m(t, QM_STAGE_BP_SWEEP, sweeps);
-    for (int l = 3; l >= 0; --l) {
-      const int iters = p->bp_iters[l];
-      if (iters == 0) {
-        launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
-        continue;
+  switch (stage) {
+    case QM_STAGE_QUADTREE: {
+      Mark m(t, QM_STAGE_QUADTREE);
+      launch_quadtree(df, hf, n, s);
+      bool prior_depth = false;
+      for (int i = 0; i < n; ++i) prior_depth |= hf[i].prior_depth != nullptr;
+      if (prior_depth) launch_quadtree_prior(df, hf, n, s);
+      break;
+    }
+    case QM_STAGE_COST: {
+      Mark m(t, QM_STAGE_COST);
+      launch_cost(df, hf, n, N, b->table, false, tu, s);
+      bool prior = false;
+      for (int i = 0; i < n; ++i) prior |= hf[i].prior != nullptr;
+      if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
+      break;
+    }
+    case QM_STAGE_PYRAMID: {
+      Mark m(t, QM_STAGE_PYRAMID);
+      launch_pyramid(df, hf, n, N, s);
+      break;
+    }
+    case QM_STAGE_BP_SWEEP: {
+      int sweeps = 0;
+      for (int l = 0; l < 4; ++l) sweeps += p->bp_iters[l] == 0 ? 1 : p->bp_iters[l];
+      Mark m(t, QM_STAGE_BP_SWEEP, sweeps);
+      for (int l = 3; l >= 0; --l) {
+        const int iters = p->bp_iters[l];
+        if (iters == 0) {
+          launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
+          continue;
+        }
+        for (int it = 0; it < iters; ++it)
+          launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2);
       }
-      for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2);
+      break;
+    }
+    case QM_STAGE_EXTRACT: {
+      Mark m(t, QM_STAGE_EXTRACT);
+      launch_extract(df, hf, n, N, bp, tu, s);
+      break;
+    }
+    case QM_STAGE_UPSAMPLE_ROW: {
+      Mark m(t, QM_STAGE_UPSAMPLE_ROW);
+      launch_upsample_row(df, hf, n, 0, b->wlut, s);
+      break;
+    }
+    case QM_STAGE_UPSAMPLE_COL: {
+      Mark m(t, QM_STAGE_UPSAMPLE_COL);
+      launch_upsample_col(df, hf, n, 0, b->wlut, true, s);
+      break;
+    }
+    case QM_STAGE_FUSE_WARP: {
+      Mark m(t, QM_STAGE_FUSE_WARP);
+      launch_fuse_warp(df, hf, n, s);
+      break;
+    }
+    default: {
+      Mark m(t, QM_STAGE_FUSE_UPDATE);
+      launch_fuse_update(df, hf, n, s);
+      break;
     }
   }
-  marker(3);
-  { Mark m(t, QM_STAGE_EXTRACT); launch_extract(df, hf, n, N, bp, tu, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_ROW); launch_upsample_row(df, hf, n, 0, b->wlut, s); }
-  { Mark m(t, QM_STAGE_UPSAMPLE_COL); launch_upsample_col(df, hf, n, 0, b->wlut, true, s); }
-  { Mark m(t, QM_STAGE_FUSE_WARP); launch_fuse_warp(df, hf, n, s); }
-  { Mark m(t, QM_STAGE_FUSE_UPDATE); launch_fuse_update(df, hf, n, s); }
+}
+
+// All stages of the whole batch on one stream.
+void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
+  for (int st = 0; st < QM_NUM_STAGES; ++st) enqueue_stage(b, par, s, t, 0, b->n, st);
 }
 
 // One pipeline run. With the overlap setting on (and two frames or more) the
@@ -382,15 +408,29 @@ int run_pipeline(qm_batch* b, int par, hipStream_t s) {
   int rc = ensure_side(b);
   if (rc) return rc;
   const int G = plan.groups;
+  int f0[qm_batch::kMaxGroups], nf[qm_batch::kMaxGroups];
+  for (int g = 0, at = 0; g < G; ++g) {
+    nf[g] = (b->n - at) / (G - g);  // split as evenly as the count allows
+    f0[g] = at;
+    at += nf[g];
+  }
+  auto stream_of = [&](int g) { return g == 0 ? s : b->side[g - 1]; };
   QM_HIP(hipEventRecord(b->ev_fork, s));
-  int f0 = 0;
-  for (int g = 0; g < G; ++g) {
-    const int nf = (b->n - f0) / (G - g);  // split as evenly as the count allows
-    hipStream_t sg = g == 0 ? s : b->side[g - 1];
-    if (g > 0) QM_HIP(hipStreamWaitEvent(sg, plan.mark_after >= 0 ? b->ev_mark[g - 1] : b->ev_fork, 0));
-    const bool mark = g + 1 < G && plan.mark_after >= 0;
-    run_stages(b, par, sg, nullptr, f0, nf, mark ? plan.mark_after : -1, mark ? b->ev_mark[g] : nullptr);
-    f0 += nf;
+  // Stages are enqueued in the order they are expected to run: stage u of group
+  // g in slot u + g * lag (lag = stages up to the stagger point), ties by group.
+  // Group g waits on its own stream for group g - 1's mark (or the fork)
+  // before its first stage.
+  const int lag = plan.mark_after + 1;
+  const int slots = QM_NUM_STAGES + (G - 1) * lag;
+  for (int slot = 0; slot < slots; ++slot) {
+    for (int g = 0; g < G; ++g) {
+      const int u = slot - g * lag;
+      if (u < 0 || u >= QM_NUM_STAGES) continue;
+      hipStream_t sg = stream_of(g);
+      if (u == 0 && g > 0) QM_HIP(hipStreamWaitEvent(sg, lag > 0 ? b->ev_mark[g - 1] : b->ev_fork, 0));
+      enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], u);
+      if (lag > 0 && u == plan.mark_after && g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], sg));
+    }
   }
   for (int g = 1; g < G; ++g) {
     QM_HIP(hipEventRecord(b->ev_join[g - 1], b->side[g - 1]));
