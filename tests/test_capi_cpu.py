@@ -63,3 +63,26 @@ def test_abi_version_matches_binding():
     assert _lib.lib().qm_abi_version() == _lib.ABI_VERSION
     hdr = open(os.path.join(ROOT, "include", "qm.h")).read()
     assert f"#define QM_ABI_VERSION {_lib.ABI_VERSION}" in hdr
+
+
+def test_alternate_and_output_entry_points_reject_bad_arguments():
+    L = _lib.lib()
+    p = _lib.default_params()
+    assert L.qm_quadtree_prior(None, 64, 48, None, None) == -1 and b"NULL" in L.qm_last_error()
+    assert L.qm_prior_to_cost(None, 64, 48, None, C.byref(p), None, None, None) == -1
+    assert b"NULL" in L.qm_last_error()
+    bad = _lib.default_params(n_hyp=48)
+    assert L.qm_prior_to_cost(C.c_void_p(1), 64, 48, C.c_void_p(1), C.byref(bad), C.c_void_p(1), None, None) == -1
+    assert b"n_hyp" in L.qm_last_error()
+    assert L.qm_upsample_local(None, None, 64, 48, 64, None, None) == -1 and b"NULL" in L.qm_last_error()
+    assert L.qm_upsample_local(C.c_void_p(1), C.c_void_p(1), 64, 48, 32, C.c_void_p(1), None) == -1
+    assert b"pitch" in L.qm_last_error()
+    assert L.qm_sd_step(None, None, None, None, None, 64, 48, 64, None, None, None, 0, None, None, None) == -1
+    assert L.qm_batch_set_tuning(None, 3, 1) == -1 and L.qm_batch_autotune(None, 1, None, None, None) == -1
+
+
+def test_tuning_knob_names_match_header():
+    hdr = open(os.path.join(ROOT, "include", "qm.h")).read()
+    ids = dict((m.group(1).lower(), int(m.group(2))) for m in re.finditer(r"#define QM_TUNE_([A-Z_]+) (\d+)", hdr))
+    assert ids == {k: i for i, k in enumerate(_lib.TUNE_KNOBS)}
+    assert f"#define QM_NUM_TUNE {len(_lib.TUNE_KNOBS)}" in hdr
