@@ -109,15 +109,21 @@ def stage_bytes(levels, W, H, N):
     sub = levels[0:4 * (H // 4):4, 0:4 * (W // 4):4].astype(np.int64)
     ys, xs = np.mgrid[0:H // 4, 0:W // 4]
     qs = 1 << sub
-    nodes = int((((4 * xs) % qs == 0) & ((4 * ys) % qs == 0)).sum())
+    node = ((4 * xs) % qs == 0) & ((4 * ys) % qs == 0)
+    # the last level-0 half-sweep (B cells, x + y odd, after 4 half-sweeps)
+    # extracts its colour's winners; the extraction kernel does the A cells
+    last = ((xs + ys) & 1) == 1
+    nodes = int((node & ~last).sum())
+    others = int((~node & ~last).sum())
     bp, upd = bp_traffic(levels, W, H, N)
+    bp += 8 * int((node & last).sum())  # fused extraction: depth + label of the B nodes
     px = W * H
     return {
         "quadtree": 2 * px,
         "cost": 5 * px + C + 4 * N * C,
         "pyramid": 4 * N * sum(a * b for a, b in zip(lw, lh)),
         "bp_sweep": bp,
-        "extract": nodes * (20 * N + 8) + (C - nodes) * 9,
+        "extract": nodes * (20 * N + 8) + others * 9,
         "upsample_row": px + 8 * C + 8 * px,           # image, labels + depths, row results
         "upsample_col": px + 8 * px + 4 * px + 8 * px,  # image, row results, dense, z-buffer reset
         "fuse_warp": 40 * px,

@@ -114,155 +114,21 @@ __device__ __forceinline__ bool cell_skipped(const FrameDev& F, int l, int bx, i
   return (int)gp((const uint8_t*)F.levels)[(size_t)Y * F.W + X] > lev;
 }
 
-// One half-sweep (src/bpextract.cu:200-312) for a batch: ROWS work items per
-// block, one per 16-lane row. h for direction dir is the data term plus the
-// incoming messages other than the target's, added in the reference's order
-// (up, down, left, right).
-template <int N, int ROWS>
-__global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l,
-                                                        int A, int first, int active_ok, int copy_all, float p1,
-                                                        float p2) {
-  constexpr int S = N / 16;
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int j = threadIdx.x & 15;
-  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
-  const int w = F.lw[l], h = F.lh[l];
-  int x = 0, y = 0;
-  bool upd = false, copy = false;
-  if (first) {
-    if (item < w * h) {
-      y = item / w;
-      x = item - y * w;
-      const bool mine = A ? ((x & 1) == (y & 1)) : ((x & 1) != (y & 1));
-      const bool skipped = cell_skipped(F, l, x >> 1, y);
-      upd = active_ok && mine && !skipped;
-      // A cell of the other colour that the level's second half-sweep will
-      // update needs no copy: only this sweep reads its messages, and it reads
-      // them from the parent. Skipped cells keep the parent's messages.
-      copy = !upd && (copy_all || skipped);
-    }
-  } else {
-    const int hw = (w + 1) >> 1;
-    if (item < hw * h) {
-      y = item / hw;
-      const int bx = item - y * hw;
-      x = 2 * bx + (A ? (y & 1) : ((y + 1) & 1));
-      upd = x < w && !cell_skipped(F, l, bx, y);
-    }
-  }
-  const gptr<float> dst = gp(F.msg[l]) + ((size_t)y * w + x) * 4 * N;
-  const gptr<const float> P = gp((const float*)((first && l < 3) ? F.msg[l + 1] : nullptr));
-  const int wp = (l < 3) ? F.lw[l + 1] : 0;
-  if (copy) {
-    // first sweep only: the parent's four messages (zeros at the coarsest level)
-    Row<S> r[4];
-    if (P) {
-      const gptr<const float> src = P + ((size_t)(y >> 1) * wp + (x >> 1)) * 4 * N;
-#pragma unroll
-      for (int dir = 0; dir < 4; ++dir) r[dir] = load_row<S>(src + dir * N, j);
-    } else {
-#pragma unroll
-      for (int dir = 0; dir < 4; ++dir)
-#pragma unroll
-        for (int s = 0; s < S; ++s) r[dir].v[s] = 0.0f;
-    }
-#pragma unroll
-    for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, r[dir]);
-  }
-  if (!upd) return;
-  // incoming messages: from the up neighbour its "down" message, from the down
-  // neighbour its "up" message, from the left its "right", from the right its
-  // "left"; the first sweep of a level reads them from the parent cell
-  const bool has_up = y > 0, has_down = y < h - 1, has_left = x > 0, has_right = x < w - 1;
-  const gptr<const float> M = gp((const float*)F.msg[l]);
-  auto src = [&](int nx, int ny, int dir) -> gptr<const float> {
-    if (first) return P + (((size_t)(ny >> 1) * wp + (nx >> 1)) * 4 + dir) * N;
-    return M + (((size_t)ny * w + nx) * 4 + dir) * N;
-  };
-  const bool msgs = !first || P;
-  const Row<S> dv = load_row<S>(gp((const float*)F.cost[l]) + ((size_t)y * w + x) * N, j);
-  Row<S> fu, fd, fl, fr;
-#pragma unroll
-  for (int s = 0; s < S; ++s) fu.v[s] = fd.v[s] = fl.v[s] = fr.v[s] = 0.0f;
-  if (msgs && has_up) fu = load_row<S>(src(x, y - 1, 1), j);
-  if (msgs && has_down) fd = load_row<S>(src(x, y + 1, 0), j);
-  if (msgs && has_left) fl = load_row<S>(src(x - 1, y, 3), j);
-  if (msgs && has_right) fr = load_row<S>(src(x + 1, y, 2), j);
-  Row<S> hv[4];
-  if (__all(has_up && has_down && has_left && has_right)) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      hv[0].v[s] = ((dv.v[s] + fd.v[s]) + fl.v[s]) + fr.v[s];
-      hv[1].v[s] = ((dv.v[s] + fu.v[s]) + fl.v[s]) + fr.v[s];
-      hv[2].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fr.v[s];
-      hv[3].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fl.v[s];
-    }
-  } else {
-#pragma unroll
-    for (int dir = 0; dir < 4; ++dir)
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        float v = dv.v[s];
-        if (dir != 0 && has_up) v += fu.v[s];
-        if (dir != 1 && has_down) v += fd.v[s];
-        if (dir != 2 && has_left) v += fl.v[s];
-        if (dir != 3 && has_right) v += fr.v[s];
-        hv[dir].v[s] = v;
-      }
-  }
-  constexpr float inv_n = 1.0f / (float)N;
-#pragma unroll
-  for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, message_from<S>(hv[dir], p1, p2, inv_n));
+// Winner extraction of one cell (src/bpextract.cu:314-372), one 16-lane row:
+// b = the cell's belief (data + the messages from its four neighbours, added
+// in the reference's order). Argmin by the reference's tree (stage m: sample
+// d < m takes sample d + m only if strictly smaller, so ties keep the lower
+// index at every stage), parabola sub-sample refinement, depth = 1 / (step * d
+// + min) in double. Cells that are not quadtree nodes get depth 0, label -1.
+template <int N>
+__device__ __forceinline__ bool is_node(const FrameDev& F, int x, int y) {
+  const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)(4 * y) * F.W + 4 * x];
+  return ((4 * x) % qs) == 0 && ((4 * y) % qs) == 0;
 }
 
-// Winner extraction (src/bpextract.cu:314-372), one 16-lane row per cell:
-// belief = data + messages from the four neighbours, argmin by the reference's
-// tree (stage m: sample d < m takes sample d + m only if strictly smaller, so
-// ties keep the lower index at every stage), parabola sub-sample refinement,
-// depth = 1 / (step * d + min) in double.
-template <int N, int ROWS>
-__global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
-                                                       double istep) {
-  constexpr int S = N / 16;
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int lane = threadIdx.x & 63, j = lane & 15;
-  const int cell = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
-  const int w = F.lw[0], h = F.lh[0];
-  if (cell >= w * h) return;
-  const int y = cell / w, x = cell - y * w;
-  const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)(4 * y) * F.W + 4 * x];
-  if (((4 * x) % qs) != 0 || ((4 * y) % qs) != 0) {
-    if (j == 0) {
-      gp(F.dcoarse)[cell] = 0.0f;
-      gp(F.label)[cell] = -1;
-    }
-    return;
-  }
-  const gptr<const float> M = gp((const float*)F.msg[0]);
-  Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
-  Row<S> t;
-  if (x != 0) {
-    t = load_row<S>(M + (((size_t)cell - 1) * 4 + 3) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (x != w - 1) {
-    t = load_row<S>(M + (((size_t)cell + 1) * 4 + 2) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (y != 0) {
-    t = load_row<S>(M + (((size_t)cell - w) * 4 + 1) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (y != h - 1) {
-    t = load_row<S>(M + (((size_t)cell + w) * 4 + 0) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
+template <int N, int S>
+__device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int j, int lane, const Row<S>& b,
+                                               double imin, double istep) {
   // argmin tree: in-lane stages N/2 .. 16, then row_shl 8 .. 1
   float v[S];
   int id[S];
@@ -315,6 +181,190 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
   }
 }
 
+// One half-sweep (src/bpextract.cu:200-312) for a batch: ROWS work items per
+// block, one per 16-lane row. h for direction dir is the data term plus the
+// incoming messages other than the target's, added in the reference's order
+// (up, down, left, right).
+template <int N, int ROWS>
+__global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l,
+                                                        int A, int first, int active_ok, int copy_all, float p1,
+                                                        float p2, int ext, double imin, double istep) {
+  constexpr int S = N / 16;
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int j = threadIdx.x & 15;
+  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
+  const int w = F.lw[l], h = F.lh[l];
+  int x = 0, y = 0;
+  bool upd = false, copy = false, ex = false;
+  if (first) {
+    if (item < w * h) {
+      y = item / w;
+      x = item - y * w;
+      const bool mine = A ? ((x & 1) == (y & 1)) : ((x & 1) != (y & 1));
+      const bool skipped = cell_skipped(F, l, x >> 1, y);
+      upd = active_ok && mine && !skipped;
+      // A cell of the other colour that the level's second half-sweep will
+      // update needs no copy: only this sweep reads its messages, and it reads
+      // them from the parent. Skipped cells keep the parent's messages.
+      copy = !upd && (copy_all || skipped);
+    }
+  } else {
+    const int hw = (w + 1) >> 1;
+    if (item < hw * h) {
+      y = item / hw;
+      const int bx = item - y * hw;
+      x = 2 * bx + (A ? (y & 1) : ((y + 1) & 1));
+      upd = x < w && !cell_skipped(F, l, bx, y);
+      // the level-0 sweep that ends BP also extracts the winners of its
+      // colour's cells: their incoming messages are final
+      ex = ext && x < w;
+    }
+  }
+  const gptr<float> dst = gp(F.msg[l]) + ((size_t)y * w + x) * 4 * N;
+  const gptr<const float> P = gp((const float*)((first && l < 3) ? F.msg[l + 1] : nullptr));
+  const int wp = (l < 3) ? F.lw[l + 1] : 0;
+  if (copy) {
+    // first sweep only: the parent's four messages (zeros at the coarsest level)
+    Row<S> r[4];
+    if (P) {
+      const gptr<const float> src = P + ((size_t)(y >> 1) * wp + (x >> 1)) * 4 * N;
+#pragma unroll
+      for (int dir = 0; dir < 4; ++dir) r[dir] = load_row<S>(src + dir * N, j);
+    } else {
+#pragma unroll
+      for (int dir = 0; dir < 4; ++dir)
+#pragma unroll
+        for (int s = 0; s < S; ++s) r[dir].v[s] = 0.0f;
+    }
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, r[dir]);
+  }
+  if (ex && !is_node<N>(F, x, y)) {
+    if (j == 0) {
+      gp(F.dcoarse)[(size_t)y * w + x] = 0.0f;
+      gp(F.label)[(size_t)y * w + x] = -1;
+    }
+    ex = false;
+  }
+  if (!upd && !ex) return;
+  // incoming messages: from the up neighbour its "down" message, from the down
+  // neighbour its "up" message, from the left its "right", from the right its
+  // "left"; the first sweep of a level reads them from the parent cell
+  const bool has_up = y > 0, has_down = y < h - 1, has_left = x > 0, has_right = x < w - 1;
+  const gptr<const float> M = gp((const float*)F.msg[l]);
+  auto src = [&](int nx, int ny, int dir) -> gptr<const float> {
+    if (first) return P + (((size_t)(ny >> 1) * wp + (nx >> 1)) * 4 + dir) * N;
+    return M + (((size_t)ny * w + nx) * 4 + dir) * N;
+  };
+  const bool msgs = !first || P;
+  const Row<S> dv = load_row<S>(gp((const float*)F.cost[l]) + ((size_t)y * w + x) * N, j);
+  Row<S> fu, fd, fl, fr;
+#pragma unroll
+  for (int s = 0; s < S; ++s) fu.v[s] = fd.v[s] = fl.v[s] = fr.v[s] = 0.0f;
+  if (msgs && has_up) fu = load_row<S>(src(x, y - 1, 1), j);
+  if (msgs && has_down) fd = load_row<S>(src(x, y + 1, 0), j);
+  if (msgs && has_left) fl = load_row<S>(src(x - 1, y, 3), j);
+  if (msgs && has_right) fr = load_row<S>(src(x + 1, y, 2), j);
+  if (ex) {
+    // belief in the order of src/bpextract.cu:330-337: data, left, right, up, down
+    Row<S> b = dv;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (has_left) b.v[s] += fl.v[s];
+      if (has_right) b.v[s] += fr.v[s];
+      if (has_up) b.v[s] += fu.v[s];
+      if (has_down) b.v[s] += fd.v[s];
+    }
+    extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63, b, imin, istep);
+  }
+  if (!upd) return;
+  Row<S> hv[4];
+  if (__all(has_up && has_down && has_left && has_right)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      hv[0].v[s] = ((dv.v[s] + fd.v[s]) + fl.v[s]) + fr.v[s];
+      hv[1].v[s] = ((dv.v[s] + fu.v[s]) + fl.v[s]) + fr.v[s];
+      hv[2].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fr.v[s];
+      hv[3].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fl.v[s];
+    }
+  } else {
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir)
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        float v = dv.v[s];
+        if (dir != 0 && has_up) v += fu.v[s];
+        if (dir != 1 && has_down) v += fd.v[s];
+        if (dir != 2 && has_left) v += fl.v[s];
+        if (dir != 3 && has_right) v += fr.v[s];
+        hv[dir].v[s] = v;
+      }
+  }
+  constexpr float inv_n = 1.0f / (float)N;
+#pragma unroll
+  for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, message_from<S>(hv[dir], p1, p2, inv_n));
+}
+
+// Winner extraction over a batch, one 16-lane row per cell (extract_winner).
+// colour < 0: every cell; else only the cells of checkerboard colour colour
+// (0: x + y even, 1: odd), the others having been extracted by the last sweep.
+template <int N, int ROWS>
+__global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
+                                                       double istep, int colour) {
+  constexpr int S = N / 16;
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int lane = threadIdx.x & 63, j = lane & 15;
+  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
+  const int w = F.lw[0], h = F.lh[0];
+  int x, y;
+  if (colour < 0) {
+    if (item >= w * h) return;
+    y = item / w;
+    x = item - y * w;
+  } else {
+    const int hw = (w + 1) >> 1;
+    if (item >= hw * h) return;
+    y = item / hw;
+    const int bx = item - y * hw;
+    x = 2 * bx + (((y & 1) ^ colour) & 1);
+    if (x >= w) return;
+  }
+  const int cell = y * w + x;
+  if (!is_node<N>(F, x, y)) {
+    if (j == 0) {
+      gp(F.dcoarse)[cell] = 0.0f;
+      gp(F.label)[cell] = -1;
+    }
+    return;
+  }
+  const gptr<const float> M = gp((const float*)F.msg[0]);
+  Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
+  Row<S> t;
+  if (x != 0) {
+    t = load_row<S>(M + (((size_t)cell - 1) * 4 + 3) * N, j);
+#pragma unroll
+    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
+  }
+  if (x != w - 1) {
+    t = load_row<S>(M + (((size_t)cell + 1) * 4 + 2) * N, j);
+#pragma unroll
+    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
+  }
+  if (y != 0) {
+    t = load_row<S>(M + (((size_t)cell - w) * 4 + 1) * N, j);
+#pragma unroll
+    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
+  }
+  if (y != h - 1) {
+    t = load_row<S>(M + (((size_t)cell + w) * 4 + 0) * N, j);
+#pragma unroll
+    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
+  }
+  extract_winner<N, S>(F, cell, j, lane, b, imin, istep);
+}
+
 // [cell][dir][row layout] -> [dir][cell][N] in sample order, for callers that
 // want the reference's message maps
 template <int N>
@@ -342,13 +392,14 @@ __global__ __launch_bounds__(256) void k_cost_to_rows(const float* __restrict__ 
 
 template <int N, int ROWS>
 static void bp_sweep_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool A,
-                           bool first, bool active, bool copy_all, const BpLaunchParams& p) {
+                           bool first, bool active, bool copy_all, bool ext, const BpLaunchParams& p) {
   hipLaunchKernelGGL((k_bp_sweep<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, l, (int)A, (int)first,
-                     (int)active, (int)copy_all, p.p1, p.p2);
+                     (int)active, (int)copy_all, p.p1, p.p2, (int)ext, p.idepth_min, p.idepth_step);
 }
 
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all) {
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all,
+                     bool ext) {
   const int rows = (t.bp_rows == 4 || t.bp_rows == 8) ? t.bp_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
@@ -360,39 +411,43 @@ void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, in
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
-    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
-    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
-    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
   } else {
-    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
-    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
-    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, p);
+    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
   }
 }
 
 template <int N, int ROWS>
 static void extract_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
-                          const BpLaunchParams& p) {
+                          const BpLaunchParams& p, int colour) {
   hipLaunchKernelGGL((k_extract<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, p.idepth_min,
-                     p.idepth_step);
+                     p.idepth_step, colour);
 }
 
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    const Tuning& t, hipStream_t s) {
+                    const Tuning& t, hipStream_t s, int colour) {
   const int rows = (t.extract_rows == 4 || t.extract_rows == 8) ? t.extract_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].lw[0] * hframes[i].lh[0] + rows - 1) / rows;
+  for (int i = 0; i < n; ++i) {
+    const int w = hframes[i].lw[0], h = hframes[i].lh[0];
+    const int items = colour < 0 ? w * h : ((w + 1) / 2) * h;
+    counts[i] = (items + rows - 1) / rows;
+  }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
-    if (rows == 4) extract_shape<64, 4>(nb, s, dframes, bm, p);
-    else if (rows == 8) extract_shape<64, 8>(nb, s, dframes, bm, p);
-    else extract_shape<64, 16>(nb, s, dframes, bm, p);
+    if (rows == 4) extract_shape<64, 4>(nb, s, dframes, bm, p, colour);
+    else if (rows == 8) extract_shape<64, 8>(nb, s, dframes, bm, p, colour);
+    else extract_shape<64, 16>(nb, s, dframes, bm, p, colour);
   } else {
-    if (rows == 4) extract_shape<128, 4>(nb, s, dframes, bm, p);
-    else if (rows == 8) extract_shape<128, 8>(nb, s, dframes, bm, p);
-    else extract_shape<128, 16>(nb, s, dframes, bm, p);
+    if (rows == 4) extract_shape<128, 4>(nb, s, dframes, bm, p, colour);
+    else if (rows == 8) extract_shape<128, 8>(nb, s, dframes, bm, p, colour);
+    else extract_shape<128, 16>(nb, s, dframes, bm, p, colour);
   }
 }
 

@@ -325,13 +325,17 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
           continue;
         }
         for (int it = 0; it < iters; ++it)
-          launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2);
+          launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2,
+                          l == 0 && iters >= 2 && it == iters - 1);
       }
       break;
     }
     case QM_STAGE_EXTRACT: {
+      // the last level-0 sweep (when not the level's first) has extracted its
+      // colour's cells: colour 0 (x + y even) is the A set, swept at even it
       Mark m(t, QM_STAGE_EXTRACT);
-      launch_extract(df, hf, n, N, bp, tu, s);
+      const int it0 = p->bp_iters[0];
+      launch_extract(df, hf, n, N, bp, tu, s, it0 >= 2 ? ((it0 - 1) % 2 == 0 ? 1 : 0) : -1);
       break;
     }
     case QM_STAGE_UPSAMPLE_ROW: {

@@ -45,10 +45,14 @@ void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int
 // cell it does not update when copy_all, else only the cells the quadtree
 // skips, which suffices when the level has a second half-sweep);
 // active = 0 turns the first sweep into a pure initialisation.
+// ext: this level-0 (non-first) sweep also extracts the winners of its colour's
+// cells (every one of them, updated or skipped), for the sweep that ends BP.
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
-                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all = true);
+                     bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all = true,
+                     bool ext = false);
+// colour < 0: every cell; 0 / 1: only the cells with x + y even / odd
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    const Tuning& t, hipStream_t s);
+                    const Tuning& t, hipStream_t s, int colour = -1);
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
 // a cost volume in sample order -> the row layout of the pipeline
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);
