@@ -8,7 +8,7 @@
 // gradient: rbar = A^{-1} zbar, and dL/dw_e = -lambda * rbar^T M_e z summed over
 // the two right-hand sides (x and n). With out = x_col / n_col the incoming
 // gradients are g / n_col and -g * out / n_col. The forward passes are
-// recomputed first (the row pass into rowxn, the column pass into colxn); the
+// recomputed first (the row pass into the rowx planes, the column pass into colxn); the
 // adjoint runs the column solve, then the row solve.
 #include "qm_device.h"
 #include "qm_launch.h"
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
       const float4 t = F.fwd[(size_t)i * H + y];
       const float vx = t.y - t.x * vx_next;
       const float vn = t.z - t.x * vn_next;
-      const float2 z = F.rowxn[(size_t)i * H + y];
+      const float2 z = make_float2(F.rowx[(size_t)i * H + y], F.rowx[(size_t)W * H + (size_t)i * H + y]);
       if (i + 1 < s1) {
         if (F.grad_wrow)
           F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_fuse_backward(const FrameDev* __restric
 }
 
 void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
-  // recompute the forward passes (row pass -> rowxn, column pass -> colxn)
+  // recompute the forward passes (row pass -> rowx planes, column pass -> colxn)
   launch_upsample_row(dframes, hframes, n, 2, nullptr, s);
   launch_upsample_col(dframes, hframes, n, 2, nullptr, false, s);
   BlockMap bm;

@@ -1,170 +1,348 @@
 // k_back.hip -- tridiagonal smoothness upsampling (reference:
 // src/depth_upsample.cu:20-302) and Gaussian x Beta fusion (reference:
 // src/depth_fusion.cu:13-213) for a batch of keyframes.
+#include <type_traits>
+
 #include "qm_device.h"
 #include "qm_launch.h"
 
 namespace qm {
 
-// Thomas coefficients of A = I + lambda * L(w) for the unknown between couplings
-// wl (to the previous unknown) and wr (to the next); lambda is a double as in
-// the reference (upsample_lambda 10.0), the couplings are summed in float first.
-__device__ __forceinline__ void thomas_coeffs(float wl, float wr, double lam, float& a, float& b, float& c) {
-  a = (float)(-lam * (double)wl);
-  b = (float)(1.0 + lam * (double)(wl + wr));
-  c = (float)(-lam * (double)wr);
-}
-
-constexpr int kChunk = 32;
+// ---------------------------------------------------------------------------
+// Upsample solve (reference: smooth_row / smooth_col, src/depth_upsample.cu:
+// 94-302). Per line (a row, then a column) the Thomas algorithm on
+// A = I + lambda L(w) with two right-hand sides (x, n), in the reference's
+// operations and order:
+//   forward   den_k = b_k - ch_{k-1} a_k,   ch_k = c_k / den_k,
+//             dh_k = (x_k - dh_{k-1} a_k) / den_k,   nh_k likewise,
+//   backward  X_k = dh_k - ch_k X_{k+1},   N_k likewise,
+// a_k = float(-lambda w(k)), c_k = float(-lambda w(k+1)) (double products) and
+// b_k = float(1 + lambda (w(k) + w(k+1))), w(k) the coupling between steps
+// k - 1 and k of the line (0 at the line start, the edge weight at its end).
+// The chain state is 0 at the start of each of the line's two segments.
+//
+// A workgroup owns 64 lines, one per lane, and splits a step's work over waves
+// that run side by side on the CU's SIMDs, one role each:
+//   producer  loads a chunk of kUK steps and writes the chunk's a (which is
+//             also the c of the step before), b and right-hand sides to LDS;
+//   ch        the coefficient chain (it depends on the weights only);
+//   x, n      the two right-hand-side chains, reading ch_{k-1} from LDS;
+//   out       (columns) X / N and the column pass's stores.
+// The waves form a pipeline over chunks: in round r the producer fills chunk
+// r, the ch wave runs chunk r - 1 and the x / n waves chunk r - 2, a barrier
+// between rounds and rings of LDS slots between the roles. The forward values
+// of a line (ch, dh, nh per step) are not kept: each chain wave stores its
+// state at every chunk start (one float per chunk) and the backward sweep
+// walks the chunks from the end, recomputes a chunk's forward values from its
+// checkpoint (the same operations on the same inputs, so the same bits) into
+// registers and sweeps back through them.
+// ---------------------------------------------------------------------------
+constexpr int kUK = 16;       // steps per chunk
+constexpr int kUS = kUK + 1;  // LDS stride of one line's chunk (odd: lane-per-line access is conflict-free)
+constexpr int kSlots = 3;     // coefficient slots: producer, ch and x / n work on three chunks at once
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Chunked walk of one serial chain [s0, s1) in steps of kChunk, with the
-// inputs of the next chunk in flight while the current one is solved: two
-// register buffers alternate (A, B). Loads of a chunk past the end re-read the
-// last chunk (clamped indices), so every load is issued unconditionally and the
-// chain only waits for the chunk it is about to use.
-template <typename Buf, typename Load, typename Step>
-__device__ __forceinline__ void chain_forward(int s0, int s1, Load load, Step step) {
-  Buf A, B;
-  int c0 = s0;
-  load(A, c0);
-  while (true) {
-    const int c1 = c0 + kChunk;
-    const bool more1 = c1 < s1;
-    load(B, more1 ? c1 : c0);
-    step(A, c0, min(kChunk, s1 - c0));
-    if (!more1) break;
-    const int c2 = c1 + kChunk;
-    const bool more2 = c2 < s1;
-    load(A, more2 ? c2 : c1);
-    step(B, c1, min(kChunk, s1 - c1));
-    if (!more2) break;
-    c0 = c2;
+// Chunks of a line of len steps split at seg: segment 0 is [0, seg), segment
+// 1 is [seg, len); chunk j covers steps [c0, c0 + cnt) of one segment.
+struct Chunks {
+  int seg, len, n0, n;
+  __device__ Chunks(int seg_in, int len_in) {
+    len = len_in;
+    seg = seg_in < len_in ? seg_in : len_in;
+    n0 = (seg + kUK - 1) / kUK;
+    n = n0 + (len - seg + kUK - 1) / kUK;
   }
-}
-template <typename Buf, typename Load, typename Step>
-__device__ __forceinline__ void chain_backward(int s0, int s1, Load load, Step step) {
-  Buf A, B;
-  int c0 = s0 + ((s1 - 1 - s0) / kChunk) * kChunk;
-  load(A, c0);
-  while (true) {
-    const int c1 = c0 - kChunk;
-    const bool more1 = c1 >= s0;
-    load(B, more1 ? c1 : c0);
-    step(A, c0, min(kChunk, s1 - c0));
-    if (!more1) break;
-    const int c2 = c1 - kChunk;
-    const bool more2 = c2 >= s0;
-    load(A, more2 ? c2 : c1);
-    step(B, c1, kChunk);
-    if (!more2) break;
-    c0 = c2;
+  __device__ void get(int j, int& c0, int& cnt, bool& first, bool& last) const {
+    const int s0 = j < n0 ? 0 : seg, s1 = j < n0 ? seg : len;
+    c0 = s0 + kUK * (j < n0 ? j : j - n0);
+    cnt = min(kUK, s1 - c0);
+    first = c0 == s0;
+    last = c0 + cnt == s1;
   }
-}
-
-struct F4Chunk {
-  float4 t[kChunk];
 };
 
+// Two producer waves share a chunk: wave HALF writes steps [8 HALF, 8 HALF + 8)
+// of the slots (a and b of those steps, and the chunk's last c, a[kUK], for
+// HALF = 1). Couplings of the half from image bytes px[j] = I(c0 - 1 +
+// 8 HALF + j) (clamped to the line): w(c0 + 8 HALF + j) and a = float(-lambda
+// w) for j = 0..8, then b.
+constexpr int kHalf = kUK / 2;
+
+template <int HALF>
+__device__ __forceinline__ void half_couplings_lut(const uint8_t (&px)[kHalf + 2], int c0, int len, float edge,
+                                                   double lam, const float* lw, const float* lm, float* Ms,
+                                                   float* Bs) {
+  float w[kHalf + 1], m[kHalf + 1];
+#pragma unroll
+  for (int j = 0; j <= kHalf; ++j) {
+    const unsigned i = __builtin_amdgcn_sad_u8((unsigned)px[j + 1], (unsigned)px[j], 0u);  // |difference|
+    w[j] = lw[i];
+    m[j] = lm[i];
+  }
+  const int x0 = c0 + kHalf * HALF;
+  if (HALF == 0 && c0 == 0) {
+    w[0] = 0.0f;
+    m[0] = (float)(-lam * 0.0);
+  }
+  if (x0 + kHalf >= len) {
+#pragma unroll
+    for (int j = 0; j <= kHalf; ++j)
+      if (x0 + j == len) {
+        w[j] = edge;
+        m[j] = (float)(-lam * (double)edge);
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < kHalf; ++j) Ms[x0 - c0 + j] = m[j];
+  if (HALF == 1) Ms[kUK] = m[kHalf];
+#pragma unroll
+  for (int j = 0; j < kHalf; ++j) Bs[x0 - c0 + j] = (float)(1.0 + lam * (double)(w[j] + w[j + 1]));
+}
+
+// The same from explicit couplings w[j] = w(c0 + 8 HALF + j).
+template <int HALF>
+__device__ __forceinline__ void half_couplings(const float (&w)[kHalf + 1], double lam, float* Ms, float* Bs) {
+#pragma unroll
+  for (int j = 0; j < kHalf; ++j) Ms[kHalf * HALF + j] = (float)(-lam * (double)w[j]);
+  if (HALF == 1) Ms[kUK] = (float)(-lam * (double)w[kHalf]);
+#pragma unroll
+  for (int j = 0; j < kHalf; ++j) Bs[kHalf * HALF + j] = (float)(1.0 + lam * (double)(w[j] + w[j + 1]));
+}
+
+// The chain waves always run a chunk's kUK steps. Only the last chunk of a
+// segment can be shorter (cnt < kUK steps); its steps past cnt run on whatever
+// the slots hold and only feed state that is discarded: the forward state is
+// reset at the next segment's start, the backward sweep restarts from 0 at the
+// segment's last step, and the stores of those steps go to a sink.
+
+// ch wave, one chunk: den_k = b_k - ch_{k-1} a_k, ch_k = c_k / den_k. Writes
+// the ring Cr[0] = ch before the chunk, Cr[k + 1] = ch_k; returns the state.
+__device__ __forceinline__ float ch_chunk(const float* Ms, const float* Bs, float* Cr, float st) {
+  float m[kUK + 1], b[kUK];
+#pragma unroll
+  for (int k = 0; k <= kUK; ++k) m[k] = Ms[k];
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) b[k] = Bs[k];
+  Cr[0] = st;
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    const float den = b[k] - st * m[k];
+    st = m[k + 1] / den;
+    Cr[k + 1] = st;
+  }
+  return st;
+}
+
+// x / n wave, one chunk forward: v_k = (r_k - v_{k-1} a_k) / den_k with den_k
+// recomputed from ch_{k-1} = Cr[k] exactly as the ch wave formed it. DERIVE:
+// the right-hand side is n_k = (x_k > 0) from the x slot.
+template <bool DERIVE>
+__device__ __forceinline__ float rhs_chunk(const float* Ms, const float* Bs, const float* Rs, const float* Cr,
+                                           float st, float (&v)[kUK]) {
+  float m[kUK], b[kUK], r[kUK], c[kUK];
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    m[k] = Ms[k];
+    b[k] = Bs[k];
+    r[k] = Rs[k];
+    c[k] = Cr[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    const float rk = DERIVE ? (r[k] > 0.0f ? 1.0f : 0.0f) : r[k];
+    const float den = b[k] - c[k] * m[k];
+    st = (rk - st * m[k]) / den;
+    v[k] = st;
+  }
+  return st;
+}
+
+// Backward round of an x / n wave: recomputes the forward values v of chunk
+// q from its checkpoint st (as rhs_chunk) and, in the same steps, sweeps back
+// through chunk q - 1 (vb in place, its ch values cb): X_k = v_k - ch_k X_{k+1},
+// restarting from 0 at step stop. The two chains are independent, so their
+// steps interleave. Also returns chunk q's ch values (Cr[k + 1]) in c1 for the
+// next round's sweep.
+template <bool DERIVE>
+__device__ __forceinline__ float recompute_back(const float* Ms, const float* Bs, const float* Rs, const float* Cr,
+                                                float st, float (&v)[kUK], float (&c1)[kUK], int stop, float nxt,
+                                                float (&vb)[kUK], const float (&cb)[kUK]) {
+  float m[kUK], b[kUK], r[kUK], c[kUK];
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    m[k] = Ms[k];
+    b[k] = Bs[k];
+    r[k] = Rs[k];
+    c[k] = Cr[k];
+    c1[k] = Cr[k + 1];
+  }
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    const float rk = DERIVE ? (r[k] > 0.0f ? 1.0f : 0.0f) : r[k];
+    const float den = b[k] - c[k] * m[k];
+    st = (rk - st * m[k]) / den;
+    v[k] = st;
+    const int kb = kUK - 1 - k;
+    nxt = kb == stop ? 0.0f : nxt;
+    nxt = vb[kb] - cb[kb] * nxt;
+    vb[kb] = nxt;
+  }
+  return nxt;
+}
+
 // ---------------------------------------------------------------------------
-// Row pass: one lane per image row, the serial Thomas chain walked in chunks.
-// Scratch (fwd) and the result (rowxn) are column-major, [x][y], so the 64
-// lanes of a wave (64 consecutive rows) store one contiguous 1 KB / 512 B row
-// segment per step. Mode 0 composes the sparse map on the fly (BP winner
-// depth at node pixels (4x, 4y), else the semi-dense pre-fill, else 0), so the
-// full-resolution sparse map is never cleared or written unless asked for;
-// mode 1 reads an explicit sparse map; mode 2 is the generalised form with
-// caller-supplied right-hand sides and weights. Same operations, in the same
-// order, as the reference's smooth_row (src/depth_upsample.cu:94-197).
+// Loads of the producers. A wave's loads are counted in order, and the
+// compiler's waits for a chunk prepared one round earlier assume the fewest
+// loads any path could have issued since; so every load below is issued
+// unconditionally and in a fixed number (no data-dependent or optional loads,
+// pointer selects instead), and lanes past the image edge load, compute and
+// store exactly what the last line's lane does. A half chunk's 8 floats of
+// one line are read as two dword-aligned float4 from min(c0 + 8 HALF, len - 8)
+// (lines are at least 16 long); the end of a line whose length is not a
+// multiple of 8 then sits shifted by sh = c0 + 8 HALF - base in the loaded
+// values and is written to LDS with that shift.
 // ---------------------------------------------------------------------------
-template <int MODE>
-struct RowIn {
-  int px[kChunk + 2];   // image at c0 - 1 + k (modes 0, 1)
-  float sp[kChunk];     // pre-fill (mode 0), sparse map (mode 1), d (mode 2)
-  float cf[kChunk];     // confidence (mode 2)
-  float wv[kChunk + 1];  // row weights (mode 2)
-  int lb[kChunk / 4];   // BP labels of node cells (mode 0)
-  float dv[kChunk / 4];  // BP depths of node cells (mode 0)
+typedef float4 f4_dword_aligned __attribute__((aligned(4)));
+
+// A half's 8 floats of one line, read as two dword-aligned float4 from
+// min(c0 + 8 HALF, len - 8); dst[8 HALF + i - sh] = v[i] for i >= sh.
+__device__ __forceinline__ void load8(gptr<const float> line, int base, float (&v)[kHalf]) {
+  const gptr<const f4_dword_aligned> p = (gptr<const f4_dword_aligned>)(line + base);
+#pragma unroll
+  for (int q = 0; q < kHalf / 4; ++q) {
+    const float4 t = p[q];
+    v[4 * q] = t.x;
+    v[4 * q + 1] = t.y;
+    v[4 * q + 2] = t.z;
+    v[4 * q + 3] = t.w;
+  }
+}
+template <int HALF>
+__device__ __forceinline__ void store8(float* dst, int sh, const float (&v)[kHalf]) {
+  if (sh == 0) {
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) dst[kHalf * HALF + i] = v[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i)
+      if (i >= sh) dst[kHalf * HALF + i - sh] = v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass. Lines are image rows, lanes rows y0 .. y0 + 63. Results X and N
+// go to the planes rowx[x][y] and rowx[W H + x][y] (column-major, so a wave's
+// store of one step is 64 consecutive floats). Mode 0 composes the sparse map
+// on the fly (BP winner depth at node pixels (4x, 4y), else the semi-dense
+// pre-fill, else 0) and writes it to sparse_out when asked; mode 1 reads an
+// explicit sparse map; mode 2 is the generalised form with caller-supplied
+// right-hand sides, confidences and weights (and writes the row results,
+// row-major, to row_x_out / row_n_out when asked).
+// ---------------------------------------------------------------------------
+struct UpRowLds {
+  float lw[256], lm[256];
+  float M[kSlots][64 * kUS], B[kSlots][64 * kUS], R[kSlots][64 * kUS], N[kSlots][64 * kUS];
+  float C[2][64 * kUS];
 };
 
 template <int MODE>
-__global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut) {
-  __shared__ float lut[256];
-  if (MODE != 2) {
-    for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
-    __syncthreads();
-  }
+struct RowBuf {                // one producer wave's half of a chunk
+  uint8_t px[kHalf + 2];       // image at c0 - 1 + 8 HALF + j (modes 0, 1)
+  float sp[kHalf];             // pre-fill (mode 0), sparse map (mode 1), d (mode 2), from base
+  float cf[kHalf];             // confidence (mode 2), from base
+  float wv[kHalf + 1];         // row couplings (mode 2)
+  int lb[kUK / 8];             // BP labels of the half's node cells (mode 0)
+  float dv[kUK / 8];           // BP depths of the half's node cells (mode 0)
+};
+
+template <int MODE>
+__global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                const float* __restrict__ wlut) {
+  __shared__ UpRowLds L;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
-  const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
   const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
   const double lam = F.lam;
-  if (y >= H) return;
+  // waves: 0 ch, 1 and 2 producers (half a chunk each), 3 x, 4 n; waves 0 and 4
+  // (the two chains with the most idle issue slots) share a SIMD
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave: uniform
+  if (MODE != 2 && threadIdx.x < 256) {
+    const float w = wlut[threadIdx.x];
+    L.lw[threadIdx.x] = w;
+    L.lm[threadIdx.x] = (float)(-lam * (double)w);
+  }
+  const int y = min((blockIdx.x - bm.start[f]) * 64 + lane, H - 1);  // lanes past the image repeat row H - 1
   const size_t row = (size_t)y * W;
-  const gptr<const uint8_t> img = gp(F.key + (size_t)y * F.pitch);
-  const gptr<float4> fwd = gp(F.fwd);
-  const gptr<float2> rowxn = gp(F.rowxn);
-  const gptr<const float> prefill = gp(F.prefill);
-  const gptr<const float> sparse_in = gp(F.sparse_in);
-  const gptr<float> sparse_out = gp(F.sparse_out);
-  const gptr<const float> g_d = gp(F.g_d);
-  const gptr<const float> g_conf = gp(F.g_conf);
-  const gptr<const float> g_wrow = gp(F.g_wrow);
-  const gptr<float> row_x_out = gp(F.row_x_out);
-  const gptr<float> row_n_out = gp(F.row_n_out);
-  // node cells of this row (mode 0): rows 4*cy, cy < Hc; other rows read a
-  // valid cell row and ignore it
-  const bool node_row = (y & 3) == 0 && (y >> 2) < Hc;
-  const size_t cb = (size_t)min(y >> 2, Hc - 1) * Wc;
-  const gptr<const int32_t> label = gp((const int32_t*)F.label) + cb;
-  const gptr<const float> dcoarse = gp((const float*)F.dcoarse) + cb;
+  const Chunks ch(F.row_seg, W);
+  const int nch = ch.n;
   const float edge = F.edge_r ? 1.0f : 0.0f;
-  const int seg = F.row_seg < W ? F.row_seg : W;
-  auto load = [&](RowIn<MODE>& b, int c0) {
-    if (MODE == 2) {
-#pragma unroll
-      for (int k = 0; k <= kChunk; ++k) b.wv[k] = g_wrow[(size_t)y * (W + 1) + min(c0 + k, W)];
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k) b.sp[k] = g_d[row + min(c0 + k, W - 1)];
-      if (g_conf) {
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) b.cf[k] = g_conf[row + min(c0 + k, W - 1)];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kChunk + 2; ++k) b.px[k] = img[clampi(c0 - 1 + k, 0, W - 1)];
-      if (MODE == 1) {
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) b.sp[k] = sparse_in[row + min(c0 + k, W - 1)];
-      } else {
-        if (prefill) {
-#pragma unroll
-          for (int k = 0; k < kChunk; ++k) b.sp[k] = prefill[row + min(c0 + k, W - 1)];
-        }
-        // node pixels (4x, 4y): chunk starts are multiples of 4 (0 or 16 * (W / 16), plus 32 k)
-#pragma unroll
-        for (int q = 0; q < kChunk / 4; ++q) {
-          const int cx = min((c0 >> 2) + q, Wc - 1);
-          b.lb[q] = label[cx];
-          b.dv[q] = dcoarse[cx];
-        }
-      }
-    }
-  };
-  for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = sg == 0 ? 0 : seg, s1 = sg == 0 ? seg : W;
-    if (s1 <= s0) continue;
-    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    chain_forward<RowIn<MODE>>(s0, s1, load, [&](RowIn<MODE>& b, int c0, int n) {
-      float w[kChunk + 1], rx[kChunk], rn[kChunk];
+  const gptr<float> ck = gp((float*)F.fwd);  // checkpoints [role][chunk][y]
+  const int lo = lane * kUS;
+  __syncthreads();
+
+  auto producer = [&](auto half_tag) __attribute__((always_inline)) {
+    constexpr int HALF = decltype(half_tag)::value;
+    constexpr int h0 = kHalf * HALF;  // the half's first step
+    const gptr<const uint8_t> img = gp(F.key + (size_t)y * F.pitch);
+    const bool have_prefill = F.prefill != nullptr;
+    // mode 0 without a pre-fill reads (and ignores) the dense buffer instead
+    const gptr<const float> src =
+        gp(MODE == 2 ? F.g_d : (MODE == 1 ? F.sparse_in : (have_prefill ? F.prefill : (const float*)F.dense)));
+    const bool have_conf = F.g_conf != nullptr;
+    const gptr<const float> conf = gp(have_conf ? F.g_conf : F.g_d);
+    const gptr<const float> g_wrow = gp(F.g_wrow);
+    const bool node_row = (y & 3) == 0 && (y >> 2) < Hc;
+    const size_t cb = (size_t)min(y >> 2, Hc - 1) * Wc;
+    const gptr<const int32_t> label = gp((const int32_t*)F.label) + cb;
+    const gptr<const float> dcoarse = gp((const float*)F.dcoarse) + cb;
+    auto load = [&](RowBuf<MODE>& b, int j) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
+      const int base = min(c0 + h0, W - kHalf);
       if (MODE == 2) {
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) w[k] = b.wv[k];
+        for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wrow[(size_t)y * (W + 1) + min(c0 + h0 + k, W)];
+        load8(src + row, base, b.sp);
+        load8(conf + row, base, b.cf);
+      } else {
+        const int x0 = c0 + h0 - 1;  // first image byte of the half
+        if (x0 >= 0 && x0 + kHalf + 2 <= W) {
+          const gptr<const uint8_t> p = img + x0;
 #pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-          if (g_conf) {
+          for (int k = 0; k < kHalf + 2; ++k) b.px[k] = p[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kHalf + 2; ++k) b.px[k] = img[clampi(x0 + k, 0, W - 1)];
+        }
+        load8(src + row, base, b.sp);
+        if (MODE == 0) {
+          // node pixels (4 cx, 4 cy); chunk starts are multiples of 4 here
+#pragma unroll
+          for (int q = 0; q < kUK / 8; ++q) {
+            const int cx = min((c0 >> 2) + 2 * HALF + q, Wc - 1);
+            b.lb[q] = label[cx];
+            b.dv[q] = dcoarse[cx];
+          }
+        }
+      }
+    };
+    auto produce = [&](RowBuf<MODE>& b, int j, int slot) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
+      const int sh = c0 + h0 - min(c0 + h0, W - kHalf);
+      float* Ms = L.M[slot] + lo;
+      float* Bs = L.B[slot] + lo;
+      float* Rs = L.R[slot] + lo;
+      float* Ns = L.N[slot] + lo;
+      if (MODE == 2) {
+        half_couplings<HALF>(b.wv, lam, Ms, Bs);
+        float rx[kHalf], rn[kHalf];
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+          if (have_conf) {
             rx[k] = b.cf[k] * b.sp[k];
             rn[k] = b.cf[k];
           } else {
@@ -172,184 +350,411 @@ __global__ __launch_bounds__(64) void k_up_row(const FrameDev* __restrict__ fram
             rn[k] = b.sp[k] > 0.0f ? 1.0f : 0.0f;
           }
         }
+        store8<HALF>(Rs, sh, rx);
+        store8<HALF>(Ns, sh, rn);
       } else {
+        half_couplings_lut<HALF>(b.px, c0, W, edge, lam, L.lw, L.lm, Ms, Bs);
+        float sv[kHalf];
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) {
-          const int x = c0 + k;
-          w[k] = (x == 0) ? 0.0f : (x == W ? edge : lut[abs(b.px[k + 1] - b.px[k])]);
-        }
-        float sv[kChunk];
-        if (MODE == 0) {
+        for (int k = 0; k < kHalf; ++k) sv[k] = (MODE == 1 || have_prefill) ? b.sp[k] : 0.0f;
+        store8<HALF>(Rs, sh, sv);
+        if (MODE == 0 && node_row) {
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) sv[k] = prefill ? b.sp[k] : 0.0f;
-          if (node_row) {
-#pragma unroll
-            for (int q = 0; q < kChunk / 4; ++q)
-              if (4 * q < n && (c0 >> 2) + q < Wc && b.lb[q] >= 0) sv[4 * q] = b.dv[q];
+          for (int q = 0; q < kUK / 8; ++q) {
+            const int k = h0 + 4 * q;
+            if (k < cnt && (c0 >> 2) + 2 * HALF + q < Wc && b.lb[q] >= 0) Rs[k] = b.dv[q];
           }
-          if (sparse_out) {
-#pragma unroll
-            for (int k = 0; k < kChunk; ++k)
-              if (k < n) sparse_out[row + c0 + k] = sv[k];
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < kChunk; ++k) sv[k] = b.sp[k];
-        }
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-          rx[k] = sv[k];
-          rn[k] = sv[k] > 0.0f ? 1.0f : 0.0f;
         }
       }
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        if (k >= n) continue;
-        float a, bb, c;
-        thomas_coeffs(w[k], w[k + 1], lam, a, bb, c);
-        const float den = bb - fc * a;
-        const float ch = c / den;
-        const float dh = (rx[k] - fd * a) / den;
-        const float nh = (rn[k] - fn * a) / den;
-        fwd[(size_t)(c0 + k) * H + y] = make_float4(ch, dh, nh, 0.0f);
-        fc = ch;
-        fd = dh;
-        fn = nh;
+    };
+    // forward: chunk r in round r; backward: chunk nch - 1 - r in round r. The
+    // next round's inputs are loaded while this round's chunk is produced (the
+    // two buffers alternate; past the last chunk the loads repeat it).
+    RowBuf<MODE> A, Bf;
+    for (int pass = 0; pass < 2; ++pass) {
+      auto chunk_at = [&](int r) __attribute__((always_inline)) { return pass == 0 ? r : nch - 1 - r; };
+      const int rounds = nch + 2 + pass;  // the backward sweep's stores trail by one round
+      load(A, chunk_at(0));
+      for (int r = 0; r < rounds; r += 2) {
+        load(Bf, chunk_at(min(r + 1, nch - 1)));
+        if (r < nch) produce(A, chunk_at(r), r % kSlots);
+        __syncthreads();
+        if (r + 1 < rounds) {
+          load(A, chunk_at(min(r + 2, nch - 1)));
+          if (r + 1 < nch) produce(Bf, chunk_at(r + 1), (r + 1) % kSlots);
+          __syncthreads();
+        }
       }
-    });
-    fd = 0.0f;
-    fn = 0.0f;
-    chain_backward<F4Chunk>(
-        s0, s1,
-        [&](F4Chunk& b, int c0) {
+    }
+  };
+
+  if (wave == 1) {
+    producer(std::integral_constant<int, 0>{});
+  } else if (wave == 2) {
+    producer(std::integral_constant<int, 1>{});
+  } else if (wave == 0) {
+    // ---------------- ch chain ----------------
+    float st = 0.0f;
+    for (int r = 0; r < nch + 2; ++r) {
+      const int j = r - 1;
+      if (j >= 0 && j < nch) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(j, c0, cnt, first, last);
+        if (first) st = 0.0f;
+        ck[(size_t)j * H + y] = st;
+        st = ch_chunk(L.M[j % kSlots] + lo, L.B[j % kSlots] + lo, L.C[j & 1] + lo, st);
+      }
+      __syncthreads();
+    }
+    // backward: the checkpoint of round r + 1 is loaded in round r (two
+    // registers alternate)
+    auto ck_at = [&](int q) __attribute__((always_inline)) { return ck[(size_t)(nch - 1 - clampi(q, 0, nch - 1)) * H + y]; };
+    auto back_round = [&](int q, float s0) __attribute__((always_inline)) {
+      if (q >= 0 && q < nch) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(nch - 1 - q, c0, cnt, first, last);
+        ch_chunk(L.M[q % kSlots] + lo, L.B[q % kSlots] + lo, L.C[q & 1] + lo, s0);
+      }
+    };
+    const int rounds = nch + 3;
+    float cA = ck_at(-1), cB;
+    for (int r = 0; r < rounds; r += 2) {
+      cB = ck_at(r);
+      back_round(r - 1, cA);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        cA = ck_at(r + 1);
+        back_round(r, cB);
+        __syncthreads();
+      }
+    }
+  } else {
+    // ---------------- x (wave 3) and n (wave 4) chains ----------------
+    const int role = wave == 3 ? 1 : 2;  // checkpoint plane 1 (x) or 2 (n)
+    const gptr<float> out = gp(F.rowx) + (role == 2 ? (size_t)W * H : 0);
+    const gptr<float> out_rm = gp(role == 2 ? F.row_n_out : F.row_x_out);
+    const gptr<float> sparse_out = gp(F.sparse_out);
+    const float* Rsrc = (role == 2 && MODE == 2) ? &L.N[0][0] : &L.R[0][0];
+    const bool derive = role == 2 && MODE != 2;
+    const size_t ckb = (size_t)role * nch * H;
+    float st = 0.0f, v[kUK];
+    for (int r = 0; r < nch + 2; ++r) {
+      const int j = r - 2;
+      if (j >= 0) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(j, c0, cnt, first, last);
+        if (first) st = 0.0f;
+        ck[ckb + (size_t)j * H + y] = st;
+        const int s = j % kSlots;
+        const float* Rs = Rsrc + s * 64 * kUS + lo;
+        if (derive)
+          st = rhs_chunk<true>(L.M[s] + lo, L.B[s] + lo, Rs, L.C[j & 1] + lo, st, v);
+        else
+          st = rhs_chunk<false>(L.M[s] + lo, L.B[s] + lo, Rs, L.C[j & 1] + lo, st, v);
+        // the composed sparse map, once (forward sweep, x wave)
+        if (MODE == 0 && role == 1 && sparse_out) {
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) b.t[k] = fwd[(size_t)min(c0 + k, W - 1) * H + y];
-        },
-        [&](F4Chunk& b, int c0, int n) {
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) sparse_out[row + c0 + k] = Rs[k];
+        }
+      }
+      __syncthreads();
+    }
+    // backward: round r recomputes chunk q = r - 2 and sweeps back through
+    // chunk q - 1 (kept in registers from round r - 1), then stores chunk
+    // q - 1; checkpoints are loaded a round ahead. Register sets alternate
+    // between rounds. A partial chunk's steps past its end store to a sink
+    // slot past the checkpoints, so every round issues the same stores.
+    float nxt = 0.0f;
+    const gptr<float> sink = ck + (size_t)3 * nch * H + y;
+    auto ck_at = [&](int q) __attribute__((always_inline)) {
+      return ck[ckb + (size_t)(nch - 1 - clampi(q, 0, nch - 1)) * H + y];
+    };
+    float va[kUK], ca[kUK], vb[kUK], cb[kUK];
 #pragma unroll
-          for (int k = kChunk - 1; k >= 0; --k) {
-            if (k >= n) continue;
-            const float xv = b.t[k].y - b.t[k].x * fd;
-            const float nv = b.t[k].z - b.t[k].x * fn;
-            rowxn[(size_t)(c0 + k) * H + y] = make_float2(xv, nv);
-            if (MODE == 2 && row_x_out) {
-              row_x_out[row + c0 + k] = xv;
-              row_n_out[row + c0 + k] = nv;
-            }
-            fd = xv;
-            fn = nv;
+    for (int k = 0; k < kUK; ++k) va[k] = ca[k] = vb[k] = cb[k] = 0.0f;
+    auto back_round = [&](int q, float s0, float (&vn)[kUK], float (&cn)[kUK], float (&vp)[kUK],
+                          const float (&cp)[kUK]) __attribute__((always_inline)) {
+      if (q >= 0 && q <= nch) {
+        const int qc = min(q, nch - 1);  // past the last chunk: recompute a valid chunk, unused
+        const int s = qc % kSlots;
+        int c0, cnt;
+        bool first, last;
+        ch.get(nch - q, c0, cnt, first, last);  // chunk q - 1 (valid when q >= 1)
+        const int stop = last ? cnt - 1 : -1;
+        if (derive)
+          nxt = recompute_back<true>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[qc & 1] + lo, s0, vn,
+                                     cn, stop, nxt, vp, cp);
+        else
+          nxt = recompute_back<false>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[qc & 1] + lo, s0, vn,
+                                      cn, stop, nxt, vp, cp);
+        if (q >= 1) {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k) *(k < cnt ? out + (size_t)(c0 + k) * H + y : sink) = vp[k];
+          if (MODE == 2 && out_rm) {
+#pragma unroll
+            for (int k = 0; k < kUK; ++k)
+              if (k < cnt) out_rm[row + c0 + k] = vp[k];
           }
-        });
+        }
+      }
+    };
+    const int rounds = nch + 3;
+    float sA = ck_at(-2), sB;
+    for (int r = 0; r < rounds; r += 2) {
+      sB = ck_at(r - 1);
+      back_round(r - 2, sA, va, ca, vb, cb);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        sA = ck_at(r);
+        back_round(r - 1, sB, vb, cb, va, ca);
+        __syncthreads();
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Column pass: one lane per column, the same chunked chain down the rows. The
-// row results are read from the lane's own column (column-major rowxn);
-// coefficients, outputs and the image are row-major and coalesced across the
-// wave. Output x / n (NaN where n = 0, as the reference,
-// src/depth_upsample.cu:198-302). Also resets the fusion z-buffer column when
-// asked (the fusion stage runs next).
+// Column pass. Lines are image columns, lanes columns x0 .. x0 + 63; the
+// right-hand sides are the row pass's X and N planes. The out wave writes
+// dense = X / N (NaN where N = 0, as the reference) row-major, the column
+// results (X, N) to colxn when asked, and, during the forward sweep, resets the
+// fusion z-buffer when asked (the fusion stage runs next).
 // ---------------------------------------------------------------------------
-template <int MODE>
-struct ColIn {
-  int px[kChunk + 2];   // key image at rows c0 - 1 + k (modes 0, 1)
-  float wv[kChunk + 1];  // column weights (mode 2)
-  float2 r[kChunk];     // row pass (x, n)
+struct UpColLds {
+  float lw[256], lm[256];
+  float M[kSlots][64 * kUS], B[kSlots][64 * kUS], R[kSlots][64 * kUS], N[kSlots][64 * kUS];
+  float C[2][64 * kUS];
+  float XO[2][64 * kUS], NO[2][64 * kUS];
 };
 
 template <int MODE>
-__global__ __launch_bounds__(64) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
-                                               const float* __restrict__ wlut, int clear_zbuf) {
-  __shared__ float lut[256];
-  if (MODE != 2) {
-    for (int i = threadIdx.x; i < 256; i += 64) lut[i] = wlut[i];
-    __syncthreads();
-  }
+struct ColBuf {                  // one producer wave's half of a chunk
+  uint8_t px[kHalf + 2];         // key image at rows c0 - 1 + 8 HALF + j (modes 0, 1)
+  float wv[kHalf + 1];           // column couplings (mode 2)
+  float rx[kHalf], rn[kHalf];    // row pass X, N of the half, from base
+};
+
+template <int MODE>
+__global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                const float* __restrict__ wlut, int clear_zbuf) {
+  __shared__ UpColLds L;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
-  const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
-  const int W = F.W, H = F.H, pitch = F.pitch;
+  const int W = F.W, H = F.H;
   const double lam = F.lam;
-  if (x >= W) return;
-  const gptr<const uint8_t> key = gp(F.key) + x;
-  const gptr<float4> fwd = gp(F.fwd);
-  const gptr<const float2> colin = gp((const float2*)F.rowxn + (size_t)x * H);
-  const gptr<float> dense = gp(F.dense);
-  const gptr<float> dense_out = gp(F.dense_out);
-  const gptr<float2> colxn = gp(F.colxn);
-  const gptr<const float> g_wcol = gp(F.g_wcol);
-  if (clear_zbuf) {
-    const gptr<unsigned long long> zbuf = gp(F.zbuf);
-    for (int i = 0; i < H; ++i) zbuf[(size_t)i * W + x] = ~0ull;
+  // waves: 0 and 1 producers (half a chunk each), 2 ch, 3 x, 4 n, 5 out; waves 0
+  // and 4, 1 and 5 share SIMDs
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave: uniform
+  if (MODE != 2 && threadIdx.x < 256) {
+    const float w = wlut[threadIdx.x];
+    L.lw[threadIdx.x] = w;
+    L.lm[threadIdx.x] = (float)(-lam * (double)w);
   }
+  const int x = min((blockIdx.x - bm.start[f]) * 64 + lane, W - 1);  // lanes past the image repeat column W - 1
+  const Chunks ch(F.col_seg, H);
+  const int nch = ch.n;
   const float edge = F.edge_c ? 1.0f : 0.0f;
-  const int seg = F.col_seg < H ? F.col_seg : H;
-  auto load = [&](ColIn<MODE>& b, int c0) {
-    if (MODE == 2) {
-#pragma unroll
-      for (int k = 0; k <= kChunk; ++k) b.wv[k] = g_wcol[(size_t)min(c0 + k, H) * W + x];
-    } else {
-#pragma unroll
-      for (int k = 0; k < kChunk + 2; ++k) b.px[k] = key[(size_t)clampi(c0 - 1 + k, 0, H - 1) * pitch];
-    }
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k) b.r[k] = colin[min(c0 + k, H - 1)];
-  };
-  for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = sg == 0 ? 0 : seg, s1 = sg == 0 ? seg : H;
-    if (s1 <= s0) continue;
-    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    chain_forward<ColIn<MODE>>(s0, s1, load, [&](ColIn<MODE>& b, int c0, int n) {
-      float w[kChunk + 1];
+  const gptr<float> ck = gp((float*)F.fwd);  // checkpoints [role][chunk][x]
+  const int lo = lane * kUS;
+  __syncthreads();
+
+  auto producer = [&](auto half_tag) __attribute__((always_inline)) {
+    constexpr int HALF = decltype(half_tag)::value;
+    constexpr int h0 = kHalf * HALF;
+    const int pitch = F.pitch;
+    const gptr<const float> g_wcol = gp(F.g_wcol);
+    const gptr<const float> colx = gp((const float*)F.rowx) + (size_t)x * H;
+    const gptr<const float> coln = colx + (size_t)W * H;
+    auto load = [&](ColBuf<MODE>& b, int j) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
       if (MODE == 2) {
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) w[k] = b.wv[k];
+        for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wcol[(size_t)min(c0 + h0 + k, H) * W + x];
       } else {
+        // rows of the key image: uniform row pointers, the lane's column as offset
 #pragma unroll
-        for (int k = 0; k <= kChunk; ++k) {
-          const int yy = c0 + k;
-          w[k] = (yy == 0) ? 0.0f : (yy == H ? edge : lut[abs(b.px[k + 1] - b.px[k])]);
+        for (int k = 0; k < kHalf + 2; ++k)
+          b.px[k] = (gp(F.key) + (size_t)clampi(c0 + h0 - 1 + k, 0, H - 1) * pitch)[(unsigned)x];
+      }
+      const int base = min(c0 + h0, H - kHalf);
+      load8(colx, base, b.rx);
+      load8(coln, base, b.rn);
+    };
+    auto produce = [&](ColBuf<MODE>& b, int j, int slot) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
+      const int sh = c0 + h0 - min(c0 + h0, H - kHalf);
+      if (MODE == 2)
+        half_couplings<HALF>(b.wv, lam, L.M[slot] + lo, L.B[slot] + lo);
+      else
+        half_couplings_lut<HALF>(b.px, c0, H, edge, lam, L.lw, L.lm, L.M[slot] + lo, L.B[slot] + lo);
+      store8<HALF>(L.R[slot] + lo, sh, b.rx);
+      store8<HALF>(L.N[slot] + lo, sh, b.rn);
+    };
+    ColBuf<MODE> A, Bf;
+    for (int pass = 0; pass < 2; ++pass) {
+      auto chunk_at = [&](int r) __attribute__((always_inline)) { return pass == 0 ? r : nch - 1 - r; };
+      const int rounds = nch + 2 + 2 * pass;  // backward: the out wave trails by two rounds
+      load(A, chunk_at(0));
+      for (int r = 0; r < rounds; r += 2) {
+        load(Bf, chunk_at(min(r + 1, nch - 1)));
+        if (r < nch) produce(A, chunk_at(r), r % kSlots);
+        __syncthreads();
+        if (r + 1 < rounds) {
+          load(A, chunk_at(min(r + 2, nch - 1)));
+          if (r + 1 < nch) produce(Bf, chunk_at(r + 1), (r + 1) % kSlots);
+          __syncthreads();
         }
       }
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        if (k >= n) continue;
-        float a, bb, c;
-        thomas_coeffs(w[k], w[k + 1], lam, a, bb, c);
-        const float den = bb - fc * a;
-        const float ch = c / den;
-        const float dh = (b.r[k].x - fd * a) / den;
-        const float nh = (b.r[k].y - fn * a) / den;
-        fwd[(size_t)(c0 + k) * W + x] = make_float4(ch, dh, nh, 0.0f);
-        fc = ch;
-        fd = dh;
-        fn = nh;
+    }
+  };
+
+  if (wave == 0) {
+    producer(std::integral_constant<int, 0>{});
+  } else if (wave == 1) {
+    producer(std::integral_constant<int, 1>{});
+  } else if (wave == 2) {
+    // ---------------- ch chain ----------------
+    float st = 0.0f;
+    for (int r = 0; r < nch + 2; ++r) {
+      const int j = r - 1;
+      if (j >= 0 && j < nch) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(j, c0, cnt, first, last);
+        if (first) st = 0.0f;
+        ck[(size_t)j * W + x] = st;
+        st = ch_chunk(L.M[j % kSlots] + lo, L.B[j % kSlots] + lo, L.C[j & 1] + lo, st);
       }
-    });
-    fd = 0.0f;
-    fn = 0.0f;
-    chain_backward<F4Chunk>(
-        s0, s1,
-        [&](F4Chunk& b, int c0) {
+      __syncthreads();
+    }
+    auto ck_at = [&](int q) __attribute__((always_inline)) { return ck[(size_t)(nch - 1 - clampi(q, 0, nch - 1)) * W + x]; };
+    auto back_round = [&](int q, float s0) __attribute__((always_inline)) {
+      if (q >= 0 && q < nch) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(nch - 1 - q, c0, cnt, first, last);
+        ch_chunk(L.M[q % kSlots] + lo, L.B[q % kSlots] + lo, L.C[q & 1] + lo, s0);
+      }
+    };
+    const int rounds = nch + 4;
+    float cA = ck_at(-1), cB;
+    for (int r = 0; r < rounds; r += 2) {
+      cB = ck_at(r);
+      back_round(r - 1, cA);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        cA = ck_at(r + 1);
+        back_round(r, cB);
+        __syncthreads();
+      }
+    }
+  } else if (wave == 3 || wave == 4) {
+    // ---------------- x (wave 3) and n (wave 4) chains ----------------
+    const int role = wave == 3 ? 1 : 2;
+    const float* Rsrc = role == 2 ? &L.N[0][0] : &L.R[0][0];
+    float* Osrc = role == 2 ? &L.NO[0][0] : &L.XO[0][0];
+    const size_t ckb = (size_t)role * nch * W;
+    float st = 0.0f, v[kUK];
+    for (int r = 0; r < nch + 2; ++r) {
+      const int j = r - 2;
+      if (j >= 0) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(j, c0, cnt, first, last);
+        if (first) st = 0.0f;
+        ck[ckb + (size_t)j * W + x] = st;
+        const int s = j % kSlots;
+        st = rhs_chunk<false>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[j & 1] + lo, st, v);
+      }
+      __syncthreads();
+    }
+    // backward: round r recomputes chunk q = r - 2 and sweeps back through
+    // chunk q - 1 (registers from round r - 1) into the out wave's slot
+    float nxt = 0.0f;
+    auto ck_at = [&](int q) __attribute__((always_inline)) {
+      return ck[ckb + (size_t)(nch - 1 - clampi(q, 0, nch - 1)) * W + x];
+    };
+    float va[kUK], ca[kUK], vb[kUK], cb[kUK];
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) b.t[k] = fwd[(size_t)min(c0 + k, H - 1) * W + x];
-        },
-        [&](F4Chunk& b, int c0, int n) {
+    for (int k = 0; k < kUK; ++k) va[k] = ca[k] = vb[k] = cb[k] = 0.0f;
+    auto back_round = [&](int q, float s0, float (&vn)[kUK], float (&cn)[kUK], float (&vp)[kUK],
+                          const float (&cp)[kUK]) __attribute__((always_inline)) {
+      if (q >= 0 && q <= nch) {
+        const int qc = min(q, nch - 1);
+        const int s = qc % kSlots;
+        int c0, cnt;
+        bool first, last;
+        ch.get(nch - q, c0, cnt, first, last);  // chunk q - 1 (valid when q >= 1)
+        const int stop = last ? cnt - 1 : -1;
+        nxt = recompute_back<false>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[qc & 1] + lo, s0, vn, cn,
+                                    stop, nxt, vp, cp);
+        if (q >= 1) {
+          float* O = Osrc + ((q - 1) & 1) * 64 * kUS + lo;
 #pragma unroll
-          for (int k = kChunk - 1; k >= 0; --k) {
-            if (k >= n) continue;
-            const float xv = b.t[k].y - b.t[k].x * fd;
-            const float nv = b.t[k].z - b.t[k].x * fn;
-            const float out = xv / nv;
+          for (int k = 0; k < kUK; ++k) O[k] = vp[k];
+        }
+      }
+    };
+    const int rounds = nch + 4;
+    float sA = ck_at(-2), sB;
+    for (int r = 0; r < rounds; r += 2) {
+      sB = ck_at(r - 1);
+      back_round(r - 2, sA, va, ca, vb, cb);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        sA = ck_at(r);
+        back_round(r - 1, sB, vb, cb, va, ca);
+        __syncthreads();
+      }
+    }
+  } else {
+    // ---------------- out ----------------
+    const gptr<float> dense = gp(F.dense);
+    const gptr<float> dense_out = gp(F.dense_out);
+    const gptr<float2> colxn = gp(F.colxn);
+    const gptr<unsigned long long> zbuf = gp(F.zbuf);
+    for (int r = 0; r < nch + 2; ++r) {
+      if (clear_zbuf && r < nch) {
+        int c0, cnt;
+        bool first, last;
+        ch.get(r, c0, cnt, first, last);
+#pragma unroll
+        for (int k = 0; k < kUK; ++k)
+          if (k < cnt) zbuf[(size_t)(c0 + k) * W + x] = ~0ull;
+      }
+      __syncthreads();
+    }
+    for (int r = 0; r < nch + 4; ++r) {
+      const int q = r - 4;
+      if (q >= 0) {
+        const int j = nch - 1 - q;
+        int c0, cnt;
+        bool first, last;
+        ch.get(j, c0, cnt, first, last);
+        const float* XO = L.XO[q & 1] + lo;
+        const float* NO = L.NO[q & 1] + lo;
+#pragma unroll
+        for (int k = 0; k < kUK; ++k) {
+          if (k < cnt) {
+            const float xv = XO[k], nv = NO[k];
             const size_t o = (size_t)(c0 + k) * W + x;
+            const float d = xv / nv;
             if (colxn) colxn[o] = make_float2(xv, nv);
-            dense[o] = out;
-            if (dense_out) dense_out[o] = out;
-            fd = xv;
-            fn = nv;
+            dense[o] = d;
+            if (dense_out) dense_out[o] = d;
           }
-        });
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -537,11 +942,11 @@ void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n
   const int nb = rows_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
+    hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
+    hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
   else
-    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut);
+    hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
@@ -550,9 +955,9 @@ void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n
   const int nb = cols_map(bm, hframes, n);
   if (nb <= 0) return;
   if (mode == 2)
-    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, (int)clear_zbuf);
   else
-    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, (int)clear_zbuf);
+    hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, (int)clear_zbuf);
 }
 
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {

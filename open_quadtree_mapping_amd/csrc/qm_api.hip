@@ -181,7 +181,7 @@ size_t frame_bytes(const FrameDev& F, int N) {
   const size_t cc = (size_t)F.Wc * F.Hc;
   b += 2 * align256(cc * 4);               // dcoarse, label
   b += align256(px * 16);                  // fwd
-  b += align256(px * 8);                   // rowxn
+  b += align256(px * 8);                   // rowx (X and N planes)
   b += align256(px * 4);                   // dense
   b += 2 * align256(px * 16);              // seeds x2
   b += align256(px * 8);                   // zbuf
@@ -622,7 +622,7 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
   if ((rc = sc.get(&rxn, (size_t)W * H * 8))) return rc;
   if ((rc = sc.get(&dl, sizeof(lut)))) return rc;
   F.fwd = (float4*)fwd;
-  F.rowxn = (float2*)rxn;
+  F.rowx = (float*)rxn;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dl, lut, sizeof(lut), hipMemcpyHostToDevice, s));
   launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, false, s);
@@ -666,7 +666,7 @@ int qm_upsample_general_batch(const qm_upsample_problem* probs, int n, void* str
     if ((rc = sc.get(&fwd, px * 16))) return rc;
     if ((rc = sc.get(&rxn, px * 8))) return rc;
     F.fwd = (float4*)fwd;
-    F.rowxn = (float2*)rxn;
+    F.rowx = (float*)rxn;
   }
   QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
   launch_upsample((FrameDev*)df, hf.data(), n, 2, nullptr, false, s);
@@ -713,7 +713,7 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
     if ((rc = sc.get(&axn, px * 8))) return rc;
     if ((rc = sc.get(&dense, px * 4))) return rc;
     F.fwd = (float4*)fwd;
-    F.rowxn = (float2*)rxn;
+    F.rowx = (float*)rxn;
     F.colxn = (float2*)cxn;
     F.adjxn = (float2*)axn;
     F.dense = (float*)dense;
@@ -985,7 +985,7 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
     F.dcoarse = cv.take<float>(cc * 4);
     F.label = cv.take<int32_t>(cc * 4);
     F.fwd = cv.take<float4>(px * 16);
-    F.rowxn = cv.take<float2>(px * 8);
+    F.rowx = cv.take<float>(px * 8);
     F.dense = cv.take<float>(px * 4);
     b->seeds[0][i] = cv.take<float4>(px * 16);
     b->seeds[1][i] = cv.take<float4>(px * 16);

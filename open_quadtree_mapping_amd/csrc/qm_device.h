@@ -35,8 +35,8 @@ struct FrameDev {
   int32_t* label;       // Hc x Wc
   int32_t* count;       // Hc x Wc or null
   float* sparse_out;    // H x W or null
-  float4* fwd;          // H x W Thomas forward coefficients
-  float2* rowxn;        // H x W row pass result
+  float4* fwd;          // H x W scratch: upsample chunk checkpoints, adjoint Thomas coefficients
+  float* rowx;          // row pass result: X then N, each a W x H plane [x][y]
   float* dense;         // H x W
   float* dense_out;     // H x W copy for the caller or null
   float4* seeds;        // H x W (warped in place)
