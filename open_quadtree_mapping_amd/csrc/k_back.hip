@@ -158,11 +158,10 @@ __device__ __forceinline__ float rhs_chunk(const float* Ms, const float* Bs, con
   return st;
 }
 
-// Backward round of an x / n wave: recomputes the forward values v of chunk
-// q from its checkpoint st (as rhs_chunk) and, in the same steps, sweeps back
-// through chunk q - 1 (vb in place, its ch values cb): X_k = v_k - ch_k X_{k+1},
-// restarting from 0 at step stop. The two chains are independent, so their
-// steps interleave. Also returns chunk q's ch values (Cr[k + 1]) in c1 for the
+// Backward round of an x / n wave: sweeps back through chunk q - 1 (vb in
+// place, its ch values cb): X_k = v_k - ch_k X_{k+1}, restarting from 0 at step
+// stop, then recomputes the forward values v of chunk q from its checkpoint st
+// (as rhs_chunk). Also returns chunk q's ch values (Cr[k + 1]) in c1 for the
 // next round's sweep.
 template <bool DERIVE>
 __device__ __forceinline__ float recompute_back(const float* Ms, const float* Bs, const float* Rs, const float* Cr,
@@ -178,15 +177,17 @@ __device__ __forceinline__ float recompute_back(const float* Ms, const float* Bs
     c1[k] = Cr[k + 1];
   }
 #pragma unroll
+  for (int kb = kUK - 1; kb >= 0; --kb) {
+    nxt = kb == stop ? 0.0f : nxt;
+    nxt = vb[kb] - cb[kb] * nxt;
+    vb[kb] = nxt;
+  }
+#pragma unroll
   for (int k = 0; k < kUK; ++k) {
     const float rk = DERIVE ? (r[k] > 0.0f ? 1.0f : 0.0f) : r[k];
     const float den = b[k] - c[k] * m[k];
     st = (rk - st * m[k]) / den;
     v[k] = st;
-    const int kb = kUK - 1 - k;
-    nxt = kb == stop ? 0.0f : nxt;
-    nxt = vb[kb] - cb[kb] * nxt;
-    vb[kb] = nxt;
   }
   return nxt;
 }
