@@ -273,7 +273,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
           // the row's three taps are consecutive bytes (offsets o, o + 1, o + 2,
           // with o + 4 inside the image), and byte gathers, in a divergent
           // branch, for the rows of other lanes; t keeps the offsets, replaced
-          // by the texels in such rows. All loads are issued before any SAD.
+          // by the texels in such rows, which are then packed into the row's
+          // dword. All loads are issued before any SAD.
           unsigned t[4][3][3], v[4][3];
           bool rw[4][3];
 #pragma unroll
@@ -293,15 +294,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
+            // a gathered row's three texels packed like a dword row (divergent:
+            // skipped when no lane of the wave gathered that row)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              if (!rw[r][j]) v[r][j] = t[r][j][0] | (t[r][j][1] << 8) | (t[r][j][2] << 16);
             unsigned sad = 0;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              const unsigned a = __builtin_amdgcn_sad_u8(v[r][j] & 0xFFFFFFu, prow[j], 0u);
-              const unsigned b = __builtin_amdgcn_sad_u8(
-                  t[r][j][2], patch[2][j],
-                  __builtin_amdgcn_sad_u8(t[r][j][1], patch[1][j], __builtin_amdgcn_sad_u8(t[r][j][0], patch[0][j], 0u)));
-              sad += rw[r][j] ? a : b;
-            }
+            for (int j = 0; j < 3; ++j) sad = __builtin_amdgcn_sad_u8(v[r][j] & 0xFFFFFFu, prow[j], sad);
             cr[r] = nr[r] != 0.0f ? (float)sad : 0.0f;
           }
         } else {
