@@ -120,7 +120,8 @@ def stage_bytes(levels, W, H, N):
     px = W * H
     return {
         "quadtree": 2 * px,
-        "cost": 5 * px + C + 4 * N * C,
+        # level 0, and levels 1-3 from the cost kernel's tiles (its fused pyramid)
+        "cost": 5 * px + C + 4 * N * C + 4 * N * sum(a * b for a, b in zip(lw[1:], lh[1:])),
         "pyramid": 4 * N * sum(a * b for a, b in zip(lw, lh)),
         "bp_sweep": bp,
         "extract": nodes * (20 * N + 8) + others * 9,
@@ -248,6 +249,8 @@ def main():
     for k, (ms, cnt) in acc.items():
         per_run_ms = ms / reps
         launches = cnt // reps
+        if launches == 0:
+            continue  # a stage folded into another (the pyramid, into the cost volume)
         gbs = stage_alg[k] / (per_run_ms * 1e-3) / 1e9 if per_run_ms > 0 else 0.0
         stages[k] = {"ms_per_run": round(per_run_ms, 4), "launches": launches,
                      "avg_launch_us": round(1e3 * per_run_ms / max(launches, 1), 2),

@@ -192,26 +192,87 @@ __device__ __forceinline__ float patch_sad_clamped(gptr<const uint8_t> ref, int 
   return sad;
 }
 
+// Levels 1-3 of the cost pyramid (reference: src/bpextract.cu:59-78,169-198)
+// for the 8 x 8 level-0 tile (tx, ty) held in LDS (cell c = 8 row + column at
+// t0[c * stride], samples in order): its 4 x 4 level-1, 2 x 2 level-2 cells and
+// its level-3 cell. Children are summed from 0.0f in the order (2x,2y),
+// (2x,2y+1), (2x+1,2y), (2x+1,2y+1), skipping those outside their level, as the
+// stand-alone pyramid kernel does; the results go to the same positions (row
+// layout or sample order) as level 0.
+template <int N, int THREADS>
+__device__ __forceinline__ void cost_pyramid_tile(const FrameDev& F, const float* t0, int stride, int tx, int ty,
+                                                  int natural) {
+  __shared__ float t1[16 * N], t2[4 * N];
+  const int w0 = F.lw[0], h0 = F.lh[0], w1 = F.lw[1], h1 = F.lh[1], w2 = F.lw[2], h2 = F.lh[2];
+  auto pos = [&](int d) __attribute__((always_inline)) { return natural ? d : row_pos<N>(d); };
+  // level 1
+  for (int i = threadIdx.x; i < 16 * N; i += THREADS) {
+    const int c = i / N, d = i - c * N;
+    const int lx = c & 3, ly = c >> 2;  // in the tile's 4 x 4 level-1 cells
+    float sum = 0.0f;
+#pragma unroll
+    for (int i0 = 0; i0 < 2; ++i0)
+#pragma unroll
+      for (int j0 = 0; j0 < 2; ++j0) {
+        const int cx = 2 * lx + i0, cy = 2 * ly + j0;
+        if (8 * tx + cx < w0 && 8 * ty + cy < h0) sum += t0[(8 * cy + cx) * stride + d];
+      }
+    t1[c * N + d] = sum;
+    const int x1 = 4 * tx + lx, y1 = 4 * ty + ly;
+    if (x1 < w1 && y1 < h1) gp(F.cost[1])[((size_t)y1 * w1 + x1) * N + pos(d)] = sum;
+  }
+  __syncthreads();
+  // level 2
+  for (int i = threadIdx.x; i < 4 * N; i += THREADS) {
+    const int c = i / N, d = i - c * N;
+    const int lx = c & 1, ly = c >> 1;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+      for (int j1 = 0; j1 < 2; ++j1) {
+        const int cx = 2 * lx + i1, cy = 2 * ly + j1;
+        if (4 * tx + cx < w1 && 4 * ty + cy < h1) sum += t1[(4 * cy + cx) * N + d];
+      }
+    t2[c * N + d] = sum;
+    const int x2 = 2 * tx + lx, y2 = 2 * ty + ly;
+    if (x2 < w2 && y2 < h2) gp(F.cost[2])[((size_t)y2 * w2 + x2) * N + pos(d)] = sum;
+  }
+  __syncthreads();
+  // level 3: one cell, the tile's own
+  for (int d = threadIdx.x; d < N; d += THREADS) {
+    float sum = 0.0f;
+#pragma unroll
+    for (int i2 = 0; i2 < 2; ++i2)
+#pragma unroll
+      for (int j2 = 0; j2 < 2; ++j2)
+        if (2 * tx + i2 < w2 && 2 * ty + j2 < h2) sum += t2[(2 * j2 + i2) * N + d];
+    gp(F.cost[3])[((size_t)ty * F.lw[3] + tx) * N + pos(d)] = sum;
+  }
+}
+
 template <int N, bool FAST, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
-                                                     const float* __restrict__ depth_table, int nblocks, int natural) {
-  // block = 64 consecutive cells x N samples; wave w takes samples
-  // [w*N/WAVES, (w+1)*N/WAVES) of every cell
+                                                     const float* __restrict__ depth_table, int nblocks, int natural,
+                                                     int pyramid) {
+  // block = an 8 x 8 tile of cells x N samples (lane = cell); wave w takes
+  // samples [w*N/WAVES, (w+1)*N/WAVES) of every cell
   constexpr int kPerWave = N / WAVES;
   constexpr int kRow = N + 1;  // LDS row stride in floats (conflict-free column writes)
   __shared__ float tile[64 * kRow];
   const int lb_global = xcd_logical_block(blockIdx.x, nblocks);
   const int f = find_frame(bm, lb_global);
   const FrameDev& F = frames[f];
-  const int W = F.W, H = F.H, Wc = F.Wc;
-  const int ncell = Wc * F.Hc;
-  const int cell0 = (lb_global - bm.start[f]) * 64;
+  const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
+  const int tiles_x = (Wc + 7) >> 3;
+  const int tl = lb_global - bm.start[f];
+  const int ty = tl / tiles_x, tx = tl - ty * tiles_x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cell = cell0 + lane;
+  const int cxx = 8 * tx + (lane & 7), cyy = 8 * ty + (lane >> 3);
+  const int cell = cyy * Wc + cxx;
   const int dbase = kPerWave * w;
   float* trow = tile + lane * kRow + dbase;
-  if (cell < ncell) {
-    const int cyy = cell / Wc, cxx = cell - cyy * Wc;
+  if (cxx < Wc && cyy < Hc) {
     const int x = 4 * cxx, y = 4 * cyy;
     const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)y * W + x];
     const bool node = !(x >= W - 1 || y >= H - 1 || x <= 0 || y <= 0 || (x % qs) != 0 || (y % qs) != 0);
@@ -320,15 +381,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
     }
   }
   __syncthreads();
-  // the block's 64 cost rows are contiguous in the volume: written as float4,
-  // in the row layout of the pipeline or, for callers, in sample order
-  const gptr<float4> out = (gptr<float4>)(gp(F.cost[0]) + (size_t)cell0 * N);
+  // each tile row's 8 cost rows are contiguous in the volume: written as
+  // float4, in the row layout of the pipeline or, for callers, in sample order
   constexpr int kQ = N / 4;  // float4 per row
 #pragma unroll
   for (int k = 0; k < (16 * N) / (64 * WAVES); ++k) {
-    const int i = threadIdx.x + 64 * WAVES * k;  // float4 index in the 64-row tile
+    const int i = threadIdx.x + 64 * WAVES * k;  // float4 index in the tile
     const int c = i / kQ, q = i - (i / kQ) * kQ;
-    if (cell0 + c < ncell) {
+    const int ox = 8 * tx + (c & 7), oy = 8 * ty + (c >> 3);
+    if (ox < Wc && oy < Hc) {
       const float* src = tile + c * kRow;
       float4 v;
       if (natural) {
@@ -337,9 +398,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
         v = make_float4(src[row_sample<N>(4 * q)], src[row_sample<N>(4 * q + 1)], src[row_sample<N>(4 * q + 2)],
                         src[row_sample<N>(4 * q + 3)]);
       }
-      out[i] = v;
+      ((gptr<float4>)(gp(F.cost[0]) + ((size_t)oy * Wc + ox) * N))[q] = v;
     }
   }
+  if (pyramid) cost_pyramid_tile<N, 64 * WAVES>(F, tile, kRow, tx, ty, natural);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,32 +468,33 @@ void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hi
 
 template <int N, bool FAST>
 static void launch_cost_shape(int waves, int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
-                              const float* table, int natural) {
+                              const float* table, int natural, int pyramid) {
   if (waves == 2)
-    hipLaunchKernelGGL((k_cost<N, FAST, 2>), dim3(nb), dim3(128), 0, s, dframes, bm, table, nb, natural);
+    hipLaunchKernelGGL((k_cost<N, FAST, 2>), dim3(nb), dim3(128), 0, s, dframes, bm, table, nb, natural, pyramid);
   else if (waves == 8)
-    hipLaunchKernelGGL((k_cost<N, FAST, 8>), dim3(nb), dim3(512), 0, s, dframes, bm, table, nb, natural);
+    hipLaunchKernelGGL((k_cost<N, FAST, 8>), dim3(nb), dim3(512), 0, s, dframes, bm, table, nb, natural, pyramid);
   else
-    hipLaunchKernelGGL((k_cost<N, FAST, 4>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, natural);
+    hipLaunchKernelGGL((k_cost<N, FAST, 4>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, natural, pyramid);
 }
 
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
-                 const Tuning& t, hipStream_t s) {
+                 const Tuning& t, hipStream_t s, bool pyramid) {
   BlockMap bm;
   int counts[kMaxFrames];
   bool fast = true;
   for (int i = 0; i < n; ++i) {
-    counts[i] = (hframes[i].Wc * hframes[i].Hc + 63) / 64;
+    counts[i] = ((hframes[i].Wc + 7) / 8) * ((hframes[i].Hc + 7) / 8);  // 8 x 8 tiles of cells
     for (int r = 0; r < 4; ++r) fast = fast && cost_fast_taps(hframes[i].T[r]);
   }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
+  const int pyr = pyramid ? 1 : 0;
   if (N == 64) {
-    if (fast) launch_cost_shape<64, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
-    else launch_cost_shape<64, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
+    if (fast) launch_cost_shape<64, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
+    else launch_cost_shape<64, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
   } else {
-    if (fast) launch_cost_shape<128, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
-    else launch_cost_shape<128, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural);
+    if (fast) launch_cost_shape<128, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
+    else launch_cost_shape<128, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
   }
 }
 

@@ -303,15 +303,21 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
     }
     case QM_STAGE_COST: {
       Mark m(t, QM_STAGE_COST);
-      launch_cost(df, hf, n, N, b->table, false, tu, s);
       bool prior = false;
       for (int i = 0; i < n; ++i) prior |= hf[i].prior != nullptr;
+      // without a prior the cost kernel builds the pyramid from its tiles;
+      // the prior cost changes level 0 first, then the pyramid stage runs
+      launch_cost(df, hf, n, N, b->table, false, tu, s, !prior);
       if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
       break;
     }
     case QM_STAGE_PYRAMID: {
-      Mark m(t, QM_STAGE_PYRAMID);
-      launch_pyramid(df, hf, n, N, s);
+      bool prior = false;
+      for (int i = 0; i < n; ++i) prior |= hf[i].prior != nullptr;
+      if (prior) {
+        Mark m(t, QM_STAGE_PYRAMID);
+        launch_pyramid(df, hf, n, N, s);
+      }
       break;
     }
     case QM_STAGE_BP_SWEEP: {
@@ -420,20 +426,29 @@ int run_pipeline(qm_batch* b, int par, hipStream_t s) {
   }
   auto stream_of = [&](int g) { return g == 0 ? s : b->side[g - 1]; };
   QM_HIP(hipEventRecord(b->ev_fork, s));
-  // Stages are enqueued in the order they are expected to run: stage u of group
-  // g in slot u + g * lag (lag = stages up to the stagger point), ties by group.
-  // Group g waits on its own stream for group g - 1's mark (or the fork)
-  // before its first stage.
-  const int lag = plan.mark_after + 1;
-  const int slots = QM_NUM_STAGES + (G - 1) * lag;
+  // The stages that launch anything (without a prior cost the pyramid is part
+  // of the cost volume). They are enqueued in the order they are expected to
+  // run: active stage u of group g in slot u + g * lag (lag = stages up to the
+  // stagger point), ties by group, so that a group's next stage is always
+  // enqueued before the next group's first one (a captured graph keeps a
+  // node's first dependent on the node's own queue). Group g waits on its own
+  // stream for group g - 1's mark (or the fork) before its first stage.
+  int active[QM_NUM_STAGES], na = 0;
+  for (int u = 0; u < QM_NUM_STAGES; ++u)
+    if (u != QM_STAGE_PYRAMID || any_prior(b)) active[na++] = u;
+  int mark = -1;  // the stagger point (a stage number) as an index into active
+  for (int u = 0; u < na; ++u)
+    if (active[u] == plan.mark_after) mark = u;
+  const int lag = mark + 1;
+  const int slots = na + (G - 1) * lag;
   for (int slot = 0; slot < slots; ++slot) {
     for (int g = 0; g < G; ++g) {
       const int u = slot - g * lag;
-      if (u < 0 || u >= QM_NUM_STAGES) continue;
+      if (u < 0 || u >= na) continue;
       hipStream_t sg = stream_of(g);
       if (u == 0 && g > 0) QM_HIP(hipStreamWaitEvent(sg, lag > 0 ? b->ev_mark[g - 1] : b->ev_fork, 0));
-      enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], u);
-      if (lag > 0 && u == plan.mark_after && g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], sg));
+      enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[u]);
+      if (lag > 0 && u == mark && g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], sg));
     }
   }
   for (int g = 1; g < G; ++g) {
