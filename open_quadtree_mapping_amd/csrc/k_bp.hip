@@ -339,28 +339,21 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
     }
     return;
   }
+  // all five rows are loaded at once (a missing neighbour's row is replaced by
+  // a valid one and not added), so their latencies overlap
   const gptr<const float> M = gp((const float*)F.msg[0]);
+  const bool hl = x != 0, hr = x != w - 1, hu = y != 0, hd = y != h - 1;
   Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
-  Row<S> t;
-  if (x != 0) {
-    t = load_row<S>(M + (((size_t)cell - 1) * 4 + 3) * N, j);
+  const Row<S> tl = load_row<S>(M + (((size_t)(hl ? cell - 1 : cell)) * 4 + 3) * N, j);
+  const Row<S> tr = load_row<S>(M + (((size_t)(hr ? cell + 1 : cell)) * 4 + 2) * N, j);
+  const Row<S> tu = load_row<S>(M + (((size_t)(hu ? cell - w : cell)) * 4 + 1) * N, j);
+  const Row<S> td = load_row<S>(M + (((size_t)(hd ? cell + w : cell)) * 4 + 0) * N, j);
 #pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (x != w - 1) {
-    t = load_row<S>(M + (((size_t)cell + 1) * 4 + 2) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (y != 0) {
-    t = load_row<S>(M + (((size_t)cell - w) * 4 + 1) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
-  }
-  if (y != h - 1) {
-    t = load_row<S>(M + (((size_t)cell + w) * 4 + 0) * N, j);
-#pragma unroll
-    for (int s = 0; s < S; ++s) b.v[s] += t.v[s];
+  for (int s = 0; s < S; ++s) {
+    b.v[s] = hl ? b.v[s] + tl.v[s] : b.v[s];
+    b.v[s] = hr ? b.v[s] + tr.v[s] : b.v[s];
+    b.v[s] = hu ? b.v[s] + tu.v[s] : b.v[s];
+    b.v[s] = hd ? b.v[s] + td.v[s] : b.v[s];
   }
   extract_winner<N, S>(F, cell, j, lane, b, imin, istep);
 }
