@@ -176,11 +176,19 @@ __device__ __forceinline__ float recompute_back(const float* Ms, const float* Bs
     c[k] = Cr[k];
     c1[k] = Cr[k + 1];
   }
+  if (stop < 0) {  // no segment end in the chunk (the usual case)
 #pragma unroll
-  for (int kb = kUK - 1; kb >= 0; --kb) {
-    nxt = kb == stop ? 0.0f : nxt;
-    nxt = vb[kb] - cb[kb] * nxt;
-    vb[kb] = nxt;
+    for (int kb = kUK - 1; kb >= 0; --kb) {
+      nxt = vb[kb] - cb[kb] * nxt;
+      vb[kb] = nxt;
+    }
+  } else {
+#pragma unroll
+    for (int kb = kUK - 1; kb >= 0; --kb) {
+      nxt = kb == stop ? 0.0f : nxt;
+      nxt = vb[kb] - cb[kb] * nxt;
+      vb[kb] = nxt;
+    }
   }
 #pragma unroll
   for (int k = 0; k < kUK; ++k) {
@@ -493,8 +501,15 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
           nxt = recompute_back<false>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[qc & 1] + lo, s0, vn,
                                       cn, stop, nxt, vp, cp);
         if (q >= 1) {
+          if (cnt == kUK) {
+            // 32-bit offsets from the plane's base (W H < 2^28 floats)
+            const unsigned o0 = (unsigned)c0 * (unsigned)H + (unsigned)y;
 #pragma unroll
-          for (int k = 0; k < kUK; ++k) *(k < cnt ? out + (size_t)(c0 + k) * H + y : sink) = vp[k];
+            for (int k = 0; k < kUK; ++k) out[o0 + (unsigned)k * (unsigned)H] = vp[k];
+          } else {
+#pragma unroll
+            for (int k = 0; k < kUK; ++k) *(k < cnt ? out + (size_t)(c0 + k) * H + y : sink) = vp[k];
+          }
           if (MODE == 2 && out_rm) {
 #pragma unroll
             for (int k = 0; k < kUK; ++k)
@@ -742,16 +757,32 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
         ch.get(j, c0, cnt, first, last);
         const float* XO = L.XO[q & 1] + lo;
         const float* NO = L.NO[q & 1] + lo;
+        float xv[kUK], nv[kUK], d[kUK];
 #pragma unroll
         for (int k = 0; k < kUK; ++k) {
-          if (k < cnt) {
-            const float xv = XO[k], nv = NO[k];
-            const size_t o = (size_t)(c0 + k) * W + x;
-            const float d = xv / nv;
-            if (colxn) colxn[o] = make_float2(xv, nv);
-            dense[o] = d;
-            if (dense_out) dense_out[o] = d;
-          }
+          xv[k] = XO[k];
+          nv[k] = NO[k];
+          d[k] = xv[k] / nv[k];
+        }
+        // 32-bit offsets (W H < 2^28); the optional outputs in branches of their own
+        const unsigned o0 = (unsigned)c0 * (unsigned)W + (unsigned)x;
+        if (cnt == kUK) {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k) dense[o0 + (unsigned)k * (unsigned)W] = d[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) dense[o0 + (unsigned)k * (unsigned)W] = d[k];
+        }
+        if (colxn) {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) colxn[o0 + (unsigned)k * (unsigned)W] = make_float2(xv[k], nv[k]);
+        }
+        if (dense_out) {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) dense_out[o0 + (unsigned)k * (unsigned)W] = d[k];
         }
       }
       __syncthreads();
