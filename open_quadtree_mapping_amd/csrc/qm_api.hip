@@ -1181,10 +1181,9 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
   const FrameDev* hf = b->hf[par].data();
   const int n = b->n, N = b->p.n_hyp;
   const BpLaunchParams bp = bp_params(&b->p);
-  // inputs of the timed stages: quadtree, cost, pyramid (the seeds are untouched)
+  // inputs of the timed stages: quadtree, cost and its pyramid (the seeds are untouched)
   launch_quadtree(df, hf, n, s);
-  launch_cost(df, hf, n, N, b->table, false, b->tune, s);
-  launch_pyramid(df, hf, n, N, s);
+  launch_cost(df, hf, n, N, b->table, false, b->tune, s, true);
   hipEvent_t e0, e1;
   QM_HIP(hipEventCreate(&e0));
   QM_HIP(hipEventCreate(&e1));
@@ -1205,7 +1204,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     std::function<void(const Tuning&)> stage;
   };
   const Knob knobs[3] = {
-      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, t, s); }},
+      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, t, s, true); }},
       {QM_TUNE_BP_ROWS, {4, 8, 16}, [&](const Tuning& t) { run_sweeps(df, hf, n, &b->p, t, s); }},
       {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
   };
