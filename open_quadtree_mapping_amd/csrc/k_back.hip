@@ -23,19 +23,21 @@ namespace qm {
 //
 // A workgroup owns 64 lines, one per lane, and splits a step's work over waves
 // that run side by side on the CU's SIMDs, one role each:
-//   producer  loads a chunk of kUK steps and writes the chunk's a (which is
-//             also the c of the step before), b and right-hand sides to LDS;
+//   producers two waves, each for half a chunk of kUK steps: load it and write
+//             its a (which is also the c of the step before), b and
+//             right-hand sides to LDS;
 //   ch        the coefficient chain (it depends on the weights only);
 //   x, n      the two right-hand-side chains, reading ch_{k-1} from LDS;
 //   out       (columns) X / N and the column pass's stores.
-// The waves form a pipeline over chunks: in round r the producer fills chunk
+// The waves form a pipeline over chunks: in round r the producers fill chunk
 // r, the ch wave runs chunk r - 1 and the x / n waves chunk r - 2, a barrier
 // between rounds and rings of LDS slots between the roles. The forward values
 // of a line (ch, dh, nh per step) are not kept: each chain wave stores its
 // state at every chunk start (one float per chunk) and the backward sweep
-// walks the chunks from the end, recomputes a chunk's forward values from its
+// walks the chunks from the end, recomputing a chunk's forward values from its
 // checkpoint (the same operations on the same inputs, so the same bits) into
-// registers and sweeps back through them.
+// registers: a backward round of an x / n wave sweeps back through the chunk
+// it recomputed in the round before, then recomputes the next one.
 // ---------------------------------------------------------------------------
 constexpr int kUK = 16;       // steps per chunk
 constexpr int kUS = kUK + 1;  // LDS stride of one line's chunk (odd: lane-per-line access is conflict-free)
