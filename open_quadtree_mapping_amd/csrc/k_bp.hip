@@ -128,7 +128,7 @@ __device__ __forceinline__ bool is_node(const FrameDev& F, int x, int y) {
 
 template <int N, int S>
 __device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int j, int lane, const Row<S>& b,
-                                               double imin, double istep) {
+                                               double imin, double istep, bool node = true) {
   // argmin tree: in-lane stages N/2 .. 16, then row_shl 8 .. 1
   float v[S];
   int id[S];
@@ -176,8 +176,10 @@ __device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int 
       const float ba = bb / a;
       if (isfinite(ba)) disp = (float)bi - ba / 2.0f;
     }
-    gp(F.dcoarse)[cell] = (float)(1.0 / (istep * (double)disp + imin));
-    gp(F.label)[cell] = bi;
+    // a cell that is not a node gets 0 and label -1
+    const float depth = (float)(1.0 / (istep * (double)disp + imin));
+    gp(F.dcoarse)[cell] = node ? depth : 0.0f;
+    gp(F.label)[cell] = node ? bi : -1;
   }
 }
 
@@ -332,15 +334,11 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
     if (x >= w) return;
   }
   const int cell = y * w + x;
-  if (!is_node<N>(F, x, y)) {
-    if (j == 0) {
-      gp(F.dcoarse)[cell] = 0.0f;
-      gp(F.label)[cell] = -1;
-    }
-    return;
-  }
-  // all five rows are loaded at once (a missing neighbour's row is replaced by
-  // a valid one and not added), so their latencies overlap
+  // the quadtree level and all five rows are loaded at once (a missing
+  // neighbour's row is replaced by a valid one and not added; a cell that is
+  // not a node runs the extraction too and stores 0 and -1), so their
+  // latencies overlap
+  const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)(4 * y) * F.W + 4 * x];
   const gptr<const float> M = gp((const float*)F.msg[0]);
   const bool hl = x != 0, hr = x != w - 1, hu = y != 0, hd = y != h - 1;
   Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
@@ -355,7 +353,8 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
     b.v[s] = hu ? b.v[s] + tu.v[s] : b.v[s];
     b.v[s] = hd ? b.v[s] + td.v[s] : b.v[s];
   }
-  extract_winner<N, S>(F, cell, j, lane, b, imin, istep);
+  const bool node = ((4 * x) % qs) == 0 && ((4 * y) % qs) == 0;  // is_node
+  extract_winner<N, S>(F, cell, j, lane, b, imin, istep, node);
 }
 
 // [cell][dir][row layout] -> [dir][cell][N] in sample order, for callers that
