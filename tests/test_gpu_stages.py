@@ -178,6 +178,29 @@ def test_upsample_general(H, W, rs, cs):
     assert np.array_equal(host(got), ref)
 
 
+# right-hand sides below 2^-70 and above 2^70 and couplings that push the
+# pivots above 2^20, beside ordinary values: the chains' chunks that meet them
+# take the plain division, the others the unscaled one
+def test_upsample_general_extreme_magnitudes():
+    H, W = 40, 72
+    rng = np.random.default_rng(9)
+    d = (rng.uniform(1, 10, (H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float32)
+    pick = rng.random((H, W))
+    d[pick < 0.02] = 1e-25
+    d[(pick >= 0.02) & (pick < 0.04)] = 1e24
+    d[(pick >= 0.04) & (pick < 0.05)] = 3e-38
+    conf = rng.uniform(1e-3, 1.0, (H, W)).astype(np.float32)
+    wr = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H, W + 1))).astype(np.float32)
+    wc = np.exp(rng.uniform(np.log(1e-3), np.log(10), (H + 1, W))).astype(np.float32)
+    wr[rng.random((H, W + 1)) < 0.02] = 1e6
+    wc[rng.random((H + 1, W)) < 0.02] = 1e6
+    ref, rx, rn = O.upsample(d, wr, wc, conf=conf, want_rows=True)
+    got, gx, gn = S.upsample_general(dev(d), dev(wr), dev(wc), conf=dev(conf), want_rows=True)
+    assert np.array_equal(host(gx), rx, equal_nan=True)
+    assert np.array_equal(host(gn), rn, equal_nan=True)
+    assert np.array_equal(host(got), ref, equal_nan=True)
+
+
 def fusion_case(W, H, seed):
     rng = np.random.default_rng(seed)
     ks = keyframe_set(W, H, 3 if W == 376 else 5)
