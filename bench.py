@@ -53,6 +53,13 @@ def parse():
                     help="force the frame-group overlap setting (0-7) after autotuning, e.g. for a profiled run")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the batched launches without hipGraph replay (for counter collection)")
+    ap.add_argument("--workload", default="keyframes32",
+                    choices=["keyframes32", "uneven12", "diff32", "upsample16"],
+                    help="keyframes32: the headline workload (default). The others print one supplementary "
+                         "line for another shape: uneven12 = 4 x 376x240 + 4 x 752x480 + 4 x 1280x720 "
+                         "streams, N = 64, batched; diff32 = 32 streams at 752x480, N = 128 over 0.3-50 m, "
+                         "batched forward plus the backward to each stream's lambda; upsample16 = 16 "
+                         "upsample problems at 752x480, forward plus adjoint")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -150,6 +157,88 @@ def measured_traffic(stage, W, H, N, B):
     return int(rec["hbm_bytes_per_launch"]), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
 
 
+def run_workload(args):
+    """Supplementary workloads (--workload): one JSON line with the rate of one
+    step, timed like the headline workload (warm-up steps, then args.steps
+    steps between two device synchronisations)."""
+    import numpy as np
+    import torch
+
+    from open_quadtree_mapping_amd import autograd as QA
+    from open_quadtree_mapping_amd import default_params
+    from open_quadtree_mapping_amd.batch import Batch
+    from open_quadtree_mapping_amd.scene import camera_for, forward_lateral_trajectory, make_scene, render
+    from open_quadtree_mapping_amd.se3 import relative_pose
+
+    dev = torch.device("cuda:0")
+
+    def keyframes(W, H, n, seed0):
+        cam = camera_for(W, H)
+        out = []
+        for s in range(n):
+            traj = forward_lateral_trajectory(6, seed=seed0 + s)
+            sc = make_scene(seed0 + s)
+            imgs = [torch.from_numpy(render(sc, cam, T)[0]).to(dev) for T in traj[1:]]
+            T = np.stack([relative_pose(traj[4 - i], traj[5]) for i in range(4)])
+            out.append(dict(key=imgs[4], refs=[imgs[3], imgs[2], imgs[1], imgs[0]], T=T,
+                            cam=(cam.fx, cam.fy, cam.cx, cam.cy)))
+        return out
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    wl = args.workload
+    if wl == "uneven12":
+        frames = keyframes(376, 240, 4, 300) + keyframes(752, 480, 4, 310) + keyframes(1280, 720, 4, 320)
+        batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0), outputs=("depth",))
+        batch.autotune(reps=args.tune_reps)
+        sec = timed(lambda: batch.run(graph=True))
+        n, unit, what = 12, "depth maps/s", "batched graph replay of 12 uneven streams, N = 64"
+        batch.close()
+    elif wl == "diff32":
+        frames = [dict(f, key=f["key"].clone(), refs=[r.clone() for r in f["refs"]])
+                  for f in keyframes(752, 480, 8, 100) for _ in range(4)]
+        batch = Batch(frames, default_params(n_hyp=128, idepth_min=1.0 / 50.0, idepth_max=1.0 / 0.3),
+                      outputs=("depth",))
+        batch.autotune(reps=args.tune_reps)
+        g = [torch.randn(480, 752, device=dev) for _ in range(32)]
+
+        def step():
+            batch.run(graph=True)
+            batch.backward(g)  # synchronises the stream
+        sec = timed(step)
+        n, unit, what = 32, "depth maps/s", "batched forward (graph) plus backward to 32 lambdas, N = 128"
+        batch.close()
+    else:  # upsample16
+        B, H, W = 16, 480, 752
+        gen = torch.Generator(device="cpu").manual_seed(3)
+        d = torch.rand(B, H, W, generator=gen) * 9 + 1
+        d = (d * (torch.rand(B, H, W, generator=gen) < 0.3)).to(dev)
+        conf = (1.0 - torch.rand(B, H, W, generator=gen)).to(dev)
+        wr = torch.exp(torch.empty(B, H, W + 1).uniform_(np.log(1e-3), np.log(10), generator=gen)).to(dev)
+        wc = torch.exp(torch.empty(B, H + 1, W).uniform_(np.log(1e-3), np.log(10), generator=gen)).to(dev)
+        lams = torch.full((B,), 10.0, device=dev)
+        g = torch.randn(B, H, W, device=dev)
+        leaves = [t.requires_grad_() for t in (d, conf, wr, wc, lams)]
+
+        def step():
+            for t in leaves:
+                t.grad = None
+            QA.upsample(d, wr, wc, conf=conf, lam=lams).backward(g)
+        sec = timed(step)
+        n, unit, what = B, "problems/s", "16 upsample problems, forward plus adjoint (all inputs and lambda)"
+    print(json.dumps({"workload": wl, "value": round(n / sec, 2), "unit": unit, "ms_per_step": round(sec * 1e3, 4),
+                      "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": what}}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -167,6 +256,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
+
+    if args.workload != "keyframes32":
+        run_workload(args)
         return
 
     from open_quadtree_mapping_amd import default_params

@@ -58,6 +58,9 @@ const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
 /* Fault injection for tests: refuse device allocations beyond `bytes` (0 = no cap). */
 void qm_set_alloc_cap(size_t bytes);
+/* Device bytes the library holds. This includes the scratch of the stage-level
+ * calls below, which is kept for reuse by later calls (at most 8 GiB; released
+ * when an allocation is refused). */
 size_t qm_device_bytes_in_use(void);
 
 /* ---------------- stage level (device pointers) ---------------------------- */

@@ -37,9 +37,45 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-int dev_alloc(void** p, size_t bytes) {
-  *p = nullptr;
-  if (bytes == 0) bytes = 256;
+void dev_free(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sizes.find(p);
+    if (it != g_sizes.end()) {
+      g_bytes -= it->second;
+      g_sizes.erase(it);
+    }
+  }
+  (void)hipFree(p);
+}
+
+// Scratch blocks of the stage-level calls, kept for the next call: a hipMalloc
+// and hipFree pair costs far more than the kernels of one such call. A block
+// enters the cache only after its call has synchronised its stream, so no
+// kernel can still use it; the cache holds at most kCacheMax bytes, and it is
+// emptied when an allocation is refused.
+struct CachedBlock {
+  int dev;
+  size_t bytes;
+  void* p;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;
+size_t g_cache_bytes = 0;
+constexpr size_t kCacheMax = size_t(8) << 30;
+
+void flush_scratch_cache() {
+  std::vector<CachedBlock> blocks;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    blocks.swap(g_cache);
+    g_cache_bytes = 0;
+  }
+  for (const CachedBlock& b : blocks) dev_free(b.p);
+}
+
+static int dev_alloc_once(void** p, size_t bytes) {
   const size_t cap = g_cap.load();
   if (cap && g_bytes.load() + bytes > cap)
     return fail(QM_ERR_ALLOC, "device allocation of " + std::to_string(bytes) + " bytes refused by the cap");
@@ -54,33 +90,90 @@ int dev_alloc(void** p, size_t bytes) {
   return QM_OK;
 }
 
-void dev_free(void* p) {
-  if (!p) return;
+int dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  int rc = dev_alloc_once(p, bytes);
+  if (rc == QM_OK) return rc;
+  bool had_cache;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    had_cache = !g_cache.empty();
+  }
+  if (!had_cache) return rc;
+  (void)hipGetLastError();  // a failed hipMalloc leaves its error for the next check
+  flush_scratch_cache();
+  return dev_alloc_once(p, bytes);
+}
+
+// the smallest cached block of the current device holding bytes, at most
+// twice that (else a new allocation)
+int scratch_take(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    int best = -1;
+    for (int i = 0; i < (int)g_cache.size(); ++i) {
+      const CachedBlock& b = g_cache[i];
+      if (b.dev == dev && b.bytes >= bytes && b.bytes <= 2 * bytes && (best < 0 || b.bytes < g_cache[best].bytes))
+        best = i;
+    }
+    if (best >= 0) {
+      *p = g_cache[best].p;
+      g_cache_bytes -= g_cache[best].bytes;
+      g_cache[best] = g_cache.back();
+      g_cache.pop_back();
+      return QM_OK;
+    }
+  }
+  return dev_alloc(p, bytes);
+}
+
+void scratch_return(void* p) {
+  size_t bytes = 0;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sizes.find(p);
-    if (it != g_sizes.end()) {
-      g_bytes -= it->second;
-      g_sizes.erase(it);
+    if (it != g_sizes.end()) bytes = it->second;
+  }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (bytes && g_cache_bytes + bytes <= kCacheMax) {
+      g_cache.push_back({dev, bytes, p});
+      g_cache_bytes += bytes;
+      return;
     }
   }
-  (void)hipFree(p);
+  dev_free(p);
 }
 
 }  // namespace qm
 
 namespace {
 
-// RAII scratch list for stage-level calls
+// RAII scratch list for stage-level calls. done(): the call has synchronised
+// its stream, so its blocks go back to the cache; a call that returns before
+// that (an error) frees them (hipFree waits for the device).
 struct Scratch {
   std::vector<void*> ptrs;
+  bool synced = false;
   int get(void** p, size_t bytes) {
-    int rc = dev_alloc(p, bytes);
+    int rc = scratch_take(p, bytes);
     if (rc == QM_OK) ptrs.push_back(*p);
     return rc;
   }
+  void done() { synced = true; }
   ~Scratch() {
-    for (void* p : ptrs) dev_free(p);
+    for (void* p : ptrs) {
+      if (synced)
+        scratch_return(p);
+      else
+        dev_free(p);
+    }
   }
 };
 
@@ -539,6 +632,7 @@ int qm_quadtree(const uint8_t* img, int W, int H, int pitch, uint8_t* levels, vo
   launch_quadtree((FrameDev*)df, &F, 1, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -577,6 +671,7 @@ int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int 
   launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, Tuning(), s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -612,6 +707,7 @@ int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const 
   if (messages) launch_copy_messages((FrameDev*)df, &F, 1, N, messages, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -643,6 +739,7 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
   launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, false, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -687,6 +784,7 @@ int qm_upsample_general_batch(const qm_upsample_problem* probs, int n, void* str
   launch_upsample((FrameDev*)df, hf.data(), n, 2, nullptr, false, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -744,6 +842,7 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
   launch_upsample_backward((FrameDev*)df, hf.data(), n, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -805,6 +904,7 @@ int qm_quadtree_prior(const float* prior_depth, int W, int H, uint8_t* levels, v
   launch_quadtree_prior((FrameDev*)df, &F, 1, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -829,6 +929,7 @@ int qm_prior_to_cost(const float* prior, int W, int H, const uint8_t* levels, co
   launch_prior_cost((FrameDev*)df, &F, 1, p->n_hyp, true, bp.idepth_min, bp.idepth_step, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -893,6 +994,7 @@ int qm_sd_step(float* seeds, const uint8_t* key, const float* key_grad, const ui
           T_inc_world, switch_keyframe != 0, prefill, (float2*)prior, (unsigned long long*)u64, (float4*)info, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -918,6 +1020,7 @@ int qm_fuse_backward(const float* seeds_warped, const int32_t* table, const floa
   launch_fuse_backward((FrameDev*)df, &F, 1, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
@@ -948,6 +1051,7 @@ int qm_fuse(float* seeds, const float* depth_in, int W, int H, const float cam[4
   launch_fuse((FrameDev*)df, &F, 1, true, s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
+  sc.done();
   return QM_OK;
 }
 
