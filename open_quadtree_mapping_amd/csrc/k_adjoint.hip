@@ -15,6 +15,12 @@
 
 namespace qm {
 
+// The adjoint kernels walk each line in chunks of kAK steps: a chunk's inputs
+// are loaded together (indices clamped to the segment, so every load is in
+// bounds) before its steps run, which puts one memory latency on the chain per
+// chunk instead of one per step.
+constexpr int kAK = 16;
+
 __device__ __forceinline__ void adj_coeffs(float wl, float wr, double lam, float& a, float& b, float& c) {
   a = (float)(-lam * (double)wl);
   b = (float)(1.0 + lam * (double)(wl + wr));
@@ -40,49 +46,80 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
     float wl = weight(s0);
-    for (int i = s0; i < s1; ++i) {
-      const float wr = weight(i + 1);
-      float a, b, c;
-      adj_coeffs(wl, wr, lam, a, b, c);
-      const float2 z = F.colxn[(size_t)i * W + x];
-      const float g = F.g_out[(size_t)i * W + x];
-      float gx = 0.0f, gn = 0.0f;
-      if (z.y != 0.0f) {
-        gx = g / z.y;
-        gn = -gx * (z.x / z.y);
+    for (int i0 = s0; i0 < s1; i0 += kAK) {
+      float wv[kAK], gv[kAK];
+      float2 zv[kAK];
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        const int i = min(i0 + j, s1 - 1);
+        wv[j] = weight(i + 1);
+        zv[j] = F.colxn[(size_t)i * W + x];
+        gv[j] = F.g_out[(size_t)i * W + x];
       }
-      const float den = b - fc * a;
-      const float ch = c / den;
-      const float dh = (gx - fd * a) / den;
-      const float nh = (gn - fn * a) / den;
-      F.fwd[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
-      fc = ch;
-      fd = dh;
-      fn = nh;
-      wl = wr;
+      const int n = min(kAK, s1 - i0);
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        if (j < n) {
+          const int i = i0 + j;
+          const float wr = wv[j];
+          float a, b, c;
+          adj_coeffs(wl, wr, lam, a, b, c);
+          const float2 z = zv[j];
+          const float g = gv[j];
+          float gx = 0.0f, gn = 0.0f;
+          if (z.y != 0.0f) {
+            gx = g / z.y;
+            gn = -gx * (z.x / z.y);
+          }
+          const float den = b - fc * a;
+          const float ch = c / den;
+          const float dh = (gx - fd * a) / den;
+          const float nh = (gn - fn * a) / den;
+          F.fwd[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
+          fc = ch;
+          fd = dh;
+          fn = nh;
+          wl = wr;
+        }
+      }
     }
     float vx_next = 0.0f, vn_next = 0.0f;
     float2 z_next = make_float2(0.0f, 0.0f);
-    for (int i = s1 - 1; i >= s0; --i) {
-      const float4 t = F.fwd[(size_t)i * W + x];
-      const float vx = t.y - t.x * vx_next;
-      const float vn = t.z - t.x * vn_next;
-      F.adjxn[(size_t)x * H + i] = make_float2(vx, vn);
-      const float2 z = F.colxn[(size_t)i * W + x];
-      if (i + 1 < s1) {
-        F.grad_wcol[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
-      } else {
-        const float e = -flam * (vx * z.x + vn * z.y);
-        if (s1 == H) F.grad_wcol[(size_t)H * W + x] = e;
-        else carry = e;
+    for (int i1 = s1; i1 > s0; i1 -= kAK) {
+      float4 tv[kAK];
+      float2 zv[kAK];
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        const int i = max(i1 - 1 - j, s0);
+        tv[j] = F.fwd[(size_t)i * W + x];
+        zv[j] = F.colxn[(size_t)i * W + x];
       }
-      if (i == s0) {
-        const float e = -flam * (vx * z.x + vn * z.y);
-        F.grad_wcol[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
+      const int n = min(kAK, i1 - s0);
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        if (j < n) {
+          const int i = i1 - 1 - j;
+          const float4 t = tv[j];
+          const float vx = t.y - t.x * vx_next;
+          const float vn = t.z - t.x * vn_next;
+          F.adjxn[(size_t)x * H + i] = make_float2(vx, vn);
+          const float2 z = zv[j];
+          if (i + 1 < s1) {
+            F.grad_wcol[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+          } else {
+            const float e = -flam * (vx * z.x + vn * z.y);
+            if (s1 == H) F.grad_wcol[(size_t)H * W + x] = e;
+            else carry = e;
+          }
+          if (i == s0) {
+            const float e = -flam * (vx * z.x + vn * z.y);
+            F.grad_wcol[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
+          }
+          vx_next = vx;
+          vn_next = vn;
+          z_next = z;
+        }
       }
-      vx_next = vx;
-      vn_next = vn;
-      z_next = z;
     }
   }
 }
@@ -108,56 +145,89 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
     float wl = weight(s0);
-    for (int i = s0; i < s1; ++i) {
-      const float wr = weight(i + 1);
-      float a, b, c;
-      adj_coeffs(wl, wr, lam, a, b, c);
-      const float2 r = F.adjxn[(size_t)i * H + y];
-      const float den = b - fc * a;
-      const float ch = c / den;
-      const float dh = (r.x - fd * a) / den;
-      const float nh = (r.y - fn * a) / den;
-      F.fwd[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
-      fc = ch;
-      fd = dh;
-      fn = nh;
-      wl = wr;
+    for (int i0 = s0; i0 < s1; i0 += kAK) {
+      float wv[kAK];
+      float2 rv[kAK];
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        const int i = min(i0 + j, s1 - 1);
+        wv[j] = weight(i + 1);
+        rv[j] = F.adjxn[(size_t)i * H + y];
+      }
+      const int n = min(kAK, s1 - i0);
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        if (j < n) {
+          const int i = i0 + j;
+          const float wr = wv[j];
+          float a, b, c;
+          adj_coeffs(wl, wr, lam, a, b, c);
+          const float2 r = rv[j];
+          const float den = b - fc * a;
+          const float ch = c / den;
+          const float dh = (r.x - fd * a) / den;
+          const float nh = (r.y - fn * a) / den;
+          F.fwd[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
+          fc = ch;
+          fd = dh;
+          fn = nh;
+          wl = wr;
+        }
+      }
     }
     float vx_next = 0.0f, vn_next = 0.0f;
     float2 z_next = make_float2(0.0f, 0.0f);
-    for (int i = s1 - 1; i >= s0; --i) {
-      const float4 t = F.fwd[(size_t)i * H + y];
-      const float vx = t.y - t.x * vx_next;
-      const float vn = t.z - t.x * vn_next;
-      const float2 z = make_float2(F.rowx[(size_t)i * H + y], F.rowx[(size_t)W * H + (size_t)i * H + y]);
-      if (i + 1 < s1) {
-        if (F.grad_wrow)
-          F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
-      } else {
-        const float e = -flam * (vx * z.x + vn * z.y);
-        if (s1 == W) {
-          if (F.grad_wrow) F.grad_wrow[wrow + W] = e;
-        } else {
-          carry = e;
+    for (int i1 = s1; i1 > s0; i1 -= kAK) {
+      float4 tv[kAK];
+      float2 zv[kAK];
+      float dv[kAK], cv[kAK];
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        const int i = max(i1 - 1 - j, s0);
+        tv[j] = F.fwd[(size_t)i * H + y];
+        zv[j] = make_float2(F.rowx[(size_t)i * H + y], F.rowx[(size_t)W * H + (size_t)i * H + y]);
+        dv[j] = F.g_d[row + i];
+        cv[j] = F.g_conf ? F.g_conf[row + i] : 0.0f;
+      }
+      const int n = min(kAK, i1 - s0);
+#pragma unroll
+      for (int j = 0; j < kAK; ++j) {
+        if (j < n) {
+          const int i = i1 - 1 - j;
+          const float4 t = tv[j];
+          const float vx = t.y - t.x * vx_next;
+          const float vn = t.z - t.x * vn_next;
+          const float2 z = zv[j];
+          if (i + 1 < s1) {
+            if (F.grad_wrow)
+              F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+          } else {
+            const float e = -flam * (vx * z.x + vn * z.y);
+            if (s1 == W) {
+              if (F.grad_wrow) F.grad_wrow[wrow + W] = e;
+            } else {
+              carry = e;
+            }
+          }
+          if (i == s0 && F.grad_wrow) {
+            const float e = -flam * (vx * z.x + vn * z.y);
+            F.grad_wrow[wrow + s0] = (s0 == 0) ? e : carry + e;
+          }
+          // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
+          const float d = dv[j];
+          if (F.g_conf) {
+            const float c = cv[j];
+            if (F.grad_d) F.grad_d[row + i] = c * vx;
+            if (F.grad_conf) F.grad_conf[row + i] = d * vx + vn;
+          } else {
+            if (F.grad_d) F.grad_d[row + i] = vx;
+            if (F.grad_conf) F.grad_conf[row + i] = 0.0f;
+          }
+          vx_next = vx;
+          vn_next = vn;
+          z_next = z;
         }
       }
-      if (i == s0 && F.grad_wrow) {
-        const float e = -flam * (vx * z.x + vn * z.y);
-        F.grad_wrow[wrow + s0] = (s0 == 0) ? e : carry + e;
-      }
-      // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
-      const float d = F.g_d[row + i];
-      if (F.g_conf) {
-        const float c = F.g_conf[row + i];
-        if (F.grad_d) F.grad_d[row + i] = c * vx;
-        if (F.grad_conf) F.grad_conf[row + i] = d * vx + vn;
-      } else {
-        if (F.grad_d) F.grad_d[row + i] = vx;
-        if (F.grad_conf) F.grad_conf[row + i] = 0.0f;
-      }
-      vx_next = vx;
-      vn_next = vn;
-      z_next = z;
     }
   }
 }
