@@ -15,10 +15,11 @@
 
 namespace qm {
 
-// The adjoint kernels walk each line in chunks of kAK steps: a chunk's inputs
+// The adjoint kernels walk each line in chunks of kAK steps. A chunk's inputs
 // are loaded together (indices clamped to the segment, so every load is in
-// bounds) before its steps run, which puts one memory latency on the chain per
-// chunk instead of one per step.
+// bounds; the steps past the segment's end are skipped), one chunk ahead of the
+// steps that use them: two buffers alternate, so the loads of chunk k + 1 are
+// in flight while chunk k runs.
 constexpr int kAK = 16;
 
 __device__ __forceinline__ void adj_coeffs(float wl, float wr, double lam, float& a, float& b, float& c) {
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
   if (x >= W) return;
   const double lam = F.lam;
   const float flam = (float)lam;
-  auto weight = [&](int y) -> float { return F.g_wcol[(size_t)y * W + x]; };
+  auto weight = [&](int y) -> float { return gp(F.g_wcol)[(size_t)y * W + x]; };
   const int seg = F.col_seg < H ? F.col_seg : H;
   const int bounds[3] = {0, seg, H};
   float carry = 0.0f;
@@ -46,26 +47,30 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
     float wl = weight(s0);
-    for (int i0 = s0; i0 < s1; i0 += kAK) {
-      float wv[kAK], gv[kAK];
-      float2 zv[kAK];
+    struct FwdIn {
+      float w[kAK], g[kAK];
+      float2 z[kAK];
+    };
+    auto load_f = [&](FwdIn& c, int i0) {
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         const int i = min(i0 + j, s1 - 1);
-        wv[j] = weight(i + 1);
-        zv[j] = F.colxn[(size_t)i * W + x];
-        gv[j] = F.g_out[(size_t)i * W + x];
+        c.w[j] = weight(i + 1);
+        c.z[j] = gp(F.colxn)[(size_t)i * W + x];
+        c.g[j] = gp(F.g_out)[(size_t)i * W + x];
       }
+    };
+    auto run_f = [&](const FwdIn& cin, int i0) {
       const int n = min(kAK, s1 - i0);
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         if (j < n) {
           const int i = i0 + j;
-          const float wr = wv[j];
+          const float wr = cin.w[j];
           float a, b, c;
           adj_coeffs(wl, wr, lam, a, b, c);
-          const float2 z = zv[j];
-          const float g = gv[j];
+          const float2 z = cin.z[j];
+          const float g = cin.g[j];
           float gx = 0.0f, gn = 0.0f;
           if (z.y != 0.0f) {
             gx = g / z.y;
@@ -75,50 +80,74 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
           const float ch = c / den;
           const float dh = (gx - fd * a) / den;
           const float nh = (gn - fn * a) / den;
-          F.fwd[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
+          gp(F.fwd)[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
           fc = ch;
           fd = dh;
           fn = nh;
           wl = wr;
         }
       }
+    };
+    {
+      FwdIn A, B;
+      load_f(A, s0);
+      for (int i0 = s0; i0 < s1; i0 += 2 * kAK) {
+        load_f(B, i0 + kAK);
+        run_f(A, i0);
+        load_f(A, i0 + 2 * kAK);
+        run_f(B, i0 + kAK);
+      }
     }
     float vx_next = 0.0f, vn_next = 0.0f;
     float2 z_next = make_float2(0.0f, 0.0f);
-    for (int i1 = s1; i1 > s0; i1 -= kAK) {
-      float4 tv[kAK];
-      float2 zv[kAK];
+    struct BwdIn {
+      float4 t[kAK];
+      float2 z[kAK];
+    };
+    auto load_b = [&](BwdIn& c, int i1) {
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         const int i = max(i1 - 1 - j, s0);
-        tv[j] = F.fwd[(size_t)i * W + x];
-        zv[j] = F.colxn[(size_t)i * W + x];
+        c.t[j] = gp(F.fwd)[(size_t)i * W + x];
+        c.z[j] = gp(F.colxn)[(size_t)i * W + x];
       }
+    };
+    auto run_b = [&](const BwdIn& cin, int i1) {
       const int n = min(kAK, i1 - s0);
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         if (j < n) {
           const int i = i1 - 1 - j;
-          const float4 t = tv[j];
+          const float4 t = cin.t[j];
           const float vx = t.y - t.x * vx_next;
           const float vn = t.z - t.x * vn_next;
-          F.adjxn[(size_t)x * H + i] = make_float2(vx, vn);
-          const float2 z = zv[j];
+          gp(F.adjxn)[(size_t)x * H + i] = make_float2(vx, vn);
+          const float2 z = cin.z[j];
           if (i + 1 < s1) {
-            F.grad_wcol[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+            gp(F.grad_wcol)[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
           } else {
             const float e = -flam * (vx * z.x + vn * z.y);
-            if (s1 == H) F.grad_wcol[(size_t)H * W + x] = e;
+            if (s1 == H) gp(F.grad_wcol)[(size_t)H * W + x] = e;
             else carry = e;
           }
           if (i == s0) {
             const float e = -flam * (vx * z.x + vn * z.y);
-            F.grad_wcol[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
+            gp(F.grad_wcol)[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
           }
           vx_next = vx;
           vn_next = vn;
           z_next = z;
         }
+      }
+    };
+    {
+      BwdIn A, B;
+      load_b(A, s1);
+      for (int i1 = s1; i1 > s0; i1 -= 2 * kAK) {
+        load_b(B, i1 - kAK);
+        run_b(A, i1);
+        load_b(A, i1 - 2 * kAK);
+        run_b(B, i1 - kAK);
       }
     }
   }
@@ -136,7 +165,7 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
   const float flam = (float)lam;
   const size_t row = (size_t)y * W;
   const size_t wrow = (size_t)y * (W + 1);
-  auto weight = [&](int x) -> float { return F.g_wrow[wrow + x]; };
+  auto weight = [&](int x) -> float { return gp(F.g_wrow)[wrow + x]; };
   const int seg = F.row_seg < W ? F.row_seg : W;
   const int bounds[3] = {0, seg, W};
   float carry = 0.0f;
@@ -145,88 +174,119 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
     if (s1 <= s0) continue;
     float fc = 0.0f, fd = 0.0f, fn = 0.0f;
     float wl = weight(s0);
-    for (int i0 = s0; i0 < s1; i0 += kAK) {
-      float wv[kAK];
-      float2 rv[kAK];
+    struct FwdIn {
+      float w[kAK];
+      float2 r[kAK];
+    };
+    auto load_f = [&](FwdIn& c, int i0) {
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         const int i = min(i0 + j, s1 - 1);
-        wv[j] = weight(i + 1);
-        rv[j] = F.adjxn[(size_t)i * H + y];
+        c.w[j] = weight(i + 1);
+        c.r[j] = gp(F.adjxn)[(size_t)i * H + y];
       }
+    };
+    auto run_f = [&](const FwdIn& cin, int i0) {
       const int n = min(kAK, s1 - i0);
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         if (j < n) {
           const int i = i0 + j;
-          const float wr = wv[j];
+          const float wr = cin.w[j];
           float a, b, c;
           adj_coeffs(wl, wr, lam, a, b, c);
-          const float2 r = rv[j];
+          const float2 r = cin.r[j];
           const float den = b - fc * a;
           const float ch = c / den;
           const float dh = (r.x - fd * a) / den;
           const float nh = (r.y - fn * a) / den;
-          F.fwd[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
+          gp(F.fwd)[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
           fc = ch;
           fd = dh;
           fn = nh;
           wl = wr;
         }
       }
+    };
+    {
+      FwdIn A, B;
+      load_f(A, s0);
+      for (int i0 = s0; i0 < s1; i0 += 2 * kAK) {
+        load_f(B, i0 + kAK);
+        run_f(A, i0);
+        load_f(A, i0 + 2 * kAK);
+        run_f(B, i0 + kAK);
+      }
     }
     float vx_next = 0.0f, vn_next = 0.0f;
     float2 z_next = make_float2(0.0f, 0.0f);
-    for (int i1 = s1; i1 > s0; i1 -= kAK) {
-      float4 tv[kAK];
-      float2 zv[kAK];
-      float dv[kAK], cv[kAK];
+    struct BwdIn {
+      float4 t[kAK];
+      float2 z[kAK];
+      float d[kAK], c[kAK];
+    };
+    // the confidences' row, or (unused) the depths' when there is none: every
+    // load is made, so the prefetch's waits count a fixed number of loads
+    const float* confp = F.g_conf ? F.g_conf : F.g_d;
+    auto load_b = [&](BwdIn& c, int i1) {
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         const int i = max(i1 - 1 - j, s0);
-        tv[j] = F.fwd[(size_t)i * H + y];
-        zv[j] = make_float2(F.rowx[(size_t)i * H + y], F.rowx[(size_t)W * H + (size_t)i * H + y]);
-        dv[j] = F.g_d[row + i];
-        cv[j] = F.g_conf ? F.g_conf[row + i] : 0.0f;
+        c.t[j] = gp(F.fwd)[(size_t)i * H + y];
+        c.z[j] = make_float2(gp(F.rowx)[(size_t)i * H + y], gp(F.rowx)[(size_t)W * H + (size_t)i * H + y]);
+        c.d[j] = gp(F.g_d)[row + i];
+        c.c[j] = gp(confp)[row + i];
       }
+    };
+    auto run_b = [&](const BwdIn& cin, int i1) {
       const int n = min(kAK, i1 - s0);
 #pragma unroll
       for (int j = 0; j < kAK; ++j) {
         if (j < n) {
           const int i = i1 - 1 - j;
-          const float4 t = tv[j];
+          const float4 t = cin.t[j];
           const float vx = t.y - t.x * vx_next;
           const float vn = t.z - t.x * vn_next;
-          const float2 z = zv[j];
+          const float2 z = cin.z[j];
           if (i + 1 < s1) {
             if (F.grad_wrow)
-              F.grad_wrow[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+              gp(F.grad_wrow)[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
           } else {
             const float e = -flam * (vx * z.x + vn * z.y);
             if (s1 == W) {
-              if (F.grad_wrow) F.grad_wrow[wrow + W] = e;
+              if (F.grad_wrow) gp(F.grad_wrow)[wrow + W] = e;
             } else {
               carry = e;
             }
           }
           if (i == s0 && F.grad_wrow) {
             const float e = -flam * (vx * z.x + vn * z.y);
-            F.grad_wrow[wrow + s0] = (s0 == 0) ? e : carry + e;
+            gp(F.grad_wrow)[wrow + s0] = (s0 == 0) ? e : carry + e;
           }
           // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
-          const float d = dv[j];
+          const float d = cin.d[j];
           if (F.g_conf) {
-            const float c = cv[j];
-            if (F.grad_d) F.grad_d[row + i] = c * vx;
-            if (F.grad_conf) F.grad_conf[row + i] = d * vx + vn;
+            const float c = cin.c[j];
+            if (F.grad_d) gp(F.grad_d)[row + i] = c * vx;
+            if (F.grad_conf) gp(F.grad_conf)[row + i] = d * vx + vn;
           } else {
-            if (F.grad_d) F.grad_d[row + i] = vx;
-            if (F.grad_conf) F.grad_conf[row + i] = 0.0f;
+            if (F.grad_d) gp(F.grad_d)[row + i] = vx;
+            if (F.grad_conf) gp(F.grad_conf)[row + i] = 0.0f;
           }
           vx_next = vx;
           vn_next = vn;
           z_next = z;
         }
+      }
+    };
+    {
+      BwdIn A, B;
+      load_b(A, s1);
+      for (int i1 = s1; i1 > s0; i1 -= 2 * kAK) {
+        load_b(B, i1 - kAK);
+        run_b(A, i1);
+        load_b(A, i1 - 2 * kAK);
+        run_b(B, i1 - kAK);
       }
     }
   }
