@@ -201,6 +201,26 @@ def test_upsample_general_extreme_magnitudes():
     assert np.array_equal(host(got), ref, equal_nan=True)
 
 
+# the stage calls keep their scratch for the next call: repeated calls of one
+# size reuse it (device bytes held stay flat) and give the same results
+def test_stage_scratch_is_reused():
+    from open_quadtree_mapping_amd._lib import lib
+    H, W = 64, 96
+    rng = np.random.default_rng(4)
+    d = (rng.uniform(1, 10, (H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float32)
+    wr = np.ones((H, W + 1), np.float32)
+    wc = np.ones((H + 1, W), np.float32)
+    conf = np.ones((H, W), np.float32)
+    ref = O.upsample(d, wr, wc, conf=conf)
+    held = []
+    for _ in range(3):
+        got = S.upsample_general(dev(d), dev(wr), dev(wc), conf=dev(conf))
+        torch.cuda.synchronize()
+        assert np.array_equal(host(got), ref, equal_nan=True)
+        held.append(lib().qm_device_bytes_in_use())
+    assert held[1] == held[2], held
+
+
 def fusion_case(W, H, seed):
     rng = np.random.default_rng(seed)
     ks = keyframe_set(W, H, 3 if W == 376 else 5)
