@@ -60,6 +60,10 @@ def parse():
                          "streams, N = 64, batched; diff32 = 32 streams at 752x480, N = 128 over 0.3-50 m, "
                          "batched forward plus the backward to each stream's lambda; upsample16 = 16 "
                          "upsample problems at 752x480, forward plus adjoint")
+    ap.add_argument("--cost-arith", choices=["fused", "reference"], default="fused",
+                    help="projection arithmetic of the cost volume (qm_params.cost_arith): fused (hoisted "
+                         "products, FMA, one exact reciprocal) or the reference's operation order; both are "
+                         "bit-identical to the oracle's matching mode")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -137,6 +141,34 @@ def stage_bytes(levels, W, H, N):
         "fuse_warp": 40 * px,
         "fuse_update": 48 * px,
     }, upd
+
+
+FP32_PEAK_TFLOPS = 157.3
+COST_FLOP_PER_SLOT = 40  # FP32 operations per (node cell, sample, slot) of the reference's cost kernel
+
+
+def cost_flops(levels, W, H, N):
+    """Algorithmic FP32 operations of the cost volume for one keyframe: the
+    reference's image_to_cost (src/depth_extract.cu:175-219) does about 40 per
+    (node cell, sample, slot) -- the projection through R|t and K (27) plus
+    the patch-tap sums of the nine taps -- over 5 slots."""
+    import numpy as np
+    sub = levels[0:4 * (H // 4):4, 0:4 * (W // 4):4].astype(np.int64)
+    ys, xs = np.mgrid[0:H // 4, 0:W // 4]
+    qs = 1 << sub
+    x, y = 4 * xs, 4 * ys
+    node = (x % qs == 0) & (y % qs == 0) & (x > 0) & (y > 0) & (x < W - 1) & (y < H - 1)
+    return int(node.sum()) * N * 5 * COST_FLOP_PER_SLOT
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def measured_traffic(stage, W, H, N, B):
@@ -241,6 +273,9 @@ def run_workload(args):
 
 def main():
     args = parse()
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; libgomp reads
+    # this when the oracle library loads, so it is set before anything imports it
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(streams shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -272,7 +307,8 @@ def main():
     dev = torch.device("cuda:0")
     cam = camera_for(W, H)
     imin, imax = 0.02, 2.0  # 0.5 - 50 m
-    params = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    arith = 1 if args.cost_arith == "fused" else 0
+    params = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=arith)
 
     t_setup = time.time()
     nsc = min(args.scenes, B)
@@ -349,6 +385,21 @@ def main():
                      "avg_launch_us": round(1e3 * per_run_ms / max(launches, 1), 2),
                      "alg_bytes_per_run": int(stage_alg[k]), "achieved_GBs": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # the cost stage's FP32 work (BASELINE's FLOP fraction) and its time in the
+    # other projection arithmetic, from a batch of the same frames
+    flops = sum(cost_flops(lv, W, H, N) for lv in levels_per_stream)
+    if "cost" in stages:
+        c = stages["cost"]
+        tflops = flops / (c["ms_per_run"] * 1e-3) / 1e12
+        c.update({"alg_gflop_per_run": round(flops / 1e9, 3), "achieved_TFLOPs": round(tflops, 2),
+                  "flop_frac": round(tflops / FP32_PEAK_TFLOPS, 4), "arith": args.cost_arith})
+        other = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=1 - arith)
+        ob = Batch(frames, other, outputs=("depth",))
+        ob.set_tuning(**batch.tuning())
+        ob.run_timed()
+        oms = sum(ob.run_timed()["cost"][0] for _ in range(reps)) / reps
+        ob.close()
+        c["other_arith_ms_per_run"] = round(oms, 4)
     dom = max(stages, key=lambda k: stages[k]["ms_per_run"])
     d = stages[dom]
     traffic, traffic_src = measured_traffic(dom, W, H, N, B)
@@ -379,6 +430,8 @@ def main():
     batch.close()
 
     # ---- CPU baseline: FP32 OpenMP oracle on a bounded sample -------------------
+    # one warm-up keyframe, then keyframes until the time budget is spent (at
+    # least 5); per-stage and end-to-end medians as BASELINE.md's protocol asks
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -387,18 +440,29 @@ def main():
         imgs, T, traj = scenes[0]
         seeds = np.zeros((H, W, 4), np.float32)
         eye = np.eye(3, 4, dtype=np.float32)
-        O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye)  # warm
-        n_done = 0
+        kf = lambda: O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye,
+                                arith=arith)
+        kf()  # warm-up
+        per_stage, ends = [], []
         tc0 = time.perf_counter()
         while True:
-            O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye)
-            n_done += 1
-            if time.perf_counter() - tc0 > args.cpu_seconds or n_done >= 400:
+            t_a = time.perf_counter()
+            r = kf()
+            ends.append(1e3 * (time.perf_counter() - t_a))
+            per_stage.append(r["stage_ms"])
+            if len(ends) >= 5 and (time.perf_counter() - tc0 > args.cpu_seconds or len(ends) >= 400):
                 break
         tc = time.perf_counter() - tc0
-        cpu = {"value": round(n_done / tc, 3), "unit": "depth maps/s", "cores": threads, "kind": "port",
-               "sample": f"{n_done} keyframes of {W}x{H}x{N}, full pipeline, FP32 OpenMP oracle, {tc:.1f}s"}
-        print(f"[bench] cpu oracle: {cpu['value']} maps/s on {threads} threads", file=sys.stderr, flush=True)
+        med = np.median(np.array(per_stage), axis=0)
+        cpu = {"value": round(len(ends) / tc, 3), "unit": "depth maps/s", "cores": threads, "kind": "port",
+               "sample": f"{len(ends)} keyframes of {W}x{H}x{N} after one warm-up, full pipeline, FP32 OpenMP "
+                         f"oracle ({args.cost_arith} cost arithmetic), {tc:.1f}s",
+               "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+               "median_ms": round(float(np.median(ends)), 3),
+               "stage_median_ms": {k: round(float(v), 3) for k, v in
+                                   zip(("quadtree", "cost", "bp", "upsample", "fusion"), med)}}
+        print(f"[bench] cpu oracle: {cpu['value']} maps/s on {threads} threads ({cpu['cpu_model']}), "
+              f"stage medians {cpu['stage_median_ms']}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -416,7 +480,7 @@ def main():
         "config": {"workload": f"{B} streams x {W}x{H} keyframes, {N} inverse-depth hypotheses, full pipeline "
                                "(quadtree, cost volume, HBP, upsample, fusion), "
                                + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
-                   "streams": B, "width": W, "height": H, "hypotheses": N,
+                   "streams": B, "width": W, "height": H, "hypotheses": N, "cost_arith": args.cost_arith,
                    "launch_shapes": batch_tuning},
         "plain_maps_per_s": None if plain is None else round(plain, 2),
         "batched_over_plain": None if plain is None else round(maps_per_s / plain, 3),

@@ -48,11 +48,28 @@ typedef struct qm_params {
   int max_refs;         /* reference deque length, KEYFRAME_NUM = 60 */
   float ref_cos;        /* add a reference if cos(z, z_ref) < 0.95 ... */
   float ref_baseline;   /* ... or the baseline exceeds 0.03 m */
+  int cost_arith;       /* projection arithmetic of the cost volume, QM_COST_*: both give
+                           the same algebra, and both are bit-identical to the oracle's
+                           matching mode; default QM_COST_REFERENCE */
 } qm_params;
+
+/* Cost-volume projection arithmetic (qm_params.cost_arith).
+ * QM_COST_REFERENCE: the reference's operation order
+ *   (src/depth_extract.cu:178-197): P = dir * depth, q = R P + t with every
+ *   product rounded, px = (fx q.x) / q.z + cx by IEEE division, texel
+ *   (int)(px + i u2u + j v2u + 0.5).
+ * QM_COST_FUSED: the same projection with the per-cell products hoisted out of
+ *   the sample loop and fused multiply-adds: a = (fx R0.dir, fy R1.dir, R2.dir),
+ *   U = fma(depth, a, (fx t0, fy t1, t2)), r = 1 / U.z (correctly rounded),
+ *   px + 0.5 = fma(U.x, r, cx + 0.5), texel floor(px + 0.5 + i u2u + j v2u).
+ *   4 roundings instead of 12 on the way to each pixel; about a third fewer
+ *   instructions per (cell, sample). */
+#define QM_COST_REFERENCE 0
+#define QM_COST_FUSED 1
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 6
+#define QM_ABI_VERSION 7
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);

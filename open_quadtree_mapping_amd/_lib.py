@@ -18,6 +18,7 @@ D = C.c_double
 STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", "upsample_col", "fuse_warp",
           "fuse_update")
 TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows", "overlap")  # QM_TUNE_* order
+COST_REFERENCE, COST_FUSED = 0, 1  # QM_COST_*: qm_params.cost_arith
 
 
 class QmError(RuntimeError):
@@ -37,6 +38,7 @@ class qm_params(C.Structure):
         ("max_refs", C.c_int),
         ("ref_cos", C.c_float),
         ("ref_baseline", C.c_float),
+        ("cost_arith", C.c_int),
     ]
 
 
@@ -82,7 +84,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 6  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 7  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {

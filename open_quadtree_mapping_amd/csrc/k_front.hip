@@ -133,6 +133,24 @@ __global__ __launch_bounds__(256) void k_quadtree(const FrameDev* __restrict__ f
 // the rounding is a single v_cvt_rpi_i32_f32 (floor(v + 0.5), exact on the
 // hardware for every float; the argument is >= -0.5 here, where floor and
 // truncation agree). Other transforms take the general path with the clamp.
+//
+// Two projection arithmetics, each shared bit for bit with the oracle:
+//  * reference order (QM_COST_REFERENCE): q = R (dir * depth) + t summed as
+//    ((R0 p.x + R1 p.y) + R2 p.z) + t, then two IEEE divisions
+//    px = (fx q.x) / q.z + cx, py = (fy q.y) / q.z + cy, the order of
+//    src/depth_extract.cu:178-191 with every product rounded.
+//  * fused (QM_COST_FUSED): the per-cell products are hoisted out of the
+//    sample loop, a = (fx (R0.dir), fy (R1.dir), R2.dir) with each dot product
+//    as fma(R2, d.z, fma(R1, d.y, R0 d.x)), b = (fx t0, fy t1, t2); per sample
+//    U = fma(depth, a, b) for the three rows, r = RN(1 / U.z) and
+//    (px + 0.5, py + 0.5) = (fma(U.x, r, cx + 0.5), fma(U.y, r, cy + 0.5));
+//    the texel of a tap is floor of the tap sum formed from px + 0.5 in the
+//    reference's order ((p + u2u i) + v2u j). The same algebra in 4 roundings
+//    instead of 12, so its errors against an FP64 evaluation are no larger.
+//    RN(1 / z) is v_rcp_f32 refined by one FMA Newton step, which equals the
+//    IEEE quotient for every normal |z| in [2^-126, 2^126) (checked
+//    exhaustively over all such floats on gfx950); outside that range, and
+//    for zero, infinities and NaN, the IEEE division is used.
 // ---------------------------------------------------------------------------
 // The tap bound above, evaluated on the host for every reference of a launch
 // (same float arithmetic as the device); false for NaN.
@@ -145,6 +163,65 @@ __device__ __forceinline__ int cvt_rpi(float v) {
   asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
   return r;
 }
+// floor(v) as an int (saturating, NaN -> 0)
+__device__ __forceinline__ int cvt_flr(float v) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+// Texel index of a tap sum: round-half-up of the reference's pixel in the
+// reference order, floor of (pixel + 0.5) in the fused arithmetic.
+template <bool FUSED>
+__device__ __forceinline__ int tap_index(float v) {
+  return FUSED ? cvt_flr(v) : cvt_rpi(v);
+}
+
+// RN(1 / z): one Newton step on the hardware reciprocal, with the IEEE
+// division where that step is not exact (|z| outside [2^-126, 2^126), zero,
+// infinities, NaN). The branch is wave-uniform in practice: depths in the
+// reference frame are far from both limits.
+__device__ __forceinline__ float recip_rn(float z) {
+  const float y = __builtin_amdgcn_rcpf(z);
+  const float e = fmaf(-z, y, 1.0f);
+  float r = fmaf(e, y, y);
+  const float az = fabsf(z);
+  if (__builtin_expect(!(az >= 0x1p-126f && az < 0x1p126f), 0)) r = 1.0f / z;
+  return r;
+}
+
+// RN(1 / v) for v in [1, 5] (a normal number: the Newton step is exact there).
+__device__ __forceinline__ float recip_small(float v) {
+  const float y = __builtin_amdgcn_rcpf(v);
+  return fmaf(fmaf(-v, y, 1.0f), y, y);
+}
+// RN(a / b) from y = RN(1 / b): q = a y, then one correction with the exact
+// residual a - b q (used where the whole input domain has been checked).
+__device__ __forceinline__ float div_by_recip(float a, float b, float y) {
+  const float q = a * y;
+  return fmaf(fmaf(-b, q, a), y, q);
+}
+constexpr float kRecip255 = 1.0f / 255.0f;  // RN(1 / 255), folded by the compiler
+
+// The fused arithmetic's floor of a tap sum on the general path, saturating
+// as the oracle's (NaN -> below 0, clamped to texel 0 there).
+__device__ __forceinline__ int floor_sat(float v) {
+  v = fminf(fmaxf(v, -1.0e9f), 1.0e9f);
+  return (int)floorf(v);
+}
+
+// Image bytes through a buffer resource: a tap's address is a 32-bit byte
+// offset (no 64-bit address arithmetic per load), and an offset past the
+// image reads 0 instead of touching memory outside it.
+typedef __amdgpu_buffer_rsrc_t ImgRsrc;
+__device__ __forceinline__ ImgRsrc img_rsrc(const uint8_t* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ unsigned img_dword(ImgRsrc r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ unsigned img_byte(ImgRsrc r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
 
 // Fast path: the three taps of a patch row (fixed j) usually land on three
 // consecutive texels of one image row. Such a row is read with one unaligned
@@ -155,7 +232,9 @@ __device__ __forceinline__ int cvt_rpi(float v) {
 typedef unsigned u32_unaligned __attribute__((aligned(1)));
 
 // Offsets y * pitch + x of the nine taps, [j][i] for tap (i - 1, j - 1), on
-// the fast path (no clamp).
+// the fast path (no clamp); (px, py) is the pixel (reference order) or the
+// pixel + 0.5 (fused).
+template <bool FUSED>
 __device__ __forceinline__ void tap_offsets(float px, float py, const float* Tr, int pitch, unsigned (&o)[3][3]) {
   const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
   float bx[3], by[3];
@@ -171,11 +250,12 @@ __device__ __forceinline__ void tap_offsets(float px, float py, const float* Tr,
     for (int i = 0; i < 3; ++i) {
       const float sx = j == 1 ? bx[i] : bx[i] + (j < 1 ? -v2u : v2u);
       const float sy = j == 1 ? by[i] : by[i] + (j < 1 ? -v2v : v2v);
-      o[j][i] = __umul24((unsigned)cvt_rpi(sy), (unsigned)pitch) + (unsigned)cvt_rpi(sx);
+      o[j][i] = __umul24((unsigned)tap_index<FUSED>(sy), (unsigned)pitch) + (unsigned)tap_index<FUSED>(sx);
     }
 }
 
 // General path: clamped taps, float SAD in the reference's tap order.
+template <bool FUSED>
 __device__ __forceinline__ float patch_sad_clamped(gptr<const uint8_t> ref, int W, int H, int pitch, float px,
                                                    float py, const float* Tr, const unsigned (&patch)[3][3]) {
   const float u2u = Tr[0], u2v = Tr[4], v2u = Tr[1], v2v = Tr[5];
@@ -184,8 +264,10 @@ __device__ __forceinline__ float patch_sad_clamped(gptr<const uint8_t> ref, int 
   for (int j = -1; j <= 1; ++j) {
 #pragma unroll
     for (int i = -1; i <= 1; ++i) {
-      const int chx = min(max(round_half_up((px + u2u * (float)i) + v2u * (float)j), 0), W - 1);
-      const int chy = min(max(round_half_up((py + u2v * (float)i) + v2v * (float)j), 0), H - 1);
+      const float sx = (px + u2u * (float)i) + v2u * (float)j;
+      const float sy = (py + u2v * (float)i) + v2v * (float)j;
+      const int chx = min(max(FUSED ? floor_sat(sx) : round_half_up(sx), 0), W - 1);
+      const int chy = min(max(FUSED ? floor_sat(sy) : round_half_up(sy), 0), H - 1);
       sad += fabsf((float)patch[i + 1][j + 1] - (float)ref[(size_t)chy * pitch + chx]);
     }
   }
@@ -251,7 +333,7 @@ __device__ __forceinline__ void cost_pyramid_tile(const FrameDev& F, const float
   }
 }
 
-template <int N, bool FAST, int WAVES>
+template <int N, bool FAST, int WAVES, bool FUSED>
 __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict__ frames, BlockMap bm,
                                                      const float* __restrict__ depth_table, int nblocks, int natural,
                                                      int pyramid) {
@@ -267,7 +349,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
   const int tiles_x = (Wc + 7) >> 3;
   const int tl = lb_global - bm.start[f];
   const int ty = tl / tiles_x, tx = tl - ty * tiles_x;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave index is uniform: the sample loop then reads the depth table
+  // through scalar loads
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int cxx = 8 * tx + (lane & 7), cyy = 8 * ty + (lane >> 3);
   const int cell = cyy * Wc + cxx;
   const int dbase = kPerWave * w;
@@ -289,9 +374,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
       for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int i = 0; i < 12; ++i) T[r][i] = F.T[r][i];
+      const unsigned img_bytes = (unsigned)H * (unsigned)pitch;
       gptr<const uint8_t> ref[4];
+      ImgRsrc rs[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ref[r] = gp(F.ref[r]);
+      for (int r = 0; r < 4; ++r) {
+        ref[r] = gp(F.ref[r]);
+        rs[r] = img_rsrc(F.ref[r], img_bytes);
+      }
       const gptr<const uint8_t> key = gp(F.key);
       unsigned patch[3][3], prow[3];
 #pragma unroll
@@ -302,31 +392,60 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
 #pragma unroll
       for (int j = 0; j < 3; ++j) prow[j] = patch[0][j] | (patch[1][j] << 8) | (patch[2][j] << 16);
       const V3 dir = ray_dir(x, y, fx, fy, cx, cy);
-      const float wlim = (float)W - 2.5f, hlim = (float)H - 2.5f;  // exact for W, H <= 16384
-      const unsigned last4 = (unsigned)H * (unsigned)pitch - 4u;  // last offset a dword may start at
+      // fused arithmetic: the per-cell rows a = (fx R0.dir, fy R1.dir, R2.dir)
+      float ax[4], ay[4], az[4], bx[4], by[4];
+      if (FUSED) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          bx[r] = fx * T[r][3];
+          by[r] = fy * T[r][7];
+          ax[r] = fx * fmaf(T[r][2], dir.z, fmaf(T[r][1], dir.y, T[r][0] * dir.x));
+          ay[r] = fy * fmaf(T[r][6], dir.z, fmaf(T[r][5], dir.y, T[r][4] * dir.x));
+          az[r] = fmaf(T[r][10], dir.z, fmaf(T[r][9], dir.y, T[r][8] * dir.x));
+        }
+      }
+      // in-view bounds: 2 <= (int)((double)p + 0.5) < W - 2 is 1.5 <= p < W - 2.5
+      // on the reference-order pixel and 2 <= p5 < W - 2 on the fused p + 0.5
+      // (exact for W, H <= 16384; false for NaN)
+      const float lo = FUSED ? 2.0f : 1.5f;
+      const float wlim = FUSED ? (float)W - 2.0f : (float)W - 2.5f;
+      const float hlim = FUSED ? (float)H - 2.0f : (float)H - 2.5f;
+      const float safe = FUSED ? 2.5f : 2.0f;  // an in-bounds centre for lanes out of view
+      const float cx5 = cx + 0.5f, cy5 = cy + 0.5f;
+      const unsigned last4 = img_bytes - 4u;  // last offset a dword may start at
 #pragma unroll 2
       for (int t = 0; t < kPerWave; ++t) {
         const float depth = depth_table[dbase + t];
-        V3 P;
-        P.x = dir.x * depth;
-        P.y = dir.y * depth;
-        P.z = dir.z * depth;
         // all four projections and view tests first, then all taps: the texel
         // loads and their SADs then sit in one basic block. nr[r] is 1 for a
         // reference that sees the sample, else 0.
         float pxs[4], pys[4], nr[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const V3 q = se3_apply(T[r], P);
-          const float pxf = (fx * q.x) / q.z + cx;
-          const float pyf = (fy * q.y) / q.z + cy;
-          // 2 <= (int)((double)p + 0.5) < W - 2  <=>  1.5 <= p < W - 2.5 (false for NaN)
-          const bool in = pxf >= 1.5f && pxf < wlim && pyf >= 1.5f && pyf < hlim;
+          float pxf, pyf;
+          if (FUSED) {
+            const float U = fmaf(depth, ax[r], bx[r]);
+            const float V = fmaf(depth, ay[r], by[r]);
+            const float Z = fmaf(depth, az[r], T[r][11]);
+            const float rz = recip_rn(Z);
+            pxf = fmaf(U, rz, cx5);
+            pyf = fmaf(V, rz, cy5);
+          } else {
+            V3 P;
+            P.x = dir.x * depth;
+            P.y = dir.y * depth;
+            P.z = dir.z * depth;
+            const V3 q = se3_apply(T[r], P);
+            pxf = (fx * q.x) / q.z + cx;
+            pyf = (fy * q.y) / q.z + cy;
+          }
+          const bool in = pxf >= lo && pxf < wlim && pyf >= lo && pyf < hlim;
           nr[r] = in ? 1.0f : 0.0f;
-          // out of view, the fast path samples around (2, 2), always in bounds,
-          // and drops the result, so the taps of all references issue together
-          pxs[r] = (FAST && !in) ? 2.0f : pxf;
-          pys[r] = (FAST && !in) ? 2.0f : pyf;
+          // out of view, the fast path samples around a safe centre, always in
+          // bounds, and drops the result, so the taps of all references issue
+          // together
+          pxs[r] = (FAST && !in) ? safe : pxf;
+          pys[r] = (FAST && !in) ? safe : pyf;
         }
         float cr[4];
         if (FAST) {
@@ -340,17 +459,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
           bool rw[4][3];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            tap_offsets(pxs[r], pys[r], T[r], pitch, t[r]);
+            tap_offsets<FUSED>(pxs[r], pys[r], T[r], pitch, t[r]);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
               rw[r][j] = (t[r][j][1] == t[r][j][0] + 1u) & (t[r][j][2] == t[r][j][0] + 2u) & (t[r][j][0] <= last4);
-              v[r][j] = *reinterpret_cast<gptr<const u32_unaligned>>(ref[r] + (rw[r][j] ? t[r][j][0] : 0u));
+              v[r][j] = img_dword(rs[r], rw[r][j] ? t[r][j][0] : 0u);
             }
 #pragma unroll
             for (int j = 0; j < 3; ++j)
               if (!rw[r][j]) {
 #pragma unroll
-                for (int i = 0; i < 3; ++i) t[r][j][i] = ref[r][t[r][j][i]];
+                for (int i = 0; i < 3; ++i) t[r][j][i] = img_byte(rs[r], t[r][j][i]);
               }
           }
 #pragma unroll
@@ -368,15 +487,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            cr[r] = nr[r] != 0.0f ? patch_sad_clamped(ref[r], W, H, pitch, pxs[r], pys[r], T[r], patch) : 0.0f;
+            cr[r] = nr[r] != 0.0f ? patch_sad_clamped<FUSED>(ref[r], W, H, pitch, pxs[r], pys[r], T[r], patch) : 0.0f;
         }
         const float tot = ((cr[0] + cr[3]) + cr[1]) + (cr[0] + cr[2]);
         const float n = ((nr[0] + nr[3]) + nr[1]) + (nr[0] + nr[2]);
-        // normalize_the_cost divides by add_num, which each of the 64 sample
-        // threads of a node's block incremented once: an exact / 64
-        // branch-free: the quotient is computed for every lane (n >= 1 where used)
-        const float q = (tot / fmaxf(n, 1.0f)) / 255.0f;
-        trow[t] = (n > 0.0f ? q : 100.0f) / 64.0f;
+        // (tot / n) / 255 with both quotients correctly rounded, as FMA-corrected
+        // products with the reciprocal (exact for every integer tot <= 5*9*255
+        // and n in 1..5, and for every such quotient over 255: checked over the
+        // whole domain); normalize_the_cost divides by add_num, which each of
+        // the 64 sample threads of a node's block incremented once: an exact
+        // / 64. Branch-free: the quotient is computed for every lane (n >= 1
+        // where used).
+        const float nn = fmaxf(n, 1.0f);
+        const float q = div_by_recip(div_by_recip(tot, nn, recip_small(nn)), 255.0f, kRecip255);
+        trow[t] = (n > 0.0f ? q : 100.0f) * 0.015625f;
       }
     }
   }
@@ -466,19 +590,34 @@ void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hi
   if (nb > 0) hipLaunchKernelGGL(k_quadtree, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
-template <int N, bool FAST>
+template <int N, bool FAST, bool FUSED>
 static void launch_cost_shape(int waves, int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
                               const float* table, int natural, int pyramid) {
   if (waves == 2)
-    hipLaunchKernelGGL((k_cost<N, FAST, 2>), dim3(nb), dim3(128), 0, s, dframes, bm, table, nb, natural, pyramid);
+    hipLaunchKernelGGL((k_cost<N, FAST, 2, FUSED>), dim3(nb), dim3(128), 0, s, dframes, bm, table, nb, natural,
+                       pyramid);
   else if (waves == 8)
-    hipLaunchKernelGGL((k_cost<N, FAST, 8>), dim3(nb), dim3(512), 0, s, dframes, bm, table, nb, natural, pyramid);
+    hipLaunchKernelGGL((k_cost<N, FAST, 8, FUSED>), dim3(nb), dim3(512), 0, s, dframes, bm, table, nb, natural,
+                       pyramid);
   else
-    hipLaunchKernelGGL((k_cost<N, FAST, 4>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, natural, pyramid);
+    hipLaunchKernelGGL((k_cost<N, FAST, 4, FUSED>), dim3(nb), dim3(256), 0, s, dframes, bm, table, nb, natural,
+                       pyramid);
+}
+
+template <int N>
+static void launch_cost_n(bool fast, bool fused, int waves, int nb, hipStream_t s, const FrameDev* dframes,
+                          const BlockMap& bm, const float* table, int natural, int pyramid) {
+  if (fused) {
+    if (fast) launch_cost_shape<N, true, true>(waves, nb, s, dframes, bm, table, natural, pyramid);
+    else launch_cost_shape<N, false, true>(waves, nb, s, dframes, bm, table, natural, pyramid);
+  } else {
+    if (fast) launch_cost_shape<N, true, false>(waves, nb, s, dframes, bm, table, natural, pyramid);
+    else launch_cost_shape<N, false, false>(waves, nb, s, dframes, bm, table, natural, pyramid);
+  }
 }
 
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
-                 const Tuning& t, hipStream_t s, bool pyramid) {
+                 int arith, const Tuning& t, hipStream_t s, bool pyramid) {
   BlockMap bm;
   int counts[kMaxFrames];
   bool fast = true;
@@ -489,13 +628,9 @@ void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N,
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   const int pyr = pyramid ? 1 : 0;
-  if (N == 64) {
-    if (fast) launch_cost_shape<64, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
-    else launch_cost_shape<64, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
-  } else {
-    if (fast) launch_cost_shape<128, true>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
-    else launch_cost_shape<128, false>(t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
-  }
+  const bool fused = arith == QM_COST_FUSED;
+  if (N == 64) launch_cost_n<64>(fast, fused, t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
+  else launch_cost_n<128>(fast, fused, t.cost_waves, nb, s, dframes, bm, table, (int)natural, pyr);
 }
 
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s) {

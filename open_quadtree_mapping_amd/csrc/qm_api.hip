@@ -180,6 +180,8 @@ struct Scratch {
 int check_params(const qm_params* p) {
   if (!p) return fail(QM_ERR_INVALID, "params is NULL");
   if (p->n_hyp != 64 && p->n_hyp != 128) return fail(QM_ERR_INVALID, "n_hyp must be 64 or 128");
+  if (p->cost_arith != QM_COST_REFERENCE && p->cost_arith != QM_COST_FUSED)
+    return fail(QM_ERR_INVALID, "cost_arith must be QM_COST_REFERENCE or QM_COST_FUSED");
   if (!(p->idepth_max > p->idepth_min) || !(p->idepth_min > 0.0))
     return fail(QM_ERR_INVALID, "need 0 < idepth_min < idepth_max");
   for (int i = 0; i < 4; ++i)
@@ -400,7 +402,7 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
       for (int i = 0; i < n; ++i) prior |= hf[i].prior != nullptr;
       // without a prior the cost kernel builds the pyramid from its tiles;
       // the prior cost changes level 0 first, then the pyramid stage runs
-      launch_cost(df, hf, n, N, b->table, false, tu, s, !prior);
+      launch_cost(df, hf, n, N, b->table, false, p->cost_arith, tu, s, !prior);
       if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
       break;
     }
@@ -611,6 +613,7 @@ void qm_default_params(qm_params* p) {
   p->max_refs = 60;
   p->ref_cos = 0.95f;
   p->ref_baseline = 0.03f;
+  p->cost_arith = QM_COST_REFERENCE;
 }
 
 void qm_set_alloc_cap(size_t bytes) { g_cap.store(bytes); }
@@ -668,7 +671,7 @@ int qm_cost_volume(const uint8_t* key, const uint8_t* const refs[4], int W, int 
   if ((rc = sc.get(&dt, sizeof(float) * p->n_hyp))) return rc;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dt, tab.data(), sizeof(float) * p->n_hyp, hipMemcpyHostToDevice, s));
-  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, Tuning(), s);
+  launch_cost((FrameDev*)df, &F, 1, p->n_hyp, (const float*)dt, true, p->cost_arith, Tuning(), s);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   sc.done();
@@ -1287,7 +1290,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
   const BpLaunchParams bp = bp_params(&b->p);
   // inputs of the timed stages: quadtree, cost and its pyramid (the seeds are untouched)
   launch_quadtree(df, hf, n, s);
-  launch_cost(df, hf, n, N, b->table, false, b->tune, s, true);
+  launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, b->tune, s, true);
   hipEvent_t e0, e1;
   QM_HIP(hipEventCreate(&e0));
   QM_HIP(hipEventCreate(&e1));
@@ -1308,7 +1311,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     std::function<void(const Tuning&)> stage;
   };
   const Knob knobs[3] = {
-      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, t, s, true); }},
+      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, t, s, true); }},
       {QM_TUNE_BP_ROWS, {4, 8, 16}, [&](const Tuning& t) { run_sweeps(df, hf, n, &b->p, t, s); }},
       {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
   };

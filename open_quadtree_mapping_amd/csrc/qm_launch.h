@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include "qm_device.h"
+#include "../../include/qm.h"
 
 namespace qm {
 
@@ -37,9 +38,10 @@ struct Tuning {
 
 void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // natural = write the volume in sample order (stage API) instead of the row layout
+// arith = QM_COST_REFERENCE or QM_COST_FUSED (the projection arithmetic)
 // pyramid: the cost kernel also writes levels 1-3 (no stand-alone pyramid needed)
 void launch_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const float* table, bool natural,
-                 const Tuning& t, hipStream_t s, bool pyramid = false);
+                 int arith, const Tuning& t, hipStream_t s, bool pyramid = false);
 void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int N, hipStream_t s);
 // one half-sweep of level l; first = the level's first sweep (also initialises
 // cells' messages from the parent level, or zero at the coarsest level: every

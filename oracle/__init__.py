@@ -48,8 +48,8 @@ def lib():
         L.qmo_quadtree.argtypes = [P, I, I, I, P]
         L.qmo_depth_table_f32.argtypes = [I, D, D, P]
         L.qmo_depth_table_f64.argtypes = [I, D, D, P]
-        L.qmo_cost_f32.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P]
-        L.qmo_cost_f64.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P]
+        L.qmo_cost_f32.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P, I]
+        L.qmo_cost_f64.argtypes = [P, P, I, I, I, P, F, F, F, F, I, P, P, P, P, I]
         L.qmo_bp_extract_f32.argtypes = [P, I, I, P, I, F, F, P, D, D, P, P, P]
         L.qmo_bp_extract_f64.argtypes = [P, I, I, P, I, F, F, P, D, D, P, P, P]
         L.qmo_weights_f32.argtypes = [P, I, I, I, P, P]
@@ -64,7 +64,7 @@ def lib():
         L.qmo_fuse_update_f64.argtypes = [I, I, P, P, P, P, P]
         L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
         L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
-        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P, P, P]
+        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P, P, P, I]
         L.qmo_sd_gradient.argtypes = [P, I, I, I, P]
         L.qmo_sd_init.argtypes = [P, I, I, P]
         L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P, P]
@@ -111,8 +111,13 @@ def depth_table(N: int, imin: float, imax: float, fp64: bool = False) -> np.ndar
     return out
 
 
-def cost_volume(key, refs, T, cam, N, imin, imax, levels, fp64=False):
-    """refs: 4 uint8 images; T: (4,3,4) float32 T_ref_key. Returns (cost HcxWcxN, count HcxWc)."""
+COST_REFERENCE, COST_FUSED = 0, 1  # projection arithmetic (the library's QM_COST_*)
+
+
+def cost_volume(key, refs, T, cam, N, imin, imax, levels, fp64=False, arith=COST_REFERENCE):
+    """refs: 4 uint8 images; T: (4,3,4) float32 T_ref_key. Returns (cost HcxWcxN, count HcxWc).
+    arith: COST_REFERENCE (the reference's operation order) or COST_FUSED
+    (hoisted per-cell products, fused multiply-adds, one reciprocal)."""
     key = np.ascontiguousarray(key, dtype=np.uint8)
     refs = [np.ascontiguousarray(r, dtype=np.uint8) for r in refs]
     assert len(refs) == 4
@@ -124,7 +129,8 @@ def cost_volume(key, refs, T, cam, N, imin, imax, levels, fp64=False):
     count = np.zeros((H // 4, W // 4), np.int32)
     fn = lib().qmo_cost_f64 if fp64 else lib().qmo_cost_f32
     rp = _ptr_array(refs)
-    fn(_p(key), rp, W, H, W, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, _p(table), _p(levels), _p(cost), _p(count))
+    fn(_p(key), rp, W, H, W, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, _p(table), _p(levels), _p(cost), _p(count),
+       int(arith))
     return cost, count
 
 
@@ -221,7 +227,8 @@ def compose_sparse(W, H, depth_coarse, label, prefill=None, fp64=False):
     return out
 
 
-def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None, prior=None, prior_depth=None):
+def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None, prior=None, prior_depth=None,
+             arith=COST_REFERENCE):
     """Full FP32 keyframe. seeds (HxWx4 float32) is updated in place.
     Returns dict(depth, sparse, levels, label, stage_ms)."""
     key = np.ascontiguousarray(key, dtype=np.uint8)
@@ -240,7 +247,7 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     ms = np.zeros(5, np.float64)
     lib().qmo_keyframe_f32(_p(key), _ptr_array(refs), W, H, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, imin, imax,
                            _p(pf), _p(pr), _p(pd), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label),
-                           _p(ms))
+                           _p(ms), int(arith))
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
 
 
@@ -405,7 +412,7 @@ class DepthmapSequence:
     """Oracle of quadmap::Depthmap over a frame sequence (FP32)."""
 
     def __init__(self, cam, N=64, imin=0.01, imax=2.0, ratio=1, min_refs=11, max_refs=60, ref_cos=0.95,
-                 ref_baseline=0.03, semidense=True, prior_cost=False, prior_quadtree=False):
+                 ref_baseline=0.03, semidense=True, prior_cost=False, prior_quadtree=False, arith=COST_REFERENCE):
         self.cam, self.N, self.imin, self.imax, self.ratio = cam, N, imin, imax, ratio
         self.min_refs, self.max_refs = min_refs, max_refs
         self.ref_cos, self.ref_baseline = np.float32(ref_cos), np.float32(ref_baseline)
@@ -418,6 +425,7 @@ class DepthmapSequence:
         self.T_world_ref = np.eye(3, 4, dtype=np.float32)
         self.semidense = semidense
         self.prior_cost, self.prior_quadtree = prior_cost, prior_quadtree
+        self.arith = arith
         self.prior = None
         self.sd_key = self.sd_key_grad = self.sd_key_T = self.sd_seeds = None
         self.prefill = None
@@ -465,7 +473,8 @@ class DepthmapSequence:
             res = keyframe(img, [self.refs[r][0] for r in range(4)], T, self.cam, self.N, self.imin, self.imax,
                            self.seeds, T_last, prefill=self.prefill if self.semidense else None,
                            prior=self.prior if self.semidense and self.prior_cost else None,
-                           prior_depth=self.prefill if self.semidense and self.prior_quadtree else None)
+                           prior_depth=self.prefill if self.semidense and self.prior_quadtree else None,
+                           arith=self.arith)
         if self._need_add_reference(T_curr_world):
             self.refs.insert(0, (img.copy(), T_curr_world.copy()))
             if len(self.refs) > self.max_refs:

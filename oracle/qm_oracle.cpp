@@ -163,10 +163,26 @@ void quadtree_impl(const uint8_t* img, int W, int H, int pitch, uint8_t* levels)
 // Slots f = 0..4 read reference (int)(0.9 * f) = {0,0,1,2,3}
 // (src/depth_extract.cu:153-166 with the age table that is never written).
 // ---------------------------------------------------------------------------
+// Two projection arithmetics (the library's qm_params.cost_arith):
+//   arith 0, reference order: P = dir * depth, q = R P + t with every product
+//     rounded, px = (fx q.x) / q.z + cx (src/depth_extract.cu:178-191), texel
+//     (int)((px + u2u i) + v2u j + 0.5) (:195-197);
+//   arith 1, fused: per cell a = (fx (R0.dir), fy (R1.dir), R2.dir), each dot
+//     product fma(R2, d.z, fma(R1, d.y, R0 d.x)); per sample U = fma(depth, a,
+//     (fx t0, fy t1, t2)), r = 1 / U.z, p5 = fma(U.xy, r, c + 0.5) (the pixel
+//     + 0.5), texel floor((p5 + u2u i) + v2u j), in view when 2 <= p5 < W - 2.
+//     The same algebra with fewer roundings; the HIP kernel computes the same
+//     bits (its reciprocal is exact where it replaces the division).
+template <typename Real> inline Real fma_r(Real a, Real b, Real c) { return std::fma(a, b, c); }
+template <typename Real> inline int floor_sat(Real v) {
+  v = std::fmin(std::fmax(v, Real(-1.0e9)), Real(1.0e9));
+  return (int)std::floor(v);
+}
+
 template <typename Real>
 void cost_impl(const uint8_t* key, const uint8_t* const* refs, int W, int H, int pitch,
                const float* T /*4x12*/, const Cam& cam, int N, const Real* depth_table,
-               const uint8_t* levels, Real* cost, int32_t* count) {
+               const uint8_t* levels, Real* cost, int32_t* count, int arith) {
   const int Wc = W / 4, Hc = H / 4;
 #pragma omp parallel for schedule(dynamic, 4)
   for (int cy = 0; cy < Hc; ++cy) {
@@ -181,6 +197,13 @@ void cost_impl(const uint8_t* key, const uint8_t* const* refs, int W, int H, int
       for (int j = -1; j <= 1; ++j)
         for (int i = -1; i <= 1; ++i) patch[i + 1][j + 1] = (Real)key[(size_t)(y + j) * pitch + (x + i)];
       const V3<Real> dir = ray_dir<Real>(x, y, cam);
+      Real ax[4], ay[4], az[4];
+      for (int r = 0; r < 4; ++r) {
+        const float* Tr = T + 12 * r;
+        ax[r] = (Real)cam.fx * fma_r<Real>(Tr[2], dir.z, fma_r<Real>(Tr[1], dir.y, (Real)Tr[0] * dir.x));
+        ay[r] = (Real)cam.fy * fma_r<Real>(Tr[6], dir.z, fma_r<Real>(Tr[5], dir.y, (Real)Tr[4] * dir.x));
+        az[r] = fma_r<Real>(Tr[10], dir.z, fma_r<Real>(Tr[9], dir.y, (Real)Tr[8] * dir.x));
+      }
       for (int d = 0; d < N; ++d) {
         const Real depth = depth_table[d];
         V3<Real> P;
@@ -190,6 +213,34 @@ void cost_impl(const uint8_t* key, const uint8_t* const* refs, int W, int H, int
         Real cr[4], nr[4];
         for (int r = 0; r < 4; ++r) {
           const float* Tr = T + 12 * r;
+          if (arith == 1) {
+            const Real U = fma_r<Real>(depth, ax[r], (Real)cam.fx * (Real)Tr[3]);
+            const Real V = fma_r<Real>(depth, ay[r], (Real)cam.fy * (Real)Tr[7]);
+            const Real Z = fma_r<Real>(depth, az[r], (Real)Tr[11]);
+            const Real rz = Real(1) / Z;
+            const Real px5 = fma_r<Real>(U, rz, (Real)cam.cx + Real(0.5));
+            const Real py5 = fma_r<Real>(V, rz, (Real)cam.cy + Real(0.5));
+            if (px5 >= Real(2) && px5 < (Real)(W - 2) && py5 >= Real(2) && py5 < (Real)(H - 2)) {
+              const Real u2u = (Real)Tr[0], u2v = (Real)Tr[4], v2u = (Real)Tr[1], v2v = (Real)Tr[5];
+              const uint8_t* ref = refs[r];
+              Real sad = Real(0);
+              for (int j = -1; j <= 1; ++j) {
+                for (int i = -1; i <= 1; ++i) {
+                  int chx = floor_sat<Real>((px5 + u2u * (Real)i) + v2u * (Real)j);
+                  int chy = floor_sat<Real>((py5 + u2v * (Real)i) + v2v * (Real)j);
+                  chx = chx < 0 ? 0 : (chx > W - 1 ? W - 1 : chx);
+                  chy = chy < 0 ? 0 : (chy > H - 1 ? H - 1 : chy);
+                  sad += std::fabs(patch[i + 1][j + 1] - (Real)ref[(size_t)chy * pitch + chx]);
+                }
+              }
+              cr[r] = sad;
+              nr[r] = Real(1);
+            } else {
+              cr[r] = Real(0);
+              nr[r] = Real(0);
+            }
+            continue;
+          }
           const V3<Real> q = se3_apply<Real>(Tr, P);
           const Real px = ((Real)cam.fx * q.x) / q.z + (Real)cam.cx;
           const Real py = ((Real)cam.fy * q.y) / q.z + (Real)cam.cy;
@@ -728,15 +779,15 @@ void qmo_depth_table_f64(int N, double imin, double imax, double* out) {
 
 void qmo_cost_f32(const uint8_t* key, const uint8_t* const* refs, int W, int H, int pitch, const float* T, float fx,
                   float fy, float cx, float cy, int N, const float* table, const uint8_t* levels, float* cost,
-                  int32_t* count) {
+                  int32_t* count, int arith) {
   Cam c{fx, fy, cx, cy};
-  cost_impl<float>(key, refs, W, H, pitch, T, c, N, table, levels, cost, count);
+  cost_impl<float>(key, refs, W, H, pitch, T, c, N, table, levels, cost, count, arith);
 }
 void qmo_cost_f64(const uint8_t* key, const uint8_t* const* refs, int W, int H, int pitch, const float* T, float fx,
                   float fy, float cx, float cy, int N, const double* table, const uint8_t* levels, double* cost,
-                  int32_t* count) {
+                  int32_t* count, int arith) {
   Cam c{fx, fy, cx, cy};
-  cost_impl<double>(key, refs, W, H, pitch, T, c, N, table, levels, cost, count);
+  cost_impl<double>(key, refs, W, H, pitch, T, c, N, table, levels, cost, count, arith);
 }
 
 static BpParams make_bp(int N, float P1, float P2, const int* iters, double imin, double imax) {
@@ -823,8 +874,9 @@ void qmo_compose_sparse_f64(int W, int H, const double* depth_coarse, const int3
 // seeds is the persistent fusion state (updated: it receives the new seeds).
 void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int H, const float* T_refs, float fx,
                       float fy, float cx, float cy, int N, double imin, double imax, const float* prefill,
-                      const float* prior, const float* prior_depth, float* seeds, const float* T_last_to_cur, float* depth_out, float* sparse_out,
-                      uint8_t* levels_out, int32_t* label_out, double* stage_ms) {
+                      const float* prior, const float* prior_depth, float* seeds, const float* T_last_to_cur,
+                      float* depth_out, float* sparse_out, uint8_t* levels_out, int32_t* label_out, double* stage_ms,
+                      int arith) {
   using clk = std::chrono::steady_clock;
   auto ms = [](clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
@@ -857,7 +909,7 @@ void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int
   quadtree_impl(key, W, H, W, levels);
   if (prior_depth) qmo_quadtree_prior(prior_depth, W, H, levels);
   auto t1 = clk::now();
-  cost_impl<float>(key, refs, W, H, W, T_refs, c, N, table.data(), levels, cost.data(), nullptr);
+  cost_impl<float>(key, refs, W, H, W, T_refs, c, N, table.data(), levels, cost.data(), nullptr, arith);
   if (prior) qmo_prior_to_cost(prior, W, H, levels, N, imin, imax, cost.data(), nullptr);
   auto t2 = clk::now();
   bp_extract_impl<float>(cost.data(), W, H, levels, make_bp(N, 0.003f, 0.01f, iters, imin, imax), dcoarse.data(),

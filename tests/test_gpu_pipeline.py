@@ -34,27 +34,28 @@ def keyframe_inputs(cam, traj, imgs, k, dev_imgs):
     return fr, T, T_last
 
 
+@pytest.mark.parametrize("arith", [O.COST_REFERENCE, O.COST_FUSED])
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("N,imin,imax", [(64, 0.02, 2.0), (128, 0.02, 1.0 / 0.3)])
-def test_batch_matches_oracle(graph, N, imin, imax):
+def test_batch_matches_oracle(graph, N, imin, imax, arith):
     sizes = [(376, 240, 3), (752, 480, 5), (376, 240, 7)]
     streams = [stream_frames(W, H, s) for (W, H, s) in sizes]
     dev_imgs = [[torch.from_numpy(im).to(DEV) for im in imgs] for (_, _, imgs) in streams]
-    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=arith)
     first_k = 4
     frames = [keyframe_inputs(cam, traj, imgs, first_k, dev_imgs[i])[0] for i, (cam, traj, imgs) in
               enumerate(streams)]
     b = Batch(frames, p, outputs=("depth", "dense", "sparse"))
     seeds = [np.zeros((H, W, 4), np.float32) for (W, H, _) in sizes]
     for k in range(first_k, first_k + 3):
-        print(f"batch keyframe {k} graph={graph} N={N}", flush=True)
+        print(f"batch keyframe {k} graph={graph} N={N} arith={arith}", flush=True)
         ref = []
         for i, (cam, traj, imgs) in enumerate(streams):
             fr, T, T_last = keyframe_inputs(cam, traj, imgs, k, dev_imgs[i])
             if k > first_k:
                 b.set_frame(i, fr)
             ref.append(O.keyframe(imgs[k], [imgs[k - 1 - j] for j in range(4)], T, cam, N, imin, imax, seeds[i],
-                                  T_last))
+                                  T_last, arith=arith))
         b.run(graph=graph)
         torch.cuda.synchronize()
         for i in range(len(sizes)):

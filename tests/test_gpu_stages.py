@@ -80,23 +80,28 @@ def test_quadtree_scene(W, H, seed):
     assert np.array_equal(got, O.quadtree(ks["key"]))
 
 
+ARITHS = [O.COST_REFERENCE, O.COST_FUSED]
+
+
+@pytest.mark.parametrize("arith", ARITHS)
 @pytest.mark.parametrize("W,H,seed,N,imin,imax", [(376, 240, 3, 64, 0.02, 2.0), (752, 480, 5, 64, 0.01, 2.0),
                                                   (376, 240, 4, 128, 0.02, 1.0 / 0.3)])
-def test_cost_volume(W, H, seed, N, imin, imax):
+def test_cost_volume(W, H, seed, N, imin, imax, arith):
     ks = keyframe_set(W, H, seed)
-    print(f"cost volume {W}x{H} N={N}", flush=True)
+    print(f"cost volume {W}x{H} N={N} arith={arith}", flush=True)
     levels = O.quadtree(ks["key"])
-    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=arith)
     cost, count = S.cost_volume(dev(ks["key"]), [dev(r) for r in ks["refs"]], ks["T"], cam_tuple(ks["cam"]),
                                 dev(levels), p, want_count=True)
-    rc, rn = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], N, imin, imax, levels)
+    rc, rn = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], N, imin, imax, levels, arith=arith)
     assert np.array_equal(host(count), rn)
     got = host(cost)
     assert np.array_equal(got, rc), f"max abs diff {np.abs(got - rc).max()}"
 
 
+@pytest.mark.parametrize("arith", ARITHS)
 @pytest.mark.parametrize("N", [64, 128])
-def test_cost_volume_general_transform(N):
+def test_cost_volume_general_transform(N, arith):
     # transforms whose rotation block is not a rotation (rows scaled by 2, so
     # |u2u| + |v2u| exceeds 1.75) take the kernel's clamped general tap path;
     # one such reference switches the whole launch to it
@@ -105,11 +110,11 @@ def test_cost_volume_general_transform(N):
     T = ks["T"].copy()
     T[1, :2, :3] *= np.float32(2.0)
     T[3, :2, :3] *= np.float32(0.7)
-    print(f"cost volume general transform N={N}", flush=True)
+    print(f"cost volume general transform N={N} arith={arith}", flush=True)
     levels = O.quadtree(ks["key"])
-    p = default_params(n_hyp=N, idepth_min=0.02, idepth_max=2.0)
+    p = default_params(n_hyp=N, idepth_min=0.02, idepth_max=2.0, cost_arith=arith)
     cost = S.cost_volume(dev(ks["key"]), [dev(r) for r in ks["refs"]], T, cam_tuple(ks["cam"]), dev(levels), p)
-    rc, _ = O.cost_volume(ks["key"], ks["refs"], T, ks["cam"], N, 0.02, 2.0, levels)
+    rc, _ = O.cost_volume(ks["key"], ks["refs"], T, ks["cam"], N, 0.02, 2.0, levels, arith=arith)
     assert np.array_equal(host(cost), rc)
 
 

@@ -125,11 +125,15 @@ def _nodes(levels, W, H):
     return m
 
 
-def test_cost_identical_images_identity_pose_is_zero():
+ARITHS = [O.COST_REFERENCE, O.COST_FUSED]
+
+
+@pytest.mark.parametrize("arith", ARITHS)
+def test_cost_identical_images_identity_pose_is_zero(arith):
     rng = np.random.default_rng(1)
     key = rng.integers(0, 256, (64, 96)).astype(np.uint8)
     lv = O.quadtree(key)
-    cost, cnt = O.cost_volume(key, [key] * 4, EYE, CAM, 64, 0.02, 2.0, lv)
+    cost, cnt = O.cost_volume(key, [key] * 4, EYE, CAM, 64, 0.02, 2.0, lv, arith=arith)
     m = _nodes(lv, 96, 64)
     assert np.array_equal(cnt == 64, m) and np.all((cnt == 0) | (cnt == 64))
     assert np.all(cost[~m] == 0)
@@ -140,12 +144,13 @@ def test_cost_identical_images_identity_pose_is_zero():
     assert np.all(cost[inner] == 0)
 
 
-def test_cost_slot_multiset_is_0_0_1_2_3():
+@pytest.mark.parametrize("arith", ARITHS)
+def test_cost_slot_multiset_is_0_0_1_2_3(arith):
     key = np.full((64, 96), 100, np.uint8)
     vals = [90, 130, 100, 40]
     refs = [np.full((64, 96), v, np.uint8) for v in vals]
     lv = O.quadtree(key)
-    cost, _ = O.cost_volume(key, refs, EYE, CAM, 64, 0.02, 2.0, lv)
+    cost, _ = O.cost_volume(key, refs, EYE, CAM, 64, 0.02, 2.0, lv, arith=arith)
     sads = [9.0 * abs(100 - v) for v in vals]
     # normalize_the_cost: add_num = 64, one increment per sample thread of the node
     want = np.float32((sads[0] + sads[3] + sads[1] + sads[0] + sads[2]) / 5.0 / 255.0 / 64.0)
@@ -155,41 +160,48 @@ def test_cost_slot_multiset_is_0_0_1_2_3():
     assert np.allclose(cost[cy, cx], want, rtol=1e-6, atol=0)
 
 
-def test_cost_out_of_view_is_100_over_64():
+@pytest.mark.parametrize("arith", ARITHS)
+def test_cost_out_of_view_is_100_over_64(arith):
     key = np.full((64, 96), 100, np.uint8)
     T = EYE.copy()
     T[:, 0, 3] = 1e4  # everything projects far outside
     lv = O.quadtree(key)
-    cost, cnt = O.cost_volume(key, [key] * 4, T, CAM, 64, 0.02, 2.0, lv)
+    cost, cnt = O.cost_volume(key, [key] * 4, T, CAM, 64, 0.02, 2.0, lv, arith=arith)
     m = cnt == 64
     assert m.any() and np.all(cost[m] == 1.5625)
 
 
-def test_cost_translation_minimum_at_true_disparity():
+@pytest.mark.parametrize("arith", ARITHS)
+@pytest.mark.parametrize("tx", [0.25, -0.17, 0.4])
+def test_cost_translation_minimum_at_true_disparity(arith, tx):
     # pure x translation: the centre pixel (on the optical axis) at depth z shows
     # in the reference at x + fx*tx/z; a textured reference shifted by that
-    # disparity makes the SAD exactly 0 at the matching sample.
+    # disparity makes the SAD exactly 0 at the samples whose projection rounds
+    # to it. The projection is taken in exact FP64 (x = 48 + 60 tx / z, z the
+    # float sample depth); samples within 1e-4 px of a rounding boundary, where
+    # FP32 roundings may legitimately go either way, are not asserted.
     W, H = 96, 64
     rng = np.random.default_rng(4)
     key = rng.integers(0, 256, (H, W)).astype(np.uint8)
     N, imin, imax = 64, 0.02, 2.0
     table = O.depth_table(N, imin, imax)
-    tx = 0.25
+    tx32 = float(np.float32(tx))
     dstar = 20
-    disp = 60.0 * tx / float(table[dstar])
-    shift = int(disp + 0.5)
+    shift = int(np.floor(60.0 * tx32 / float(table[dstar]) + 0.5))
     ref = np.roll(key, shift, axis=1)
     T = EYE.copy()
     T[:, 0, 3] = tx
     lv = np.full((H, W), 2, np.uint8)
-    cost, _ = O.cost_volume(key, [ref] * 4, T, CAM, N, imin, imax, lv)
+    cost, _ = O.cost_volume(key, [ref] * 4, T, CAM, N, imin, imax, lv, arith=arith)
     c = cost[32 // 4, 48 // 4]
-    f32 = np.float32
-    proj = [float(f32(f32(60.0) * f32(tx)) / f32(table[d]) + f32(48.0)) for d in range(N)]
-    hits = [d for d in range(N) if int(proj[d] + 0.5) == 48 + shift]
-    assert dstar in hits
-    assert np.all(c[hits] == 0)
-    assert np.all(c[[d for d in range(N) if d not in hits]] > 0)
+    px = 48.0 + 60.0 * tx32 / table.astype(np.float64)
+    frac = px + 0.5 - np.floor(px + 0.5)
+    clear = (frac > 1e-4) & (frac < 1 - 1e-4)
+    hit = np.floor(px + 0.5) == 48 + shift
+    assert hit[dstar] and clear[dstar]
+    assert np.all(c[hit & clear] == 0)
+    assert np.all(c[~hit & clear] > 0)
+    assert clear.sum() >= N - 2
 
 
 # ---------------------------------------------------------------------------
