@@ -228,6 +228,16 @@ int qm_point_cloud(const float* depth, const uint8_t* ref_img, int W, int H, con
 int qm_fuse(float* seeds, const float* depth_in, int W, int H, const float cam[4], const float T_last_to_cur[12],
             float* depth_out, float* seeds_out, int32_t* table, void* stream);
 
+/* qm_fuse without any device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate (tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and synchronisation as qm_fuse;
+ * its scratch (descriptor and z-buffer) is one static device array of
+ * 16 MiB + 4 KiB per GPU, held until the process ends, so W * H is at most
+ * 2097152 pixels (QM_ERR_INVALID beyond). Calls on one GPU share that array:
+ * calls that overlap on one GPU (other host threads, other streams) race. */
+int qm_fuse_noalloc(float* seeds, const float* depth_in, int W, int H, const float cam[4],
+                    const float T_last_to_cur[12], float* depth_out, float* seeds_out, int32_t* table, void* stream);
+
 /* Adjoint of the per-pixel fusion update (src/depth_fusion.cu:146-213) with
  * respect to its dense depth input, for the warped seed state and hole-filled
  * table qm_fuse produced: grad = g_depth_out * d(depth_out)/d(depth) +

@@ -102,6 +102,7 @@ SIGNATURES = {
     "qm_upsample_general_batch": (I, [P, I, P]),
     "qm_upsample_backward_batch": (I, [P, I, P]),
     "qm_fuse": (I, [P, P, I, I, P, P, P, P, P, P]),
+    "qm_fuse_noalloc": (I, [P, P, I, I, P, P, P, P, P, P]),
     "qm_fuse_backward": (I, [P, P, P, I, I, P, P, P, P]),
     "qm_point_cloud": (I, [P, P, I, I, P, P, P, P, P]),
     "qm_upsample_local": (I, [P, P, I, I, I, P, P]),

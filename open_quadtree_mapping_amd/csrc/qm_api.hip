@@ -151,6 +151,26 @@ void scratch_return(void* p) {
   dev_free(p);
 }
 
+void init_geometry(FrameDev& F, int W, int H, int pitch) {
+  std::memset(&F, 0, sizeof(F));
+  F.W = W;
+  F.H = H;
+  F.pitch = pitch;
+  F.Wc = W / 4;
+  F.Hc = H / 4;
+  F.lw[0] = F.Wc;
+  F.lh[0] = F.Hc;
+  for (int l = 1; l < 4; ++l) {
+    F.lw[l] = (F.lw[l - 1] + 1) / 2;
+    F.lh[l] = (F.lh[l - 1] + 1) / 2;
+  }
+  F.row_seg = 16 * (W / 16);
+  F.col_seg = 8 * (H / 8);
+  F.edge_r = (W % 16) ? 1 : 0;
+  F.edge_c = (H % 16) ? 1 : 0;
+  F.lam = 10.0;
+}
+
 }  // namespace qm
 
 namespace {
@@ -210,26 +230,6 @@ void host_weight_lut(float* out) {
     const float arg = (float)(-(double)(d * d) / 400.0);
     out[d] = (float)std::exp((double)arg);
   }
-}
-
-void init_geometry(FrameDev& F, int W, int H, int pitch) {
-  std::memset(&F, 0, sizeof(F));
-  F.W = W;
-  F.H = H;
-  F.pitch = pitch;
-  F.Wc = W / 4;
-  F.Hc = H / 4;
-  F.lw[0] = F.Wc;
-  F.lh[0] = F.Hc;
-  for (int l = 1; l < 4; ++l) {
-    F.lw[l] = (F.lw[l - 1] + 1) / 2;
-    F.lh[l] = (F.lh[l - 1] + 1) / 2;
-  }
-  F.row_seg = 16 * (W / 16);
-  F.col_seg = 8 * (H / 8);
-  F.edge_r = (W % 16) ? 1 : 0;
-  F.edge_c = (H % 16) ? 1 : 0;
-  F.lam = 10.0;
 }
 
 BpLaunchParams bp_params(const qm_params* p) {

@@ -11,6 +11,10 @@
   } while (0)
 
 namespace qm {
+struct FrameDev;
+// geometry fields of a stage-level descriptor (sizes, pyramid levels, chain
+// splits, weight edges); every other field zero
+void init_geometry(FrameDev& F, int W, int H, int pitch);
 // set the thread-local message returned by qm_last_error and pass the code through
 int fail(int code, const std::string& msg);
 // device allocation that honours the test cap and the byte accounting

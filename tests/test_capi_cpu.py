@@ -86,3 +86,12 @@ def test_tuning_knob_names_match_header():
     ids = dict((m.group(1).lower(), int(m.group(2))) for m in re.finditer(r"#define QM_TUNE_([A-Z_]+) (\d+)", hdr))
     assert ids == {k: i for i, k in enumerate(_lib.TUNE_KNOBS)}
     assert f"#define QM_NUM_TUNE {len(_lib.TUNE_KNOBS)}" in hdr
+
+
+def test_fuse_noalloc_rejects_sizes_beyond_its_static_scratch():
+    L = _lib.lib()
+    one = C.c_void_p(1)
+    rc = L.qm_fuse_noalloc(one, one, 2048, 1088, one, one, one, one, None, None)
+    assert rc == -1 and b"2097152" in L.qm_last_error()
+    rc = L.qm_fuse_noalloc(None, one, 752, 480, one, one, one, one, None, None)
+    assert rc == -1 and b"NULL" in L.qm_last_error()
