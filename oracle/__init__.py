@@ -24,7 +24,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libqm_oracle.so")
+# QM_ORACLE_LIB names another build of the oracle (used to check that the
+# oracle's pins fail on a build with a deliberately planted mistake)
+_LIB_PATH = os.environ.get("QM_ORACLE_LIB") or os.path.join(_HERE, "libqm_oracle.so")
 _lib = None
 
 P = C.c_void_p
@@ -68,6 +70,13 @@ def lib():
         L.qmo_sd_gradient.argtypes = [P, I, I, I, P]
         L.qmo_sd_init.argtypes = [P, I, I, P]
         L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P, P]
+        L.qmo_sd_search_point.argtypes = [I, I, I, I, F, F, F, F, F, F, F, F, F, F, P, P, P, P, P, P]
+        L.qmo_sd_search_point.restype = F
+        L.qmo_sd_fill_holes.argtypes = [P, I, I, P]
+        L.qmo_sd_tex_bilinear.argtypes = [P, I, I, F, F]
+        L.qmo_sd_tex_bilinear.restype = F
+        L.qmo_sd_regularize.argtypes = [P, I, I, I]
+        L.qmo_sd_new_keyframe.argtypes = [P, I, I, F, F, F, F, P, P, P, P, P, P]
         L.qmo_quadtree_prior.argtypes = [P, I, I, P]
         L.qmo_prior_to_cost.argtypes = [P, I, I, P, I, D, D, P, P]
         L.qmo_set_threads.argtypes = [I]
@@ -364,6 +373,59 @@ def sd_step(seeds, key, key_grad, inc, inc_grad, cam, T_key_world, T_inc_world, 
                       _p(np.ascontiguousarray(inc_grad, np.float32)), W, H, W, cam.fx, cam.fy, cam.cx, cam.cy,
                       _p(Tk), _p(Ti), int(bool(switch_keyframe)), _p(prefill), _p(prior))
     return (prefill, prior) if want_prior else prefill
+
+
+def sd_search_point(x, y, epx, epy, gradient_max, my_gradient, min_idep, max_idep, cam, k2i, key, key_grad, inc,
+                    inc_grad):
+    """search_point (src/lsd_semidense.cu:529-753) of one key pixel. Returns
+    (score or failure code -1/-2/-3, inverse depth, variance, epipolar length)."""
+    key = np.ascontiguousarray(key, np.uint8)
+    H, W = key.shape
+    out = np.zeros(3, np.float32)
+    err = lib().qmo_sd_search_point(int(x), int(y), W, H, epx, epy, gradient_max, my_gradient, min_idep, max_idep,
+                                    cam.fx, cam.fy, cam.cx, cam.cy,
+                                    _p(np.ascontiguousarray(k2i, np.float32).reshape(12)), _p(key),
+                                    _p(np.ascontiguousarray(key_grad, np.float32)),
+                                    _p(np.ascontiguousarray(inc, np.uint8)),
+                                    _p(np.ascontiguousarray(inc_grad, np.float32)), _p(out))
+    return float(err), float(out[0]), float(out[1]), float(out[2])
+
+
+def sd_tex_bilinear(img, u, v):
+    """The epipolar search's bilinear texture fetch at (u, v), texel centres at +0.5."""
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    return float(lib().qmo_sd_tex_bilinear(_p(img), W, H, float(u), float(v)))
+
+
+def sd_fill_holes(seeds, inc_grad):
+    """fill_holes (src/lsd_semidense.cu:345-387) on a copy of (H, W, 8) seeds."""
+    s = np.array(seeds, np.float32, copy=True, order="C")
+    H, W = s.shape[:2]
+    lib().qmo_sd_fill_holes(_p(s), W, H, _p(np.ascontiguousarray(inc_grad, np.float32)))
+    return s
+
+
+def sd_regularize(seeds, remove_occlusions=False):
+    """regularize (src/lsd_semidense.cu:296-344) on a copy of (H, W, 8) seeds."""
+    s = np.array(seeds, np.float32, copy=True, order="C")
+    H, W = s.shape[:2]
+    lib().qmo_sd_regularize(_p(s), W, H, int(bool(remove_occlusions)))
+    return s
+
+
+def sd_new_keyframe(old_seeds, cam, T_old_to_new, key, key_grad, inc, inc_grad):
+    """Propagation into a new keyframe (src/lsd_semidense.cu:64-181 as driven by
+    src/seed_matrix.cu:247-304): returns the new (H, W, 8) seeds."""
+    old = np.ascontiguousarray(old_seeds, np.float32)
+    H, W = old.shape[:2]
+    out = np.zeros_like(old)
+    lib().qmo_sd_new_keyframe(_p(old), W, H, cam.fx, cam.fy, cam.cx, cam.cy,
+                              _p(np.ascontiguousarray(T_old_to_new, np.float32).reshape(12)),
+                              _p(np.ascontiguousarray(key, np.uint8)), _p(np.ascontiguousarray(key_grad, np.float32)),
+                              _p(np.ascontiguousarray(inc, np.uint8)), _p(np.ascontiguousarray(inc_grad, np.float32)),
+                              _p(out))
+    return out
 
 
 def axis_cos_baseline(T_a_world, T_b_world):

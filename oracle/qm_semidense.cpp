@@ -605,4 +605,51 @@ void qmo_sd_step(float* seeds, const uint8_t* key_img, const float* key_grad, co
   std::memcpy(seeds, s.data(), sizeof(Seed) * n);
 }
 
+// Single steps of the module, for the oracle's own pins (tests/test_oracle_pins.py).
+// search_point of one key pixel: out = (result inverse depth, variance,
+// epipolar length); returns the score or the -1/-2/-3 failure code.
+float qmo_sd_search_point(int x, int y, int W, int H, float epx, float epy, float gradient_max, float my_gradient,
+                          float min_idep, float max_idep, float fx, float fy, float cx, float cy, const float* k2i,
+                          const uint8_t* key_img, const float* key_grad, const uint8_t* inc_img, const float* inc_grad,
+                          float* out) {
+  const Cam c{fx, fy, cx, cy};
+  const Frame key{key_img, key_grad, W, H, W};
+  const Frame inc{inc_img, inc_grad, W, H, W};
+  float idep = 0.0f, var = 0.0f, len = 0.0f;
+  const float err =
+      search_point(x, y, W, H, epx, epy, gradient_max, my_gradient, min_idep, max_idep, c, k2i, key, inc, idep, var, len);
+  out[0] = idep;
+  out[1] = var;
+  out[2] = len;
+  return err;
+}
+// one bilinear texture fetch at (u, v) in texel-centre coordinates
+float qmo_sd_tex_bilinear(const uint8_t* img, int W, int H, float u, float v) { return tex_bilinear(img, W, H, W, u, v); }
+// hole filling (reads inc_grad) and regularisation of a W x H x 8 seed array, in place
+void qmo_sd_fill_holes(float* seeds, int W, int H, const float* inc_grad) {
+  const Frame inc{nullptr, inc_grad, W, H, W};
+  std::vector<Seed> s((size_t)W * H);
+  std::memcpy(s.data(), seeds, sizeof(Seed) * s.size());
+  fill_holes(s, W, H, inc);
+  std::memcpy(seeds, s.data(), sizeof(Seed) * s.size());
+}
+void qmo_sd_regularize(float* seeds, int W, int H, int remove_occlusions) {
+  std::vector<Seed> s((size_t)W * H);
+  std::memcpy(s.data(), seeds, sizeof(Seed) * s.size());
+  regularize(s, W, H, remove_occlusions != 0);
+  std::memcpy(seeds, s.data(), sizeof(Seed) * s.size());
+}
+// propagation of old-keyframe seeds through o2n (old -> new camera) into the new keyframe
+void qmo_sd_new_keyframe(const float* old_seeds, int W, int H, float fx, float fy, float cx, float cy, const float* o2n,
+                         const uint8_t* key_img, const float* key_grad, const uint8_t* inc_img, const float* inc_grad,
+                         float* out_seeds) {
+  const Cam c{fx, fy, cx, cy};
+  const Frame key{key_img, key_grad, W, H, W};
+  const Frame inc{inc_img, inc_grad, W, H, W};
+  std::vector<Seed> old((size_t)W * H), out;
+  std::memcpy(old.data(), old_seeds, sizeof(Seed) * old.size());
+  new_keyframe(old, W, H, c, o2n, key, inc, out);
+  std::memcpy(out_seeds, out.data(), sizeof(Seed) * out.size());
+}
+
 }  // extern "C"

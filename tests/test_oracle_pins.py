@@ -857,3 +857,343 @@ def test_sd_prior_projection_identity_pose():
     np.testing.assert_allclose(prior[m, 0], (s[..., 6] / np.sqrt(r2))[m], rtol=1e-6)
     np.testing.assert_allclose(prior[m, 1], np.maximum(r2 * s[..., 7], 0.01)[m], rtol=2e-6)
     assert np.all(prior[~strong] == 0)
+
+
+# ---------------------------------------------------------------------------
+# Fusion hole filling (src/depth_fusion.cu:121-144), pinned by a source-centric
+# simulation of the reference kernel: every source thread with 2 <= x <= W-2,
+# 2 <= y <= H-2 and a non-empty entry CASes its entry into the empty cells of
+# its 3x3 neighbourhood (y offset outer, x offset inner). The schedule
+# simulated is "every thread reads its own entry, then the threads run their
+# CAS loops one after another in raster order": one legal execution of the
+# racy kernel. With the identity pose each good seed warps onto its own pixel,
+# so the warp table before filling is known from the good/bad seed pattern.
+# ---------------------------------------------------------------------------
+def hole_fill_reference_kernel(table):
+    H, W = table.shape
+    snap = table.copy()
+    out = table.copy()
+    for y in range(H):
+        for x in range(W):
+            if x >= W - 1 or y >= H - 1 or x <= 1 or y <= 1:
+                continue
+            ti = snap[y, x]
+            if ti == 0:
+                continue
+            for i in (-1, 0, 1):
+                for j in (-1, 0, 1):
+                    if out[y + i, x + j] == 0:
+                        out[y + i, x + j] = ti
+    return out
+
+
+def _fusion_table(good, W, H):
+    cam = Camera(W, H, 40.0, 40.0, W / 2.0, H / 2.0)
+    seeds = np.zeros((H, W, 4), np.float32)
+    seeds[..., 0] = np.where(good, 10.0, 0.0)   # a < 0.001: a bad point, not warped
+    seeds[..., 1] = 1.0
+    seeds[..., 2] = 5.0
+    seeds[..., 3] = 1.0
+    depth = np.full((H, W), 5.0, np.float32)
+    _, _, _, table = O.fuse(seeds, depth, np.eye(3, 4, dtype=np.float32), cam)
+    warped = np.where(good, np.arange(W * H, dtype=np.int32).reshape(H, W), 0).astype(np.int32)
+    warped[0, 0] = 0  # source index 0 is the empty sentinel: pixel (0, 0) is never stored
+    return table, warped
+
+
+@pytest.mark.parametrize("seed,density", [(1, 0.5), (2, 0.2), (3, 0.8), (4, 0.05)])
+def test_fusion_hole_filling_matches_reference_kernel_simulation(seed, density):
+    W, H = 23, 17
+    good = np.random.default_rng(seed).random((H, W)) < density
+    good[0, 0] = True
+    table, warped = _fusion_table(good, W, H)
+    assert np.array_equal(table, hole_fill_reference_kernel(warped))
+
+
+def test_fusion_hole_filling_constructed_cases():
+    W, H = 12, 10
+    # two sources compete for one hole: the raster-first source wins
+    good = np.zeros((H, W), bool)
+    good[4, 6] = good[5, 4] = True
+    table, _ = _fusion_table(good, W, H)
+    assert table[5, 5] == 4 * W + 6          # (6, 4) precedes (4, 5) in raster order
+    assert table[4, 5] == 4 * W + 6 and table[6, 3] == 5 * W + 4
+    # no cascade: cells two steps away stay empty
+    assert table[7, 7] == 0 and table[2, 2] == 0
+    # the source band is 2 <= x <= W-2, 2 <= y <= H-2: x = 1 and x = W-1 sources fill nothing
+    good = np.zeros((H, W), bool)
+    good[5, 1] = good[5, W - 1] = good[1, 6] = good[H - 1, 6] = True
+    table, _ = _fusion_table(good, W, H)
+    filled = table != 0
+    assert filled.sum() == 4 and filled[5, 1] and filled[5, W - 1] and filled[1, 6] and filled[H - 1, 6]
+    # x = 2 and x = W-2 sources do fill
+    good = np.zeros((H, W), bool)
+    good[5, 2] = good[5, W - 2] = True
+    table, _ = _fusion_table(good, W, H)
+    assert table[4, 1] == 5 * W + 2 and table[6, W - 1] == 5 * W + W - 2
+    # the sentinel: pixel (0, 0)'s own seed stores index 0, i.e. nothing
+    good = np.zeros((H, W), bool)
+    good[0, 0] = True
+    table, _ = _fusion_table(good, W, H)
+    assert np.all(table == 0)
+
+
+# ---------------------------------------------------------------------------
+# Semi-dense epipolar search (src/lsd_semidense.cu:529-753): a pure x
+# translation of a texture that varies along x only. A key pixel at inverse
+# depth q shows in the income frame at u = x + fx tx q, so with an income image
+# that is the key shifted by an integer disparity the search returns q to a
+# small fraction of a pixel, and the variance is the reference's model in
+# closed form: alpha = d(idepth)/du = 1 / (fx tx) and
+# var = alpha^2 (0.05 [subpixel] + 0.25 gmax^2 / g^2 + 4^2 / g^2).
+# ---------------------------------------------------------------------------
+SD_W, SD_H = 160, 64
+SD_CAM = Camera(SD_W, SD_H, 100.0, 100.0, 80.0, 32.0)
+
+
+def _sd_signal(n, kind="smooth"):
+    x = np.arange(n)
+    if kind == "smooth":
+        return np.round(128 + 50 * np.sin(x / 3.1) + 30 * np.sin(x / 7.3 + 1.0)).astype(np.int64)
+    return np.where((x // 2) % 2 == 0, 90, 170).astype(np.int64)  # period 4: ambiguous
+
+
+def _sd_pair(disp_px, kind="smooth"):
+    sig = _sd_signal(SD_W + 64, kind)
+    key = np.tile(sig[32:32 + SD_W], (SD_H, 1)).astype(np.uint8)
+    inc = np.tile(sig[32 - disp_px:32 - disp_px + SD_W], (SD_H, 1)).astype(np.uint8)
+    return key, inc
+
+
+@pytest.mark.parametrize("idepth,tx", [(0.6, 0.1), (0.5, 0.08), (1.0, 0.05)])
+def test_sd_search_recovers_translation_depth_and_closed_form_variance(idepth, tx):
+    disp = 100.0 * float(np.float32(tx)) * idepth
+    assert abs(disp - round(disp)) < 1e-6
+    key, inc = _sd_pair(int(round(disp)))
+    kg, ig = O.sd_gradient(key), O.sd_gradient(inc)
+    k2i = np.eye(3, 4, dtype=np.float32)
+    k2i[0, 3] = tx
+    alpha = 1.0 / (100.0 * float(np.float32(tx)))
+    checked = 0
+    for x in range(20, 110, 3):
+        g = float(kg[32, x, 0])
+        if abs(g) < 5.0:
+            continue
+        for gmax in (abs(g), 2.0 * abs(g)):
+            err, q, var, _ = O.sd_search_point(x, 32, 1.0, 0.0, gmax, abs(g), 0.01, 2.0, SD_CAM, k2i, key, kg, inc,
+                                               ig)
+            assert err >= 0.0, (x, err)
+            assert abs(q - idepth) * 100.0 * tx <= 0.05, (x, q)      # within 0.05 px
+            want = alpha * alpha * (0.05 + 0.25 * gmax * gmax / (g * g) + 16.0 / (g * g))
+            assert var == pytest.approx(want, rel=2e-6), (x, var, want)
+            checked += 1
+    assert checked >= 30
+
+
+def test_sd_bilinear_fetch_uses_8_bit_fraction_weights():
+    # CUDA's linear filtering (Programming Guide, texture fetching appendix)
+    # stores the weights in 9-bit fixed point with 8 fractional bits. On a ramp
+    # T(i, j) = 4 i + 2 j the fetch at (i0 + 0.5 + f, j0 + 0.5 + g) is therefore
+    # 4 (i0 + round(256 f) / 256) + 2 (j0 + round(256 g) / 256), not the exact
+    # 4 (i0 + f) + 2 (j0 + g); clamped at the border.
+    W, H = 40, 30
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = (4 * xx + 2 * yy).astype(np.uint8)
+    rng = np.random.default_rng(12)
+    exact_differs = 0
+    for _ in range(400):
+        i0, j0 = int(rng.integers(0, W - 1)), int(rng.integers(0, H - 1))
+        f, g = float(np.float32(rng.random())), float(np.float32(rng.random()))
+        u, v = np.float32(i0 + 0.5 + f), np.float32(j0 + 0.5 + g)
+        fu = float(u) - 0.5 - i0  # the fraction the fetch sees
+        fv = float(v) - 0.5 - j0
+        got = O.sd_tex_bilinear(img, u, v)
+        want = 4 * (i0 + np.rint(256 * fu) / 256) + 2 * (j0 + np.rint(256 * fv) / 256)
+        assert got == pytest.approx(want, abs=1e-5), (i0, j0, f, g)
+        exact_differs += abs(got - (4 * (i0 + fu) + 2 * (j0 + fv))) > 1e-3
+    assert exact_differs > 100
+    # clamp addressing: beyond the last texel the fetch holds the edge value
+    assert O.sd_tex_bilinear(img, W + 3.0, 5.5) == float(img[5, W - 1])
+    assert O.sd_tex_bilinear(img, -2.0, 5.5) == float(img[5, 0])
+
+
+def test_sd_search_failure_codes():
+    k2i = np.eye(3, 4, dtype=np.float32)
+    k2i[0, 3] = 0.1
+    key, inc = _sd_pair(6)
+    kg, ig = O.sd_gradient(key), O.sd_gradient(inc)
+    g = float(kg[32, 20, 0])
+    # the far end of the search range leaves the image: -1
+    err, *_ = O.sd_search_point(0, 32, 1.0, 0.0, abs(g), abs(g), 0.01, 2.0, SD_CAM, k2i, key, kg, inc, ig)
+    assert err == -1.0
+    # no match anywhere (a black income image): -3
+    err, *_ = O.sd_search_point(20, 32, 1.0, 0.0, abs(g), abs(g), 0.01, 2.0, SD_CAM, k2i, key, kg, 0 * inc, ig)
+    assert err == -3.0
+    # a period-4 texture: the second best match is 4 px away and as good: -2
+    key, inc = _sd_pair(6, kind="periodic")
+    kg, ig = O.sd_gradient(key), O.sd_gradient(inc)
+    x = next(x for x in range(20, 40) if abs(kg[32, x, 0]) >= 5)
+    g = float(kg[32, x, 0])
+    err, *_ = O.sd_search_point(x, 32, 1.0, 0.0, abs(g), abs(g), 0.01, 2.0, SD_CAM, k2i, key, kg, inc, ig)
+    assert err == -2.0
+
+
+# ---------------------------------------------------------------------------
+# Semi-dense hole filling (src/lsd_semidense.cu:345-387): an invalid interior
+# pixel with enough valid neighbours in its 5x5 window (validity sum > 100, or
+# > 24 when not blacklisted below -1) and an income gradient of at least 5
+# becomes a seed with the inverse-variance weighted mean of the valid
+# neighbours: idepth = sum(q / v) / sum(1 / v), variance = 1 / sum(1 / v),
+# validity counter 0. Fills read a snapshot: no cascade.
+# ---------------------------------------------------------------------------
+def _sd_seeds(W, H):
+    return np.zeros((H, W, 8), np.float32)
+
+
+def _put_seed(s, x, y, q, var, counter, blacklist=0.0):
+    s[y, x] = [1.0, blacklist, counter, 0.0, q, var, 1.0, 4.0]
+
+
+def test_sd_fill_holes_inverse_variance_mean():
+    W, H = 20, 16
+    rng = np.random.default_rng(5)
+    s = _sd_seeds(W, H)
+    nb = [(8 + i, 7 + j) for j in range(-2, 3) for i in range(-2, 3) if (i, j) != (0, 0) and rng.random() < 0.6]
+    qs = rng.uniform(0.2, 1.5, len(nb)).astype(np.float32)
+    vs = rng.uniform(0.01, 0.5, len(nb)).astype(np.float32)
+    for (x, y), q, v in zip(nb, qs, vs):
+        _put_seed(s, x, y, q, v, 20.0)
+    grad = np.full((H, W, 2), 4.0, np.float32)  # |g|^2 = 32 >= 25 everywhere
+    out = O.sd_fill_holes(s, grad)
+    w = 1.0 / vs.astype(np.float64)
+    c = out[7, 8]
+    assert c[0] == 1.0 and c[2] == 0.0 and c[1] == 0.0
+    assert c[4] == pytest.approx(float((qs * w).sum() / w.sum()), rel=2e-6)
+    assert c[5] == pytest.approx(float(1.0 / w.sum()), rel=2e-6)
+    # valid seeds are left alone
+    for (x, y) in nb:
+        assert np.array_equal(out[y, x], s[y, x])
+
+
+def test_sd_fill_holes_thresholds_and_snapshot():
+    W, H = 24, 16
+    grad = np.full((H, W, 2), 4.0, np.float32)
+    # validity sum exactly 24 does not create, 25 does
+    s = _sd_seeds(W, H)
+    _put_seed(s, 6, 6, 0.5, 0.1, 24.0)
+    out = O.sd_fill_holes(s, grad)
+    assert out[6, 7, 0] == 0.0
+    s[6, 6, 2] = 25.0
+    out = O.sd_fill_holes(s, grad)
+    assert out[6, 7, 0] == 1.0 and out[6, 7, 4] == pytest.approx(0.5, rel=1e-7)
+    # a hole blacklisted below -1 needs a sum above 100
+    s = _sd_seeds(W, H)
+    _put_seed(s, 6, 6, 0.5, 0.1, 60.0)
+    s[6, 7, 1] = -2.0
+    out = O.sd_fill_holes(s, grad)
+    assert out[6, 7, 0] == 0.0 and out[6, 8, 0] == 1.0
+    s[6, 6, 2] = 101.0
+    out = O.sd_fill_holes(s, grad)
+    assert out[6, 7, 0] == 1.0
+    # an income gradient below 5 creates nothing
+    low = grad.copy()
+    low[6, 7] = 3.0  # |g|^2 = 18
+    out = O.sd_fill_holes(s, low)
+    assert out[6, 7, 0] == 0.0 and out[6, 8, 0] == 1.0
+    # snapshot: a fill two pixels beyond the window of the only source stays empty
+    assert out[6, 9, 0] == 0.0 and out[6, 8, 0] == 1.0 and out[6, 4, 0] == 1.0
+
+
+# ---------------------------------------------------------------------------
+# Keyframe propagation (src/lsd_semidense.cu:64-181 via src/seed_matrix.cu:
+# 247-304): seeds of a fronto-parallel plane at inverse depth 0.5 (a power of
+# two, so the regularisation's weighted means of a constant field are exact)
+# land where the motion puts them with the inverse depth it implies; of two
+# seeds that land on one pixel the larger inverse depth wins.
+# ---------------------------------------------------------------------------
+def _sd_textured(W, H):
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = (100 + 10 * ((xx + yy) % 4)).astype(np.uint8)  # |gx| = |gy| = 10 everywhere inside
+    return img, O.sd_gradient(img)
+
+
+def test_sd_propagation_identity_and_forward_motion():
+    W, H = 48, 40
+    cam = Camera(W, H, 40.0, 40.0, 24.0, 20.0)
+    img, grad = _sd_textured(W, H)
+    old = _sd_seeds(W, H)
+    for y in range(8, 32):
+        for x in range(8, 40):
+            old[y, x] = [1.0, 0.0, 30.0, 0.0, 0.5, 0.01, 0.5, 0.01]
+    out = O.sd_new_keyframe(old, cam, np.eye(3, 4, dtype=np.float32), img, grad, img, grad)
+    inner = (slice(10, 30), slice(10, 38))
+    assert np.all(out[inner][..., 0] == 1.0)
+    assert np.all(out[inner][..., 4] == 0.5) and np.all(out[inner][..., 6] == 0.5)
+    # forward by 0.5 m: depth 2 -> 1.5, so inverse depth 1 / 1.5 wherever a seed lands
+    T = np.eye(3, 4, dtype=np.float32)
+    T[2, 3] = -0.5
+    out = O.sd_new_keyframe(old, cam, T, img, grad, img, grad)
+    v = out[..., 0] == 1.0
+    assert v.sum() > 300
+    assert np.allclose(out[v][:, 4], 1.0 / 1.5, rtol=1e-6, atol=0)
+    # the points spread out from the principal point by 2 / 1.5
+    ys, xs = np.nonzero(v)
+    assert xs.min() < 8 - 2 and xs.max() > 40 + 2
+
+
+def test_sd_propagation_collision_larger_inverse_depth_wins():
+    W, H = 48, 40
+    cam = Camera(W, H, 40.0, 40.0, 24.0, 20.0)
+    img, grad = _sd_textured(W, H)
+    T = np.eye(3, 4, dtype=np.float32)
+    T[0, 3] = 0.2  # fx tx = 8: inverse depth q moves 8 q pixels
+    for qa, qb in [(0.5, 0.25), (0.25, 0.5)]:
+        old = _sd_seeds(W, H)
+        # A at x = 20 moves 8 qa, B at x = 20 + 8 (qa - qb) moves 8 qb: both land on x = 20 + 8 qa
+        xa, xb, y = 20, 20 + int(8 * (qa - qb)), 20
+        old[y, xa] = [1.0, 0.0, 30.0, 0.0, qa, 0.01, qa, 0.01]
+        old[y, xb] = [1.0, 0.0, 30.0, 0.0, qb, 0.01, qb, 0.01]
+        out = O.sd_new_keyframe(old, cam, T, img, grad, img, grad)
+        tx_ = 20 + int(8 * qa)
+        assert out[y, tx_, 0] == 1.0
+        assert out[y, tx_, 4] == max(qa, qb)
+
+
+# ---------------------------------------------------------------------------
+# Point cloud (src/publisher.cpp:129-164): under the identity pose every point
+# lies at distance d along its pixel's ray; a translation shifts every point
+# by it; a rotation by 90 degrees about z maps (x, y, z) to (-y, x, z); pixels
+# below 0.1 are left out, NaN kept, raster order.
+# ---------------------------------------------------------------------------
+def test_point_cloud_closed_forms():
+    W, H = 24, 16
+    cam = Camera(W, H, 20.0, 22.0, 11.5, 7.5)
+    rng = np.random.default_rng(9)
+    depth = rng.uniform(0.2, 9.0, (H, W)).astype(np.float32)
+    depth[3, 4] = 0.05
+    depth[5, 6] = -1.0
+    depth[7, 8] = np.nan
+    img = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    I = np.eye(3, 4, dtype=np.float32)
+    pts = O.point_cloud(depth, img, cam, I)
+    keep = ~(depth.astype(np.float64) < 0.1)
+    assert len(pts) == keep.sum() == W * H - 2
+    ys, xs = np.nonzero(keep)
+    d = depth[ys, xs].astype(np.float64)
+    fin = np.isfinite(d)
+    assert np.allclose(np.linalg.norm(pts[fin, :3].astype(np.float64), axis=1), d[fin], rtol=1e-6)
+    ray = np.stack([(xs - 11.5) / 20.0, (ys - 7.5) / 22.0, np.ones_like(d)], 1)
+    ray /= np.linalg.norm(ray, axis=1, keepdims=True)
+    assert np.allclose(pts[fin, :3], ray[fin] * d[fin, None], rtol=1e-5, atol=1e-6)
+    assert np.isnan(pts[~fin, :3]).all()
+    assert np.array_equal(pts[:, 3], img[ys, xs].astype(np.float32))
+    T = I.copy()
+    T[:, 3] = [0.5, -2.0, 3.0]
+    pt = O.point_cloud(depth, img, cam, T)
+    assert np.allclose(pt[fin, :3], pts[fin, :3] + T[:, 3], rtol=1e-6, atol=1e-6)
+    R = np.zeros((3, 4), np.float32)
+    R[0, 1], R[1, 0], R[2, 2] = -1.0, 1.0, 1.0
+    pr = O.point_cloud(depth, img, cam, R)
+    assert np.allclose(pr[fin, 0], -pts[fin, 1], rtol=1e-6, atol=1e-7)
+    assert np.allclose(pr[fin, 1], pts[fin, 0], rtol=1e-6, atol=1e-7)
+    assert np.allclose(pr[fin, 2], pts[fin, 2], rtol=1e-6, atol=1e-7)
