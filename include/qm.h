@@ -69,7 +69,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 7
+#define QM_ABI_VERSION 8
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -322,8 +322,14 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int chosen[QM_NUM_TUN
  * Goes through the fusion update adjoint (prior seeds and warp table held
  * fixed) and the upsample adjoint (its forward recomputed, not stored), one
  * launch per stage over all frames, then synchronises the stream. Scratch is
- * allocated on the first call. No counterpart in the reference. */
+ * allocated on the first call. QM_ERR_STATE before the first run and after a
+ * qm_batch_set_frame, qm_batch_reset_seeds or qm_batch_autotune that followed
+ * the last run (they change what that run's state the backward reads). A
+ * later run replaces the state: a caller that keeps a run for a later
+ * backward compares qm_batch_runs. No counterpart in the reference. */
 int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad_lambda, void* stream);
+/* Number of runs (qm_batch_run, qm_batch_run_timed) since qm_batch_create. */
+int qm_batch_runs(qm_batch* b, long long* runs);
 /* Device pointer of an internal buffer of frame i, for inspection:
  * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state),
  * 6 hole-filled warp table of the last run (int32), 7 warped prior seeds of the last run.

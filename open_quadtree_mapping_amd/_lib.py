@@ -84,7 +84,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 7  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 8  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {
@@ -119,6 +119,7 @@ SIGNATURES = {
     "qm_batch_read_buffer": (I, [P, I, I, P, C.c_size_t]),
     "qm_batch_reset_seeds": (I, [P]),
     "qm_batch_backward": (I, [P, P, P, P]),
+    "qm_batch_runs": (I, [P, P]),
     "qm_batch_set_tuning": (I, [P, I, I]),
     "qm_batch_get_tuning": (I, [P, I, P]),
     "qm_batch_autotune": (I, [P, I, P, P, P]),

@@ -5,7 +5,8 @@ UpsampleFunction: out = x_col / n_col of the generalised upsample
   forward sweep and runs the transposed (= same, A is symmetric) tridiagonal
   solves. Gradients w.r.t. sparse depths, confidences, per-pixel smoothness
   weights, and the smoothness scale lambda when it is given as a tensor
-  (dL/dlambda = sum_e w_e dL/dw_e / lambda, since A = I + lambda L(w)).
+  (dL/dlambda = sum_e w_e dL/dw_e / lambda, since A = I + lambda L(w); such a
+  lambda must be > 0).
 FuseFunction: the Gaussian x Beta update of one keyframe (qm_fuse) with
   backward qm_fuse_backward w.r.t. the dense depth input (prior seeds fixed).
 BatchDepthFunction: one batched pipeline run (every stream through every
@@ -73,6 +74,9 @@ class UpsampleFunction(torch.autograd.Function):
             lams = [float(lam)] * B
         if len(lams) != B:
             raise ValueError("lam must be a scalar or hold one value per problem")
+        if ctx.needs_input_grad[4] and not all(v > 0.0 for v in lams):
+            # dL/dlambda is formed from dL/dw as sum_e w_e dL/dw_e / lambda
+            raise ValueError("a differentiable lam must be > 0")
         out = torch.empty_like(d_)
         probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg)
         for b in range(B):
@@ -164,11 +168,18 @@ class BatchDepthFunction(torch.autograd.Function):
         batch.set_lambdas([float(v) for v in lams.detach().double().cpu()])
         batch.run(graph=graph)
         ctx.batch = batch
+        ctx.run_id = batch.runs()
         ctx.dtype, ctx.device = lams.dtype, lams.device
         return tuple(o["depth"].clone() for o in batch.outputs)
 
     @staticmethod
     def backward(ctx, *grads):
+        # the backward differentiates the state the batch holds now: it must
+        # still be this forward's run (the library also refuses after a
+        # set_frame / reset_seeds / autotune that followed the run)
+        if ctx.batch.runs() != ctx.run_id:
+            raise RuntimeError("batch_depth: the batch ran again after this forward; its backward needs the state "
+                               "of the run it differentiates (call backward before the next run)")
         g = ctx.batch.backward(list(grads))
         return torch.tensor(g, dtype=ctx.dtype, device=ctx.device), None, None
 

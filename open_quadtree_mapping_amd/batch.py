@@ -81,7 +81,14 @@ class Batch:
         check(lib().qm_batch_set_frame(self._h, i, C.byref(d)))
 
     def set_lambdas(self, lams):
-        """Per-stream smoothness weights for the next run."""
+        """Per-stream smoothness weights for the next run (each finite and > 0;
+        the library reads a value <= 0 as "use the params' lambda")."""
+        lams = [float(v) for v in lams]
+        if len(lams) != self.n:
+            raise ValueError(f"need {self.n} lambdas, got {len(lams)}")
+        bad = [v for v in lams if not (np.isfinite(v) and v > 0.0)]
+        if bad:
+            raise ValueError(f"lambdas must be finite and > 0, got {bad}")
         for i, v in enumerate(lams):
             fr = dict(self.frames[i])
             fr["lambda"] = float(v)
@@ -96,6 +103,12 @@ class Batch:
         out = (C.c_double * self.n)()
         check(lib().qm_batch_backward(self._h, arr, out, C.c_void_p(s.cuda_stream)))
         return np.array(out[:], dtype=np.float64)
+
+    def runs(self) -> int:
+        """Number of runs of this batch so far (qm_batch_runs)."""
+        r = C.c_longlong()
+        check(lib().qm_batch_runs(self._h, C.byref(r)))
+        return r.value
 
     def run(self, graph: bool = False, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream()

@@ -38,6 +38,9 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
   if (x >= W) return;
   const double lam = F.lam;
   const float flam = (float)lam;
+  // dL/dw_e = lambda dL/d(lambda w_e); the pipeline backward takes the
+  // unscaled term so that dL/dlambda = sum_e w_e dL/d(lambda w_e) needs no division
+  const float gscale = F.grad_w_unscaled ? 1.0f : flam;
   auto weight = [&](int y) -> float { return gp(F.g_wcol)[(size_t)y * W + x]; };
   const int seg = F.col_seg < H ? F.col_seg : H;
   const int bounds[3] = {0, seg, H};
@@ -124,14 +127,14 @@ __global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ fra
           gp(F.adjxn)[(size_t)x * H + i] = make_float2(vx, vn);
           const float2 z = cin.z[j];
           if (i + 1 < s1) {
-            gp(F.grad_wcol)[(size_t)(i + 1) * W + x] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+            gp(F.grad_wcol)[(size_t)(i + 1) * W + x] = -gscale * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
           } else {
-            const float e = -flam * (vx * z.x + vn * z.y);
+            const float e = -gscale * (vx * z.x + vn * z.y);
             if (s1 == H) gp(F.grad_wcol)[(size_t)H * W + x] = e;
             else carry = e;
           }
           if (i == s0) {
-            const float e = -flam * (vx * z.x + vn * z.y);
+            const float e = -gscale * (vx * z.x + vn * z.y);
             gp(F.grad_wcol)[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
           }
           vx_next = vx;
@@ -163,6 +166,9 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
   if (y >= H) return;
   const double lam = F.lam;
   const float flam = (float)lam;
+  // dL/dw_e = lambda dL/d(lambda w_e); the pipeline backward takes the
+  // unscaled term so that dL/dlambda = sum_e w_e dL/d(lambda w_e) needs no division
+  const float gscale = F.grad_w_unscaled ? 1.0f : flam;
   const size_t row = (size_t)y * W;
   const size_t wrow = (size_t)y * (W + 1);
   auto weight = [&](int x) -> float { return gp(F.g_wrow)[wrow + x]; };
@@ -250,9 +256,9 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
           const float2 z = cin.z[j];
           if (i + 1 < s1) {
             if (F.grad_wrow)
-              gp(F.grad_wrow)[wrow + i + 1] = -flam * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
+              gp(F.grad_wrow)[wrow + i + 1] = -gscale * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
           } else {
-            const float e = -flam * (vx * z.x + vn * z.y);
+            const float e = -gscale * (vx * z.x + vn * z.y);
             if (s1 == W) {
               if (F.grad_wrow) gp(F.grad_wrow)[wrow + W] = e;
             } else {
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ fra
             }
           }
           if (i == s0 && F.grad_wrow) {
-            const float e = -flam * (vx * z.x + vn * z.y);
+            const float e = -gscale * (vx * z.x + vn * z.y);
             gp(F.grad_wrow)[wrow + s0] = (s0 == 0) ? e : carry + e;
           }
           // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
@@ -437,9 +443,11 @@ __global__ __launch_bounds__(256) void k_compose_inputs(const FrameDev* __restri
   }
 }
 
-// dL/dlambda = sum over edges of w_e * dL/dw_e / lambda (A depends on lambda
-// and w only through lambda * w). Fixed-order block sums in double, one
-// partial per block; the host adds a frame's partials in block order.
+// dL/dlambda = sum over edges of w_e * dL/d(lambda w_e) (A depends on lambda
+// and w only through lambda * w; the adjoint kernels wrote the unscaled
+// dL/d(lambda w_e) for this, so lambda = 0 needs no division). Fixed-order
+// block sums in double, one partial per block; the host adds a frame's
+// partials in block order.
 __global__ __launch_bounds__(256) void k_lambda_partials(const FrameDev* __restrict__ frames, BlockMap bm,
                                                          double* __restrict__ partial) {
   __shared__ double red[256];
@@ -457,7 +465,7 @@ __global__ __launch_bounds__(256) void k_lambda_partials(const FrameDev* __restr
     if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] / F.lam;
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 int lambda_blocks(const FrameDev& F) {

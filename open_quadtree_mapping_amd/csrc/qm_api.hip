@@ -308,7 +308,11 @@ struct qm_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int parity = 0;
-  int runs = 0;
+  long long runs = 0;
+  // the state a backward differentiates (the last run's descriptors, warp
+  // tables and warped prior seeds) is intact: set by a run, cleared by
+  // set_frame, reset_seeds and autotune
+  bool run_state_valid = false;
   Tuning tune;
   // extra streams and fork / mark / join events of the frame-group overlap
   static constexpr int kMaxGroups = 8;
@@ -1154,6 +1158,7 @@ int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
     return fail(QM_ERR_INVALID, "frame size differs from the planned one");
   const bool had_prior = any_prior(b), had_prior_depth = any_prior_depth(b);
   for (int par = 0; par < 2; ++par) apply_desc(b->hf[par][i], *d, b->p);
+  b->run_state_valid = false;
   // the alternates' launches are part of a captured graph only when some frame has the prior
   if (had_prior != any_prior(b) || had_prior_depth != any_prior_depth(b)) drop_graphs(b);
   return upload_frames(b);
@@ -1191,6 +1196,13 @@ int qm_batch_run(qm_batch* b, int mode, void* stream) {
   }
   b->parity ^= 1;
   b->runs += 1;
+  b->run_state_valid = true;
+  return QM_OK;
+}
+
+int qm_batch_runs(qm_batch* b, long long* runs) {
+  if (!b || !runs) return fail(QM_ERR_INVALID, "NULL pointer");
+  *runs = b->runs;
   return QM_OK;
 }
 
@@ -1215,6 +1227,7 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double* stage_ms, int* stage_l
   }
   b->parity ^= 1;
   b->runs += 1;
+  b->run_state_valid = true;
   return QM_OK;
 }
 
@@ -1282,6 +1295,7 @@ int qm_batch_get_tuning(qm_batch* b, int knob, int* value) {
 int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* best_ms) {
   if (!b) return fail(QM_ERR_INVALID, "NULL batch");
   if (reps < 1 || reps > 1000) return fail(QM_ERR_INVALID, "reps must be in [1, 1000]");
+  b->run_state_valid = false;  // its timing runs overwrite the last run's intermediate buffers
   hipStream_t s = (hipStream_t)stream;
   const int par = b->parity;
   const FrameDev* df = b->df[par];
@@ -1400,6 +1414,7 @@ int qm_batch_read_buffer(qm_batch* b, int i, int which, void* dst, size_t bytes)
 
 int qm_batch_reset_seeds(qm_batch* b) {
   if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  b->run_state_valid = false;
   for (int i = 0; i < b->n; ++i) {
     const size_t px = (size_t)b->hf[0][i].W * b->hf[0][i].H;
     QM_HIP(hipMemset(b->seeds[0][i], 0, px * 16));
@@ -1435,7 +1450,11 @@ int qm_batch_destroy(qm_batch* b) {
 
 int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad_lambda, void* stream) {
   if (!b || !grad_lambda) return fail(QM_ERR_INVALID, "NULL pointer");
-  if (b->runs == 0) return fail(QM_ERR_INVALID, "no forward run to differentiate");
+  if (b->runs == 0) return fail(QM_ERR_STATE, "no forward run to differentiate");
+  if (!b->run_state_valid)
+    return fail(QM_ERR_STATE,
+                "the batch changed after its last run (set_frame, reset_seeds or autotune); run it again before "
+                "the backward");
   hipStream_t s = (hipStream_t)stream;
   const int n = b->n, last = b->parity ^ 1;
   int rc;
@@ -1497,6 +1516,7 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
     F.grad_conf = nullptr;
     F.grad_wrow = w.gwrow;
     F.grad_wcol = w.gwcol;
+    F.grad_w_unscaled = 1;
     F.colxn = w.colxn;
     F.adjxn = w.adjxn;
     F.row_x_out = nullptr;

@@ -59,6 +59,7 @@ struct FrameDev {
   float* grad_conf;
   float* grad_wrow;     // H x (W+1)
   float* grad_wcol;     // (H+1) x W
+  int grad_w_unscaled;  // 1: grad_w* receive dL/d(lambda w_e) (no lambda factor), for dL/dlambda
   // adjoint of the fusion update
   const int32_t* table_in;     // hole-filled warp table
   const float* g_depth_out;    // dL/d(depth_out)

@@ -59,8 +59,6 @@ def test_batch_lambda_gradient_vs_fp64(n_streams):
         assert np.array_equal(dense, O.upsample(sparse, wr, wc, lam=lam, row_seg=rs, col_seg=cs), equal_nan=True)
         d32, _ = O.fuse_update(table, prior, dense)
         assert np.array_equal(depth, d32, equal_nan=True)
-        if (depth > 0).mean() < 0.05:
-            continue
 
         def f(lv):
             de = O.upsample(sparse.astype(np.float64), wr.astype(np.float64), wc.astype(np.float64), lam=lv,
@@ -69,6 +67,88 @@ def test_batch_lambda_gradient_vs_fp64(n_streams):
             return float(np.sum(np.where(do == -1.0, 0.0, do) * g[i]))
         fd = (f(lam + 1e-6) - f(lam - 1e-6)) / 2e-6
         assert abs(grad[i] - fd) <= 5e-3 * abs(fd) + 1e-6, f"stream {i}: {grad[i]} vs {fd}"
-        checked += 1
+        checked += 1 if (depth > 0).mean() >= 0.05 else 0
+    # every stream is asserted; most also carry a substantial confident area
     assert checked >= n_streams // 2
+    batch.close()
+
+
+def _small_batch(n=2, lam=None):
+    W, H = 376, 240
+    sets = [keyframe_set(W, H, 60 + k) for k in range(n)]
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    frames = [dict(key=dev(ks["key"]), refs=[dev(r) for r in ks["refs"]], T=ks["T"], cam=cam_tuple(ks["cam"]))
+              for ks in sets]
+    kw = {} if lam is None else {"lambda": lam}
+    p = default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, **kw)
+    return Batch(frames, p, outputs=("depth", "dense", "sparse")), sets, frames
+
+
+def test_batch_backward_refuses_a_changed_batch():
+    from open_quadtree_mapping_amd._lib import QmError
+    batch, sets, frames = _small_batch()
+    g = [torch.ones(240, 376, device="cuda") for _ in range(2)]
+    with pytest.raises(QmError, match="no forward run"):
+        batch.backward(g)
+    batch.run()
+    assert batch.runs() == 1
+    batch.backward(g)  # the last run is intact
+    batch.set_frame(0, frames[0])
+    with pytest.raises(QmError, match="changed after its last run"):
+        batch.backward(g)
+    batch.run(graph=True)
+    batch.backward(g)
+    batch.autotune(reps=1)
+    with pytest.raises(QmError, match="changed after its last run"):
+        batch.backward(g)
+    batch.run()
+    batch.reset_seeds()
+    with pytest.raises(QmError, match="changed after its last run"):
+        batch.backward(g)
+    # the autograd function notices a later run
+    lams = torch.tensor([9.0, 11.0], dtype=torch.float64, device="cuda", requires_grad=True)
+    outs = QA.batch_depth(lams, batch)
+    batch.run()
+    with pytest.raises(RuntimeError, match="ran again"):
+        sum(o.sum() for o in outs).backward()
+    for bad in ([0.0, 1.0], [1.0, -2.0], [float("nan"), 1.0]):
+        with pytest.raises(ValueError, match="> 0"):
+            batch.set_lambdas(bad)
+    batch.close()
+
+
+@pytest.mark.parametrize("lam", [0.0, 1e-3])
+def test_batch_lambda_gradient_at_small_lambda(lam):
+    """params.lambda = 0 or 1e-3. The lambda gradient is formed as
+    sum_e w_e dL/d(lambda w_e), with no division by lambda: at 0 it is finite
+    (the upsample there returns sparse / indicator, NaN off the sparse pixels,
+    and any lambda != 0 makes those finite, so no difference quotient across 0
+    exists to compare with); at 1e-3 it matches the FP64 central difference."""
+    batch, sets, frames = _small_batch(lam=lam)
+    W, H = 376, 240
+    for _ in range(16):
+        batch.run()
+    batch.run()
+    rng = np.random.default_rng(8)
+    g = [rng.standard_normal((H, W)).astype(np.float32) for _ in range(2)]
+    grad = batch.backward([torch.tensor(gi, device="cuda") for gi in g])
+    assert np.all(np.isfinite(grad))
+    if lam == 0.0:
+        batch.close()
+        return
+    for i in range(2):
+        ks = sets[i]
+        sparse = batch.outputs[i]["sparse"].cpu().numpy()
+        table = batch.read_buffer(i, Batch.BUF_TABLE)
+        prior = batch.read_buffer(i, Batch.BUF_PRIOR)
+        wr, wc = O.weights(ks["key"])
+        rs, cs = O.ref_segments(W, H)
+
+        def f(lv):
+            de = O.upsample(sparse.astype(np.float64), wr.astype(np.float64), wc.astype(np.float64), lam=lv,
+                            row_seg=rs, col_seg=cs, fp64=True)
+            do, _ = O.fuse_update(table, prior.astype(np.float64), de, fp64=True)
+            return float(np.sum(np.where(do == -1.0, 0.0, do) * g[i].astype(np.float64)))
+        fd = (f(lam + 1e-6) - f(lam - 1e-6)) / 2e-6
+        assert abs(grad[i] - fd) <= 5e-3 * abs(fd) + 1e-6, f"stream {i}: {grad[i]} vs {fd}"
     batch.close()
