@@ -76,9 +76,13 @@ void qm_default_params(qm_params* p);
 /* Fault injection for tests: refuse device allocations beyond `bytes` (0 = no cap). */
 void qm_set_alloc_cap(size_t bytes);
 /* Device bytes the library holds. This includes the scratch of the stage-level
- * calls below, which is kept for reuse by later calls (at most 8 GiB; released
- * when an allocation is refused). */
+ * calls below, which is kept for reuse by later calls: at most 1 GiB over all
+ * devices, the oldest blocks freed first beyond that, everything freed when a
+ * device allocation of the library is refused or qm_release_scratch is called. */
 size_t qm_device_bytes_in_use(void);
+/* Free the kept stage-level scratch now (for a caller whose own allocator ran
+ * out of memory, or before teardown). Safe to call at any time between calls. */
+int qm_release_scratch(void);
 
 /* ---------------- stage level (device pointers) ---------------------------- */
 

@@ -93,6 +93,7 @@ SIGNATURES = {
     "qm_default_params": (None, [P]),
     "qm_set_alloc_cap": (None, [C.c_size_t]),
     "qm_device_bytes_in_use": (C.c_size_t, []),
+    "qm_release_scratch": (I, []),
     "qm_quadtree": (I, [P, I, I, I, P, P]),
     "qm_cost_volume": (I, [P, P, I, I, I, P, P, P, P, P, P, P]),
     "qm_bp_extract": (I, [P, I, I, P, P, P, P, P, P]),
@@ -168,6 +169,22 @@ def lib():
             raise QmError(f"{LIB_PATH} has ABI version {got}, the binding expects {ABI_VERSION}: rebuild it")
         _lib = L
     return _lib
+
+
+def release_scratch():
+    """Free the library's kept stage-level scratch (qm_release_scratch)."""
+    check(lib().qm_release_scratch())
+
+
+def device_empty(shape, dtype, device):
+    """torch.empty on the device; when torch runs out of memory, the library's
+    kept scratch is released and the allocation tried once more."""
+    import torch
+    try:
+        return torch.empty(shape, dtype=dtype, device=device)
+    except torch.cuda.OutOfMemoryError:
+        release_scratch()
+        return torch.empty(shape, dtype=dtype, device=device)
 
 
 def check(rc: int):

@@ -21,7 +21,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import check, lib, qm_upsample_problem
+from ._lib import check, device_empty, lib, qm_upsample_problem
 from . import stages as S
 
 __all__ = ["UpsampleFunction", "FuseFunction", "BatchDepthFunction", "upsample", "fuse", "batch_depth"]
@@ -77,7 +77,7 @@ class UpsampleFunction(torch.autograd.Function):
         if ctx.needs_input_grad[4] and not all(v > 0.0 for v in lams):
             # dL/dlambda is formed from dL/dw as sum_e w_e dL/dw_e / lambda
             raise ValueError("a differentiable lam must be > 0")
-        out = torch.empty_like(d_)
+        out = device_empty(d_.shape, d_.dtype, d_.device)
         probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg)
         for b in range(B):
             probs[b].dense = out[b].data_ptr()
@@ -97,11 +97,11 @@ class UpsampleFunction(torch.autograd.Function):
         conf = saved[3] if ctx.has_conf else None
         B = d.shape[0]
         g = (g if ctx.batched else g[None]).contiguous()
-        gd = torch.empty_like(d) if ctx.needs_input_grad[0] else None
-        gc = torch.empty_like(d) if (conf is not None and ctx.needs_input_grad[1]) else None
+        gd = device_empty(d.shape, d.dtype, d.device) if ctx.needs_input_grad[0] else None
+        gc = device_empty(d.shape, d.dtype, d.device) if (conf is not None and ctx.needs_input_grad[1]) else None
         need_w = ctx.needs_input_grad[2] or ctx.needs_input_grad[3] or ctx.needs_input_grad[4]
-        gwr = torch.empty_like(w_row) if need_w else None
-        gwc = torch.empty_like(w_col) if need_w else None
+        gwr = device_empty(w_row.shape, w_row.dtype, w_row.device) if need_w else None
+        gwc = device_empty(w_col.shape, w_col.dtype, w_col.device) if need_w else None
         probs = _problems(d, conf, w_row, w_col, ctx.lams, *ctx.segs)
         for b in range(B):
             q = probs[b]
@@ -144,7 +144,7 @@ class FuseFunction(torch.autograd.Function):
     def backward(ctx, g_depth, g_seeds):
         d, warped, table = ctx.saved_tensors
         H, W = d.shape
-        grad = torch.empty_like(d)
+        grad = device_empty(d.shape, d.dtype, d.device)
         gd = None if g_depth is None else g_depth.contiguous()
         gs = None if g_seeds is None else g_seeds.contiguous()
         if gd is None and gs is None:

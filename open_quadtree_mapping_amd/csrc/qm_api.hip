@@ -63,7 +63,19 @@ struct CachedBlock {
 std::mutex g_cache_mu;
 std::vector<CachedBlock> g_cache;
 size_t g_cache_bytes = 0;
-constexpr size_t kCacheMax = size_t(8) << 30;
+constexpr size_t kCacheMax = size_t(1) << 30;  // 1 GiB over all devices
+
+// frees each block with its own device current
+void free_blocks(const std::vector<CachedBlock>& blocks) {
+  if (blocks.empty()) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (const CachedBlock& b : blocks) {
+    if (b.dev != cur) (void)hipSetDevice(b.dev);
+    dev_free(b.p);
+    if (b.dev != cur) (void)hipSetDevice(cur);
+  }
+}
 
 void flush_scratch_cache() {
   std::vector<CachedBlock> blocks;
@@ -72,7 +84,7 @@ void flush_scratch_cache() {
     blocks.swap(g_cache);
     g_cache_bytes = 0;
   }
-  for (const CachedBlock& b : blocks) dev_free(b.p);
+  free_blocks(blocks);
 }
 
 static int dev_alloc_once(void** p, size_t bytes) {
@@ -131,6 +143,8 @@ int scratch_take(void** p, size_t bytes) {
   return dev_alloc(p, bytes);
 }
 
+// keeps a block for reuse; when the cache would exceed its cap the oldest
+// blocks are freed first (a block larger than the cap is freed at once)
 void scratch_return(void* p) {
   size_t bytes = 0;
   {
@@ -138,17 +152,23 @@ void scratch_return(void* p) {
     auto it = g_sizes.find(p);
     if (it != g_sizes.end()) bytes = it->second;
   }
+  if (!bytes || bytes > kCacheMax) {
+    dev_free(p);
+    return;
+  }
   int dev = 0;
   (void)hipGetDevice(&dev);
+  std::vector<CachedBlock> evicted;
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    if (bytes && g_cache_bytes + bytes <= kCacheMax) {
-      g_cache.push_back({dev, bytes, p});
-      g_cache_bytes += bytes;
-      return;
-    }
+    size_t k = 0;
+    while (k < g_cache.size() && g_cache_bytes + bytes > kCacheMax) g_cache_bytes -= g_cache[k++].bytes;
+    evicted.assign(g_cache.begin(), g_cache.begin() + k);
+    g_cache.erase(g_cache.begin(), g_cache.begin() + k);
+    g_cache.push_back({dev, bytes, p});
+    g_cache_bytes += bytes;
   }
-  dev_free(p);
+  free_blocks(evicted);
 }
 
 void init_geometry(FrameDev& F, int W, int H, int pitch) {
@@ -621,6 +641,10 @@ void qm_default_params(qm_params* p) {
 }
 
 void qm_set_alloc_cap(size_t bytes) { g_cap.store(bytes); }
+int qm_release_scratch(void) {
+  flush_scratch_cache();
+  return QM_OK;
+}
 size_t qm_device_bytes_in_use(void) { return g_bytes.load(); }
 
 int qm_quadtree(const uint8_t* img, int W, int H, int pitch, uint8_t* levels, void* stream) {

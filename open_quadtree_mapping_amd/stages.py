@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import check, default_params, lib, qm_params
+from ._lib import check, default_params, device_empty, lib, qm_params
 
 __all__ = ["quadtree", "cost_volume", "bp_extract", "upsample", "upsample_general", "fuse", "default_params"]
 
@@ -47,7 +47,7 @@ def quadtree(img: torch.Tensor) -> torch.Tensor:
     """uint8 HxW keyframe -> uint8 HxW quadtree level in {2, 3, 4}."""
     _need(img, torch.uint8, name="img")
     H, W = img.shape
-    out = torch.empty((H, W), dtype=torch.uint8, device=img.device)
+    out = device_empty((H, W), torch.uint8, img.device)
     check(lib().qm_quadtree(_ptr(img), W, H, W, _ptr(out), _stream()))
     return out
 
@@ -65,8 +65,8 @@ def cost_volume(key, refs, T_ref_key, cam, levels, params: qm_params | None = No
     _need(levels, torch.uint8, (H, W), name="levels")
     T = np.ascontiguousarray(T_ref_key, dtype=np.float32).reshape(48)
     camv = np.ascontiguousarray(cam, dtype=np.float32).reshape(4)
-    cost = torch.empty((H // 4, W // 4, p.n_hyp), dtype=torch.float32, device=key.device)
-    count = torch.empty((H // 4, W // 4), dtype=torch.int32, device=key.device) if want_count else None
+    cost = device_empty((H // 4, W // 4, p.n_hyp), torch.float32, key.device)
+    count = device_empty((H // 4, W // 4), torch.int32, key.device) if want_count else None
     refp = (C.c_void_p * 4)(*[r.data_ptr() for r in refs])
     check(lib().qm_cost_volume(_ptr(key), refp, W, H, W, T.ctypes.data_as(C.c_void_p),
                                camv.ctypes.data_as(C.c_void_p), C.byref(p), _ptr(levels), _ptr(cost),
@@ -80,9 +80,9 @@ def bp_extract(cost, W, H, levels, params: qm_params | None = None, want_message
     _need(cost, torch.float32, (H // 4, W // 4, p.n_hyp), name="cost")
     _need(levels, torch.uint8, (H, W), name="levels")
     dev = cost.device
-    depth = torch.empty((H // 4, W // 4), dtype=torch.float32, device=dev)
-    label = torch.empty((H // 4, W // 4), dtype=torch.int32, device=dev)
-    msg = torch.empty((4, H // 4, W // 4, p.n_hyp), dtype=torch.float32, device=dev) if want_messages else None
+    depth = device_empty((H // 4, W // 4), torch.float32, dev)
+    label = device_empty((H // 4, W // 4), torch.int32, dev)
+    msg = device_empty((4, H // 4, W // 4, p.n_hyp), torch.float32, dev) if want_messages else None
     check(lib().qm_bp_extract(_ptr(cost), W, H, _ptr(levels), C.byref(p), _ptr(depth), _ptr(label), _ptr(msg),
                               _stream()))
     return (depth, label, msg) if want_messages else (depth, label)
@@ -94,7 +94,7 @@ def upsample(sparse, img, params: qm_params | None = None):
     _need(img, torch.uint8, name="img")
     H, W = img.shape
     _need(sparse, torch.float32, (H, W), name="sparse")
-    out = torch.empty((H, W), dtype=torch.float32, device=img.device)
+    out = device_empty((H, W), torch.float32, img.device)
     check(lib().qm_upsample(_ptr(sparse), _ptr(img), W, H, W, C.byref(p), _ptr(out), _stream()))
     return out
 
@@ -106,9 +106,9 @@ def upsample_general(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg
     _need(w_row, torch.float32, (H, W + 1), name="w_row")
     _need(w_col, torch.float32, (H + 1, W), name="w_col")
     _need(conf, torch.float32, (H, W), name="conf")
-    out = torch.empty((H, W), dtype=torch.float32, device=d.device)
-    rx = torch.empty_like(out) if want_rows else None
-    rn = torch.empty_like(out) if want_rows else None
+    out = device_empty((H, W), torch.float32, d.device)
+    rx = device_empty(out.shape, out.dtype, out.device) if want_rows else None
+    rn = device_empty(out.shape, out.dtype, out.device) if want_rows else None
     check(lib().qm_upsample_general(_ptr(d), _ptr(conf), _ptr(w_row), _ptr(w_col), W, H, float(lam),
                                     W if row_seg is None else int(row_seg), H if col_seg is None else int(col_seg),
                                     _ptr(out), _ptr(rx), _ptr(rn), _stream()))
@@ -122,9 +122,9 @@ def fuse(seeds, depth_in, cam, T_last_to_cur, want_table=False):
     _need(seeds, torch.float32, (H, W, 4), name="seeds")
     camv = np.ascontiguousarray(cam, dtype=np.float32).reshape(4)
     T = np.ascontiguousarray(T_last_to_cur, dtype=np.float32).reshape(12)
-    dout = torch.empty((H, W), dtype=torch.float32, device=seeds.device)
-    sout = torch.empty((H, W, 4), dtype=torch.float32, device=seeds.device)
-    table = torch.empty((H, W), dtype=torch.int32, device=seeds.device) if want_table else None
+    dout = device_empty((H, W), torch.float32, seeds.device)
+    sout = device_empty((H, W, 4), torch.float32, seeds.device)
+    table = device_empty((H, W), torch.int32, seeds.device) if want_table else None
     check(lib().qm_fuse(_ptr(seeds), _ptr(depth_in), W, H, camv.ctypes.data_as(C.c_void_p),
                         T.ctypes.data_as(C.c_void_p), _ptr(dout), _ptr(sout), _ptr(table), _stream()))
     return (dout, sout, table) if want_table else (dout, sout)
@@ -138,7 +138,7 @@ def point_cloud(depth, ref_img, cam, T_world_ref):
     _need(ref_img, torch.uint8, (H, W), name="ref_img")
     camv = np.ascontiguousarray(cam, dtype=np.float32).reshape(4)
     T = np.ascontiguousarray(T_world_ref, dtype=np.float32).reshape(12)
-    out = torch.empty((H * W, 4), dtype=torch.float32, device=depth.device)
+    out = device_empty((H * W, 4), torch.float32, depth.device)
     n = C.c_int()
     check(lib().qm_point_cloud(_ptr(depth), _ptr(ref_img), W, H, camv.ctypes.data_as(C.c_void_p),
                                T.ctypes.data_as(C.c_void_p), _ptr(out), C.byref(n), _stream()))
@@ -150,7 +150,7 @@ def upsample_local(sparse, img):
     _need(img, torch.uint8, name="img")
     H, W = img.shape
     _need(sparse, torch.float32, (H, W), name="sparse")
-    out = torch.empty((H, W), dtype=torch.float32, device=img.device)
+    out = device_empty((H, W), torch.float32, img.device)
     check(lib().qm_upsample_local(_ptr(sparse), _ptr(img), W, H, W, _ptr(out), _stream()))
     return out
 

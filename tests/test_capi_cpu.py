@@ -95,3 +95,7 @@ def test_fuse_noalloc_rejects_sizes_beyond_its_static_scratch():
     assert rc == -1 and b"2097152" in L.qm_last_error()
     rc = L.qm_fuse_noalloc(None, one, 752, 480, one, one, one, one, None, None)
     assert rc == -1 and b"NULL" in L.qm_last_error()
+
+
+def test_release_scratch_without_work_is_a_no_op():
+    assert _lib.lib().qm_release_scratch() == 0

@@ -224,6 +224,11 @@ def test_stage_scratch_is_reused():
         assert np.array_equal(host(got), ref, equal_nan=True)
         held.append(lib().qm_device_bytes_in_use())
     assert held[1] == held[2], held
+    # qm_release_scratch frees what the calls kept; the next call allocates afresh
+    assert lib().qm_release_scratch() == 0
+    assert lib().qm_device_bytes_in_use() < held[2]
+    got = S.upsample_general(dev(d), dev(wr), dev(wc), conf=dev(conf))
+    assert np.array_equal(host(got), ref, equal_nan=True)
 
 
 def fusion_case(W, H, seed):
