@@ -1,5 +1,16 @@
 // quadmap/se3.h -- SE3<float> for callers of quadmap::Depthmap, plain C++.
 //
+// The interface of this file (the constructors, data(r, c), inv(), the products
+// and their operation order) follows include/quadmap/se3.cuh of the reference,
+// which is part of REMODE - REgularized MOnocular Depth Estimation:
+//   Copyright (C) 2014 Matia Pizzoli <matia dot pizzoli at gmail dot com>
+//   Robotics and Perception Group, University of Zurich, Switzerland
+//   http://rpg.ifi.uzh.ch
+// REMODE is free software under the GNU General Public License, version 3 or
+// any later version; this file is distributed under the same license, WITHOUT
+// ANY WARRANTY, without even the implied warranty of MERCHANTABILITY or FITNESS
+// FOR A PARTICULAR PURPOSE (see <http://www.gnu.org/licenses/>).
+//
 // Same surface as the reference's include/quadmap/se3.cuh:26-201 (identity and
 // quaternion constructors, data(r, c), inv(), products with SE3 and float3,
 // getTranslation) so caller code compiles unchanged; the operation order of

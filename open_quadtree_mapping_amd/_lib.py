@@ -6,7 +6,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqm.so")
+# QM_LIB names another build of the same library (the sanitizer build of the
+# host-side checks); by default the in-tree libqm.so
+LIB_PATH = os.environ.get("QM_LIB") or os.path.join(_HERE, "libqm.so")
 _lib = None
 
 P = C.c_void_p
