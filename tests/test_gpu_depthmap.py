@@ -132,34 +132,3 @@ def test_point_cloud_matches_oracle():
     want = O.point_cloud(dm.getDepthmap(), dm.getReferenceImage(), cam, dm.getT_world_ref())
     assert len(pc) > 1000 and np.array_equal(pc, want, equal_nan=True)
     dm.close()
-
-
-def test_full_resolution_ratio3_sequence_matches_oracle():
-    """Full-resolution 752x480 (fx = fy = 365), forward-plus-lateral motion of
-    5 cm per frame, semi2dense_ratio = 3, 64 hypotheses over 0.5-50 m: every
-    third frame is a keyframe candidate, and depth maps appear once more than
-    ten references are held (frames 35, 38, 41, 44 of 45)."""
-    from open_quadtree_mapping_amd.scene import forward_lateral_trajectory
-    W, H = 752, 480
-    cam = camera_for(W, H)
-    n = 45
-    traj = forward_lateral_trajectory(n, seed=41)
-    sc = make_scene(41)
-    imgs = [render(sc, cam, T)[0] for T in traj]
-    p = default_params(idepth_min=0.02, idepth_max=2.0)
-    dm = Depthmap(W, H, cam.fx, cam.cx, cam.fy, cam.cy, semi2dense_ratio=3, params=p)
-    ref = O.DepthmapSequence(cam, 64, 0.02, 2.0, ratio=3)
-    outputs = []
-    for k in range(n):
-        got = dm.add_frames(imgs[k], traj[k])
-        want = ref.add_frames(imgs[k], traj[k])
-        assert got == want, f"frame {k}: has_result {got} vs {want}"
-        assert dm.num_references() == len(ref.refs)
-        if got and k > 0:
-            print(f"full-resolution ratio-3 keyframe at frame {k}", flush=True)
-            assert np.array_equal(dm.getDebugmap(), ref.debug)
-            assert np.array_equal(dm.getDepthmap(), ref.depth, equal_nan=True)
-            assert np.array_equal(dm.getT_world_ref(), ref.T_world_ref)
-            outputs.append(k)
-    assert outputs == [35, 38, 41, 44]
-    dm.close()
