@@ -189,6 +189,22 @@ def measured_traffic(stage, W, H, N, B):
     return int(rec["hbm_bytes_per_launch"]), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
 
 
+def measured_cost_valu(W, H, N, arith):
+    """VALU instructions per (cell, sample) of the cost kernel in this
+    arithmetic, from the committed rocprofv3 issue-counter summary; (None, None)
+    when no summary covers it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"pipeline_{W}x{H}x{N}_b8_issue_counters.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
+    if not m:
+        return None, None
+    rec = json.loads(m.group(1))
+    v = rec.get("valu_per_cell_sample", {}).get(arith)
+    return v, os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
+
+
 def run_workload(args):
     """Supplementary workloads (--workload): one JSON line with the rate of one
     step, timed like the headline workload (warm-up steps, then args.steps
@@ -393,6 +409,9 @@ def main():
         tflops = flops / (c["ms_per_run"] * 1e-3) / 1e12
         c.update({"alg_gflop_per_run": round(flops / 1e9, 3), "achieved_TFLOPs": round(tflops, 2),
                   "flop_frac": round(tflops / FP32_PEAK_TFLOPS, 4), "arith": args.cost_arith})
+        valu, vsrc = measured_cost_valu(W, H, N, args.cost_arith)
+        if valu is not None:
+            c.update({"valu_per_cell_sample": valu, "valu_source": vsrc})
         other = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=1 - arith)
         ob = Batch(frames, other, outputs=("depth",))
         ob.set_tuning(**batch.tuning())
