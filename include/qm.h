@@ -69,7 +69,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 8
+#define QM_ABI_VERSION 9
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -129,9 +129,10 @@ int qm_upsample_general(const float* d, const float* conf, const float* w_row, c
  * dL/dconf (zeros when conf is NULL), dL/dw_row (H x (W+1)) and dL/dw_col
  * ((H+1) x W); any gradient output may be NULL. The forward passes are
  * recomputed, not stored; A = I + lambda L(w) is symmetric, so each adjoint is a
- * tridiagonal solve with the forward matrices. Pixels where the forward
- * denominator n is 0 (output NaN) contribute no gradient. No counterpart in the
- * reference (its upsample is forward-only). */
+ * tridiagonal solve with the forward matrices (run in the forward passes' wave
+ * pipeline). Pixels where the forward denominator n is 0 (output NaN) contribute
+ * no gradient. The batched form can also return dL/dlambda (grad_lambda of a
+ * problem). No counterpart in the reference (its upsample is forward-only). */
 int qm_upsample_backward(const float* d, const float* conf, const float* w_row, const float* w_col, int W, int H,
                          double lambda, int row_seg, int col_seg, const float* g, float* grad_d, float* grad_conf,
                          float* grad_w_row, float* grad_w_col, void* stream);
@@ -158,6 +159,7 @@ typedef struct qm_upsample_problem {
   float* grad_conf;
   float* grad_w_row;
   float* grad_w_col;
+  double* grad_lambda;         /* backward output dL/dlambda: one double in device memory, or NULL */
 } qm_upsample_problem;
 int qm_upsample_general_batch(const qm_upsample_problem* problems, int n, void* stream);
 int qm_upsample_backward_batch(const qm_upsample_problem* problems, int n, void* stream);

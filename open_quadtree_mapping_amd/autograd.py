@@ -5,8 +5,8 @@ UpsampleFunction: out = x_col / n_col of the generalised upsample
   forward sweep and runs the transposed (= same, A is symmetric) tridiagonal
   solves. Gradients w.r.t. sparse depths, confidences, per-pixel smoothness
   weights, and the smoothness scale lambda when it is given as a tensor
-  (dL/dlambda = sum_e w_e dL/dw_e / lambda, since A = I + lambda L(w); such a
-  lambda must be > 0).
+  (dL/dlambda = sum_e w_e dL/d(lambda w_e), since A = I + lambda L(w), summed on
+  the device in a fixed order; such a lambda must be > 0).
 FuseFunction: the Gaussian x Beta update of one keyframe (qm_fuse) with
   backward qm_fuse_backward w.r.t. the dense depth input (prior seeds fixed).
 BatchDepthFunction: one batched pipeline run (every stream through every
@@ -75,7 +75,6 @@ class UpsampleFunction(torch.autograd.Function):
         if len(lams) != B:
             raise ValueError("lam must be a scalar or hold one value per problem")
         if ctx.needs_input_grad[4] and not all(v > 0.0 for v in lams):
-            # dL/dlambda is formed from dL/dw as sum_e w_e dL/dw_e / lambda
             raise ValueError("a differentiable lam must be > 0")
         out = device_empty(d_.shape, d_.dtype, d_.device)
         probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg)
@@ -99,9 +98,10 @@ class UpsampleFunction(torch.autograd.Function):
         g = (g if ctx.batched else g[None]).contiguous()
         gd = device_empty(d.shape, d.dtype, d.device) if ctx.needs_input_grad[0] else None
         gc = device_empty(d.shape, d.dtype, d.device) if (conf is not None and ctx.needs_input_grad[1]) else None
-        need_w = ctx.needs_input_grad[2] or ctx.needs_input_grad[3] or ctx.needs_input_grad[4]
-        gwr = device_empty(w_row.shape, w_row.dtype, w_row.device) if need_w else None
-        gwc = device_empty(w_col.shape, w_col.dtype, w_col.device) if need_w else None
+        gwr = device_empty(w_row.shape, w_row.dtype, w_row.device) if ctx.needs_input_grad[2] else None
+        gwc = device_empty(w_col.shape, w_col.dtype, w_col.device) if ctx.needs_input_grad[3] else None
+        want_lam = ctx.lam_shape is not None and ctx.needs_input_grad[4]
+        per = device_empty((B,), torch.float64, d.device) if want_lam else None
         probs = _problems(d, conf, w_row, w_col, ctx.lams, *ctx.segs)
         for b in range(B):
             q = probs[b]
@@ -110,17 +110,15 @@ class UpsampleFunction(torch.autograd.Function):
             q.grad_conf = None if gc is None else gc[b].data_ptr()
             q.grad_w_row = None if gwr is None else gwr[b].data_ptr()
             q.grad_w_col = None if gwc is None else gwc[b].data_ptr()
+            # dL/dlambda = sum_e w_e dL/d(lambda w_e), summed on the device
+            q.grad_lambda = None if per is None else per[b:].data_ptr()
         check(lib().qm_upsample_backward_batch(probs, B, _stream()))
         glam = None
-        if ctx.lam_shape is not None and ctx.needs_input_grad[4]:
-            lam_t = torch.tensor(ctx.lams, dtype=torch.float64, device=d.device)
-            per = ((w_row.double() * gwr.double()).sum(dim=(1, 2)) +
-                   (w_col.double() * gwc.double()).sum(dim=(1, 2))) / lam_t
+        if want_lam:
             one_per_problem = len(ctx.lam_shape) == 1 and ctx.lam_shape[0] == B
             glam = (per if one_per_problem else per.sum()).to(torch.float32).reshape(ctx.lam_shape)
         unb = (lambda t: t) if ctx.batched else (lambda t: None if t is None else t[0])
-        return (unb(gd), unb(gc), unb(gwr) if ctx.needs_input_grad[2] else None,
-                unb(gwc) if ctx.needs_input_grad[3] else None, glam, None, None)
+        return unb(gd), unb(gc), unb(gwr), unb(gwc), glam, None, None
 
 
 def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None):

@@ -8,295 +8,13 @@
 // gradient: rbar = A^{-1} zbar, and dL/dw_e = -lambda * rbar^T M_e z summed over
 // the two right-hand sides (x and n). With out = x_col / n_col the incoming
 // gradients are g / n_col and -g * out / n_col. The forward passes are
-// recomputed first (the row pass into the rowx planes, the column pass into colxn); the
-// adjoint runs the column solve, then the row solve.
+// recomputed first (the row pass into the rowx planes, the column pass into
+// colxn); the adjoint solves then run in the forward passes' wave pipeline
+// (k_up_col / k_up_row mode 3, k_back.hip): columns, then rows.
 #include "qm_device.h"
 #include "qm_launch.h"
 
 namespace qm {
-
-// The adjoint kernels walk each line in chunks of kAK steps. A chunk's inputs
-// are loaded together (indices clamped to the segment, so every load is in
-// bounds; the steps past the segment's end are skipped), one chunk ahead of the
-// steps that use them: two buffers alternate, so the loads of chunk k + 1 are
-// in flight while chunk k runs.
-constexpr int kAK = 16;
-
-__device__ __forceinline__ void adj_coeffs(float wl, float wr, double lam, float& a, float& b, float& c) {
-  a = (float)(-lam * (double)wl);
-  b = (float)(1.0 + lam * (double)(wl + wr));
-  c = (float)(-lam * (double)wr);
-}
-
-// Column adjoint: lane per column. rhs from g and the recomputed column pass,
-// result (xbar, nbar) into adjxn, dL/dw_col into grad_wcol.
-__global__ __launch_bounds__(64) void k_adj_col(const FrameDev* __restrict__ frames, BlockMap bm) {
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int x = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
-  const int W = F.W, H = F.H;
-  if (x >= W) return;
-  const double lam = F.lam;
-  const float flam = (float)lam;
-  // dL/dw_e = lambda dL/d(lambda w_e); the pipeline backward takes the
-  // unscaled term so that dL/dlambda = sum_e w_e dL/d(lambda w_e) needs no division
-  const float gscale = F.grad_w_unscaled ? 1.0f : flam;
-  auto weight = [&](int y) -> float { return gp(F.g_wcol)[(size_t)y * W + x]; };
-  const int seg = F.col_seg < H ? F.col_seg : H;
-  const int bounds[3] = {0, seg, H};
-  float carry = 0.0f;
-  for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = bounds[sg], s1 = bounds[sg + 1];
-    if (s1 <= s0) continue;
-    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    float wl = weight(s0);
-    struct FwdIn {
-      float w[kAK], g[kAK];
-      float2 z[kAK];
-    };
-    auto load_f = [&](FwdIn& c, int i0) {
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        const int i = min(i0 + j, s1 - 1);
-        c.w[j] = weight(i + 1);
-        c.z[j] = gp(F.colxn)[(size_t)i * W + x];
-        c.g[j] = gp(F.g_out)[(size_t)i * W + x];
-      }
-    };
-    auto run_f = [&](const FwdIn& cin, int i0) {
-      const int n = min(kAK, s1 - i0);
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        if (j < n) {
-          const int i = i0 + j;
-          const float wr = cin.w[j];
-          float a, b, c;
-          adj_coeffs(wl, wr, lam, a, b, c);
-          const float2 z = cin.z[j];
-          const float g = cin.g[j];
-          float gx = 0.0f, gn = 0.0f;
-          if (z.y != 0.0f) {
-            gx = g / z.y;
-            gn = -gx * (z.x / z.y);
-          }
-          const float den = b - fc * a;
-          const float ch = c / den;
-          const float dh = (gx - fd * a) / den;
-          const float nh = (gn - fn * a) / den;
-          gp(F.fwd)[(size_t)i * W + x] = make_float4(ch, dh, nh, 0.0f);
-          fc = ch;
-          fd = dh;
-          fn = nh;
-          wl = wr;
-        }
-      }
-    };
-    {
-      FwdIn A, B;
-      load_f(A, s0);
-      for (int i0 = s0; i0 < s1; i0 += 2 * kAK) {
-        load_f(B, i0 + kAK);
-        run_f(A, i0);
-        load_f(A, i0 + 2 * kAK);
-        run_f(B, i0 + kAK);
-      }
-    }
-    float vx_next = 0.0f, vn_next = 0.0f;
-    float2 z_next = make_float2(0.0f, 0.0f);
-    struct BwdIn {
-      float4 t[kAK];
-      float2 z[kAK];
-    };
-    auto load_b = [&](BwdIn& c, int i1) {
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        const int i = max(i1 - 1 - j, s0);
-        c.t[j] = gp(F.fwd)[(size_t)i * W + x];
-        c.z[j] = gp(F.colxn)[(size_t)i * W + x];
-      }
-    };
-    auto run_b = [&](const BwdIn& cin, int i1) {
-      const int n = min(kAK, i1 - s0);
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        if (j < n) {
-          const int i = i1 - 1 - j;
-          const float4 t = cin.t[j];
-          const float vx = t.y - t.x * vx_next;
-          const float vn = t.z - t.x * vn_next;
-          gp(F.adjxn)[(size_t)x * H + i] = make_float2(vx, vn);
-          const float2 z = cin.z[j];
-          if (i + 1 < s1) {
-            gp(F.grad_wcol)[(size_t)(i + 1) * W + x] = -gscale * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
-          } else {
-            const float e = -gscale * (vx * z.x + vn * z.y);
-            if (s1 == H) gp(F.grad_wcol)[(size_t)H * W + x] = e;
-            else carry = e;
-          }
-          if (i == s0) {
-            const float e = -gscale * (vx * z.x + vn * z.y);
-            gp(F.grad_wcol)[(size_t)s0 * W + x] = (s0 == 0) ? e : carry + e;
-          }
-          vx_next = vx;
-          vn_next = vn;
-          z_next = z;
-        }
-      }
-    };
-    {
-      BwdIn A, B;
-      load_b(A, s1);
-      for (int i1 = s1; i1 > s0; i1 -= 2 * kAK) {
-        load_b(B, i1 - kAK);
-        run_b(A, i1);
-        load_b(A, i1 - 2 * kAK);
-        run_b(B, i1 - kAK);
-      }
-    }
-  }
-}
-
-// Row adjoint: lane per row. rhs = column-adjoint solution; z = recomputed row
-// pass. Gives dL/dd, dL/dconf and dL/dw_row.
-__global__ __launch_bounds__(64) void k_adj_row(const FrameDev* __restrict__ frames, BlockMap bm) {
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int y = (blockIdx.x - bm.start[f]) * 64 + threadIdx.x;
-  const int W = F.W, H = F.H;
-  if (y >= H) return;
-  const double lam = F.lam;
-  const float flam = (float)lam;
-  // dL/dw_e = lambda dL/d(lambda w_e); the pipeline backward takes the
-  // unscaled term so that dL/dlambda = sum_e w_e dL/d(lambda w_e) needs no division
-  const float gscale = F.grad_w_unscaled ? 1.0f : flam;
-  const size_t row = (size_t)y * W;
-  const size_t wrow = (size_t)y * (W + 1);
-  auto weight = [&](int x) -> float { return gp(F.g_wrow)[wrow + x]; };
-  const int seg = F.row_seg < W ? F.row_seg : W;
-  const int bounds[3] = {0, seg, W};
-  float carry = 0.0f;
-  for (int sg = 0; sg < 2; ++sg) {
-    const int s0 = bounds[sg], s1 = bounds[sg + 1];
-    if (s1 <= s0) continue;
-    float fc = 0.0f, fd = 0.0f, fn = 0.0f;
-    float wl = weight(s0);
-    struct FwdIn {
-      float w[kAK];
-      float2 r[kAK];
-    };
-    auto load_f = [&](FwdIn& c, int i0) {
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        const int i = min(i0 + j, s1 - 1);
-        c.w[j] = weight(i + 1);
-        c.r[j] = gp(F.adjxn)[(size_t)i * H + y];
-      }
-    };
-    auto run_f = [&](const FwdIn& cin, int i0) {
-      const int n = min(kAK, s1 - i0);
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        if (j < n) {
-          const int i = i0 + j;
-          const float wr = cin.w[j];
-          float a, b, c;
-          adj_coeffs(wl, wr, lam, a, b, c);
-          const float2 r = cin.r[j];
-          const float den = b - fc * a;
-          const float ch = c / den;
-          const float dh = (r.x - fd * a) / den;
-          const float nh = (r.y - fn * a) / den;
-          gp(F.fwd)[(size_t)i * H + y] = make_float4(ch, dh, nh, 0.0f);
-          fc = ch;
-          fd = dh;
-          fn = nh;
-          wl = wr;
-        }
-      }
-    };
-    {
-      FwdIn A, B;
-      load_f(A, s0);
-      for (int i0 = s0; i0 < s1; i0 += 2 * kAK) {
-        load_f(B, i0 + kAK);
-        run_f(A, i0);
-        load_f(A, i0 + 2 * kAK);
-        run_f(B, i0 + kAK);
-      }
-    }
-    float vx_next = 0.0f, vn_next = 0.0f;
-    float2 z_next = make_float2(0.0f, 0.0f);
-    struct BwdIn {
-      float4 t[kAK];
-      float2 z[kAK];
-      float d[kAK], c[kAK];
-    };
-    // the confidences' row, or (unused) the depths' when there is none: every
-    // load is made, so the prefetch's waits count a fixed number of loads
-    const float* confp = F.g_conf ? F.g_conf : F.g_d;
-    auto load_b = [&](BwdIn& c, int i1) {
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        const int i = max(i1 - 1 - j, s0);
-        c.t[j] = gp(F.fwd)[(size_t)i * H + y];
-        c.z[j] = make_float2(gp(F.rowx)[(size_t)i * H + y], gp(F.rowx)[(size_t)W * H + (size_t)i * H + y]);
-        c.d[j] = gp(F.g_d)[row + i];
-        c.c[j] = gp(confp)[row + i];
-      }
-    };
-    auto run_b = [&](const BwdIn& cin, int i1) {
-      const int n = min(kAK, i1 - s0);
-#pragma unroll
-      for (int j = 0; j < kAK; ++j) {
-        if (j < n) {
-          const int i = i1 - 1 - j;
-          const float4 t = cin.t[j];
-          const float vx = t.y - t.x * vx_next;
-          const float vn = t.z - t.x * vn_next;
-          const float2 z = cin.z[j];
-          if (i + 1 < s1) {
-            if (F.grad_wrow)
-              gp(F.grad_wrow)[wrow + i + 1] = -gscale * ((vx - vx_next) * (z.x - z_next.x) + (vn - vn_next) * (z.y - z_next.y));
-          } else {
-            const float e = -gscale * (vx * z.x + vn * z.y);
-            if (s1 == W) {
-              if (F.grad_wrow) gp(F.grad_wrow)[wrow + W] = e;
-            } else {
-              carry = e;
-            }
-          }
-          if (i == s0 && F.grad_wrow) {
-            const float e = -gscale * (vx * z.x + vn * z.y);
-            gp(F.grad_wrow)[wrow + s0] = (s0 == 0) ? e : carry + e;
-          }
-          // rhs_x = conf * d, rhs_n = conf (reference mode: rhs_x = d, rhs_n = 1{d > 0})
-          const float d = cin.d[j];
-          if (F.g_conf) {
-            const float c = cin.c[j];
-            if (F.grad_d) gp(F.grad_d)[row + i] = c * vx;
-            if (F.grad_conf) gp(F.grad_conf)[row + i] = d * vx + vn;
-          } else {
-            if (F.grad_d) gp(F.grad_d)[row + i] = vx;
-            if (F.grad_conf) gp(F.grad_conf)[row + i] = 0.0f;
-          }
-          vx_next = vx;
-          vn_next = vn;
-          z_next = z;
-        }
-      }
-    };
-    {
-      BwdIn A, B;
-      load_b(A, s1);
-      for (int i1 = s1; i1 > s0; i1 -= 2 * kAK) {
-        load_b(B, i1 - kAK);
-        run_b(A, i1);
-        load_b(A, i1 - 2 * kAK);
-        run_b(B, i1 - kAK);
-      }
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Fusion update adjoint w.r.t. the dense depth input, by forward-mode dual
@@ -385,18 +103,39 @@ __global__ __launch_bounds__(256) void k_fuse_backward(const FrameDev* __restric
   F.grad_depth_in[idx] = gsum;
 }
 
+// dL/dlambda of each frame with a grad_lambda output: the sum of its H + W
+// per-line partials (rows, then columns), in one fixed order: thread t adds
+// lines t, t + 256, ... in turn, then a fixed pairwise tree over the threads.
+__global__ __launch_bounds__(256) void k_lambda_reduce(const FrameDev* __restrict__ frames, BlockMap bm) {
+  __shared__ double red[256];
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int n = F.H + F.W;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += F.lam_part[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int m = 128; m >= 1; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *F.grad_lambda = red[0];
+}
+
 void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
-  // recompute the forward passes (row pass -> rowx planes, column pass -> colxn)
+  // recompute the forward passes (row pass -> rowx planes, column pass ->
+  // colxn), then the adjoint solves (the same wave pipeline, mode 3): columns
+  // (-> adjxn, dL/dw_col), then rows (-> dL/dd, dL/dconf, dL/dw_row), each
+  // with its per-line lambda partials
   launch_upsample_row(dframes, hframes, n, 2, nullptr, s);
   launch_upsample_col(dframes, hframes, n, 2, nullptr, false, s);
+  launch_upsample_col(dframes, hframes, n, 3, nullptr, false, s);
+  launch_upsample_row(dframes, hframes, n, 3, nullptr, s);
   BlockMap bm;
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;
-  int nb = make_block_map(bm, counts, n);
-  if (nb > 0) hipLaunchKernelGGL(k_adj_col, dim3(nb), dim3(64), 0, s, dframes, bm);
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 63) / 64;
-  nb = make_block_map(bm, counts, n);
-  if (nb > 0) hipLaunchKernelGGL(k_adj_row, dim3(nb), dim3(64), 0, s, dframes, bm);
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].grad_lambda && hframes[i].lam_part) ? 1 : 0;
+  const int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_lambda_reduce, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,35 +182,6 @@ __global__ __launch_bounds__(256) void k_compose_inputs(const FrameDev* __restri
   }
 }
 
-// dL/dlambda = sum over edges of w_e * dL/d(lambda w_e) (A depends on lambda
-// and w only through lambda * w; the adjoint kernels wrote the unscaled
-// dL/d(lambda w_e) for this, so lambda = 0 needs no division). Fixed-order
-// block sums in double, one partial per block; the host adds a frame's
-// partials in block order.
-__global__ __launch_bounds__(256) void k_lambda_partials(const FrameDev* __restrict__ frames, BlockMap bm,
-                                                         double* __restrict__ partial) {
-  __shared__ double red[256];
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int W = F.W, H = F.H;
-  const size_t nrow = (size_t)H * (W + 1), ncol = (size_t)(H + 1) * W;
-  const size_t i = (size_t)(blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
-  double v = 0.0;
-  if (i < nrow) v = (double)F.g_wrow[i] * (double)F.grad_wrow[i];
-  else if (i < nrow + ncol) v = (double)F.g_wcol[i - nrow] * (double)F.grad_wcol[i - nrow];
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int m = 128; m >= 1; m >>= 1) {
-    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-int lambda_blocks(const FrameDev& F) {
-  return (int)((((size_t)F.H * (F.W + 1) + (size_t)(F.H + 1) * F.W) + 255) / 256);
-}
-
 void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
                            hipStream_t s) {
   BlockMap bm;
@@ -479,17 +189,6 @@ void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int
   for (int i = 0; i < n; ++i) counts[i] = ((hframes[i].W + 1) * (hframes[i].H + 1) + 255) / 256;
   const int nb = make_block_map(bm, counts, n);
   if (nb > 0) hipLaunchKernelGGL(k_compose_inputs, dim3(nb), dim3(256), 0, s, dframes, bm, wlut);
-}
-
-int launch_lambda_partials(const FrameDev* dframes, const FrameDev* hframes, int n, double* partial,
-                           int* first_block, hipStream_t s) {
-  BlockMap bm;
-  int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = lambda_blocks(hframes[i]);
-  const int nb = make_block_map(bm, counts, n);
-  for (int i = 0; i <= n; ++i) first_block[i] = bm.start[i];
-  if (nb > 0) hipLaunchKernelGGL(k_lambda_partials, dim3(nb), dim3(256), 0, s, dframes, bm, partial);
-  return nb;
 }
 
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {

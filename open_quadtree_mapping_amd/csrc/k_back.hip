@@ -256,6 +256,22 @@ struct UpRowLds {
   float M[kSlots][64 * kUS], B[kSlots][64 * kUS], R[kSlots][64 * kUS], N[kSlots][64 * kUS];
   float C[2][64 * kUS];
 };
+// the adjoint row pass (mode 3) also hands the x / n chains' solutions to an
+// out wave
+struct UpRowLdsAdj : UpRowLds {
+  float XO[2][64 * kUS], NO[2][64 * kUS];
+};
+__device__ __forceinline__ float* adj_xo(UpRowLdsAdj& L) { return &L.XO[0][0]; }
+__device__ __forceinline__ float* adj_no(UpRowLdsAdj& L) { return &L.NO[0][0]; }
+__device__ __forceinline__ float* adj_xo(UpRowLds&) { return nullptr; }  // modes 0-2: no out wave
+__device__ __forceinline__ float* adj_no(UpRowLds&) { return nullptr; }
+
+// The adjoint out wave's inputs for one chunk (loaded a round ahead): the
+// forward result z at the chunk's steps and the coupling w(i + 1) of each
+// step's upper edge; the row adjoint also reads the depths and confidences.
+struct AdjOutIn {
+  float zx[kUK], zy[kUK], w[kUK], d[kUK], c[kUK];
+};
 
 template <int MODE>
 struct RowBuf {                // one producer wave's half of a chunk
@@ -265,20 +281,24 @@ struct RowBuf {                // one producer wave's half of a chunk
   float wv[kHalf + 1];         // row couplings (mode 2)
   int lb[kUK / 8];             // BP labels of the half's node cells (mode 0)
   float dv[kUK / 8];           // BP depths of the half's node cells (mode 0)
+  float ax[kHalf], an[kHalf];  // the column adjoint's solution (X, N) at c0 + 8 HALF + j (mode 3)
 };
 
 template <int MODE>
-__global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
-                                                const float* __restrict__ wlut) {
-  __shared__ UpRowLds L;
+__global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                                  const float* __restrict__ wlut) {
+  __shared__ typename std::conditional<MODE == 3, UpRowLdsAdj, UpRowLds>::type L;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
   const double lam = F.lam;
   // waves: 0 ch, 1 and 2 producers (half a chunk each), 3 x, 4 n; waves 0 and 4
-  // (the two chains with the most idle issue slots) share a SIMD
+  // (the two chains with the most idle issue slots) share a SIMD; mode 3
+  // (the adjoint) adds wave 5, the out wave
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave: uniform
-  if (MODE != 2 && threadIdx.x < 256) {
+  // the adjoint's out wave trails the x / n chains by one more round
+  constexpr int kTail = MODE == 3 ? 1 : 0;
+  if (MODE < 2 && threadIdx.x < 256) {
     const float w = wlut[threadIdx.x];
     L.lw[threadIdx.x] = w;
     L.lm[threadIdx.x] = (float)(-lam * (double)w);
@@ -307,11 +327,24 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
     const size_t cb = (size_t)min(y >> 2, Hc - 1) * Wc;
     const gptr<const int32_t> label = gp((const int32_t*)F.label) + cb;
     const gptr<const float> dcoarse = gp((const float*)F.dcoarse) + cb;
+    const gptr<const float2> adjr = gp((const float2*)F.adjxn) + y;
     auto load = [&](RowBuf<MODE>& b, int j) __attribute__((always_inline)) {
       int c0, cnt;
       bool first, last;
       ch.get(j, c0, cnt, first, last);
       const int base = min(c0 + h0, W - kHalf);
+      if (MODE == 3) {
+        // every load unconditional, at columns clamped to the line
+#pragma unroll
+        for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wrow[(size_t)y * (W + 1) + min(c0 + h0 + k, W)];
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+          const float2 r = adjr[(size_t)min(c0 + h0 + k, W - 1) * H];
+          b.ax[k] = r.x;
+          b.an[k] = r.y;
+        }
+        return;
+      }
       if (MODE == 2) {
 #pragma unroll
         for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wrow[(size_t)y * (W + 1) + min(c0 + h0 + k, W)];
@@ -348,6 +381,15 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
       float* Bs = L.B[slot] + lo;
       float* Rs = L.R[slot] + lo;
       float* Ns = L.N[slot] + lo;
+      if (MODE == 3) {
+        half_couplings<HALF>(b.wv, lam, Ms, Bs);
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+          Rs[h0 + k] = b.ax[k];
+          Ns[h0 + k] = b.an[k];
+        }
+        return;
+      }
       if (MODE == 2) {
         half_couplings<HALF>(b.wv, lam, Ms, Bs);
         float rx[kHalf], rn[kHalf];
@@ -384,7 +426,7 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
     RowBuf<MODE> A, Bf;
     for (int pass = 0; pass < 2; ++pass) {
       auto chunk_at = [&](int r) __attribute__((always_inline)) { return pass == 0 ? r : nch - 1 - r; };
-      const int rounds = nch + 2 + pass;  // the backward sweep's stores trail by one round
+      const int rounds = nch + 2 + (1 + kTail) * pass;  // the backward sweep's stores trail by one round
       load(A, chunk_at(0));
       for (int r = 0; r < rounds; r += 2) {
         load(Bf, chunk_at(min(r + 1, nch - 1)));
@@ -429,7 +471,7 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
         ch_chunk(L.M[q % kSlots] + lo, L.B[q % kSlots] + lo, L.C[q & 1] + lo, s0);
       }
     };
-    const int rounds = nch + 3;
+    const int rounds = nch + 3 + kTail;
     float cA = ck_at(-1), cB;
     for (int r = 0; r < rounds; r += 2) {
       cB = ck_at(r);
@@ -441,14 +483,14 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
         __syncthreads();
       }
     }
-  } else {
+  } else if (wave == 3 || wave == 4) {
     // ---------------- x (wave 3) and n (wave 4) chains ----------------
     const int role = wave == 3 ? 1 : 2;  // checkpoint plane 1 (x) or 2 (n)
     const gptr<float> out = gp(F.rowx) + (role == 2 ? (size_t)W * H : 0);
     const gptr<float> out_rm = gp(role == 2 ? F.row_n_out : F.row_x_out);
     const gptr<float> sparse_out = gp(F.sparse_out);
-    const float* Rsrc = (role == 2 && MODE == 2) ? &L.N[0][0] : &L.R[0][0];
-    const bool derive = role == 2 && MODE != 2;
+    const float* Rsrc = (role == 2 && MODE >= 2) ? &L.N[0][0] : &L.R[0][0];
+    const bool derive = role == 2 && MODE < 2;
     const size_t ckb = (size_t)role * nch * H;
     float st = 0.0f, v[kUK];
     for (int r = 0; r < nch + 2; ++r) {
@@ -502,7 +544,12 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
         else
           nxt = recompute_back<false>(L.M[s] + lo, L.B[s] + lo, Rsrc + s * 64 * kUS + lo, L.C[qc & 1] + lo, s0, vn,
                                       cn, stop, nxt, vp, cp);
-        if (q >= 1) {
+        if (q >= 1 && MODE == 3) {
+          // the adjoint hands chunk q - 1 to the out wave
+          float* O = (role == 2 ? adj_no(L) : adj_xo(L)) + ((q - 1) & 1) * 64 * kUS + lo;
+#pragma unroll
+          for (int k = 0; k < kUK; ++k) O[k] = vp[k];
+        } else if (q >= 1) {
           if (cnt == kUK) {
             // 32-bit offsets from the plane's base (W H < 2^28 floats)
             const unsigned o0 = (unsigned)c0 * (unsigned)H + (unsigned)y;
@@ -520,7 +567,7 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
         }
       }
     };
-    const int rounds = nch + 3;
+    const int rounds = nch + 3 + kTail;
     float sA = ck_at(-2), sB;
     for (int r = 0; r < rounds; r += 2) {
       sB = ck_at(r - 1);
@@ -532,6 +579,150 @@ __global__ __launch_bounds__(320) void k_up_row(const FrameDev* __restrict__ fra
         __syncthreads();
       }
     }
+  } else if (MODE == 3) {
+    // ---------------- adjoint out (wave 5) ----------------
+    // From the row adjoint's solutions v (x / n waves) and the forward row
+    // results z (the rowx planes): dL/dd = conf vx (vx without confidences),
+    // dL/dconf = d vx + vn (0 without), and dL/dw of every edge as in the
+    // column pass's adjoint out wave; the per-row lambda partial. Chunks come
+    // from the row's end; a round's inputs are loaded a round ahead.
+    const gptr<const float> zxp = gp((const float*)F.rowx) + y;
+    const gptr<const float> zyp = zxp + (size_t)W * H;
+    const size_t wrow = (size_t)y * (W + 1);
+    const gptr<const float> wr = gp(F.g_wrow) + wrow;
+    const bool have_conf = F.g_conf != nullptr;
+    const gptr<const float> dp = gp(F.g_d) + row;
+    const gptr<const float> cp = gp(have_conf ? F.g_conf : F.g_d) + row;
+    const gptr<float> gwr = gp(F.grad_wrow), gd = gp(F.grad_d), gc = gp(F.grad_conf);
+    const float gscale = F.grad_w_unscaled ? 1.0f : (float)lam;
+    for (int r = 0; r < nch + 2; ++r) __syncthreads();
+    AdjOutIn A, B;
+    // z from the column-major planes (a wave's load of one step is 64
+    // consecutive floats); d, conf and w(i + 1) as the lane's own 16
+    // consecutive floats (four dword-aligned float4) from min(c0, W - 16), which
+    // is c0 unless the chunk ends within 16 of the row's end (then out_round
+    // reloads them one by one). Every load unconditional.
+    auto load4x4 = [&](gptr<const float> p, float (&v)[kUK]) __attribute__((always_inline)) {
+      const gptr<const f4_dword_aligned> q4 = (gptr<const f4_dword_aligned>)p;
+#pragma unroll
+      for (int k = 0; k < kUK / 4; ++k) {
+        const float4 t = q4[k];
+        v[4 * k] = t.x;
+        v[4 * k + 1] = t.y;
+        v[4 * k + 2] = t.z;
+        v[4 * k + 3] = t.w;
+      }
+    };
+    auto load_o = [&](AdjOutIn& b, int q) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(nch - 1 - clampi(q, 0, nch - 1), c0, cnt, first, last);
+#pragma unroll
+      for (int k = 0; k < kUK; ++k) {
+        const int i = min(c0 + k, W - 1);
+        b.zx[k] = zxp[(size_t)i * H];
+        b.zy[k] = zyp[(size_t)i * H];
+      }
+      const int base = min(c0, W - kUK);
+      load4x4(wr + base + 1, b.w);
+      load4x4(dp + base, b.d);
+      load4x4(cp + base, b.c);
+    };
+    float vxn = 0.0f, vnn = 0.0f, zxn = 0.0f, zyn = 0.0f, carry = 0.0f;
+    double part = 0.0;  // sum over the row's edges of w dL/d(lambda w)
+    auto out_round = [&](const AdjOutIn& b, int q) __attribute__((always_inline)) {
+      if (q < 0 || q >= nch) return;
+      const int j = nch - 1 - q;
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
+      const bool seg1 = j >= ch.n0;
+      const int s0 = seg1 ? ch.seg : 0, s1 = seg1 ? ch.len : ch.seg;
+      const float* XO = adj_xo(L) + (q & 1) * 64 * kUS + lo;
+      const float* NO = adj_no(L) + (q & 1) * 64 * kUS + lo;
+      float wv[kUK], dv[kUK], cv[kUK];
+      if (c0 <= W - kUK) {
+#pragma unroll
+        for (int k = 0; k < kUK; ++k) {
+          wv[k] = b.w[k];
+          dv[k] = b.d[k];
+          cv[k] = b.c[k];
+        }
+      } else {  // the row's last chunk: its own loads (once per row)
+#pragma unroll
+        for (int k = 0; k < kUK; ++k) {
+          const int i = min(c0 + k, W - 1);
+          wv[k] = wr[min(c0 + k + 1, W)];
+          dv[k] = dp[i];
+          cv[k] = cp[i];
+        }
+      }
+      // edge c0 + k + 1, dL/dd and dL/dconf of step c0 + k
+      float ge[kUK], gdv[kUK], gcv[kUK];
+#pragma unroll
+      for (int k = kUK - 1; k >= 0; --k) {
+        ge[k] = gdv[k] = gcv[k] = 0.0f;
+        if (k < cnt) {
+          const int i = c0 + k;
+          const float vx = XO[k], vn = NO[k], zx = b.zx[k], zy = b.zy[k];
+          if (i + 1 < s1) {
+            const float t = (vx - vxn) * (zx - zxn) + (vn - vnn) * (zy - zyn);
+            ge[k] = -gscale * t;
+            part += (double)wv[k] * (double)(-t);
+          } else {
+            const float t = vx * zx + vn * zy;
+            const float e = -gscale * t;
+            part += (double)wv[k] * (double)(-t);
+            ge[k] = s1 == W ? e : e + carry;
+          }
+          if (i == s0) {
+            const float t = vx * zx + vn * zy;
+            const float e = -gscale * t;
+            part += (double)wr[s0] * (double)(-t);
+            if (s0 == 0) {
+              if (gwr) gwr[wrow] = e;
+            } else {
+              carry = e;  // segment 1 is swept first: its share of the split edge
+            }
+          }
+          // rhs_x = conf d, rhs_n = conf (without confidences: rhs_x = d, rhs_n = 1{d > 0})
+          gdv[k] = have_conf ? cv[k] * vx : vx;
+          gcv[k] = have_conf ? dv[k] * vx + vn : 0.0f;
+          vxn = vx;
+          vnn = vn;
+          zxn = zx;
+          zyn = zy;
+        }
+      }
+      // the lane's 16 consecutive results as four float4 (a whole chunk), else one by one
+      auto store = [&](gptr<float> p, const float (&v)[kUK]) __attribute__((always_inline)) {
+        if (cnt == kUK) {
+          const gptr<f4_dword_aligned> q4 = (gptr<f4_dword_aligned>)p;
+#pragma unroll
+          for (int k = 0; k < kUK / 4; ++k) q4[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) p[k] = v[k];
+        }
+      };
+      if (gwr) store(gwr + wrow + c0 + 1, ge);
+      if (gd) store(gd + row + c0, gdv);
+      if (gc) store(gc + row + c0, gcv);
+    };
+    const int rounds = nch + 4;
+    load_o(A, 0);
+    for (int r = 0; r < rounds; r += 2) {
+      load_o(B, r - 3);
+      out_round(A, r - 4);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        load_o(A, r - 2);
+        out_round(B, r - 3);
+        __syncthreads();
+      }
+    }
+    if (F.lam_part) F.lam_part[y] = part;
   }
 }
 
@@ -552,9 +743,12 @@ struct UpColLds {
 template <int MODE>
 struct ColBuf {                  // one producer wave's half of a chunk
   uint8_t px[kHalf + 2];         // key image at rows c0 - 1 + 8 HALF + j (modes 0, 1)
-  float wv[kHalf + 1];           // column couplings (mode 2)
-  float rx[kHalf], rn[kHalf];    // row pass X, N of the half, from base
+  float wv[kHalf + 1];           // column couplings (modes 2, 3)
+  float rx[kHalf], rn[kHalf];    // row pass X, N of the half, from base (modes 0-2)
+  float g[kHalf];                // dL/d(dense) (mode 3), row c0 + 8 HALF + j
+  float zx[kHalf], zy[kHalf];    // the forward column result (X, N) there (mode 3)
 };
+
 
 template <int MODE>
 __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ frames, BlockMap bm,
@@ -567,7 +761,7 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
   // waves: 0 and 1 producers (half a chunk each), 2 ch, 3 x, 4 n, 5 out; waves 0
   // and 4, 1 and 5 share SIMDs
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave: uniform
-  if (MODE != 2 && threadIdx.x < 256) {
+  if (MODE < 2 && threadIdx.x < 256) {
     const float w = wlut[threadIdx.x];
     L.lw[threadIdx.x] = w;
     L.lm[threadIdx.x] = (float)(-lam * (double)w);
@@ -587,10 +781,26 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
     const gptr<const float> g_wcol = gp(F.g_wcol);
     const gptr<const float> colx = gp((const float*)F.rowx) + (size_t)x * H;
     const gptr<const float> coln = colx + (size_t)W * H;
+    const gptr<const float> g_out = gp(F.g_out);
+    const gptr<const float2> colz = gp((const float2*)F.colxn);
     auto load = [&](ColBuf<MODE>& b, int j) __attribute__((always_inline)) {
       int c0, cnt;
       bool first, last;
       ch.get(j, c0, cnt, first, last);
+      if (MODE == 3) {
+        // every load unconditional, at rows clamped to the line
+#pragma unroll
+        for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wcol[(size_t)min(c0 + h0 + k, H) * W + x];
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+          const size_t o = (size_t)min(c0 + h0 + k, H - 1) * W + x;
+          b.g[k] = g_out[o];
+          const float2 z = colz[o];
+          b.zx[k] = z.x;
+          b.zy[k] = z.y;
+        }
+        return;
+      }
       if (MODE == 2) {
 #pragma unroll
         for (int k = 0; k <= kHalf; ++k) b.wv[k] = g_wcol[(size_t)min(c0 + h0 + k, H) * W + x];
@@ -609,6 +819,24 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
       bool first, last;
       ch.get(j, c0, cnt, first, last);
       const int sh = c0 + h0 - min(c0 + h0, H - kHalf);
+      if (MODE == 3) {
+        // adjoint right-hand sides: out = X / N, so dL/dX = g / N and
+        // dL/dN = -(g / N) (X / N); pixels with N = 0 (NaN output) give none
+        half_couplings<HALF>(b.wv, lam, L.M[slot] + lo, L.B[slot] + lo);
+        float* Rs = L.R[slot] + lo;
+        float* Ns = L.N[slot] + lo;
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+          float gx = 0.0f, gn = 0.0f;
+          if (b.zy[k] != 0.0f) {
+            gx = b.g[k] / b.zy[k];
+            gn = -gx * (b.zx[k] / b.zy[k]);
+          }
+          Rs[h0 + k] = gx;
+          Ns[h0 + k] = gn;
+        }
+        return;
+      }
       if (MODE == 2)
         half_couplings<HALF>(b.wv, lam, L.M[slot] + lo, L.B[slot] + lo);
       else
@@ -733,6 +961,101 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
         __syncthreads();
       }
     }
+  } else if (MODE == 3) {
+    // ---------------- adjoint out ----------------
+    // (X, N) of the adjoint solve into adjxn ([column][row]); dL/dw of every
+    // edge from the solutions v and the forward results z of its two steps,
+    // -scale ((vx_i - vx_{i+1}) (zx_i - zx_{i+1}) + (vn_i - vn_{i+1}) (zy_i - zy_{i+1})),
+    // or -scale (vx zx + vn zy) for an edge on a segment's end; the two
+    // segments' terms of the split edge are summed. Chunks come from the line's
+    // end; the inputs of a round are loaded a round ahead (two buffers).
+    const gptr<float2> adj = gp(F.adjxn) + (size_t)x * H;
+    const gptr<const float2> colz = gp((const float2*)F.colxn);
+    const gptr<const float> wcol = gp(F.g_wcol);
+    const gptr<float> gw = gp(F.grad_wcol);
+    const float gscale = F.grad_w_unscaled ? 1.0f : (float)lam;
+    for (int r = 0; r < nch + 2; ++r) __syncthreads();
+    AdjOutIn A, B;
+    auto load_o = [&](AdjOutIn& b, int q) __attribute__((always_inline)) {
+      int c0, cnt;
+      bool first, last;
+      ch.get(nch - 1 - clampi(q, 0, nch - 1), c0, cnt, first, last);
+#pragma unroll
+      for (int k = 0; k < kUK; ++k) {
+        const float2 z = colz[(size_t)min(c0 + k, H - 1) * W + x];
+        b.zx[k] = z.x;
+        b.zy[k] = z.y;
+        b.w[k] = wcol[(size_t)min(c0 + k + 1, H) * W + x];
+      }
+    };
+    float vxn = 0.0f, vnn = 0.0f, zxn = 0.0f, zyn = 0.0f, carry = 0.0f;
+    double part = 0.0;  // sum over the column's edges of w dL/d(lambda w)
+    auto out_round = [&](const AdjOutIn& b, int q) __attribute__((always_inline)) {
+      if (q < 0 || q >= nch) return;
+      const int j = nch - 1 - q;
+      int c0, cnt;
+      bool first, last;
+      ch.get(j, c0, cnt, first, last);
+      const bool seg1 = j >= ch.n0;
+      const int s0 = seg1 ? ch.seg : 0, s1 = seg1 ? ch.len : ch.seg;
+      const float* XO = L.XO[q & 1] + lo;
+      const float* NO = L.NO[q & 1] + lo;
+      // the solutions as the lane's 16 consecutive float2 (eight float4 for a
+      // whole chunk, else one by one)
+      if (cnt == kUK) {
+        const gptr<f4_dword_aligned> q4 = (gptr<f4_dword_aligned>)(adj + c0);
+#pragma unroll
+        for (int k = 0; k < kUK / 2; ++k) q4[k] = make_float4(XO[2 * k], NO[2 * k], XO[2 * k + 1], NO[2 * k + 1]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kUK; ++k)
+          if (k < cnt) adj[c0 + k] = make_float2(XO[k], NO[k]);
+      }
+#pragma unroll
+      for (int k = kUK - 1; k >= 0; --k) {
+        if (k < cnt) {
+          const int i = c0 + k;
+          const float vx = XO[k], vn = NO[k], zx = b.zx[k], zy = b.zy[k];
+          if (i + 1 < s1) {
+            const float t = (vx - vxn) * (zx - zxn) + (vn - vnn) * (zy - zyn);
+            if (gw) gw[(size_t)(i + 1) * W + x] = -gscale * t;
+            part += (double)b.w[k] * (double)(-t);
+          } else {
+            const float t = vx * zx + vn * zy;
+            const float e = -gscale * t;
+            part += (double)b.w[k] * (double)(-t);
+            if (gw) gw[(size_t)s1 * W + x] = s1 == H ? e : e + carry;
+          }
+          if (i == s0) {
+            const float t = vx * zx + vn * zy;
+            const float e = -gscale * t;
+            part += (double)wcol[(size_t)s0 * W + x] * (double)(-t);
+            if (s0 == 0) {
+              if (gw) gw[x] = e;
+            } else {
+              carry = e;  // segment 1 is swept first: its share of the split edge
+            }
+          }
+          vxn = vx;
+          vnn = vn;
+          zxn = zx;
+          zyn = zy;
+        }
+      }
+    };
+    const int rounds = nch + 4;
+    load_o(A, 0);
+    for (int r = 0; r < rounds; r += 2) {
+      load_o(B, r - 3);
+      out_round(A, r - 4);
+      __syncthreads();
+      if (r + 1 < rounds) {
+        load_o(A, r - 2);
+        out_round(B, r - 3);
+        __syncthreads();
+      }
+    }
+    if (F.lam_part) F.lam_part[(size_t)H + x] = part;
   } else {
     // ---------------- out ----------------
     const gptr<float> dense = gp(F.dense);
@@ -979,8 +1302,10 @@ void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n
     hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
   else if (mode == 1)
     hipLaunchKernelGGL(k_up_row<1>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
-  else
+  else if (mode == 2)
     hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
+  else
+    hipLaunchKernelGGL(k_up_row<3>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut);
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
@@ -990,6 +1315,8 @@ void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n
   if (nb <= 0) return;
   if (mode == 2)
     hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, (int)clear_zbuf);
+  else if (mode == 3)
+    hipLaunchKernelGGL(k_up_col<3>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, 0);
   else
     hipLaunchKernelGGL(k_up_col<0>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, (int)clear_zbuf);
 }

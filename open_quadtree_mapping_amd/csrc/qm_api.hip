@@ -346,8 +346,9 @@ struct qm_batch {
   double* partial = nullptr;
   int partial_n = 0;
   struct Bwd {
-    float *sparse, *wrow, *wcol, *gdense, *gwrow, *gwcol;
+    float *sparse, *wrow, *wcol, *gdense;
     float2 *colxn, *adjxn;
+    double* lam_part;  // H + W per-line partials of dL/dlambda
   };
   std::vector<Bwd> bwd;
 };
@@ -849,6 +850,12 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
     F.grad_d = q.grad_d;
     F.grad_conf = q.grad_conf;
     F.grad_wrow = q.grad_w_row;
+    if (q.grad_lambda) {
+      void* lp;
+      if ((rc = sc.get(&lp, sizeof(double) * (size_t)(q.W + q.H)))) return rc;
+      F.lam_part = (double*)lp;
+      F.grad_lambda = q.grad_lambda;
+    }
     const size_t px = (size_t)q.W * q.H;
     void *fwd, *rxn, *cxn, *axn, *dense;
     if ((rc = sc.get(&fwd, px * 16))) return rc;
@@ -861,13 +868,7 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
     F.colxn = (float2*)cxn;
     F.adjxn = (float2*)axn;
     F.dense = (float*)dense;
-    if (q.grad_w_col) {
-      F.grad_wcol = q.grad_w_col;
-    } else {
-      void* gwc;
-      if ((rc = sc.get(&gwc, (size_t)(q.H + 1) * q.W * 4))) return rc;
-      F.grad_wcol = (float*)gwc;
-    }
+    F.grad_wcol = q.grad_w_col;
   }
   QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
   launch_upsample_backward((FrameDev*)df, hf.data(), n, s);
@@ -1487,13 +1488,12 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
     for (int i = 0; i < n; ++i) {
       const FrameDev& F = b->hf[0][i];
       const size_t px = (size_t)F.W * F.H, er = (size_t)F.H * (F.W + 1), ec = (size_t)(F.H + 1) * F.W;
-      total += 2 * align256(px * 4) + 2 * align256(er * 4) + 2 * align256(ec * 4) + 2 * align256(px * 8);
+      total += 2 * align256(px * 4) + align256(er * 4) + align256(ec * 4) + 2 * align256(px * 8) +
+               align256(sizeof(double) * (F.W + F.H));
     }
-    int nblocks = 0;
-    for (int i = 0; i < n; ++i) nblocks += lambda_blocks(b->hf[0][i]);
     void *arena = nullptr, *dfb = nullptr, *partial = nullptr;
     if ((rc = dev_alloc(&arena, total)) || (rc = dev_alloc(&dfb, sizeof(FrameDev) * n)) ||
-        (rc = dev_alloc(&partial, sizeof(double) * nblocks))) {
+        (rc = dev_alloc(&partial, sizeof(double) * n))) {
       dev_free(arena);
       dev_free(dfb);
       dev_free(partial);
@@ -1501,8 +1501,8 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
     }
     b->bwd_arena = arena;
     b->dfb = (FrameDev*)dfb;
-    b->partial = (double*)partial;
-    b->partial_n = nblocks;
+    b->partial = (double*)partial;  // dL/dlambda of each frame
+    b->partial_n = n;
     Carver cv{(char*)arena};
     b->bwd.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -1512,11 +1512,10 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
       w.sparse = cv.take<float>(px * 4);
       w.gdense = cv.take<float>(px * 4);
       w.wrow = cv.take<float>(er * 4);
-      w.gwrow = cv.take<float>(er * 4);
       w.wcol = cv.take<float>(ec * 4);
-      w.gwcol = cv.take<float>(ec * 4);
       w.colxn = cv.take<float2>(px * 8);
       w.adjxn = cv.take<float2>(px * 8);
+      w.lam_part = cv.take<double>(sizeof(double) * (F.W + F.H));
     }
   }
   std::vector<FrameDev> hb = b->hf[last];
@@ -1538,9 +1537,11 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
     F.g_out = w.gdense;
     F.grad_d = nullptr;
     F.grad_conf = nullptr;
-    F.grad_wrow = w.gwrow;
-    F.grad_wcol = w.gwcol;
+    F.grad_wrow = nullptr;  // only dL/dlambda is wanted: the adjoint sums w dL/d(lambda w) per line
+    F.grad_wcol = nullptr;
     F.grad_w_unscaled = 1;
+    F.lam_part = w.lam_part;
+    F.grad_lambda = b->partial + i;
     F.colxn = w.colxn;
     F.adjxn = w.adjxn;
     F.row_x_out = nullptr;
@@ -1552,17 +1553,9 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
   launch_fuse_backward(b->dfb, hb.data(), n, s);
   launch_compose_inputs(b->dfb, hb.data(), n, b->wlut, s);
   launch_upsample_backward(b->dfb, hb.data(), n, s);
-  int first[kMaxFrames + 1];
-  const int nb = launch_lambda_partials(b->dfb, hb.data(), n, b->partial, first, s);
   QM_HIP(hipGetLastError());
-  std::vector<double> part(nb);
-  QM_HIP(hipMemcpyAsync(part.data(), b->partial, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  QM_HIP(hipMemcpyAsync(grad_lambda, b->partial, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   QM_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < n; ++i) {
-    double acc = 0.0;
-    for (int k = first[i]; k < first[i + 1]; ++k) acc += part[k];
-    grad_lambda[i] = acc;
-  }
   return QM_OK;
 }
 

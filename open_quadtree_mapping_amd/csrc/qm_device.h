@@ -60,6 +60,8 @@ struct FrameDev {
   float* grad_wrow;     // H x (W+1)
   float* grad_wcol;     // (H+1) x W
   int grad_w_unscaled;  // 1: grad_w* receive dL/d(lambda w_e) (no lambda factor), for dL/dlambda
+  double* lam_part;     // H + W per-line sums of w_e dL/d(lambda w_e) (rows, then columns) or null
+  double* grad_lambda;  // dL/dlambda of the frame (one double) or null
   // adjoint of the fusion update
   const int32_t* table_in;     // hole-filled warp table
   const float* g_depth_out;    // dL/d(depth_out)

@@ -84,12 +84,8 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
 void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // pipeline backward: the keyframe upsample's inputs written as explicit arrays
-// into each descriptor's g_d / g_wrow / g_wcol, and the per-block partial sums
-// of dL/dlambda
+// into each descriptor's g_d / g_wrow / g_wcol
 void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
                            hipStream_t s);
-int lambda_blocks(const FrameDev& F);
-int launch_lambda_partials(const FrameDev* dframes, const FrameDev* hframes, int n, double* partial,
-                           int* first_block, hipStream_t s);
 
 }  // namespace qm
