@@ -70,13 +70,26 @@ def parse():
     return ap.parse_args()
 
 
-def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10)):
+def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10), pairs=True):
     """Algorithmic bytes of the BP half-sweeps for one keyframe, following the
-    skip rule of the kernels (quadtree lookup at (bx << lev, y << lev)).
-    Active update: data + 4 incoming + 4 outgoing messages = 36N bytes; a cell
+    skip rule of the kernels (quadtree lookup at (bx << lev, y << lev)) and
+    their schedule (the extraction of the last level-0 sweep's colour included
+    there, as in the pipeline).
+    One launch per half-sweep (pairs=False): an update reads the cell's data
+    row and 4 incoming messages and writes 4 outgoing ones, 36N bytes; a cell
     initialised from its parent at a level's first sweep (the cells the
     quadtree skips, or every cell not updated when the level has fewer than two
-    half-sweeps): 16N read + 16N write (16N write only at the coarsest level)."""
+    half-sweeps) costs 16N read + 16N write (16N write only at the coarsest
+    level).
+    Fused pairs (pairs=True): the first half-sweep's updates read 20N (data +
+    4 incoming, from the parent level in a level's first pair) and write their
+    4 messages (16N) only in the level's last pair, and not at level 0; the
+    second half-sweep's updates read their data row (4N; the incoming messages
+    come from LDS) and write 16N; a skipped cell initialised at a level's first
+    pair is written to both message buffers (16N read + 32N write, 32N write
+    at the coarsest level); an odd last half-sweep runs alone (36N). The halo
+    cells a pair recomputes are not algorithmic bytes: the HBM counters show
+    them."""
     import numpy as np
     Wc, Hc = W // 4, H // 4
     lw, lh = [Wc], [Hc]
@@ -99,18 +112,40 @@ def bp_traffic(levels, W, H, N, iters=(4, 10, 10, 10)):
         colA = (xs & 1) == (ys & 1)
         nA = int((active & colA).sum())
         nB = int((active & ~colA).sum())
+        n_skip = int((~active).sum())
         cells = w * h
-        for it in range(iters[l]):
-            n_upd = nA if it % 2 == 0 else nB
-            updates += n_upd
-            total += 36 * N * n_upd
+        it_l = iters[l]
+        if it_l == 0:
+            total += (16 if l == 3 else 32) * N * cells
+            continue
+        if not pairs or it_l < 2:
+            for it in range(it_l):
+                n_upd = nA if it % 2 == 0 else nB
+                updates += n_upd
+                total += 36 * N * n_upd
+                if it == 0:
+                    copied = (cells - n_upd) if it_l < 2 else n_skip
+                    total += (16 if l == 3 else 32) * N * copied
+            continue
+        for it in range(0, it_l - 1, 2):
+            last = it + 2 == it_l
+            updates += nA + nB
+            total += 20 * N * nA + 20 * N * nB
+            if last and l > 0:
+                total += 16 * N * nA
             if it == 0:
-                copied = (cells - n_upd) if iters[l] < 2 else int((~active).sum())
-                total += (16 if l == 3 else 32) * N * copied
+                total += (32 if l == 3 else 48) * N * n_skip
+        if it_l % 2:
+            updates += nA
+            total += 36 * N * nA
     return total, updates
 
 
-def stage_bytes(levels, W, H, N):
+def bp_launches(iters=(4, 10, 10, 10), pairs=True):
+    return sum(1 if it == 0 else (it // 2 + it % 2 if pairs and it >= 2 else it) for it in iters)
+
+
+def stage_bytes(levels, W, H, N, pairs=True):
     import numpy as np
     C = (W // 4) * (H // 4)
     lw, lh = [W // 4], [H // 4]
@@ -126,7 +161,7 @@ def stage_bytes(levels, W, H, N):
     last = ((xs + ys) & 1) == 1
     nodes = int((node & ~last).sum())
     others = int((~node & ~last).sum())
-    bp, upd = bp_traffic(levels, W, H, N)
+    bp, upd = bp_traffic(levels, W, H, N, pairs=pairs)
     bp += 8 * int((node & last).sum())  # fused extraction: depth + label of the B nodes
     px = W * H
     return {
@@ -184,7 +219,7 @@ def measured_traffic(stage, W, H, N, B):
     if not m:
         return None, None
     rec = json.loads(m.group(1))
-    if rec.get("kernel") != "k_" + stage:
+    if rec.get("stage") != stage:
         return None, None
     return int(rec["hbm_bytes_per_launch"]), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
 
@@ -243,9 +278,11 @@ def run_workload(args):
         return (time.perf_counter() - t0) / args.steps
 
     wl = args.workload
+    arith = 1 if args.cost_arith == "fused" else 0
     if wl == "uneven12":
         frames = keyframes(376, 240, 4, 300) + keyframes(752, 480, 4, 310) + keyframes(1280, 720, 4, 320)
-        batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0), outputs=("depth",))
+        batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, cost_arith=arith),
+                      outputs=("depth",))
         batch.autotune(reps=args.tune_reps)
         sec = timed(lambda: batch.run(graph=True))
         n, unit, what = 12, "depth maps/s", "batched graph replay of 12 uneven streams, N = 64"
@@ -253,7 +290,7 @@ def run_workload(args):
     elif wl == "diff32":
         frames = [dict(f, key=f["key"].clone(), refs=[r.clone() for r in f["refs"]])
                   for f in keyframes(752, 480, 8, 100) for _ in range(4)]
-        batch = Batch(frames, default_params(n_hyp=128, idepth_min=1.0 / 50.0, idepth_max=1.0 / 0.3),
+        batch = Batch(frames, default_params(n_hyp=128, idepth_min=1.0 / 50.0, idepth_max=1.0 / 0.3, cost_arith=arith),
                       outputs=("depth",))
         batch.autotune(reps=args.tune_reps)
         g = [torch.randn(480, 752, device=dev) for _ in range(32)]
@@ -282,9 +319,12 @@ def run_workload(args):
             QA.upsample(d, wr, wc, conf=conf, lam=lams).backward(g)
         sec = timed(step)
         n, unit, what = B, "problems/s", "16 upsample problems, forward plus adjoint (all inputs and lambda)"
+    cfg = {"workload": what}
+    if wl != "upsample16":
+        cfg["cost_arith"] = args.cost_arith
     print(json.dumps({"workload": wl, "value": round(n / sec, 2), "unit": unit, "ms_per_step": round(sec * 1e3, 4),
                       "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
-                      "config": {"workload": what}}), flush=True)
+                      "config": cfg}), flush=True)
 
 
 def main():
@@ -385,9 +425,12 @@ def main():
             a[0] += ms
             a[1] += cnt
     levels_per_stream = [S.quadtree(f["key"]).cpu().numpy() for f in frames]
+    pairs = batch.tuning()["bp_pair"] != 0
     stage_alg = {}
+    bp_updates = 0
     for lv in levels_per_stream:
-        sb, _ = stage_bytes(lv, W, H, N)
+        sb, upd = stage_bytes(lv, W, H, N, pairs=pairs)
+        bp_updates += upd
         for k, v in sb.items():
             stage_alg[k] = stage_alg.get(k, 0) + v
     stages = {}
@@ -419,6 +462,14 @@ def main():
         oms = sum(ob.run_timed()["cost"][0] for _ in range(reps)) / reps
         ob.close()
         c["other_arith_ms_per_run"] = round(oms, 4)
+    if "bp_sweep" in stages and bp_updates:
+        # bytes per cell update in units of N (36 for one launch per half-sweep)
+        stages["bp_sweep"].update({"updates_per_run": bp_updates, "fused_pairs": pairs,
+                                   "alg_bytes_per_update_N": round(stage_alg["bp_sweep"] / bp_updates / N, 2)})
+        tr, _ = measured_traffic("bp_sweep", W, H, N, B)
+        if tr is not None:
+            stages["bp_sweep"]["hbm_bytes_per_update_N"] = round(
+                tr * stages["bp_sweep"]["launches"] / bp_updates / N, 2)
     dom = max(stages, key=lambda k: stages[k]["ms_per_run"])
     d = stages[dom]
     traffic, traffic_src = measured_traffic(dom, W, H, N, B)

@@ -69,7 +69,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 9
+#define QM_ABI_VERSION 10
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -302,7 +302,7 @@ int qm_batch_run(qm_batch* b, int mode, void* stream);
 int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES], int stage_launches[QM_NUM_STAGES]);
 /* Launch-shape knobs of a batch; every value gives bit-identical results.
  * Changing one drops the batch's captured graphs (recaptured on the next
- * graph run). Defaults: 4, 16, 16, 0. */
+ * graph run). Defaults: 4, 16, 16, 0, 8, 8, 2. */
 #define QM_TUNE_COST_WAVES 0   /* 2, 4 or 8 waves per cost-volume block of 64 cells */
 #define QM_TUNE_BP_ROWS 1      /* 4, 8 or 16 cells (16-lane rows) per BP half-sweep block */
 #define QM_TUNE_EXTRACT_ROWS 2 /* 4, 8 or 16 cells per winner-extraction block */
@@ -312,7 +312,14 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
                                   groups, each starting after the previous group's cost volume;
                                   7: four groups, each after the previous group's BP (results
                                   identical in every case) */
-#define QM_NUM_TUNE 4
+#define QM_TUNE_BP_PAIR 4      /* 0: one launch per BP half-sweep; 8 or 16: the half-sweeps of a
+                                  level in fused pairs, one launch per pair, over strips of 14 or
+                                  30 cell columns (the first half-sweep's messages pass to the
+                                  second through LDS) */
+#define QM_TUNE_BP_SEG 5       /* 8, 16, 32 or 64 cell rows per strip segment of a fused pair */
+#define QM_TUNE_BP_PAIR_LEVELS 6 /* 1-4: the fused pairs run at levels 0 .. value - 1 (the coarser
+                                    levels, with few cells, keep one launch per half-sweep) */
+#define QM_NUM_TUNE 7
 int qm_batch_set_tuning(qm_batch* b, int knob, int value);
 int qm_batch_get_tuning(qm_batch* b, int knob, int* value);
 /* Autotuner: on this batch's own frames, time every value of every knob

@@ -19,7 +19,7 @@ D = C.c_double
 
 STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", "upsample_col", "fuse_warp",
           "fuse_update")
-TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows", "overlap")  # QM_TUNE_* order
+TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows", "overlap", "bp_pair", "bp_seg", "bp_pair_levels")  # QM_TUNE_* order
 COST_REFERENCE, COST_FUSED = 0, 1  # QM_COST_*: qm_params.cost_arith
 
 
@@ -87,7 +87,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 9  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 10  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {

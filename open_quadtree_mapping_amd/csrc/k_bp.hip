@@ -11,7 +11,8 @@
 // neighbour messages straight from the parent level, and copies the parent's
 // messages into the cells it does not update that will need them: cells the
 // quadtree skips at this level, and every cell when the level has fewer than
-// two half-sweeps.
+// two half-sweeps. Consecutive half-sweeps of a level run in pairs, one launch
+// each (k_bp_pair): the first colour's messages pass to the second through LDS.
 #include "qm_device.h"
 #include "qm_launch.h"
 
@@ -43,6 +44,9 @@ __device__ __forceinline__ void store_row(gptr<float> p, int j, const Row<S>& r)
 #pragma unroll
   for (int k = 0; k < S / 4; ++k) q[k] = make_float4(r.v[4 * k], r.v[4 * k + 1], r.v[4 * k + 2], r.v[4 * k + 3]);
 }
+
+// message buffer b (0: msg, 1: msg2) of level l
+__device__ __forceinline__ float* msg_buf(const FrameDev& F, int l, int b) { return b ? F.msg2[l] : F.msg[l]; }
 
 // The reference's shared-memory halving tree over a row (stage m: sample
 // d < m combines sample d + m, m = N/2, ..., 1; src/bpextract.cu:274-302):
@@ -102,6 +106,53 @@ __device__ __forceinline__ Row<S> message_from(const Row<S>& h, float p1, float 
 #pragma unroll
   for (int s = 0; s < S; ++s) m.v[s] = m.v[s] - mean;
   return m;
+}
+
+// h of the four directions (the data term plus the incoming messages other
+// than the target's, each added only where its neighbour exists), in the
+// reference's order up, down, left, right
+template <int S>
+__device__ __forceinline__ void directional_h(const Row<S>& dv, const Row<S>& fu, const Row<S>& fd, const Row<S>& fl,
+                                              const Row<S>& fr, bool has_up, bool has_down, bool has_left,
+                                              bool has_right, Row<S> (&hv)[4]) {
+  if (__all(has_up && has_down && has_left && has_right)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      hv[0].v[s] = ((dv.v[s] + fd.v[s]) + fl.v[s]) + fr.v[s];
+      hv[1].v[s] = ((dv.v[s] + fu.v[s]) + fl.v[s]) + fr.v[s];
+      hv[2].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fr.v[s];
+      hv[3].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fl.v[s];
+    }
+  } else {
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir)
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        float v = dv.v[s];
+        if (dir != 0 && has_up) v += fu.v[s];
+        if (dir != 1 && has_down) v += fd.v[s];
+        if (dir != 2 && has_left) v += fl.v[s];
+        if (dir != 3 && has_right) v += fr.v[s];
+        hv[dir].v[s] = v;
+      }
+  }
+}
+
+// winner extraction's belief in the order of src/bpextract.cu:330-337: data,
+// left, right, up, down
+template <int S>
+__device__ __forceinline__ Row<S> belief(const Row<S>& dv, const Row<S>& fu, const Row<S>& fd, const Row<S>& fl,
+                                         const Row<S>& fr, bool has_up, bool has_down, bool has_left,
+                                         bool has_right) {
+  Row<S> b = dv;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (has_left) b.v[s] += fl.v[s];
+    if (has_right) b.v[s] += fr.v[s];
+    if (has_up) b.v[s] += fu.v[s];
+    if (has_down) b.v[s] += fd.v[s];
+  }
+  return b;
 }
 
 __device__ __forceinline__ bool cell_skipped(const FrameDev& F, int l, int bx, int y) {
@@ -190,7 +241,8 @@ __device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int 
 template <int N, int ROWS>
 __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restrict__ frames, BlockMap bm, int l,
                                                         int A, int first, int active_ok, int copy_all, float p1,
-                                                        float p2, int ext, double imin, double istep) {
+                                                        float p2, int ext, double imin, double istep, int buf,
+                                                        int pbuf) {
   constexpr int S = N / 16;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
@@ -223,8 +275,8 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
       ex = ext && x < w;
     }
   }
-  const gptr<float> dst = gp(F.msg[l]) + ((size_t)y * w + x) * 4 * N;
-  const gptr<const float> P = gp((const float*)((first && l < 3) ? F.msg[l + 1] : nullptr));
+  const gptr<float> dst = gp(msg_buf(F, l, buf)) + ((size_t)y * w + x) * 4 * N;
+  const gptr<const float> P = gp((const float*)((first && l < 3) ? msg_buf(F, l + 1, pbuf) : nullptr));
   const int wp = (l < 3) ? F.lw[l + 1] : 0;
   if (copy) {
     // first sweep only: the parent's four messages (zeros at the coarsest level)
@@ -254,7 +306,7 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
   // neighbour its "up" message, from the left its "right", from the right its
   // "left"; the first sweep of a level reads them from the parent cell
   const bool has_up = y > 0, has_down = y < h - 1, has_left = x > 0, has_right = x < w - 1;
-  const gptr<const float> M = gp((const float*)F.msg[l]);
+  const gptr<const float> M = gp((const float*)msg_buf(F, l, buf));
   auto src = [&](int nx, int ny, int dir) -> gptr<const float> {
     if (first) return P + (((size_t)(ny >> 1) * wp + (nx >> 1)) * 4 + dir) * N;
     return M + (((size_t)ny * w + nx) * 4 + dir) * N;
@@ -268,44 +320,255 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
   if (msgs && has_down) fd = load_row<S>(src(x, y + 1, 0), j);
   if (msgs && has_left) fl = load_row<S>(src(x - 1, y, 3), j);
   if (msgs && has_right) fr = load_row<S>(src(x + 1, y, 2), j);
-  if (ex) {
-    // belief in the order of src/bpextract.cu:330-337: data, left, right, up, down
-    Row<S> b = dv;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (has_left) b.v[s] += fl.v[s];
-      if (has_right) b.v[s] += fr.v[s];
-      if (has_up) b.v[s] += fu.v[s];
-      if (has_down) b.v[s] += fd.v[s];
-    }
-    extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63, b, imin, istep);
-  }
+  if (ex)
+    extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63,
+                         belief<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right), imin, istep);
   if (!upd) return;
   Row<S> hv[4];
-  if (__all(has_up && has_down && has_left && has_right)) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      hv[0].v[s] = ((dv.v[s] + fd.v[s]) + fl.v[s]) + fr.v[s];
-      hv[1].v[s] = ((dv.v[s] + fu.v[s]) + fl.v[s]) + fr.v[s];
-      hv[2].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fr.v[s];
-      hv[3].v[s] = ((dv.v[s] + fu.v[s]) + fd.v[s]) + fl.v[s];
-    }
-  } else {
-#pragma unroll
-    for (int dir = 0; dir < 4; ++dir)
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        float v = dv.v[s];
-        if (dir != 0 && has_up) v += fu.v[s];
-        if (dir != 1 && has_down) v += fd.v[s];
-        if (dir != 2 && has_left) v += fl.v[s];
-        if (dir != 3 && has_right) v += fr.v[s];
-        hv[dir].v[s] = v;
-      }
-  }
+  directional_h<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right, hv);
   constexpr float inv_n = 1.0f / (float)N;
 #pragma unroll
   for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, message_from<S>(hv[dir], p1, p2, inv_n));
+}
+
+// ---------------------------------------------------------------------------
+// Two consecutive half-sweeps of one level in one launch. The first sweep
+// updates the cells of parity ca = (x + y) & 1 (the "A" cells), the second the
+// others ("B" cells), which read only A cells' messages. A block owns a strip
+// of TW = 2 (GA - 1) columns and seg rows and walks down its rows: in step t
+// its A groups update A row y0 - 1 + t (the strip's A cells and the one A cell
+// just outside the strip that borders a B cell of the row) into LDS slot t % 4,
+// while its B groups update B row y0 + t - 3 from the A rows above, at and
+// below it (slots t - 3, t - 2, t - 1). An A cell's messages go to HBM only
+// when a later launch reads them (write_first: the level's last pair); the B
+// cells' messages go to the other message buffer (wr), so that no block reads
+// a message another block of the launch rewrites. A cells on the halo (one per
+// row, and the rows y0 - 1 and y0 + seg) are updated by every block that
+// borders them, from the same inputs with the same arithmetic, so each message
+// equals the one of two separate half-sweeps, bit for bit. Cells the quadtree
+// skips keep their messages: both buffers hold them. SRC > 0: the level's
+// first two sweeps: A cells read their neighbours' messages from the parent
+// level (SRC 1, buffer pbuf; SRC 2 at the coarsest level: zeros), and skipped
+// cells of both colours get the parent's messages in both buffers. SRC 0:
+// later sweeps, A cells read buffer rd. ext: the second sweep ends BP (level 0)
+// and also extracts the winners of its B cells.
+// ---------------------------------------------------------------------------
+constexpr int kPairMaxSeg = 64;
+
+template <int N, int GA, int SRC>
+__global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict__ frames, BlockMap bm, int l, int ca,
+                                                     int rd, int wr, int pbuf, int write_first, int seg, float p1,
+                                                     float p2, int ext, double imin, double istep) {
+  constexpr bool first = SRC > 0;
+  constexpr int S = N / 16, Q = S / 4;  // floats and float4 per lane of one row
+  constexpr int TW = 2 * (GA - 1);
+  __shared__ float4 Ls[4][GA][4][16 * Q];        // the A messages of four A rows
+  // per cell of rows y0 - 1 .. y0 + R, columns x0 - 1 .. x0 + TW: bit 0 the
+  // quadtree skips it, bit 1 it is a quadtree node (level 0 only)
+  __shared__ uint8_t Sk[kPairMaxSeg + 2][TW + 2];
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int w = F.lw[l], h = F.lh[l];
+  const int ns = (w + TW - 1) / TW;
+  const int bi = blockIdx.x - bm.start[f];
+  const int x0 = (bi % ns) * TW, y0 = (bi / ns) * seg;
+  const int R = min(seg, h - y0);
+  const int T = 3 * ((R + 5) / 3);  // R + 3 steps, rounded up to a multiple of 3
+  const int grp = threadIdx.x >> 4, j = threadIdx.x & 15;
+  for (int i = threadIdx.x; i < (R + 2) * (TW + 2); i += 32 * GA) {
+    const int r = i / (TW + 2), c = i - r * (TW + 2);
+    const int y = y0 - 1 + r, x = x0 - 1 + c;
+    const bool in = x >= 0 && x < w && y >= 0 && y < h;
+    Sk[r][c] = (uint8_t)((in && cell_skipped(F, l, x >> 1, y) ? 1 : 0) | (in && l == 0 && is_node<N>(F, x, y) ? 2 : 0));
+  }
+  __syncthreads();
+  auto skipped = [&](int x, int y) __attribute__((always_inline)) { return (Sk[y - y0 + 1][x - x0 + 1] & 1) != 0; };
+  auto node = [&](int x, int y) __attribute__((always_inline)) { return (Sk[y - y0 + 1][x - x0 + 1] & 2) != 0; };
+  const gptr<const float> cost = gp((const float*)F.cost[l]);
+  const gptr<const float> Mrd = gp((const float*)msg_buf(F, l, rd));
+  const gptr<float> Mwr = gp(msg_buf(F, l, wr));
+  const gptr<float> M0 = gp(F.msg[l]), M1 = gp(F.msg2[l]);
+  const gptr<const float> P = gp((const float*)(SRC == 1 ? msg_buf(F, l + 1, pbuf) : F.msg[l]));
+  const gptr<const float> Min = SRC == 1 ? P : Mrd;  // where the A cells' incoming messages are read
+  const int wp = l < 3 ? F.lw[l + 1] : 1;
+  constexpr float inv_n = 1.0f / (float)N;
+  auto cell_msgs = [&](int x, int y) __attribute__((always_inline)) { return ((size_t)y * w + x) * 4 * N; };
+  auto parent_msgs = [&](int x, int y) __attribute__((always_inline)) {
+    return ((size_t)(y >> 1) * wp + (x >> 1)) * 4 * N;
+  };
+  auto store4 = [&](gptr<float> dst, const Row<S> (&m)[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, m[dir]);
+  };
+  auto zero = [&](Row<S>& r) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.v[s] = 0.0f;
+  };
+
+  if (grp < GA) {
+    // ---------------- A groups: A cell x0 - sh + 2 g of row y0 - 1 + t ----------------
+    const int g = grp;
+    auto cell_at = [&](int u, int& x, int& y) __attribute__((always_inline)) {
+      y = y0 - 1 + u;
+      x = x0 - ((((x0 + y) & 1) != ca) ? 1 : 0) + 2 * g;
+      return u <= R + 1 && y >= 0 && y < h && x >= 0 && x < w;
+    };
+    struct In {
+      Row<S> dv, m[4];  // data row; incoming messages (up, down, left, right), or the cell's own when skipped
+    };
+    // step u's inputs, loaded two steps ahead: every load unconditional (an
+    // invalid cell or a missing neighbour reads a valid row, not used)
+    auto load_in = [&](int u, In& in) __attribute__((always_inline)) {
+      int x, y;
+      const bool ok = cell_at(u, x, y);
+      if (!ok) x = y = 0;
+      const bool sk = ok && skipped(x, y);
+      in.dv = load_row<S>(cost + ((size_t)y * w + x) * N, j);
+      // neighbour and direction of incoming message d: the up neighbour's
+      // down message, the down neighbour's up, the left's right, the right's left
+      const int nx[4] = {x, x, x > 0 ? x - 1 : x, x < w - 1 ? x + 1 : x};
+      const int ny[4] = {y > 0 ? y - 1 : y, y < h - 1 ? y + 1 : y, y, y};
+      const int md[4] = {1, 0, 3, 2};
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        if (SRC == 2) {
+          zero(in.m[d]);
+        } else {
+          const int cx = sk ? x : nx[d], cy = sk ? y : ny[d], dd = sk ? d : md[d];
+          in.m[d] = load_row<S>(Min + (SRC == 1 ? parent_msgs(cx, cy) : cell_msgs(cx, cy)) + dd * N, j);
+        }
+      }
+    };
+    auto step = [&](int u, const In& in) __attribute__((always_inline)) {
+      int x, y;
+      if (!cell_at(u, x, y)) return;
+      const bool interior = x >= x0 && x < x0 + TW && y >= y0 && y < y0 + R;
+      Row<S> out[4];
+      if (!skipped(x, y)) {
+        Row<S> hv[4];
+        directional_h<S>(in.dv, in.m[0], in.m[1], in.m[2], in.m[3], y > 0, y < h - 1, x > 0, x < w - 1, hv);
+#pragma unroll
+        for (int dir = 0; dir < 4; ++dir) out[dir] = message_from<S>(hv[dir], p1, p2, inv_n);
+        if (interior && write_first) store4(Mwr + cell_msgs(x, y), out);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) out[d] = in.m[d];
+        if (first && interior) {
+          store4(M0 + cell_msgs(x, y), out);
+          store4(M1 + cell_msgs(x, y), out);
+        }
+      }
+      float4* dst = &Ls[u & 3][g][0][0];
+#pragma unroll
+      for (int dir = 0; dir < 4; ++dir)
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          dst[dir * 16 * Q + j * Q + q] =
+              make_float4(out[dir].v[4 * q], out[dir].v[4 * q + 1], out[dir].v[4 * q + 2], out[dir].v[4 * q + 3]);
+    };
+    // inputs are loaded two steps ahead into three register sets, which
+    // take turns: the loop is unrolled by three, without conditions, so that
+    // no set is copied (the steps past R + 2 have no cells)
+    In i0, i1, i2;
+    load_in(0, i0);
+    load_in(1, i1);
+    for (int t = 0; t < T; t += 3) {
+      load_in(t + 2, i2);
+      step(t, i0);
+      __syncthreads();
+      load_in(t + 3, i0);
+      step(t + 1, i1);
+      __syncthreads();
+      load_in(t + 4, i1);
+      step(t + 2, i2);
+      __syncthreads();
+    }
+  } else {
+    // ---------------- B groups: B cell x0 + sh + 2 gb of row y0 + t - 3 ----------------
+    const int gb = grp - GA;
+    auto cell_at = [&](int u, int& x, int& y) __attribute__((always_inline)) {
+      y = y0 + u - 3;
+      x = x0 + ((((x0 + y) & 1) == ca) ? 1 : 0) + 2 * gb;
+      return u >= 3 && y < y0 + R && x < w && gb < GA - 1;
+    };
+    auto load_dv = [&](int u, Row<S>& r) __attribute__((always_inline)) {
+      int x, y;
+      const bool ok = cell_at(u, x, y);
+      r = load_row<S>(cost + (ok ? (size_t)y * w + x : (size_t)0) * N, j);
+    };
+    auto lds_row = [&](int slot, int ga, int dir) __attribute__((always_inline)) {
+      Row<S> r;
+      const float4* src = &Ls[slot][ga][dir][j * Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float4 v = src[q];
+        r.v[4 * q] = v.x;
+        r.v[4 * q + 1] = v.y;
+        r.v[4 * q + 2] = v.z;
+        r.v[4 * q + 3] = v.w;
+      }
+      return r;
+    };
+    auto step = [&](int u, const Row<S>& dv) __attribute__((always_inline)) {
+      int x, y;
+      if (!cell_at(u, x, y)) return;
+      const bool sk = skipped(x, y);
+      if (first && sk) {
+        // the level's first sweep gives a skipped cell its parent's messages
+        Row<S> m[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          if (SRC == 1) m[d] = load_row<S>(P + parent_msgs(x, y) + d * N, j);
+          else zero(m[d]);
+        }
+        store4(M0 + cell_msgs(x, y), m);
+        store4(M1 + cell_msgs(x, y), m);
+      }
+      bool ex = ext != 0;
+      if (ex && !node(x, y)) {
+        if (j == 0) {
+          gp(F.dcoarse)[(size_t)y * w + x] = 0.0f;
+          gp(F.label)[(size_t)y * w + x] = -1;
+        }
+        ex = false;
+      }
+      if (sk && !ex) return;
+      const bool has_up = y > 0, has_down = y < h - 1, has_left = x > 0, has_right = x < w - 1;
+      const int gv = gb + ((((x0 + y) & 1) == ca) ? 1 : 0);  // the A group of the cells above and below
+      Row<S> fu, fd, fl, fr;
+      zero(fu);
+      zero(fd);
+      zero(fl);
+      zero(fr);
+      if (has_up) fu = lds_row((u - 3) & 3, gv, 1);
+      if (has_down) fd = lds_row((u - 1) & 3, gv, 0);
+      if (has_left) fl = lds_row((u - 2) & 3, gb, 3);
+      if (has_right) fr = lds_row((u - 2) & 3, gb + 1, 2);
+      if (ex)
+        extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63,
+                             belief<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right), imin, istep);
+      if (sk) return;
+      Row<S> hv[4], out[4];
+      directional_h<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right, hv);
+#pragma unroll
+      for (int dir = 0; dir < 4; ++dir) out[dir] = message_from<S>(hv[dir], p1, p2, inv_n);
+      store4(Mwr + cell_msgs(x, y), out);
+    };
+    Row<S> d0, d1, d2;
+    load_dv(0, d0);
+    load_dv(1, d1);
+    for (int t = 0; t < T; t += 3) {
+      load_dv(t + 2, d2);
+      step(t, d0);
+      __syncthreads();
+      load_dv(t + 3, d0);
+      step(t + 1, d1);
+      __syncthreads();
+      load_dv(t + 4, d1);
+      step(t + 2, d2);
+      __syncthreads();
+    }
+  }
 }
 
 // Winner extraction over a batch, one 16-lane row per cell (extract_winner).
@@ -313,7 +576,7 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
 // (0: x + y even, 1: odd), the others having been extracted by the last sweep.
 template <int N, int ROWS>
 __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
-                                                       double istep, int colour) {
+                                                       double istep, int colour, int buf) {
   constexpr int S = N / 16;
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
@@ -339,7 +602,7 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
   // not a node runs the extraction too and stores 0 and -1), so their
   // latencies overlap
   const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)(4 * y) * F.W + 4 * x];
-  const gptr<const float> M = gp((const float*)F.msg[0]);
+  const gptr<const float> M = gp((const float*)msg_buf(F, 0, buf));
   const bool hl = x != 0, hr = x != w - 1, hu = y != 0, hd = y != h - 1;
   Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
   const Row<S> tl = load_row<S>(M + (((size_t)(hl ? cell - 1 : cell)) * 4 + 3) * N, j);
@@ -360,7 +623,7 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
 // [cell][dir][row layout] -> [dir][cell][N] in sample order, for callers that
 // want the reference's message maps
 template <int N>
-__global__ __launch_bounds__(256) void k_copy_messages(const FrameDev* __restrict__ frames, float* out) {
+__global__ __launch_bounds__(256) void k_copy_messages(const FrameDev* __restrict__ frames, float* out, int buf) {
   const FrameDev& F = frames[0];
   const int ncell = F.lw[0] * F.lh[0];
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -369,7 +632,7 @@ __global__ __launch_bounds__(256) void k_copy_messages(const FrameDev* __restric
   const size_t t = i / N;
   const int dir = (int)(t % 4);
   const size_t cell = t / 4;
-  out[((size_t)dir * ncell + cell) * N + row_sample<N>(p)] = F.msg[0][i];
+  out[((size_t)dir * ncell + cell) * N + row_sample<N>(p)] = msg_buf(F, 0, buf)[i];
 }
 
 // caller cost volume [cell][N] in sample order -> row layout
@@ -384,14 +647,15 @@ __global__ __launch_bounds__(256) void k_cost_to_rows(const float* __restrict__ 
 
 template <int N, int ROWS>
 static void bp_sweep_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool A,
-                           bool first, bool active, bool copy_all, bool ext, const BpLaunchParams& p) {
+                           bool first, bool active, bool copy_all, bool ext, const BpLaunchParams& p, int buf,
+                           int pbuf) {
   hipLaunchKernelGGL((k_bp_sweep<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, l, (int)A, (int)first,
-                     (int)active, (int)copy_all, p.p1, p.p2, (int)ext, p.idepth_min, p.idepth_step);
+                     (int)active, (int)copy_all, p.p1, p.p2, (int)ext, p.idepth_min, p.idepth_step, buf, pbuf);
 }
 
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
                      bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all,
-                     bool ext) {
+                     bool ext, int buf, int pbuf) {
   const int rows = (t.bp_rows == 4 || t.bp_rows == 8) ? t.bp_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
@@ -402,26 +666,103 @@ void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, in
   }
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
+  const bool e = ext && l == 0 && !first;
   if (N == 64) {
-    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
-    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
-    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    if (rows == 4) bp_sweep_shape<64, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
+    else if (rows == 8) bp_sweep_shape<64, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
+    else bp_sweep_shape<64, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
   } else {
-    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
-    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
-    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, ext && l == 0 && !first, p);
+    if (rows == 4) bp_sweep_shape<128, 4>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
+    else if (rows == 8) bp_sweep_shape<128, 8>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
+    else bp_sweep_shape<128, 16>(nb, s, dframes, bm, l, A, first, active, copy_all, e, p, buf, pbuf);
+  }
+}
+
+template <int N, int GA>
+static void bp_pair_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool first,
+                          int rd, int wr, int pbuf, bool write_first, int seg, bool ext, const BpLaunchParams& p) {
+  const int src = !first ? 0 : l < 3 ? 1 : 2;
+  auto k = src == 0 ? k_bp_pair<N, GA, 0> : src == 1 ? k_bp_pair<N, GA, 1> : k_bp_pair<N, GA, 2>;
+  hipLaunchKernelGGL(k, dim3(nb), dim3(32 * GA), 0, s, dframes, bm, l, 0, rd, wr, pbuf, (int)write_first, seg, p.p1,
+                     p.p2, (int)ext, p.idepth_min, p.idepth_step);
+}
+
+// sweeps it and it + 1 (it even: the first updates the x + y even cells) of level l
+static void launch_bp_pair(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool first, int rd,
+                           int wr, int pbuf, bool write_first, bool ext, const BpLaunchParams& p, const Tuning& t,
+                           hipStream_t s) {
+  const int ga = t.bp_pair == 16 ? 16 : 8;
+  const int seg = (t.bp_seg == 8 || t.bp_seg == 16 || t.bp_seg == 64) ? t.bp_seg : 32;
+  const int tw = 2 * (ga - 1);
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) {
+    const int w = hframes[i].lw[l], h = hframes[i].lh[l];
+    counts[i] = ((w + tw - 1) / tw) * ((h + seg - 1) / seg);
+  }
+  const int nb = make_block_map(bm, counts, n);
+  if (nb <= 0) return;
+  if (N == 64) {
+    if (ga == 16) bp_pair_shape<64, 16>(nb, s, dframes, bm, l, first, rd, wr, pbuf, write_first, seg, ext, p);
+    else bp_pair_shape<64, 8>(nb, s, dframes, bm, l, first, rd, wr, pbuf, write_first, seg, ext, p);
+  } else {
+    if (ga == 16) bp_pair_shape<128, 16>(nb, s, dframes, bm, l, first, rd, wr, pbuf, write_first, seg, ext, p);
+    else bp_pair_shape<128, 8>(nb, s, dframes, bm, l, first, rd, wr, pbuf, write_first, seg, ext, p);
+  }
+}
+
+bool bp_level_pairs(int l, int iters, const Tuning& t) {
+  const int levels = (t.bp_pair_levels >= 1 && t.bp_pair_levels <= 4) ? t.bp_pair_levels : 2;
+  return t.bp_pair != 0 && l < levels && iters >= 2;
+}
+
+int bp_final_buf(int l, int iters, const Tuning& t) {
+  return bp_level_pairs(l, iters, t) ? (iters / 2 - 1) & 1 : 0;
+}
+
+void launch_bp_levels(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const int iters[4],
+                      const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool ext_last, bool want_msgs) {
+  for (int l = 3; l >= 0; --l) {
+    const int it_l = iters[l];
+    const int pbuf = l < 3 ? bp_final_buf(l + 1, iters[l + 1], t) : 0;
+    const bool ext = ext_last && l == 0;
+    if (it_l == 0) {
+      launch_bp_sweep(dframes, hframes, n, N, l, true, true, false, p, t, s, true, false, 0, pbuf);
+      continue;
+    }
+    if (!bp_level_pairs(l, it_l, t)) {
+      for (int it = 0; it < it_l; ++it)
+        launch_bp_sweep(dframes, hframes, n, N, l, (it % 2) == 0, it == 0, true, p, t, s, it_l < 2,
+                        ext && it_l >= 2 && it == it_l - 1, 0, pbuf);
+      continue;
+    }
+    // pairs (0, 1), (2, 3), ...: the first writes buffer 0, each later one
+    // reads the buffer the previous pair wrote and writes the other; an odd
+    // last sweep runs alone, in place. The last pair writes its A cells' messages
+    // unless no later launch reads them (level 0 when its B cells' winners are
+    // extracted there and the A cells' by the extraction stage, which reads only
+    // B messages).
+    int cur = 0;
+    for (int it = 0; it + 1 < it_l; it += 2) {
+      const bool first = it == 0, last = it + 2 == it_l;
+      const int rd = first ? 0 : cur, wr = first ? 0 : cur ^ 1;
+      const bool write_first = last && (l > 0 || want_msgs || !ext);
+      launch_bp_pair(dframes, hframes, n, N, l, first, rd, wr, pbuf, write_first, ext && last, p, t, s);
+      cur = wr;
+    }
+    if (it_l % 2) launch_bp_sweep(dframes, hframes, n, N, l, true, false, true, p, t, s, false, ext, cur, pbuf);
   }
 }
 
 template <int N, int ROWS>
 static void extract_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm,
-                          const BpLaunchParams& p, int colour) {
+                          const BpLaunchParams& p, int colour, int buf) {
   hipLaunchKernelGGL((k_extract<N, ROWS>), dim3(nb), dim3(16 * ROWS), 0, s, dframes, bm, p.idepth_min,
-                     p.idepth_step, colour);
+                     p.idepth_step, colour, buf);
 }
 
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    const Tuning& t, hipStream_t s, int colour) {
+                    const Tuning& t, hipStream_t s, int colour, int buf) {
   const int rows = (t.extract_rows == 4 || t.extract_rows == 8) ? t.extract_rows : 16;
   BlockMap bm;
   int counts[kMaxFrames];
@@ -433,25 +774,25 @@ void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int
   const int nb = make_block_map(bm, counts, n);
   if (nb <= 0) return;
   if (N == 64) {
-    if (rows == 4) extract_shape<64, 4>(nb, s, dframes, bm, p, colour);
-    else if (rows == 8) extract_shape<64, 8>(nb, s, dframes, bm, p, colour);
-    else extract_shape<64, 16>(nb, s, dframes, bm, p, colour);
+    if (rows == 4) extract_shape<64, 4>(nb, s, dframes, bm, p, colour, buf);
+    else if (rows == 8) extract_shape<64, 8>(nb, s, dframes, bm, p, colour, buf);
+    else extract_shape<64, 16>(nb, s, dframes, bm, p, colour, buf);
   } else {
-    if (rows == 4) extract_shape<128, 4>(nb, s, dframes, bm, p, colour);
-    else if (rows == 8) extract_shape<128, 8>(nb, s, dframes, bm, p, colour);
-    else extract_shape<128, 16>(nb, s, dframes, bm, p, colour);
+    if (rows == 4) extract_shape<128, 4>(nb, s, dframes, bm, p, colour, buf);
+    else if (rows == 8) extract_shape<128, 8>(nb, s, dframes, bm, p, colour, buf);
+    else extract_shape<128, 16>(nb, s, dframes, bm, p, colour, buf);
   }
 }
 
 void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out,
-                          hipStream_t s) {
+                          hipStream_t s, int buf) {
   (void)n;
   const size_t total = (size_t)hframes[0].lw[0] * hframes[0].lh[0] * 4 * N;
   const int nb = (int)((total + 255) / 256);
   if (N == 64)
-    hipLaunchKernelGGL(k_copy_messages<64>, dim3(nb), dim3(256), 0, s, dframes, out);
+    hipLaunchKernelGGL(k_copy_messages<64>, dim3(nb), dim3(256), 0, s, dframes, out, buf);
   else
-    hipLaunchKernelGGL(k_copy_messages<128>, dim3(nb), dim3(256), 0, s, dframes, out);
+    hipLaunchKernelGGL(k_copy_messages<128>, dim3(nb), dim3(256), 0, s, dframes, out, buf);
 }
 
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s) {

@@ -261,26 +261,32 @@ BpLaunchParams bp_params(const qm_params* p) {
   return b;
 }
 
-// The coarse-to-fine half-sweeps of the hierarchical BP.
-void run_sweeps(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, const Tuning& t, hipStream_t s) {
-  const int N = p->n_hyp;
-  const BpLaunchParams bp = bp_params(p);
-  for (int l = 3; l >= 0; --l) {
-    const int iters = p->bp_iters[l];
-    if (iters == 0) {
-      launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, t, s);
-      continue;
-    }
-    for (int it = 0; it < iters; ++it) launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, t, s, iters < 2);
-  }
+// The coarse-to-fine half-sweeps of the hierarchical BP (every cell's winner
+// extracted afterwards).
+void run_sweeps(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, const Tuning& t, hipStream_t s,
+                bool want_msgs = false) {
+  launch_bp_levels(df, hf, n, p->n_hyp, p->bp_iters, bp_params(p), t, s, false, want_msgs);
 }
 
-// The whole hierarchical BP: pyramid, coarse-to-fine half-sweeps, extraction.
-void run_bp(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, hipStream_t s) {
+// launches of the BP half-sweep stage
+int bp_launches(const int iters[4], const Tuning& t) {
+  int k = 0;
+  for (int l = 0; l < 4; ++l) {
+    const int it = iters[l];
+    k += it == 0 ? 1 : !bp_level_pairs(l, it, t) ? it : it / 2 + it % 2;
+  }
+  return k;
+}
+
+// The whole hierarchical BP: pyramid, coarse-to-fine half-sweeps, extraction;
+// want_msgs: level 0's final messages are copied out afterwards (returns their buffer).
+int run_bp(const FrameDev* df, const FrameDev* hf, int n, const qm_params* p, hipStream_t s, bool want_msgs) {
   const Tuning t;
   launch_pyramid(df, hf, n, p->n_hyp, s);
-  run_sweeps(df, hf, n, p, t, s);
-  launch_extract(df, hf, n, p->n_hyp, bp_params(p), t, s);
+  run_sweeps(df, hf, n, p, t, s, want_msgs);
+  const int buf = bp_final_buf(0, p->bp_iters[0], t);
+  launch_extract(df, hf, n, p->n_hyp, bp_params(p), t, s, -1, buf);
+  return buf;
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -291,7 +297,7 @@ size_t frame_bytes(const FrameDev& F, int N) {
   size_t b = align256(px);  // levels
   for (int l = 0; l < 4; ++l) {
     const size_t c = (size_t)F.lw[l] * F.lh[l];
-    b += align256(c * N * 4) + align256(c * 4 * N * 4);
+    b += align256(c * N * 4) + 2 * align256(c * 4 * N * 4);  // cost, two message buffers
   }
   const size_t cc = (size_t)F.Wc * F.Hc;
   b += 2 * align256(cc * 4);               // dcoarse, label
@@ -441,27 +447,17 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
       break;
     }
     case QM_STAGE_BP_SWEEP: {
-      int sweeps = 0;
-      for (int l = 0; l < 4; ++l) sweeps += p->bp_iters[l] == 0 ? 1 : p->bp_iters[l];
-      Mark m(t, QM_STAGE_BP_SWEEP, sweeps);
-      for (int l = 3; l >= 0; --l) {
-        const int iters = p->bp_iters[l];
-        if (iters == 0) {
-          launch_bp_sweep(df, hf, n, N, l, true, true, false, bp, tu, s);
-          continue;
-        }
-        for (int it = 0; it < iters; ++it)
-          launch_bp_sweep(df, hf, n, N, l, (it % 2) == 0, it == 0, true, bp, tu, s, iters < 2,
-                          l == 0 && iters >= 2 && it == iters - 1);
-      }
+      Mark m(t, QM_STAGE_BP_SWEEP, bp_launches(p->bp_iters, tu));
+      // the last level-0 sweep (when not the level's first) extracts its colour's cells
+      launch_bp_levels(df, hf, n, N, p->bp_iters, bp, tu, s, true, false);
       break;
     }
     case QM_STAGE_EXTRACT: {
-      // the last level-0 sweep (when not the level's first) has extracted its
-      // colour's cells: colour 0 (x + y even) is the A set, swept at even it
+      // the other colour's cells: colour 0 (x + y even) is the A set, swept at even it
       Mark m(t, QM_STAGE_EXTRACT);
       const int it0 = p->bp_iters[0];
-      launch_extract(df, hf, n, N, bp, tu, s, it0 >= 2 ? ((it0 - 1) % 2 == 0 ? 1 : 0) : -1);
+      launch_extract(df, hf, n, N, bp, tu, s, it0 >= 2 ? ((it0 - 1) % 2 == 0 ? 1 : 0) : -1,
+                     bp_final_buf(0, it0, tu));
       break;
     }
     case QM_STAGE_UPSAMPLE_ROW: {
@@ -728,6 +724,8 @@ int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const 
     void* m;
     if ((rc = sc.get(&m, c * 4 * N * sizeof(float)))) return rc;
     F.msg[l] = (float*)m;
+    if ((rc = sc.get(&m, c * 4 * N * sizeof(float)))) return rc;
+    F.msg2[l] = (float*)m;
     void* cl;
     if ((rc = sc.get(&cl, c * N * sizeof(float)))) return rc;
     F.cost[l] = (float*)cl;
@@ -735,8 +733,8 @@ int qm_bp_extract(const float* cost, int W, int H, const uint8_t* levels, const 
   // the caller's volume is in sample order; the BP kernels read the row layout
   launch_cost_to_rows(cost, F.cost[0], (size_t)F.lw[0] * F.lh[0], N, s);
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
-  run_bp((FrameDev*)df, &F, 1, p, s);
-  if (messages) launch_copy_messages((FrameDev*)df, &F, 1, N, messages, s);
+  const int buf = run_bp((FrameDev*)df, &F, 1, p, s, messages != nullptr);
+  if (messages) launch_copy_messages((FrameDev*)df, &F, 1, N, messages, s, buf);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   sc.done();
@@ -1131,6 +1129,7 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
       const size_t c = (size_t)F.lw[l] * F.lh[l];
       F.cost[l] = cv.take<float>(c * N * 4);
       F.msg[l] = cv.take<float>(c * 4 * N * 4);
+      F.msg2[l] = cv.take<float>(c * 4 * N * 4);
     }
     const size_t cc = (size_t)F.Wc * F.Hc;
     F.dcoarse = cv.take<float>(cc * 4);
@@ -1280,6 +1279,9 @@ static int* tune_slot(Tuning& t, int knob) {
     case QM_TUNE_BP_ROWS: return &t.bp_rows;
     case QM_TUNE_EXTRACT_ROWS: return &t.extract_rows;
     case QM_TUNE_OVERLAP: return &t.overlap;
+    case QM_TUNE_BP_PAIR: return &t.bp_pair;
+    case QM_TUNE_BP_SEG: return &t.bp_seg;
+    case QM_TUNE_BP_PAIR_LEVELS: return &t.bp_pair_levels;
     default: return nullptr;
   }
 }
@@ -1300,6 +1302,9 @@ int qm_batch_set_tuning(qm_batch* b, int knob, int value) {
   if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
   const bool ok = knob == QM_TUNE_COST_WAVES ? (value == 2 || value == 4 || value == 8)
                   : knob == QM_TUNE_OVERLAP  ? (value >= 0 && value <= 7)
+                  : knob == QM_TUNE_BP_PAIR  ? (value == 0 || value == 8 || value == 16)
+                  : knob == QM_TUNE_BP_SEG   ? (value == 8 || value == 16 || value == 32 || value == 64)
+                  : knob == QM_TUNE_BP_PAIR_LEVELS ? (value >= 1 && value <= 4)
                                              : (value == 4 || value == 8 || value == 16);
   if (!ok) return fail(QM_ERR_INVALID, "value not offered for this knob");
   if (*slot != value) {
@@ -1349,9 +1354,16 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     std::vector<int> values;
     std::function<void(const Tuning&)> stage;
   };
+  // the single half-sweep's shape is timed with the pairs off (it runs alone
+  // there, at the levels without pairs, and for a level's odd last sweep)
+  auto single_sweeps = [&](const Tuning& t) {
+    Tuning u = t;
+    u.bp_pair = 0;
+    run_sweeps(df, hf, n, &b->p, u, s);
+  };
   const Knob knobs[3] = {
       {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, t, s, true); }},
-      {QM_TUNE_BP_ROWS, {4, 8, 16}, [&](const Tuning& t) { run_sweeps(df, hf, n, &b->p, t, s); }},
+      {QM_TUNE_BP_ROWS, {4, 8, 16}, single_sweeps},
       {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
   };
   int rc = QM_OK;
@@ -1370,6 +1382,35 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     }
     if (rc) break;
     if (best_ms) best_ms[k.id] = best_t;
+  }
+  // the fused pairs' three knobs depend on each other: every combination is
+  // timed over the whole sweep schedule (with the pairs off, the other two
+  // keep their values)
+  if (!rc) {
+    double best_t = 1e30;
+    const Tuning base = best;
+    for (int pair : {0, 8, 16}) {
+      for (int seg : {8, 16, 32, 64}) {
+        for (int lv = 1; lv <= 4; ++lv) {
+          if (pair == 0 && (seg != base.bp_seg || lv != base.bp_pair_levels)) continue;
+          Tuning t = base;
+          t.bp_pair = pair;
+          t.bp_seg = seg;
+          t.bp_pair_levels = lv;
+          double ms = 0.0;
+          if ((rc = time([&](const Tuning& u) { run_sweeps(df, hf, n, &b->p, u, s); }, t, &ms))) break;
+          if (ms < best_t) {
+            best_t = ms;
+            best.bp_pair = pair;
+            best.bp_seg = seg;
+            best.bp_pair_levels = lv;
+          }
+        }
+        if (rc) break;
+      }
+      if (rc) break;
+    }
+    if (best_ms) best_ms[QM_TUNE_BP_PAIR] = best_ms[QM_TUNE_BP_SEG] = best_ms[QM_TUNE_BP_PAIR_LEVELS] = best_t;
   }
   // the overlap knob: whole pipeline runs (plain launches), which move the
   // fusion state, so both seed buffers of every frame are saved first and

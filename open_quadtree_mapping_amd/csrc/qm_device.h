@@ -31,6 +31,7 @@ struct FrameDev {
   uint8_t* levels;      // H x W
   float* cost[4];       // lh*lw*N per level
   float* msg[4];        // lh*lw*4*N per level, [cell][dir][N], dir = up, down, left, right
+  float* msg2[4];       // a second message buffer per level (the fused sweep pairs read one and write the other)
   float* dcoarse;       // Hc x Wc
   int32_t* label;       // Hc x Wc
   int32_t* count;       // Hc x Wc or null

@@ -27,13 +27,18 @@ struct BpLaunchParams {
 
 // Launch shapes the autotuner chooses among (every choice gives identical
 // results): waves per cost-volume block (each wave takes N / cost_waves
-// samples of the block's 64 cells), and cells (16-lane rows) per block of the
-// BP sweep and of the extraction.
+// samples of the block's 64 cells), cells (16-lane rows) per block of the
+// single BP half-sweep and of the extraction, and the fused half-sweep pairs:
+// off (0) or A cells per strip row (8 or 16: strips of 14 or 30 columns), rows
+// per strip segment, and how many of the finest levels use them.
 struct Tuning {
   int cost_waves = 4;    // 2, 4, 8
   int bp_rows = 16;      // 4, 8, 16
   int extract_rows = 16;  // 4, 8, 16
   int overlap = 0;       // 0 one stream; 1-7 two to eight frame groups on their own streams (run_pipeline)
+  int bp_pair = 8;       // 0, 8, 16
+  int bp_seg = 8;        // 8, 16, 32, 64
+  int bp_pair_levels = 2;  // 1-4: levels 0 .. bp_pair_levels - 1
 };
 
 void launch_quadtree(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
@@ -50,13 +55,28 @@ void launch_pyramid(const FrameDev* dframes, const FrameDev* hframes, int n, int
 // active = 0 turns the first sweep into a pure initialisation.
 // ext: this level-0 (non-first) sweep also extracts the winners of its colour's
 // cells (every one of them, updated or skipped), for the sweep that ends BP.
+// buf: the level's message buffer (read and written in place); pbuf: the
+// parent level's (read by a first sweep).
 void launch_bp_sweep(const FrameDev* dframes, const FrameDev* hframes, int n, int N, int l, bool A, bool first,
                      bool active, const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool copy_all = true,
-                     bool ext = false);
-// colour < 0: every cell; 0 / 1: only the cells with x + y even / odd
+                     bool ext = false, int buf = 0, int pbuf = 0);
+// Every half-sweep of the hierarchical BP, coarse to fine (iters[l] half-sweeps
+// at level l; a level without any only initialises its messages), as single
+// half-sweeps or fused pairs (t.bp_pair). ext_last: the last level-0 sweep
+// extracts its colour's winners (the extraction stage then does the other
+// colour's); want_msgs: level 0's final messages of both colours are read
+// afterwards (copied out).
+void launch_bp_levels(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const int iters[4],
+                      const BpLaunchParams& p, const Tuning& t, hipStream_t s, bool ext_last, bool want_msgs);
+// level l's half-sweeps run as fused pairs; the message buffer holding its
+// final messages after its iters half-sweeps
+bool bp_level_pairs(int l, int iters, const Tuning& t);
+int bp_final_buf(int l, int iters, const Tuning& t);
+// colour < 0: every cell; 0 / 1: only the cells with x + y even / odd; buf: level 0's final message buffer
 void launch_extract(const FrameDev* dframes, const FrameDev* hframes, int n, int N, const BpLaunchParams& p,
-                    const Tuning& t, hipStream_t s, int colour = -1);
-void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s);
+                    const Tuning& t, hipStream_t s, int colour = -1, int buf = 0);
+void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int n, int N, float* out, hipStream_t s,
+                          int buf = 0);
 // a cost volume in sample order -> the row layout of the pipeline
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);
 // upsample: mode 0 = compose the sparse map from BP (+ pre-fill), weights from

@@ -120,7 +120,11 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
 
     b = Batch(frames, p, outputs=("sparse", "dense"))
     want = run(b)
-    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16), "overlap": tuple(range(8))}
+    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16), "overlap": tuple(range(8)),
+               "bp_pair": (0, 8, 16), "bp_seg": (8, 16, 32, 64), "bp_pair_levels": (1, 2, 3, 4)}
+    defaults = {"cost_waves": 4, "bp_rows": 16, "extract_rows": 16, "overlap": 0, "bp_pair": 8, "bp_seg": 8,
+                "bp_pair_levels": 2}
+    assert b.tuning() == defaults
     for knob, values in offered.items():
         for v in values:
             print(f"launch shape {knob}={v}", flush=True)
@@ -128,7 +132,7 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
             got = run(b)
             for (gs, gd), (ws, wd) in zip(got, want):
                 assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True), f"{knob}={v}"
-        b.set_tuning(**{knob: {"cost_waves": 4, "overlap": 0}.get(knob, 16)})
+        b.set_tuning(**{knob: defaults[knob]})
     res = b.autotune(reps=3)
     print(f"autotune: {res}", flush=True)
     for knob, (v, ms) in res.items():

@@ -138,6 +138,27 @@ def test_bp_extract(W, H, seed, N):
     assert np.nanmax(rel_err(gd[rl >= 0], rd[rl >= 0])) <= REL
 
 
+# half-sweep counts per level (fine -> coarse) that leave odd last sweeps, a
+# level with one sweep (no pair) or none (initialisation only), and pairs only
+@pytest.mark.parametrize("iters", [(3, 5, 2, 1), (2, 2, 2, 2), (1, 3, 0, 2), (5, 1, 3, 4), (0, 0, 0, 0), (6, 7, 1, 0)])
+def test_bp_schedules_match_oracle(iters):
+    W, H, N = 376, 240, 64
+    ks = keyframe_set(W, H, 7)
+    print(f"bp schedule {iters}", flush=True)
+    imin, imax = 0.02, 2.0
+    levels = O.quadtree(ks["key"])
+    rc, _ = O.cost_volume(ks["key"], ks["refs"], ks["T"], ks["cam"], N, imin, imax, levels)
+    p = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax)
+    for l in range(4):
+        p.bp_iters[l] = iters[l]
+    d, lab, msg = S.bp_extract(dev(rc), W, H, dev(levels), p, want_messages=True)
+    rd, rl, rm = O.bp_extract(rc, W, H, levels, N, imin, imax, want_messages=True, iters=iters)
+    assert np.array_equal(host(msg), rm), "messages differ"
+    assert np.array_equal(host(lab), rl)
+    gd = host(d)
+    assert np.nanmax(rel_err(gd[rl >= 0], rd[rl >= 0])) <= REL
+
+
 def test_bp_uniform_cost_gives_label_zero():
     # uniform data cost: all messages are 0 and the argmin tree keeps index 0
     W, H, N = 128, 96, 64
