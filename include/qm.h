@@ -69,7 +69,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 10
+#define QM_ABI_VERSION 11
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -160,6 +160,10 @@ typedef struct qm_upsample_problem {
   float* grad_w_row;
   float* grad_w_col;
   double* grad_lambda;         /* backward output dL/dlambda: one double in device memory, or NULL */
+  float* state;                /* optional, 4 x H x W floats in device memory: the forward keeps its row
+                                  and column pass results there, and a backward given the state its
+                                  forward wrote (same inputs) skips recomputing them; every problem of a
+                                  backward batch has one, or none does */
 } qm_upsample_problem;
 int qm_upsample_general_batch(const qm_upsample_problem* problems, int n, void* stream);
 int qm_upsample_backward_batch(const qm_upsample_problem* problems, int n, void* stream);

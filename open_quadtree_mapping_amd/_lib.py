@@ -84,10 +84,11 @@ class qm_upsample_problem(C.Structure):
         ("grad_w_row", C.c_void_p),
         ("grad_w_col", C.c_void_p),
         ("grad_lambda", C.c_void_p),
+        ("state", C.c_void_p),
     ]
 
 
-ABI_VERSION = 10  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 11  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {

@@ -1,9 +1,9 @@
 """Differentiable upsample and fusion: torch.autograd.Functions over the C ABI.
 
 UpsampleFunction: out = x_col / n_col of the generalised upsample
-  (qm_upsample_general); backward = qm_upsample_backward, which recomputes the
-  forward sweep and runs the transposed (= same, A is symmetric) tridiagonal
-  solves. Gradients w.r.t. sparse depths, confidences, per-pixel smoothness
+  (qm_upsample_general); backward = qm_upsample_backward, which runs the
+  transposed (= same, A is symmetric) tridiagonal solves on the forward's row
+  and column results (kept from the forward: 16 bytes per pixel). Gradients w.r.t. sparse depths, confidences, per-pixel smoothness
   weights, and the smoothness scale lambda when it is given as a tensor
   (dL/dlambda = sum_e w_e dL/d(lambda w_e), since A = I + lambda L(w), summed on
   the device in a fixed order; such a lambda must be > 0).
@@ -35,21 +35,30 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _problems(d, conf, w_row, w_col, lams, row_seg, col_seg):
-    """qm_upsample_problem array for a (B, H, W) batch (B = 1 for 2-D inputs)."""
+_PROBLEM = np.dtype(qm_upsample_problem)
+
+
+def _rows(t, B):
+    """Device addresses of the B leading-dimension slices of t (0 for None)."""
+    if t is None:
+        return 0
+    return t.data_ptr() + np.arange(B, dtype=np.uint64) * np.uint64(t.stride(0) * t.element_size())
+
+
+def _problems(d, conf, w_row, w_col, lams, row_seg, col_seg, **ptrs):
+    """qm_upsample_problem array for a (B, H, W) batch (B = 1 for 2-D inputs),
+    as a numpy record array laid out as the C struct; ptrs: further pointer
+    fields from (B, ...) tensors (or None)."""
     B, H, W = d.shape
-    probs = (qm_upsample_problem * B)()
-    for b in range(B):
-        q = probs[b]
-        q.W, q.H = W, H
-        q.row_seg = W if row_seg is None else int(row_seg)
-        q.col_seg = H if col_seg is None else int(col_seg)
-        q.lambda_ = float(lams[b])
-        q.d = d[b].data_ptr()
-        q.conf = None if conf is None else conf[b].data_ptr()
-        q.w_row = w_row[b].data_ptr()
-        q.w_col = w_col[b].data_ptr()
-    return probs
+    a = np.zeros(B, _PROBLEM)
+    a["W"], a["H"] = W, H
+    a["row_seg"] = W if row_seg is None else int(row_seg)
+    a["col_seg"] = H if col_seg is None else int(col_seg)
+    a["lambda_"] = np.asarray(lams, np.float64)
+    a["d"], a["conf"], a["w_row"], a["w_col"] = _rows(d, B), _rows(conf, B), _rows(w_row, B), _rows(w_col, B)
+    for k, t in ptrs.items():
+        a[k] = _rows(t, B)
+    return a
 
 
 class UpsampleFunction(torch.autograd.Function):
@@ -77,11 +86,11 @@ class UpsampleFunction(torch.autograd.Function):
         if ctx.needs_input_grad[4] and not all(v > 0.0 for v in lams):
             raise ValueError("a differentiable lam must be > 0")
         out = device_empty(d_.shape, d_.dtype, d_.device)
-        probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg)
-        for b in range(B):
-            probs[b].dense = out[b].data_ptr()
-        check(lib().qm_upsample_general_batch(probs, B, _stream()))
-        ctx.save_for_backward(d_, w_row_, w_col_, *([conf_] if conf_ is not None else []))
+        # the row and column pass results, kept for the backward
+        state = device_empty((B, 4) + tuple(d_.shape[1:]), torch.float32, d_.device)
+        probs = _problems(d_, conf_, w_row_, w_col_, lams, row_seg, col_seg, dense=out, state=state)
+        check(lib().qm_upsample_general_batch(probs.ctypes.data, B, _stream()))
+        ctx.save_for_backward(d_, w_row_, w_col_, state, *([conf_] if conf_ is not None else []))
         ctx.has_conf = conf_ is not None
         ctx.lams = lams
         ctx.lam_shape = tuple(lam.shape) if torch.is_tensor(lam) else None
@@ -92,8 +101,8 @@ class UpsampleFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         saved = ctx.saved_tensors
-        d, w_row, w_col = saved[:3]
-        conf = saved[3] if ctx.has_conf else None
+        d, w_row, w_col, state = saved[:4]
+        conf = saved[4] if ctx.has_conf else None
         B = d.shape[0]
         g = (g if ctx.batched else g[None]).contiguous()
         gd = device_empty(d.shape, d.dtype, d.device) if ctx.needs_input_grad[0] else None
@@ -101,18 +110,11 @@ class UpsampleFunction(torch.autograd.Function):
         gwr = device_empty(w_row.shape, w_row.dtype, w_row.device) if ctx.needs_input_grad[2] else None
         gwc = device_empty(w_col.shape, w_col.dtype, w_col.device) if ctx.needs_input_grad[3] else None
         want_lam = ctx.lam_shape is not None and ctx.needs_input_grad[4]
+        # dL/dlambda = sum_e w_e dL/d(lambda w_e), summed on the device
         per = device_empty((B,), torch.float64, d.device) if want_lam else None
-        probs = _problems(d, conf, w_row, w_col, ctx.lams, *ctx.segs)
-        for b in range(B):
-            q = probs[b]
-            q.g = g[b].data_ptr()
-            q.grad_d = None if gd is None else gd[b].data_ptr()
-            q.grad_conf = None if gc is None else gc[b].data_ptr()
-            q.grad_w_row = None if gwr is None else gwr[b].data_ptr()
-            q.grad_w_col = None if gwc is None else gwc[b].data_ptr()
-            # dL/dlambda = sum_e w_e dL/d(lambda w_e), summed on the device
-            q.grad_lambda = None if per is None else per[b:].data_ptr()
-        check(lib().qm_upsample_backward_batch(probs, B, _stream()))
+        probs = _problems(d, conf, w_row, w_col, ctx.lams, *ctx.segs, g=g, grad_d=gd, grad_conf=gc, grad_w_row=gwr,
+                          grad_w_col=gwc, grad_lambda=per, state=state)
+        check(lib().qm_upsample_backward_batch(probs.ctypes.data, B, _stream()))
         glam = None
         if want_lam:
             one_per_problem = len(ctx.lam_shape) == 1 and ctx.lam_shape[0] == B

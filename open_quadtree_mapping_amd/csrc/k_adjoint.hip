@@ -122,13 +122,16 @@ __global__ __launch_bounds__(256) void k_lambda_reduce(const FrameDev* __restric
   if (threadIdx.x == 0) *F.grad_lambda = red[0];
 }
 
-void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
-  // recompute the forward passes (row pass -> rowx planes, column pass ->
-  // colxn), then the adjoint solves (the same wave pipeline, mode 3): columns
-  // (-> adjxn, dL/dw_col), then rows (-> dL/dd, dL/dconf, dL/dw_row), each
-  // with its per-line lambda partials
-  launch_upsample_row(dframes, hframes, n, 2, nullptr, s);
-  launch_upsample_col(dframes, hframes, n, 2, nullptr, false, s);
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s,
+                              bool recompute) {
+  // recompute the forward passes unless the forward kept them (row pass ->
+  // rowx planes, column pass -> colxn), then the adjoint solves (the same wave
+  // pipeline, mode 3): columns (-> adjxn, dL/dw_col), then rows (-> dL/dd,
+  // dL/dconf, dL/dw_row), each with its per-line lambda partials
+  if (recompute) {
+    launch_upsample_row(dframes, hframes, n, 2, nullptr, s);
+    launch_upsample_col(dframes, hframes, n, 2, nullptr, false, s);
+  }
   launch_upsample_col(dframes, hframes, n, 3, nullptr, false, s);
   launch_upsample_row(dframes, hframes, n, 3, nullptr, s);
   BlockMap bm;

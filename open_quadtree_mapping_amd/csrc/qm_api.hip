@@ -357,6 +357,10 @@ struct qm_batch {
     double* lam_part;  // H + W per-line partials of dL/dlambda
   };
   std::vector<Bwd> bwd;
+  // once the backward buffers exist, every run's column pass also keeps its
+  // results (colxn) there: a backward of a run after run number colxn_since
+  // reuses them and the run's row planes instead of recomputing the forward
+  long long colxn_since = -1;
 };
 
 namespace {
@@ -804,11 +808,18 @@ int qm_upsample_general_batch(const qm_upsample_problem* probs, int n, void* str
     F.row_n_out = q.row_n;
     F.dense = q.dense;
     const size_t px = (size_t)q.W * q.H;
-    void *fwd, *rxn;
+    void* fwd;
     if ((rc = sc.get(&fwd, px * 16))) return rc;
-    if ((rc = sc.get(&rxn, px * 8))) return rc;
     F.fwd = (float4*)fwd;
-    F.rowx = (float*)rxn;
+    if (q.state) {
+      // the row pass's X and N planes, then the column pass's (x, n) pairs
+      F.rowx = q.state;
+      F.colxn = (float2*)(q.state + 2 * px);
+    } else {
+      void* rxn;
+      if ((rc = sc.get(&rxn, px * 8))) return rc;
+      F.rowx = (float*)rxn;
+    }
   }
   QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
   launch_upsample((FrameDev*)df, hf.data(), n, 2, nullptr, false, s);
@@ -827,7 +838,10 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
     if (!q.d || !q.w_row || !q.w_col || !q.g) return fail(QM_ERR_INVALID, "NULL pointer");
     if (q.row_seg < 0 || q.col_seg < 0) return fail(QM_ERR_INVALID, "negative segment");
     if (!(q.lambda >= 0.0)) return fail(QM_ERR_INVALID, "lambda must be >= 0");
+    if ((q.state == nullptr) != (probs[0].state == nullptr))
+      return fail(QM_ERR_INVALID, "every problem of a backward batch has a forward state, or none does");
   }
+  const bool saved = probs[0].state != nullptr;
   hipStream_t s = (hipStream_t)stream;
   std::vector<FrameDev> hf(n);
   Scratch sc;
@@ -855,21 +869,27 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
       F.grad_lambda = q.grad_lambda;
     }
     const size_t px = (size_t)q.W * q.H;
-    void *fwd, *rxn, *cxn, *axn, *dense;
+    void *fwd, *axn;
     if ((rc = sc.get(&fwd, px * 16))) return rc;
-    if ((rc = sc.get(&rxn, px * 8))) return rc;
-    if ((rc = sc.get(&cxn, px * 8))) return rc;
     if ((rc = sc.get(&axn, px * 8))) return rc;
-    if ((rc = sc.get(&dense, px * 4))) return rc;
     F.fwd = (float4*)fwd;
-    F.rowx = (float*)rxn;
-    F.colxn = (float2*)cxn;
     F.adjxn = (float2*)axn;
-    F.dense = (float*)dense;
+    if (saved) {
+      F.rowx = q.state;
+      F.colxn = (float2*)(q.state + 2 * px);
+    } else {
+      void *rxn, *cxn, *dense;
+      if ((rc = sc.get(&rxn, px * 8))) return rc;
+      if ((rc = sc.get(&cxn, px * 8))) return rc;
+      if ((rc = sc.get(&dense, px * 4))) return rc;
+      F.rowx = (float*)rxn;
+      F.colxn = (float2*)cxn;
+      F.dense = (float*)dense;
+    }
     F.grad_wcol = q.grad_w_col;
   }
   QM_HIP(hipMemcpyAsync(df, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
-  launch_upsample_backward((FrameDev*)df, hf.data(), n, s);
+  launch_upsample_backward((FrameDev*)df, hf.data(), n, s, !saved);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   sc.done();
@@ -1588,15 +1608,24 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
     F.row_x_out = nullptr;
     F.row_n_out = nullptr;
   }
+  const bool kept = b->colxn_since >= 0 && b->runs > b->colxn_since;
   QM_HIP(hipMemcpyAsync(b->dfb, hb.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
-  // fusion adjoint first: it reads the forward's dense depth, which the
+  // fusion adjoint first: it reads the forward's dense depth, which an
   // upsample recomputation then rewrites with the same values
   launch_fuse_backward(b->dfb, hb.data(), n, s);
   launch_compose_inputs(b->dfb, hb.data(), n, b->wlut, s);
-  launch_upsample_backward(b->dfb, hb.data(), n, s);
+  launch_upsample_backward(b->dfb, hb.data(), n, s, !kept);
   QM_HIP(hipGetLastError());
   QM_HIP(hipMemcpyAsync(grad_lambda, b->partial, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   QM_HIP(hipStreamSynchronize(s));
+  if (b->colxn_since < 0) {
+    // later runs keep their column results for their backward (a few MB of
+    // writes per run)
+    for (int par = 0; par < 2; ++par)
+      for (int i = 0; i < n; ++i) b->hf[par][i].colxn = b->bwd[i].colxn;
+    if ((rc = upload_frames(b))) return rc;
+    b->colxn_since = b->runs;
+  }
   return QM_OK;
 }
 

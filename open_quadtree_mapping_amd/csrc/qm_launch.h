@@ -99,9 +99,11 @@ void launch_upsample_local(const float* sparse, const uint8_t* img, int W, int H
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
-// adjoints: recompute the generalised upsample forward, then the column and row
-// adjoint solves; fusion update adjoint w.r.t. its dense depth input
-void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+// adjoints: the generalised upsample forward recomputed (recompute; else its
+// rowx planes and colxn are the forward's), then the column and row adjoint
+// solves; fusion update adjoint w.r.t. its dense depth input
+void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s,
+                              bool recompute = true);
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // pipeline backward: the keyframe upsample's inputs written as explicit arrays
 // into each descriptor's g_d / g_wrow / g_wcol

@@ -230,3 +230,33 @@ def test_differentiable_lambda_must_be_positive():
     # lambda = 0 without a gradient is a plain forward (A = I)
     out = QA.upsample(t(d), t(wr), t(wc), conf=t(conf), lam=0.0)
     assert torch.allclose(out, t(d), rtol=1e-6, atol=0)
+
+
+def test_upsample_backward_on_kept_state_equals_recomputed():
+    """The backward given the forward's kept row and column results gives the
+    gradients of the backward that recomputes them, bit for bit (and the
+    forward's output is the same with or without a state to keep)."""
+    from open_quadtree_mapping_amd._lib import check, lib
+    B, H, W = 3, 96, 136
+    probs = [problem(H, W, 60 + b) for b in range(B)]
+    t = lambda k: torch.tensor(np.stack([p[k] for p in probs]), dtype=torch.float32, device=DEV)  # noqa: E731
+    d, conf, wr, wc, g = t(0), t(1), t(2), t(3), t(4)
+    lams = [7.0, 10.0, 13.0]
+    stream = torch.cuda.current_stream().cuda_stream
+    state = torch.empty((B, 4, H, W), dtype=torch.float32, device=DEV)
+    outs, grads = [], []
+    for keep in (False, True):
+        out = torch.empty_like(d)
+        a = QA._problems(d, conf, wr, wc, lams, 128, 88, dense=out, state=state if keep else None)
+        check(lib().qm_upsample_general_batch(a.ctypes.data, B, stream))
+        gs = [torch.empty_like(d), torch.empty_like(d), torch.empty_like(wr), torch.empty_like(wc),
+              torch.zeros(B, dtype=torch.float64, device=DEV)]
+        a = QA._problems(d, conf, wr, wc, lams, 128, 88, g=g, grad_d=gs[0], grad_conf=gs[1], grad_w_row=gs[2],
+                         grad_w_col=gs[3], grad_lambda=gs[4], state=state if keep else None)
+        check(lib().qm_upsample_backward_batch(a.ctypes.data, B, stream))
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+        grads.append([x.cpu().numpy() for x in gs])
+    assert np.array_equal(outs[0], outs[1], equal_nan=True)
+    for k, (x, y) in enumerate(zip(*grads)):
+        assert np.array_equal(x, y), f"gradient {k}"

@@ -117,6 +117,25 @@ def test_batch_backward_refuses_a_changed_batch():
     batch.close()
 
 
+def test_batch_backward_on_kept_forward_equals_recomputed():
+    """The first backward of a batch recomputes the upsample forward; later
+    runs keep their column results for their backward, which then reuses them
+    and the run's row results: the same run state gives the same gradient, bit
+    for bit."""
+    batch, sets, frames = _small_batch()
+    rng = np.random.default_rng(4)
+    g = [torch.tensor(rng.standard_normal((240, 376)).astype(np.float32), device="cuda") for _ in range(2)]
+    got = []
+    for graph in (False, False, True):
+        batch.reset_seeds()
+        for _ in range(3):
+            batch.run(graph=graph)
+        got.append(np.array(batch.backward(g)))
+    assert np.all(np.isfinite(got[0]))
+    assert np.array_equal(got[0], got[1]) and np.array_equal(got[0], got[2])
+    batch.close()
+
+
 @pytest.mark.parametrize("lam", [0.0, 1e-3])
 def test_batch_lambda_gradient_at_small_lambda(lam):
     """params.lambda = 0 or 1e-3. The lambda gradient is formed as
