@@ -149,49 +149,60 @@ void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, 
 // forward pass used: row edge x of row y is 0 at x = 0, edge_r at x = W and
 // lut(|I(x) - I(x-1)|) inside; column edges likewise.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_compose_inputs(const FrameDev* __restrict__ frames, BlockMap bm,
-                                                        const float* __restrict__ wlut) {
+// One block per (row y in [0, H], 256 columns): a thread forms 4 consecutive
+// columns (row edges x in [0, W], column edges of row y, sparse values);
+// want_sparse = 0 leaves the sparse map out (an adjoint that neither
+// recomputes the forward nor returns dL/dd or dL/dconf never reads it).
+__global__ __launch_bounds__(64) void k_compose_inputs(const FrameDev* __restrict__ frames, BlockMap bm,
+                                                       const float* __restrict__ wlut, int want_sparse) {
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
-  const int i = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
   const int W = F.W, H = F.H;
-  if (i >= (W + 1) * (H + 1)) return;
-  const int y = i / (W + 1), x = i - y * (W + 1);  // x in [0, W], y in [0, H]
-  const uint8_t* key = F.key;
+  const int per_row = (W + 256) / 256;  // blocks per row: columns 0 .. W
+  const int b = blockIdx.x - bm.start[f];
+  const int y = b / per_row, x0 = (b - y * per_row) * 256 + 4 * (int)threadIdx.x;
+  const gptr<const uint8_t> key = gp(F.key);
   // the generalised inputs of this frame's descriptor are the scratch arrays
   // being filled here
-  float* wrow = const_cast<float*>(F.g_wrow);
-  float* wcol = const_cast<float*>(F.g_wcol);
-  float* sparse = const_cast<float*>(F.g_d);
-  if (y < H) {
-    float w = 0.0f;
-    if (x == W) w = F.edge_r ? 1.0f : 0.0f;
-    else if (x > 0) w = wlut[abs((int)key[(size_t)y * F.pitch + x] - (int)key[(size_t)y * F.pitch + x - 1])];
-    wrow[(size_t)y * (W + 1) + x] = w;
-  }
-  if (x < W) {
-    float w = 0.0f;
-    if (y == H) w = F.edge_c ? 1.0f : 0.0f;
-    else if (y > 0) w = wlut[abs((int)key[(size_t)y * F.pitch + x] - (int)key[(size_t)(y - 1) * F.pitch + x])];
-    wcol[(size_t)y * W + x] = w;
-  }
-  if (x < W && y < H) {
-    float v = F.prefill ? F.prefill[(size_t)y * W + x] : 0.0f;
-    if ((x & 3) == 0 && (y & 3) == 0 && (x >> 2) < F.Wc && (y >> 2) < F.Hc) {
-      const size_t c = (size_t)(y >> 2) * F.Wc + (x >> 2);
-      if (F.label[c] >= 0) v = F.dcoarse[c];
+  const gptr<float> wrow = gp(const_cast<float*>(F.g_wrow));
+  const gptr<float> wcol = gp(const_cast<float*>(F.g_wcol));
+  const gptr<float> sparse = gp(const_cast<float*>(F.g_d));
+  const gptr<const float> prefill = gp(F.prefill);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int x = x0 + k;
+    if (x > W) break;
+    const int cur = x < W && y < H ? (int)key[(size_t)y * F.pitch + x] : 0;
+    if (y < H) {
+      float w = 0.0f;
+      if (x == W) w = F.edge_r ? 1.0f : 0.0f;
+      else if (x > 0) w = wlut[abs(cur - (int)key[(size_t)y * F.pitch + x - 1])];
+      wrow[(size_t)y * (W + 1) + x] = w;
     }
-    sparse[(size_t)y * W + x] = v;
+    if (x < W) {
+      float w = 0.0f;
+      if (y == H) w = F.edge_c ? 1.0f : 0.0f;
+      else if (y > 0) w = wlut[abs(cur - (int)key[(size_t)(y - 1) * F.pitch + x])];
+      wcol[(size_t)y * W + x] = w;
+    }
+    if (want_sparse && x < W && y < H) {
+      float v = prefill ? prefill[(size_t)y * W + x] : 0.0f;
+      if ((x & 3) == 0 && (y & 3) == 0 && (x >> 2) < F.Wc && (y >> 2) < F.Hc) {
+        const size_t c = (size_t)(y >> 2) * F.Wc + (x >> 2);
+        if (gp(F.label)[c] >= 0) v = gp(F.dcoarse)[c];
+      }
+      sparse[(size_t)y * W + x] = v;
+    }
   }
 }
 
 void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
-                           hipStream_t s) {
+                           hipStream_t s, bool want_sparse) {
   BlockMap bm;
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = ((hframes[i].W + 1) * (hframes[i].H + 1) + 255) / 256;
+  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 1) * ((hframes[i].W + 256) / 256);
   const int nb = make_block_map(bm, counts, n);
-  if (nb > 0) hipLaunchKernelGGL(k_compose_inputs, dim3(nb), dim3(256), 0, s, dframes, bm, wlut);
+  if (nb > 0) hipLaunchKernelGGL(k_compose_inputs, dim3(nb), dim3(64), 0, s, dframes, bm, wlut, (int)want_sparse);
 }
 
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {

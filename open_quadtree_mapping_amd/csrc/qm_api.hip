@@ -1613,7 +1613,8 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
   // fusion adjoint first: it reads the forward's dense depth, which an
   // upsample recomputation then rewrites with the same values
   launch_fuse_backward(b->dfb, hb.data(), n, s);
-  launch_compose_inputs(b->dfb, hb.data(), n, b->wlut, s);
+  // the sparse map is read only by a recomputed forward (the adjoint wants no dL/dd here)
+  launch_compose_inputs(b->dfb, hb.data(), n, b->wlut, s, !kept);
   launch_upsample_backward(b->dfb, hb.data(), n, s, !kept);
   QM_HIP(hipGetLastError());
   QM_HIP(hipMemcpyAsync(grad_lambda, b->partial, sizeof(double) * n, hipMemcpyDeviceToHost, s));

@@ -106,8 +106,8 @@ void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, 
                               bool recompute = true);
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // pipeline backward: the keyframe upsample's inputs written as explicit arrays
-// into each descriptor's g_d / g_wrow / g_wcol
+// into each descriptor's g_d (when want_sparse) / g_wrow / g_wcol
 void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
-                           hipStream_t s);
+                           hipStream_t s, bool want_sparse = true);
 
 }  // namespace qm
