@@ -48,6 +48,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tune-cache", default=os.environ.get(
+        "QM_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "open_quadtree_mapping_amd",
+                                      "launch_shapes.json")),
+                    help="JSON file of autotuned launch shapes: cached choices for this device and batch are "
+                         "reused, new ones added ('' for none)")
+    ap.add_argument("--retune", action="store_true", help="autotune even when the cache holds this batch's shapes")
     ap.add_argument("--tune-reps", type=int, default=5)
     ap.add_argument("--overlap", type=int, default=None,
                     help="force the frame-group overlap setting (0-7) after autotuning, e.g. for a profiled run")
@@ -267,6 +273,13 @@ def run_workload(args):
                             cam=(cam.fx, cam.fy, cam.cx, cam.cy)))
         return out
 
+    def shapes_of(batch):
+        out = {f"{W}x{H}": batch.group_tuning(g) for g, (W, H) in enumerate(batch.groups())}
+        out["overlap"] = batch.tuning()["overlap"]
+        return out
+
+    shapes = None
+
     def timed(step):
         for _ in range(args.warmup):
             step()
@@ -283,7 +296,8 @@ def run_workload(args):
         frames = keyframes(376, 240, 4, 300) + keyframes(752, 480, 4, 310) + keyframes(1280, 720, 4, 320)
         batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, cost_arith=arith),
                       outputs=("depth",))
-        batch.autotune(reps=args.tune_reps)
+        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
+        shapes = shapes_of(batch)
         sec = timed(lambda: batch.run(graph=True))
         n, unit, what = 12, "depth maps/s", "batched graph replay of 12 uneven streams, N = 64"
         batch.close()
@@ -292,7 +306,8 @@ def run_workload(args):
                   for f in keyframes(752, 480, 8, 100) for _ in range(4)]
         batch = Batch(frames, default_params(n_hyp=128, idepth_min=1.0 / 50.0, idepth_max=1.0 / 0.3, cost_arith=arith),
                       outputs=("depth",))
-        batch.autotune(reps=args.tune_reps)
+        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
+        shapes = shapes_of(batch)
         g = [torch.randn(480, 752, device=dev) for _ in range(32)]
 
         def step():
@@ -322,6 +337,7 @@ def run_workload(args):
     cfg = {"workload": what}
     if wl != "upsample16":
         cfg["cost_arith"] = args.cost_arith
+        cfg["launch_shapes"] = shapes
     print(json.dumps({"workload": wl, "value": round(n / sec, 2), "unit": unit, "ms_per_step": round(sec * 1e3, 4),
                       "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
                       "config": cfg}), flush=True)
@@ -390,7 +406,7 @@ def main():
     stream = torch.cuda.current_stream()
     tuned = None
     if not args.no_autotune:
-        tuned = batch.autotune(reps=args.tune_reps)
+        tuned = batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     if args.overlap is not None:
         batch.set_tuning(overlap=args.overlap)

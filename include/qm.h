@@ -69,7 +69,7 @@ typedef struct qm_params {
 
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 11
+#define QM_ABI_VERSION 12
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);
@@ -326,12 +326,27 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
 #define QM_NUM_TUNE 7
 int qm_batch_set_tuning(qm_batch* b, int knob, int value);
 int qm_batch_get_tuning(qm_batch* b, int knob, int* value);
-/* Autotuner: on this batch's own frames, time every value of every knob
- * (reps launches each, after one warm-up; the BP knob is timed over the whole
- * coarse-to-fine sweep schedule) and keep the fastest. It first recomputes the
- * quadtree, cost volume and pyramid; the fusion state is not touched. chosen
- * and best_ms (per knob, nullable) receive the choice and its time per
- * launch set. Synchronises the stream. */
+/* Resolution groups: the frames of one size (W x H) share their launch shapes;
+ * a stage's launch covers consecutive frames whose groups agree on its shapes.
+ * qm_batch_groups gives the number of groups (in the order of their first
+ * frame) and, up to cap, their sizes (W, H nullable). qm_batch_set_tuning sets
+ * a shape knob for every group (get: group 0's); the group calls set and get
+ * one group's (not the overlap knob, which is the batch's). */
+int qm_batch_groups(qm_batch* b, int* count, int* W, int* H, int cap);
+int qm_batch_set_group_tuning(qm_batch* b, int group, int knob, int value);
+int qm_batch_get_group_tuning(qm_batch* b, int group, int knob, int* value);
+/* Autotuner, on this batch's own frames: times every value of every shape
+ * knob (reps launches each, after one warm-up; the BP knobs jointly, over the
+ * whole coarse-to-fine sweep schedule) and keeps the fastest, first with one
+ * shape for the whole batch, then (with two or more resolution groups) for
+ * each group on its own frames; per stage, the groups' own shapes are kept
+ * when the whole batch runs that stage faster with them (a launch per run of
+ * agreeing groups) than with the common shape (one launch). Then the overlap
+ * knob over whole runs with those shapes. It first recomputes the quadtree,
+ * cost volume and pyramid; the fusion state is not touched. chosen receives
+ * group 0's shapes and the overlap; best_ms the common search's time per
+ * launch set for each shape knob and the overlap's per run (nullable both);
+ * qm_batch_get_group_tuning gives every group's. Synchronises the stream. */
 int qm_batch_autotune(qm_batch* b, int reps, void* stream, int chosen[QM_NUM_TUNE], double best_ms[QM_NUM_TUNE]);
 /* Backward of the last run with respect to each stream's lambda: given
  * g_depth_out[i] = dL/d(depth_out) of frame i (device pointer, H x W; NULL

@@ -88,7 +88,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 11  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 12  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {
@@ -127,6 +127,9 @@ SIGNATURES = {
     "qm_batch_runs": (I, [P, P]),
     "qm_batch_set_tuning": (I, [P, I, I]),
     "qm_batch_get_tuning": (I, [P, I, P]),
+    "qm_batch_groups": (I, [P, P, P, P, I]),
+    "qm_batch_set_group_tuning": (I, [P, I, I, I]),
+    "qm_batch_get_group_tuning": (I, [P, I, I, P]),
     "qm_batch_autotune": (I, [P, I, P, P, P]),
     "qm_batch_destroy": (I, [P]),
     "qm_create": (I, [I, I, F, F, F, F, P, P]),

@@ -1,5 +1,6 @@
 """Batched keyframes: many camera streams of uneven resolution through every
-stage in one launch per stage (optionally one hipGraph replay per run).
+stage in one launch per stage and launch shape (optionally one hipGraph replay
+per run); the frames of one size share launch shapes (a resolution group).
 
 Thin owner of a qm_batch (include/qm.h). Inputs are torch uint8 device tensors
 kept alive by this object; per-stream fusion seeds persist across runs.
@@ -7,11 +8,35 @@ kept alive by this object; per-stream fusion seeds persist across runs.
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
+import warnings
 
 import numpy as np
 import torch
 
 from ._lib import STAGES, TUNE_KNOBS, check, default_params, lib, qm_frame_desc, qm_params
+
+
+def _load_cache(path: str) -> dict:
+    try:
+        with open(path) as f:
+            db = json.load(f)
+        return db if isinstance(db, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(path: str, db: dict):
+    d = os.path.dirname(os.path.abspath(path))
+    try:
+        os.makedirs(d, exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError as e:
+        warnings.warn(f"launch-shape cache {path} not written: {e}")
 
 __all__ = ["Batch", "make_desc"]
 
@@ -128,26 +153,83 @@ class Batch:
         check(lib().qm_batch_buffer(self._h, i, which, C.byref(p)))
         return p.value
 
-    def autotune(self, reps: int = 5, stream=None) -> dict:
-        """Pick the fastest launch shapes for this batch (qm_batch_autotune).
-        Returns {knob: (value, ms)}."""
+    def autotune(self, reps: int = 5, stream=None, cache: str | None = None, refresh: bool = False) -> dict:
+        """Pick the fastest launch shapes for this batch (qm_batch_autotune):
+        per resolution group, then the overlap knob. Returns group 0's
+        {knob: (value, ms)} (group_tuning gives the others').
+        cache: a JSON file of earlier choices, keyed by the device, each
+        group's size, frame count, hypotheses, cost arithmetic and sweep counts
+        (shape knobs), and the batch's whole composition (overlap knob). When
+        it holds every group's and the batch's entries they are applied without
+        timing (ms None); else (or with refresh) the batch is tuned and its
+        choices are written there."""
+        keys = self._cache_keys() if cache else None
+        if keys is not None and not refresh:
+            db = _load_cache(cache)
+            if all(k in db for k in keys[0]) and keys[1] in db:
+                for g, k in enumerate(keys[0]):
+                    self.set_group_tuning(g, **db[k])
+                self.set_tuning(overlap=db[keys[1]]["overlap"])
+                return {k: (v, None) for k, v in self.tuning().items()}
         s = stream if stream is not None else torch.cuda.current_stream()
         chosen = (C.c_int * len(TUNE_KNOBS))()
         ms = (C.c_double * len(TUNE_KNOBS))()
         check(lib().qm_batch_autotune(self._h, int(reps), C.c_void_p(s.cuda_stream), chosen, ms))
+        if keys is not None:
+            db = _load_cache(cache)
+            for g, k in enumerate(keys[0]):
+                db[k] = self.group_tuning(g)
+            db[keys[1]] = {"overlap": self.tuning()["overlap"]}
+            _store_cache(cache, db)
         return {k: (chosen[i], ms[i]) for i, k in enumerate(TUNE_KNOBS)}
 
+    def _cache_keys(self):
+        props = torch.cuda.get_device_properties(torch.cuda.current_device())
+        dev = f"{props.name}/{getattr(props, 'gcnArchName', '')}"
+        p = self.params
+        common = f"N{p.n_hyp}|arith{p.cost_arith}|iters{','.join(str(v) for v in p.bp_iters)}"
+        sizes = [tuple(f["key"].shape[::-1]) for f in self.frames]
+        groups = self.groups()
+        per_group = [f"{dev}|{W}x{H}|frames{sizes.count((W, H))}|{common}" for (W, H) in groups]
+        whole = f"{dev}|batch:" + ";".join(f"{W}x{H}" for (W, H) in sizes) + f"|{common}"
+        return per_group, whole
+
     def set_tuning(self, **knobs):
+        """Set knobs for the batch (shape knobs for every resolution group)."""
         for k, v in knobs.items():
             check(lib().qm_batch_set_tuning(self._h, TUNE_KNOBS.index(k), int(v)))
 
     def tuning(self) -> dict:
+        """The overlap knob and group 0's shape knobs."""
         out = {}
         for i, k in enumerate(TUNE_KNOBS):
             v = C.c_int()
             check(lib().qm_batch_get_tuning(self._h, i, C.byref(v)))
             out[k] = v.value
         return out
+
+    def groups(self) -> list:
+        """Resolution groups [(W, H), ...] in the order of their first frame."""
+        cnt = C.c_int()
+        check(lib().qm_batch_groups(self._h, C.byref(cnt), None, None, 0))
+        W, H = (C.c_int * cnt.value)(), (C.c_int * cnt.value)()
+        check(lib().qm_batch_groups(self._h, C.byref(cnt), W, H, cnt.value))
+        return [(W[g], H[g]) for g in range(cnt.value)]
+
+    def group_tuning(self, g: int) -> dict:
+        """Shape knobs of resolution group g."""
+        out = {}
+        for i, k in enumerate(TUNE_KNOBS):
+            if k == "overlap":
+                continue
+            v = C.c_int()
+            check(lib().qm_batch_get_group_tuning(self._h, int(g), i, C.byref(v)))
+            out[k] = v.value
+        return out
+
+    def set_group_tuning(self, g: int, **knobs):
+        for k, v in knobs.items():
+            check(lib().qm_batch_set_group_tuning(self._h, int(g), TUNE_KNOBS.index(k), int(v)))
 
     def read_buffer(self, i, which) -> np.ndarray:
         """Host copy of an internal buffer of stream i (BUF_* ids)."""

@@ -339,7 +339,13 @@ struct qm_batch {
   // tables and warped prior seeds) is intact: set by a run, cleared by
   // set_frame, reset_seeds and autotune
   bool run_state_valid = false;
+  // launch shapes: the frames of one size form a resolution group with shapes
+  // of its own (gtune; a stage's launch covers consecutive frames whose groups
+  // agree on its shape); tune holds the batch-wide overlap knob
   Tuning tune;
+  std::vector<int> group_of;  // frame -> group
+  std::vector<int> group_w, group_h;
+  std::vector<Tuning> gtune;
   // extra streams and fork / mark / join events of the frame-group overlap
   static constexpr int kMaxGroups = 8;
   hipStream_t side[kMaxGroups - 1] = {};
@@ -412,6 +418,34 @@ bool any_prior_depth(const qm_batch* b) {
   return false;
 }
 
+// the shapes a stage's launch depends on
+bool same_cost_shape(const Tuning& a, const Tuning& c) { return a.cost_waves == c.cost_waves; }
+bool same_bp_shape(const Tuning& a, const Tuning& c) {
+  return a.bp_rows == c.bp_rows && a.bp_pair == c.bp_pair && a.bp_seg == c.bp_seg &&
+         a.bp_pair_levels == c.bp_pair_levels;
+}
+bool same_extract_shape(const Tuning& a, const Tuning& c) {
+  return a.extract_rows == c.extract_rows && (a.bp_pair != 0) == (c.bp_pair != 0);
+}
+
+// fn(a, k, tuning) for each run [f0 + a, f0 + a + k) of consecutive frames
+// whose resolution groups share the stage's shape (same) in the group shapes gt
+template <typename Same, typename Fn>
+void for_shape_runs(const std::vector<Tuning>& gt, const std::vector<int>& group_of, int f0, int n, Same same,
+                    Fn fn) {
+  for (int a = 0; a < n;) {
+    const Tuning& ta = gt[group_of[f0 + a]];
+    int k = 1;
+    while (a + k < n && same(ta, gt[group_of[f0 + a + k]])) ++k;
+    fn(a, k, ta);
+    a += k;
+  }
+}
+template <typename Same, typename Fn>
+void for_shape_runs(const qm_batch* b, int f0, int n, Same same, Fn fn) {
+  for_shape_runs(b->gtune, b->group_of, f0, n, same, fn);
+}
+
 // One stage (QM_STAGE_* numbering: 0 quadtree, 1 cost volume, 2 pyramid,
 // 3 BP half-sweeps, 4 extraction, 5 / 6 upsample rows / columns, 7 / 8 fusion
 // warp / update) for frames [f0, f0 + n) on stream s.
@@ -421,7 +455,6 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
   const int N = b->p.n_hyp;
   const qm_params* p = &b->p;
   const BpLaunchParams bp = bp_params(p);
-  const Tuning& tu = b->tune;
   switch (stage) {
     case QM_STAGE_QUADTREE: {
       Mark m(t, QM_STAGE_QUADTREE);
@@ -432,12 +465,16 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
       break;
     }
     case QM_STAGE_COST: {
-      Mark m(t, QM_STAGE_COST);
+      int runs = 0;
+      for_shape_runs(b, f0, n, same_cost_shape, [&](int, int, const Tuning&) { ++runs; });
+      Mark m(t, QM_STAGE_COST, runs);
       bool prior = false;
       for (int i = 0; i < n; ++i) prior |= hf[i].prior != nullptr;
       // without a prior the cost kernel builds the pyramid from its tiles;
       // the prior cost changes level 0 first, then the pyramid stage runs
-      launch_cost(df, hf, n, N, b->table, false, p->cost_arith, tu, s, !prior);
+      for_shape_runs(b, f0, n, same_cost_shape, [&](int a, int k, const Tuning& tu) {
+        launch_cost(df + a, hf + a, k, N, b->table, false, p->cost_arith, tu, s, !prior);
+      });
       if (prior) launch_prior_cost(df, hf, n, N, false, bp.idepth_min, bp.idepth_step, s);
       break;
     }
@@ -451,17 +488,26 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
       break;
     }
     case QM_STAGE_BP_SWEEP: {
-      Mark m(t, QM_STAGE_BP_SWEEP, bp_launches(p->bp_iters, tu));
+      int launches = 0;
+      for_shape_runs(b, f0, n, same_bp_shape,
+                     [&](int, int, const Tuning& tu) { launches += bp_launches(p->bp_iters, tu); });
+      Mark m(t, QM_STAGE_BP_SWEEP, launches);
       // the last level-0 sweep (when not the level's first) extracts its colour's cells
-      launch_bp_levels(df, hf, n, N, p->bp_iters, bp, tu, s, true, false);
+      for_shape_runs(b, f0, n, same_bp_shape, [&](int a, int k, const Tuning& tu) {
+        launch_bp_levels(df + a, hf + a, k, N, p->bp_iters, bp, tu, s, true, false);
+      });
       break;
     }
     case QM_STAGE_EXTRACT: {
       // the other colour's cells: colour 0 (x + y even) is the A set, swept at even it
-      Mark m(t, QM_STAGE_EXTRACT);
+      int runs = 0;
+      for_shape_runs(b, f0, n, same_extract_shape, [&](int, int, const Tuning&) { ++runs; });
+      Mark m(t, QM_STAGE_EXTRACT, runs);
       const int it0 = p->bp_iters[0];
-      launch_extract(df, hf, n, N, bp, tu, s, it0 >= 2 ? ((it0 - 1) % 2 == 0 ? 1 : 0) : -1,
-                     bp_final_buf(0, it0, tu));
+      for_shape_runs(b, f0, n, same_extract_shape, [&](int a, int k, const Tuning& tu) {
+        launch_extract(df + a, hf + a, k, N, bp, tu, s, it0 >= 2 ? ((it0 - 1) % 2 == 0 ? 1 : 0) : -1,
+                       bp_final_buf(0, it0, tu));
+      });
       break;
     }
     case QM_STAGE_UPSAMPLE_ROW: {
@@ -1117,6 +1163,17 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
   qm_batch* b = new qm_batch();
   b->p = *p;
   b->n = n;
+  b->group_of.resize(n);
+  for (int i = 0; i < n; ++i) {
+    int g = 0;
+    while (g < (int)b->group_w.size() && (b->group_w[g] != frames[i].W || b->group_h[g] != frames[i].H)) ++g;
+    if (g == (int)b->group_w.size()) {
+      b->group_w.push_back(frames[i].W);
+      b->group_h.push_back(frames[i].H);
+    }
+    b->group_of[i] = g;
+  }
+  b->gtune.assign(b->group_w.size(), Tuning());
   const int N = p->n_hyp;
   std::vector<FrameDev> base(n);
   size_t total = align256(sizeof(float) * (N + 256));
@@ -1316,27 +1373,75 @@ static void drop_graphs(qm_batch* b) {
   }
 }
 
+static bool knob_value_ok(int knob, int value) {
+  switch (knob) {
+    case QM_TUNE_COST_WAVES: return value == 2 || value == 4 || value == 8;
+    case QM_TUNE_OVERLAP: return value >= 0 && value <= 7;
+    case QM_TUNE_BP_PAIR: return value == 0 || value == 8 || value == 16;
+    case QM_TUNE_BP_SEG: return value == 8 || value == 16 || value == 32 || value == 64;
+    case QM_TUNE_BP_PAIR_LEVELS: return value >= 1 && value <= 4;
+    default: return value == 4 || value == 8 || value == 16;  // bp_rows, extract_rows
+  }
+}
+
+// set one knob: the overlap for the batch, a shape knob for group g (every
+// group when g < 0); drops the captured graphs on a change
+static int set_knob(qm_batch* b, int g, int knob, int value) {
+  Tuning probe;
+  if (!tune_slot(probe, knob)) return fail(QM_ERR_INVALID, "unknown tuning knob");
+  if (!knob_value_ok(knob, value)) return fail(QM_ERR_INVALID, "value not offered for this knob");
+  bool changed = false;
+  if (knob == QM_TUNE_OVERLAP) {
+    changed = b->tune.overlap != value;
+    b->tune.overlap = value;
+  } else {
+    for (int i = 0; i < (int)b->gtune.size(); ++i) {
+      if (g >= 0 && i != g) continue;
+      int* slot = tune_slot(b->gtune[i], knob);
+      changed |= *slot != value;
+      *slot = value;
+    }
+  }
+  if (changed) drop_graphs(b);
+  return QM_OK;
+}
+
 int qm_batch_set_tuning(qm_batch* b, int knob, int value) {
   if (!b) return fail(QM_ERR_INVALID, "NULL batch");
-  int* slot = tune_slot(b->tune, knob);
-  if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
-  const bool ok = knob == QM_TUNE_COST_WAVES ? (value == 2 || value == 4 || value == 8)
-                  : knob == QM_TUNE_OVERLAP  ? (value >= 0 && value <= 7)
-                  : knob == QM_TUNE_BP_PAIR  ? (value == 0 || value == 8 || value == 16)
-                  : knob == QM_TUNE_BP_SEG   ? (value == 8 || value == 16 || value == 32 || value == 64)
-                  : knob == QM_TUNE_BP_PAIR_LEVELS ? (value >= 1 && value <= 4)
-                                             : (value == 4 || value == 8 || value == 16);
-  if (!ok) return fail(QM_ERR_INVALID, "value not offered for this knob");
-  if (*slot != value) {
-    *slot = value;
-    drop_graphs(b);
-  }
-  return QM_OK;
+  return set_knob(b, -1, knob, value);
 }
 
 int qm_batch_get_tuning(qm_batch* b, int knob, int* value) {
   if (!b || !value) return fail(QM_ERR_INVALID, "NULL pointer");
-  int* slot = tune_slot(b->tune, knob);
+  int* slot = tune_slot(knob == QM_TUNE_OVERLAP ? b->tune : b->gtune[0], knob);
+  if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
+  *value = *slot;
+  return QM_OK;
+}
+
+int qm_batch_groups(qm_batch* b, int* count, int* W, int* H, int cap) {
+  if (!b || !count) return fail(QM_ERR_INVALID, "NULL pointer");
+  const int G = (int)b->group_w.size();
+  *count = G;
+  for (int g = 0; g < G && g < cap; ++g) {
+    if (W) W[g] = b->group_w[g];
+    if (H) H[g] = b->group_h[g];
+  }
+  return QM_OK;
+}
+
+int qm_batch_set_group_tuning(qm_batch* b, int group, int knob, int value) {
+  if (!b) return fail(QM_ERR_INVALID, "NULL batch");
+  if (group < 0 || group >= (int)b->gtune.size()) return fail(QM_ERR_INVALID, "group index out of range");
+  if (knob == QM_TUNE_OVERLAP) return fail(QM_ERR_INVALID, "the overlap knob is the batch's, not a group's");
+  return set_knob(b, group, knob, value);
+}
+
+int qm_batch_get_group_tuning(qm_batch* b, int group, int knob, int* value) {
+  if (!b || !value) return fail(QM_ERR_INVALID, "NULL pointer");
+  if (group < 0 || group >= (int)b->gtune.size()) return fail(QM_ERR_INVALID, "group index out of range");
+  if (knob == QM_TUNE_OVERLAP) return fail(QM_ERR_INVALID, "the overlap knob is the batch's, not a group's");
+  int* slot = tune_slot(b->gtune[group], knob);
   if (!slot) return fail(QM_ERR_INVALID, "unknown tuning knob");
   *value = *slot;
   return QM_OK;
@@ -1352,9 +1457,10 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
   const FrameDev* hf = b->hf[par].data();
   const int n = b->n, N = b->p.n_hyp;
   const BpLaunchParams bp = bp_params(&b->p);
-  // inputs of the timed stages: quadtree, cost and its pyramid (the seeds are untouched)
+  // inputs of the timed stages: quadtree, cost and its pyramid (the seeds are
+  // untouched; every shape gives the same results)
   launch_quadtree(df, hf, n, s);
-  launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, b->tune, s, true);
+  launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, Tuning(), s, true);
   hipEvent_t e0, e1;
   QM_HIP(hipEventCreate(&e0));
   QM_HIP(hipEventCreate(&e1));
@@ -1374,43 +1480,53 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     std::vector<int> values;
     std::function<void(const Tuning&)> stage;
   };
-  // the single half-sweep's shape is timed with the pairs off (it runs alone
-  // there, at the levels without pairs, and for a level's odd last sweep)
-  auto single_sweeps = [&](const Tuning& t) {
-    Tuning u = t;
-    u.bp_pair = 0;
-    run_sweeps(df, hf, n, &b->p, u, s);
-  };
-  const Knob knobs[3] = {
-      {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) { launch_cost(df, hf, n, N, b->table, false, b->p.cost_arith, t, s, true); }},
-      {QM_TUNE_BP_ROWS, {4, 8, 16}, single_sweeps},
-      {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) { launch_extract(df, hf, n, N, bp, t, s); }},
-  };
   int rc = QM_OK;
-  Tuning best = b->tune;
-  for (const Knob& k : knobs) {
-    double best_t = 1e30;
-    for (int v : k.values) {
-      Tuning t = best;
-      *tune_slot(t, k.id) = v;
-      double ms = 0.0;
-      if ((rc = time(k.stage, t, &ms))) break;
-      if (ms < best_t) {
-        best_t = ms;
-        *tune_slot(best, k.id) = v;
+  const int G = (int)b->gtune.size();
+  // the shape knobs searched on the frames of the given runs of consecutive
+  // frames (launches of a stage cover one run each); best_ms of the shape
+  // knobs when times is given
+  auto tune_runs = [&](const std::vector<std::pair<int, int>>& runs, Tuning best, double* times) -> Tuning {
+    auto sweeps = [&](const Tuning& t) {
+      for (const auto& r : runs) run_sweeps(df + r.first, hf + r.first, r.second, &b->p, t, s);
+    };
+    // the single half-sweep's shape is timed with the pairs off (it runs
+    // alone there, at the levels without pairs, and for a level's odd last sweep)
+    auto single_sweeps = [&](const Tuning& t) {
+      Tuning u = t;
+      u.bp_pair = 0;
+      sweeps(u);
+    };
+    const Knob knobs[3] = {
+        {QM_TUNE_COST_WAVES, {2, 4, 8}, [&](const Tuning& t) {
+           for (const auto& r : runs)
+             launch_cost(df + r.first, hf + r.first, r.second, N, b->table, false, b->p.cost_arith, t, s, true);
+         }},
+        {QM_TUNE_BP_ROWS, {4, 8, 16}, single_sweeps},
+        {QM_TUNE_EXTRACT_ROWS, {4, 8, 16}, [&](const Tuning& t) {
+           for (const auto& r : runs) launch_extract(df + r.first, hf + r.first, r.second, N, bp, t, s);
+         }},
+    };
+    for (const Knob& k : knobs) {
+      double best_t = 1e30;
+      for (int v : k.values) {
+        Tuning t = best;
+        *tune_slot(t, k.id) = v;
+        double ms = 0.0;
+        if ((rc = time(k.stage, t, &ms))) return best;
+        if (ms < best_t) {
+          best_t = ms;
+          *tune_slot(best, k.id) = v;
+        }
       }
+      if (times) times[k.id] = best_t;
     }
-    if (rc) break;
-    if (best_ms) best_ms[k.id] = best_t;
-  }
-  // the fused pairs' three knobs depend on each other: every combination is
-  // timed over the whole sweep schedule (with the pairs off, the other two
-  // keep their values)
-  if (!rc) {
+    // the fused pairs' three knobs depend on each other: every combination is
+    // timed over the whole sweep schedule (with the pairs off, the other two
+    // keep their values)
     double best_t = 1e30;
     const Tuning base = best;
-    for (int pair : {0, 8, 16}) {
-      for (int seg : {8, 16, 32, 64}) {
+    for (int pair : {0, 8, 16})
+      for (int seg : {8, 16, 32, 64})
         for (int lv = 1; lv <= 4; ++lv) {
           if (pair == 0 && (seg != base.bp_seg || lv != base.bp_pair_levels)) continue;
           Tuning t = base;
@@ -1418,7 +1534,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
           t.bp_seg = seg;
           t.bp_pair_levels = lv;
           double ms = 0.0;
-          if ((rc = time([&](const Tuning& u) { run_sweeps(df, hf, n, &b->p, u, s); }, t, &ms))) break;
+          if ((rc = time(sweeps, t, &ms))) return best;
           if (ms < best_t) {
             best_t = ms;
             best.bp_pair = pair;
@@ -1426,15 +1542,78 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
             best.bp_pair_levels = lv;
           }
         }
-        if (rc) break;
+    if (times) times[QM_TUNE_BP_PAIR] = times[QM_TUNE_BP_SEG] = times[QM_TUNE_BP_PAIR_LEVELS] = best_t;
+    return best;
+  };
+  // one shape for every frame (each stage in one launch), searched on the whole batch
+  double common_ms[QM_NUM_TUNE] = {};
+  const Tuning common = tune_runs({{0, n}}, b->gtune[0], common_ms);
+  std::vector<Tuning> gbest(G, common);
+  if (G > 1 && !rc) {
+    // each resolution group on its own frames (its runs of consecutive frames)
+    std::vector<Tuning> own(G);
+    for (int g = 0; g < G && !rc; ++g) {
+      std::vector<std::pair<int, int>> runs;
+      for (int a = 0; a < n;) {
+        if (b->group_of[a] != g) {
+          ++a;
+          continue;
+        }
+        int k = 1;
+        while (a + k < n && b->group_of[a + k] == g) ++k;
+        runs.push_back({a, k});
+        a += k;
       }
-      if (rc) break;
+      own[g] = tune_runs(runs, common, nullptr);
     }
-    if (best_ms) best_ms[QM_TUNE_BP_PAIR] = best_ms[QM_TUNE_BP_SEG] = best_ms[QM_TUNE_BP_PAIR_LEVELS] = best_t;
+    // per stage, the groups' own shapes (a launch per run of agreeing groups)
+    // against the common one (one launch), both over the whole batch
+    const std::vector<Tuning> one(G, common);
+    auto cost_all = [&](const std::vector<Tuning>& gt) {
+      for_shape_runs(gt, b->group_of, 0, n, same_cost_shape, [&](int a, int k, const Tuning& tu) {
+        launch_cost(df + a, hf + a, k, N, b->table, false, b->p.cost_arith, tu, s, true);
+      });
+    };
+    auto bp_all = [&](const std::vector<Tuning>& gt) {
+      for_shape_runs(gt, b->group_of, 0, n, same_bp_shape,
+                     [&](int a, int k, const Tuning& tu) { run_sweeps(df + a, hf + a, k, &b->p, tu, s); });
+    };
+    auto extract_all = [&](const std::vector<Tuning>& gt) {
+      for_shape_runs(gt, b->group_of, 0, n, same_extract_shape,
+                     [&](int a, int k, const Tuning& tu) { launch_extract(df + a, hf + a, k, N, bp, tu, s); });
+    };
+    auto faster = [&](const std::function<void(const std::vector<Tuning>&)>& stage, bool* split) -> int {
+      double t_own = 0.0, t_one = 0.0;
+      int r;
+      if ((r = time([&](const Tuning&) { stage(own); }, common, &t_own))) return r;
+      if ((r = time([&](const Tuning&) { stage(one); }, common, &t_one))) return r;
+      *split = t_own < t_one;
+      return QM_OK;
+    };
+    bool split_cost = false, split_bp = false, split_extract = false;
+    if (!rc) rc = faster(cost_all, &split_cost);
+    if (!rc) rc = faster(bp_all, &split_bp);
+    if (!rc) rc = faster(extract_all, &split_extract);
+    for (int g = 0; g < G && !rc; ++g) {
+      Tuning& t = gbest[g];
+      if (split_cost) t.cost_waves = own[g].cost_waves;
+      if (split_bp) {
+        t.bp_rows = own[g].bp_rows;
+        t.bp_pair = own[g].bp_pair;
+        t.bp_seg = own[g].bp_seg;
+        t.bp_pair_levels = own[g].bp_pair_levels;
+      }
+      if (split_extract) t.extract_rows = own[g].extract_rows;
+    }
   }
-  // the overlap knob: whole pipeline runs (plain launches), which move the
-  // fusion state, so both seed buffers of every frame are saved first and
-  // restored afterwards (the runs' depth / dense / sparse outputs are not)
+  if (best_ms)
+    for (int k = 0; k < QM_NUM_TUNE; ++k)
+      if (k != QM_TUNE_OVERLAP) best_ms[k] = common_ms[k];
+  // the overlap knob, with the groups' chosen shapes: whole pipeline runs
+  // (plain launches), which move the fusion state, so both seed buffers of
+  // every frame are saved first and restored afterwards (the runs' depth /
+  // dense / sparse outputs are not)
+  int best_overlap = b->tune.overlap;
   if (!rc) {
     size_t total = 0;
     for (int i = 0; i < n; ++i) total += 2 * (size_t)hf[i].W * hf[i].H * sizeof(float4);
@@ -1453,18 +1632,21 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
       };
       rc = copy_seeds(true);
       const Tuning keep = b->tune;
+      const std::vector<Tuning> keep_g = b->gtune;
+      b->gtune = gbest;
       double best_t = 1e30;
       for (int v = 0; v <= 7 && !rc; ++v) {
-        Tuning t = best;
+        Tuning t = keep;
         t.overlap = v;
         double ms = 0.0;
         rc = time([&](const Tuning& tt) { b->tune = tt; (void)run_pipeline(b, par, s); }, t, &ms);
         if (!rc && ms < best_t) {
           best_t = ms;
-          best.overlap = v;
+          best_overlap = v;
         }
       }
       b->tune = keep;
+      b->gtune = keep_g;
       if (!rc) rc = copy_seeds(false);
       if (!rc) {
         const hipError_t e = hipStreamSynchronize(s);
@@ -1478,10 +1660,12 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   QM_HIP(hipGetLastError());
-  for (int k = 0; k < QM_NUM_TUNE; ++k) {
-    if (chosen) chosen[k] = *tune_slot(best, k);
-    if ((rc = qm_batch_set_tuning(b, k, *tune_slot(best, k)))) return rc;
-  }
+  for (int g = 0; g < G; ++g)
+    for (int k = 0; k < QM_NUM_TUNE; ++k)
+      if (k != QM_TUNE_OVERLAP && (rc = set_knob(b, g, k, *tune_slot(gbest[g], k)))) return rc;
+  if ((rc = set_knob(b, -1, QM_TUNE_OVERLAP, best_overlap))) return rc;
+  if (chosen)
+    for (int k = 0; k < QM_NUM_TUNE; ++k) chosen[k] = k == QM_TUNE_OVERLAP ? best_overlap : *tune_slot(gbest[0], k);
   return QM_OK;
 }
 

@@ -4,6 +4,8 @@ Several camera streams of uneven resolution run through every stage in one
 launch per stage (plain launches and hipGraph replay), three consecutive
 keyframes each, so the fusion state carries over between keyframes.
 """
+import json
+
 import numpy as np
 import pytest
 import torch
@@ -102,7 +104,7 @@ def test_twelve_uneven_streams_config4():
     b.close()
 
 
-def test_every_launch_shape_gives_identical_results_and_autotune():
+def test_every_launch_shape_gives_identical_results_and_autotune(tmp_path):
     """The autotuner's knobs change launch shapes only: every value of every
     knob reproduces the default run bit for bit; the autotuner then picks
     offered values and the batch still matches."""
@@ -133,14 +135,39 @@ def test_every_launch_shape_gives_identical_results_and_autotune():
             for (gs, gd), (ws, wd) in zip(got, want):
                 assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True), f"{knob}={v}"
         b.set_tuning(**{knob: defaults[knob]})
-    res = b.autotune(reps=3)
-    print(f"autotune: {res}", flush=True)
+    # the two sizes with shapes of their own (each stage then launches per size)
+    assert b.groups() == [(376, 240), (752, 480)]
+    for g0, g1 in [({"bp_pair": 0, "cost_waves": 2}, {"bp_pair": 16, "bp_seg": 16, "extract_rows": 4}),
+                   ({"bp_pair": 8, "bp_seg": 64, "bp_pair_levels": 4}, {"bp_pair": 0, "bp_rows": 8})]:
+        print(f"group shapes {g0} {g1}", flush=True)
+        b.set_tuning(**defaults)
+        b.set_group_tuning(0, **g0)
+        b.set_group_tuning(1, **g1)
+        assert all(b.group_tuning(0)[k] == v for k, v in g0.items())
+        assert all(b.group_tuning(1)[k] == v for k, v in g1.items())
+        got = run(b)
+        for (gs, gd), (ws, wd) in zip(got, want):
+            assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True), f"{g0} {g1}"
+    b.set_tuning(**defaults)
+    cache = str(tmp_path / "shapes.json")
+    res = b.autotune(reps=3, cache=cache)
+    print(f"autotune: {res} groups {[b.group_tuning(g) for g in range(2)]}", flush=True)
     for knob, (v, ms) in res.items():
         assert v in offered[knob] and ms > 0
     assert b.tuning() == {k: v for k, (v, _) in res.items()}
+    for g in range(2):
+        assert all(v in offered[k] for k, v in b.group_tuning(g).items())
     got = run(b)
     for (gs, gd), (ws, wd) in zip(got, want):
         assert np.array_equal(gs, ws) and np.array_equal(gd, wd, equal_nan=True)
+    # a second batch of the same composition takes the cached choices without timing
+    db = json.load(open(cache))
+    assert len(db) == 3
+    b2 = Batch(frames, p, outputs=("sparse", "dense"))
+    res2 = b2.autotune(reps=3, cache=cache)
+    assert all(ms is None for _, ms in res2.values())
+    assert b2.tuning() == b.tuning() and all(b2.group_tuning(g) == b.group_tuning(g) for g in range(2))
+    b2.close()
     b.close()
 
 
