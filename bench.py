@@ -53,7 +53,9 @@ def parse():
                                       "launch_shapes.json")),
                     help="JSON file of autotuned launch shapes: cached choices for this device and batch are "
                          "reused, new ones added ('' for none)")
-    ap.add_argument("--retune", action="store_true", help="autotune even when the cache holds this batch's shapes")
+    ap.add_argument("--retune", action="store_true", help="autotune even when a cache holds this batch's shapes")
+    ap.add_argument("--no-shipped-shapes", action="store_true",
+                    help="do not consult the launch shapes measured on MI355X that ship with the package")
     ap.add_argument("--tune-reps", type=int, default=5)
     ap.add_argument("--overlap", type=int, default=None,
                     help="force the frame-group overlap setting (0-7) after autotuning, e.g. for a profiled run")
@@ -296,7 +298,8 @@ def run_workload(args):
         frames = keyframes(376, 240, 4, 300) + keyframes(752, 480, 4, 310) + keyframes(1280, 720, 4, 320)
         batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, cost_arith=arith),
                       outputs=("depth",))
-        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
+        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune,
+                               seeds=shipped_shapes(args))
         shapes = shapes_of(batch)
         sec = timed(lambda: batch.run(graph=True))
         n, unit, what = 12, "depth maps/s", "batched graph replay of 12 uneven streams, N = 64"
@@ -306,7 +309,8 @@ def run_workload(args):
                   for f in keyframes(752, 480, 8, 100) for _ in range(4)]
         batch = Batch(frames, default_params(n_hyp=128, idepth_min=1.0 / 50.0, idepth_max=1.0 / 0.3, cost_arith=arith),
                       outputs=("depth",))
-        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
+        batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune,
+                               seeds=shipped_shapes(args))
         shapes = shapes_of(batch)
         g = [torch.randn(480, 752, device=dev) for _ in range(32)]
 
@@ -341,6 +345,16 @@ def run_workload(args):
     print(json.dumps({"workload": wl, "value": round(n / sec, 2), "unit": unit, "ms_per_step": round(sec * 1e3, 4),
                       "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
                       "config": cfg}), flush=True)
+
+
+def shipped_shapes(args):
+    """The package's JSON of launch shapes autotuned on MI355X for the bench
+    workloads (consulted after the user cache)."""
+    if args.no_shipped_shapes:
+        return ()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "open_quadtree_mapping_amd",
+                        "launch_shapes_mi355x.json")
+    return (path,) if os.path.exists(path) else ()
 
 
 def main():
@@ -406,7 +420,8 @@ def main():
     stream = torch.cuda.current_stream()
     tuned = None
     if not args.no_autotune:
-        tuned = batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune)
+        tuned = batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune,
+                               seeds=shipped_shapes(args))
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     if args.overlap is not None:
         batch.set_tuning(overlap=args.overlap)
@@ -567,7 +582,9 @@ def main():
                                "(quadtree, cost volume, HBP, upsample, fusion), "
                                + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
                    "streams": B, "width": W, "height": H, "hypotheses": N, "cost_arith": args.cost_arith,
-                   "launch_shapes": batch_tuning},
+                   "launch_shapes": batch_tuning,
+                   "launch_shapes_source": ("defaults" if tuned is None else
+                                            "cache" if all(ms is None for _, ms in tuned.values()) else "autotune")},
         "plain_maps_per_s": None if plain is None else round(plain, 2),
         "batched_over_plain": None if plain is None else round(maps_per_s / plain, 3),
         "roofline": roofline,

@@ -153,7 +153,8 @@ class Batch:
         check(lib().qm_batch_buffer(self._h, i, which, C.byref(p)))
         return p.value
 
-    def autotune(self, reps: int = 5, stream=None, cache: str | None = None, refresh: bool = False) -> dict:
+    def autotune(self, reps: int = 5, stream=None, cache: str | None = None, refresh: bool = False,
+                 seeds: tuple = ()) -> dict:
         """Pick the fastest launch shapes for this batch (qm_batch_autotune):
         per resolution group, then the overlap knob. Returns group 0's
         {knob: (value, ms)} (group_tuning gives the others').
@@ -162,10 +163,14 @@ class Batch:
         (shape knobs), and the batch's whole composition (overlap knob). When
         it holds every group's and the batch's entries they are applied without
         timing (ms None); else (or with refresh) the batch is tuned and its
-        choices are written there."""
-        keys = self._cache_keys() if cache else None
+        choices are written there. seeds: further read-only JSON files of the
+        same form, consulted after cache (earlier measured choices that ship
+        with an application)."""
+        keys = self._cache_keys() if (cache or seeds) else None
         if keys is not None and not refresh:
-            db = _load_cache(cache)
+            db = {}
+            for path in reversed([cache, *seeds] if cache else list(seeds)):
+                db.update(_load_cache(path))
             if all(k in db for k in keys[0]) and keys[1] in db:
                 for g, k in enumerate(keys[0]):
                     self.set_group_tuning(g, **db[k])
@@ -175,7 +180,7 @@ class Batch:
         chosen = (C.c_int * len(TUNE_KNOBS))()
         ms = (C.c_double * len(TUNE_KNOBS))()
         check(lib().qm_batch_autotune(self._h, int(reps), C.c_void_p(s.cuda_stream), chosen, ms))
-        if keys is not None:
+        if keys is not None and cache:
             db = _load_cache(cache)
             for g, k in enumerate(keys[0]):
                 db[k] = self.group_tuning(g)
@@ -184,8 +189,9 @@ class Batch:
         return {k: (chosen[i], ms[i]) for i, k in enumerate(TUNE_KNOBS)}
 
     def _cache_keys(self):
+        # the architecture and CU count (the device name can be empty, e.g. under a profiler)
         props = torch.cuda.get_device_properties(torch.cuda.current_device())
-        dev = f"{props.name}/{getattr(props, 'gcnArchName', '')}"
+        dev = f"{getattr(props, 'gcnArchName', '') or props.name}/{props.multi_processor_count}CU"
         p = self.params
         common = f"N{p.n_hyp}|arith{p.cost_arith}|iters{','.join(str(v) for v in p.bp_iters)}"
         sizes = [tuple(f["key"].shape[::-1]) for f in self.frames]
