@@ -134,6 +134,7 @@ void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, 
   }
   launch_upsample_col(dframes, hframes, n, 3, nullptr, false, s);
   launch_upsample_row(dframes, hframes, n, 3, nullptr, s);
+  launch_adj_dconf(dframes, hframes, n, s);  // dL/dd, dL/dconf from the row adjoint's solution planes
   BlockMap bm;
   int counts[kMaxFrames];
   for (int i = 0; i < n; ++i) counts[i] = (hframes[i].grad_lambda && hframes[i].lam_part) ? 1 : 0;

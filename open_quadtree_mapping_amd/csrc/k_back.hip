@@ -273,6 +273,71 @@ struct AdjOutIn {
   float zx[kUK], zy[kUK], w[kUK], d[kUK], c[kUK];
 };
 
+// The adjoint out wave's running state along a line: the solutions and forward
+// results of the step after the current one, the split edge's share from
+// segment 1, the line's lambda partial, and the gradient of edge 0 (once).
+struct AdjEdgeState {
+  float vxn = 0.0f, vnn = 0.0f, zxn = 0.0f, zyn = 0.0f, carry = 0.0f;
+  double part = 0.0;  // sum over the line's edges of w dL/d(lambda w)
+  float e_first = 0.0f;
+  bool has_first = false;
+};
+
+// dL/dw of edges c0 + 1 .. c0 + cnt of one chunk (ge[k] for edge c0 + k + 1)
+// from the adjoint solutions v (LDS) and forward results z of its steps, k from
+// cnt - 1 down to 0: -scale ((vx_i - vx_{i+1}) (zx_i - zx_{i+1}) + (vn_i -
+// vn_{i+1}) (zy_i - zy_{i+1})) inside the segment [s0, s1), -scale (vx zx + vn
+// zy) at its end (plus segment 1's share of the split edge), and at its first
+// step the edge before it (edge 0: e_first; the split edge: carry). The lambda
+// partial adds w dL/d(lambda w) in the same order, each term an exact double
+// product (24-bit significands), so one fma rounds as the add after it would.
+// The chunk's LDS reads are issued before its steps; the segment tests are
+// wave-uniform branches around the rare end and start steps.
+template <bool WHOLE>
+__device__ __forceinline__ void adj_edges(const float* XO, const float* NO, const AdjOutIn& b, int c0, int cnt, int s0,
+                                          int s1, int len, float gscale, float w_s0, AdjEdgeState& st,
+                                          float (&ge)[kUK]) {
+  float vx[kUK], vn[kUK];
+#pragma unroll
+  for (int k = 0; k < kUK; ++k) {
+    vx[k] = XO[k];
+    vn[k] = NO[k];
+  }
+#pragma unroll
+  for (int k = kUK - 1; k >= 0; --k) {
+    ge[k] = 0.0f;
+    if (WHOLE || k < cnt) {
+      const int i = c0 + k;
+      const float zx = b.zx[k], zy = b.zy[k];
+      float t;
+      if (i + 1 < s1) {
+        t = (vx[k] - st.vxn) * (zx - st.zxn) + (vn[k] - st.vnn) * (zy - st.zyn);
+        ge[k] = -gscale * t;
+      } else {
+        t = vx[k] * zx + vn[k] * zy;
+        const float e = -gscale * t;
+        ge[k] = s1 == len ? e : e + st.carry;
+      }
+      st.part = fma((double)b.w[k], -(double)t, st.part);
+      if (i == s0) {
+        const float t0 = vx[k] * zx + vn[k] * zy;
+        const float e0 = -gscale * t0;
+        st.part = fma((double)w_s0, -(double)t0, st.part);
+        if (s0 == 0) {
+          st.e_first = e0;
+          st.has_first = true;
+        } else {
+          st.carry = e0;  // segment 1 is swept first: its share of the split edge
+        }
+      }
+      st.vxn = vx[k];
+      st.vnn = vn[k];
+      st.zxn = zx;
+      st.zyn = zy;
+    }
+  }
+}
+
 template <int MODE>
 struct RowBuf {                // one producer wave's half of a chunk
   uint8_t px[kHalf + 2];       // image at c0 - 1 + 8 HALF + j (modes 0, 1)
@@ -486,7 +551,10 @@ __global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev
   } else if (wave == 3 || wave == 4) {
     // ---------------- x (wave 3) and n (wave 4) chains ----------------
     const int role = wave == 3 ? 1 : 2;  // checkpoint plane 1 (x) or 2 (n)
-    const gptr<float> out = gp(F.rowx) + (role == 2 ? (size_t)W * H : 0);
+    // mode 3 writes the row adjoint's solution planes (when asked) where the
+    // other modes write the row results
+    const gptr<float> out = gp(MODE == 3 ? F.adj_rowv : F.rowx) + (role == 2 ? (size_t)W * H : 0);
+    const bool store_planes = MODE != 3 || F.adj_rowv != nullptr;
     const gptr<float> out_rm = gp(role == 2 ? F.row_n_out : F.row_x_out);
     const gptr<float> sparse_out = gp(F.sparse_out);
     const float* Rsrc = (role == 2 && MODE >= 2) ? &L.N[0][0] : &L.R[0][0];
@@ -549,7 +617,8 @@ __global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev
           float* O = (role == 2 ? adj_no(L) : adj_xo(L)) + ((q - 1) & 1) * 64 * kUS + lo;
 #pragma unroll
           for (int k = 0; k < kUK; ++k) O[k] = vp[k];
-        } else if (q >= 1) {
+        }
+        if (q >= 1 && store_planes) {
           if (cnt == kUK) {
             // 32-bit offsets from the plane's base (W H < 2^28 floats)
             const unsigned o0 = (unsigned)c0 * (unsigned)H + (unsigned)y;
@@ -582,37 +651,24 @@ __global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev
   } else if (MODE == 3) {
     // ---------------- adjoint out (wave 5) ----------------
     // From the row adjoint's solutions v (x / n waves) and the forward row
-    // results z (the rowx planes): dL/dd = conf vx (vx without confidences),
-    // dL/dconf = d vx + vn (0 without), and dL/dw of every edge as in the
-    // column pass's adjoint out wave; the per-row lambda partial. Chunks come
-    // from the row's end; a round's inputs are loaded a round ahead.
+    // results z (the rowx planes): dL/dw of every edge as in the column pass's
+    // adjoint out wave, and the per-row lambda partial (dL/dd and dL/dconf
+    // follow from the solution planes the x / n waves store, k_adj_dconf).
+    // Chunks come from the row's end; a round's inputs are loaded a round
+    // ahead.
     const gptr<const float> zxp = gp((const float*)F.rowx) + y;
     const gptr<const float> zyp = zxp + (size_t)W * H;
     const size_t wrow = (size_t)y * (W + 1);
     const gptr<const float> wr = gp(F.g_wrow) + wrow;
-    const bool have_conf = F.g_conf != nullptr;
-    const gptr<const float> dp = gp(F.g_d) + row;
-    const gptr<const float> cp = gp(have_conf ? F.g_conf : F.g_d) + row;
-    const gptr<float> gwr = gp(F.grad_wrow), gd = gp(F.grad_d), gc = gp(F.grad_conf);
+    const gptr<float> gwr = gp(F.grad_wrow);
     const float gscale = F.grad_w_unscaled ? 1.0f : (float)lam;
     for (int r = 0; r < nch + 2; ++r) __syncthreads();
     AdjOutIn A, B;
     // z from the column-major planes (a wave's load of one step is 64
-    // consecutive floats); d, conf and w(i + 1) as the lane's own 16
-    // consecutive floats (four dword-aligned float4) from min(c0, W - 16), which
-    // is c0 unless the chunk ends within 16 of the row's end (then out_round
-    // reloads them one by one). Every load unconditional.
-    auto load4x4 = [&](gptr<const float> p, float (&v)[kUK]) __attribute__((always_inline)) {
-      const gptr<const f4_dword_aligned> q4 = (gptr<const f4_dword_aligned>)p;
-#pragma unroll
-      for (int k = 0; k < kUK / 4; ++k) {
-        const float4 t = q4[k];
-        v[4 * k] = t.x;
-        v[4 * k + 1] = t.y;
-        v[4 * k + 2] = t.z;
-        v[4 * k + 3] = t.w;
-      }
-    };
+    // consecutive floats); w(i + 1) as the lane's own 16 consecutive floats
+    // (four dword-aligned float4 from c0), or one by one for a chunk that ends
+    // within 16 of the row's end. All of a chunk's loads are issued a round
+    // ahead, so that no round waits for the next one's.
     auto load_o = [&](AdjOutIn& b, int q) __attribute__((always_inline)) {
       int c0, cnt;
       bool first, last;
@@ -623,13 +679,24 @@ __global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev
         b.zx[k] = zxp[(size_t)i * H];
         b.zy[k] = zyp[(size_t)i * H];
       }
-      const int base = min(c0, W - kUK);
-      load4x4(wr + base + 1, b.w);
-      load4x4(dp + base, b.d);
-      load4x4(cp + base, b.c);
+      if (c0 <= W - kUK) {
+        const gptr<const f4_dword_aligned> q4 = (gptr<const f4_dword_aligned>)(wr + c0 + 1);
+#pragma unroll
+        for (int k = 0; k < kUK / 4; ++k) {
+          const float4 t = q4[k];
+          b.w[4 * k] = t.x;
+          b.w[4 * k + 1] = t.y;
+          b.w[4 * k + 2] = t.z;
+          b.w[4 * k + 3] = t.w;
+        }
+      } else {  // the row's last chunk (once per row)
+#pragma unroll
+        for (int k = 0; k < kUK; ++k) b.w[k] = wr[min(c0 + k + 1, W)];
+      }
     };
-    float vxn = 0.0f, vnn = 0.0f, zxn = 0.0f, zyn = 0.0f, carry = 0.0f;
-    double part = 0.0;  // sum over the row's edges of w dL/d(lambda w)
+    // the weights of the edges that start the two segments
+    const float w_seg0 = wr[0], w_seg1 = wr[min(ch.seg, W)];
+    AdjEdgeState st;
     auto out_round = [&](const AdjOutIn& b, int q) __attribute__((always_inline)) {
       if (q < 0 || q >= nch) return;
       const int j = nch - 1 - q;
@@ -640,89 +707,140 @@ __global__ __launch_bounds__(MODE == 3 ? 384 : 320) void k_up_row(const FrameDev
       const int s0 = seg1 ? ch.seg : 0, s1 = seg1 ? ch.len : ch.seg;
       const float* XO = adj_xo(L) + (q & 1) * 64 * kUS + lo;
       const float* NO = adj_no(L) + (q & 1) * 64 * kUS + lo;
-      float wv[kUK], dv[kUK], cv[kUK];
-      if (c0 <= W - kUK) {
-#pragma unroll
-        for (int k = 0; k < kUK; ++k) {
-          wv[k] = b.w[k];
-          dv[k] = b.d[k];
-          cv[k] = b.c[k];
-        }
-      } else {  // the row's last chunk: its own loads (once per row)
-#pragma unroll
-        for (int k = 0; k < kUK; ++k) {
-          const int i = min(c0 + k, W - 1);
-          wv[k] = wr[min(c0 + k + 1, W)];
-          dv[k] = dp[i];
-          cv[k] = cp[i];
-        }
-      }
-      // edge c0 + k + 1, dL/dd and dL/dconf of step c0 + k
-      float ge[kUK], gdv[kUK], gcv[kUK];
-#pragma unroll
-      for (int k = kUK - 1; k >= 0; --k) {
-        ge[k] = gdv[k] = gcv[k] = 0.0f;
-        if (k < cnt) {
-          const int i = c0 + k;
-          const float vx = XO[k], vn = NO[k], zx = b.zx[k], zy = b.zy[k];
-          if (i + 1 < s1) {
-            const float t = (vx - vxn) * (zx - zxn) + (vn - vnn) * (zy - zyn);
-            ge[k] = -gscale * t;
-            part += (double)wv[k] * (double)(-t);
-          } else {
-            const float t = vx * zx + vn * zy;
-            const float e = -gscale * t;
-            part += (double)wv[k] * (double)(-t);
-            ge[k] = s1 == W ? e : e + carry;
-          }
-          if (i == s0) {
-            const float t = vx * zx + vn * zy;
-            const float e = -gscale * t;
-            part += (double)wr[s0] * (double)(-t);
-            if (s0 == 0) {
-              if (gwr) gwr[wrow] = e;
-            } else {
-              carry = e;  // segment 1 is swept first: its share of the split edge
-            }
-          }
-          // rhs_x = conf d, rhs_n = conf (without confidences: rhs_x = d, rhs_n = 1{d > 0})
-          gdv[k] = have_conf ? cv[k] * vx : vx;
-          gcv[k] = have_conf ? dv[k] * vx + vn : 0.0f;
-          vxn = vx;
-          vnn = vn;
-          zxn = zx;
-          zyn = zy;
-        }
-      }
-      // the lane's 16 consecutive results as four float4 (a whole chunk), else one by one
-      auto store = [&](gptr<float> p, const float (&v)[kUK]) __attribute__((always_inline)) {
+      float ge[kUK];  // edge c0 + k + 1
+      if (cnt == kUK)
+        adj_edges<true>(XO, NO, b, c0, cnt, s0, s1, W, gscale, seg1 ? w_seg1 : w_seg0, st, ge);
+      else
+        adj_edges<false>(XO, NO, b, c0, cnt, s0, s1, W, gscale, seg1 ? w_seg1 : w_seg0, st, ge);
+      // the lane's 16 consecutive edge gradients as four float4 (a whole chunk), else one by one
+      if (gwr) {
+        const gptr<float> p = gwr + wrow + c0 + 1;
         if (cnt == kUK) {
           const gptr<f4_dword_aligned> q4 = (gptr<f4_dword_aligned>)p;
 #pragma unroll
-          for (int k = 0; k < kUK / 4; ++k) q4[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+          for (int k = 0; k < kUK / 4; ++k) q4[k] = make_float4(ge[4 * k], ge[4 * k + 1], ge[4 * k + 2], ge[4 * k + 3]);
         } else {
 #pragma unroll
           for (int k = 0; k < kUK; ++k)
-            if (k < cnt) p[k] = v[k];
+            if (k < cnt) p[k] = ge[k];
         }
-      };
-      if (gwr) store(gwr + wrow + c0 + 1, ge);
-      if (gd) store(gd + row + c0, gdv);
-      if (gc) store(gc + row + c0, gcv);
+      }
     };
+    // round r forms chunk r - 4, whose inputs were loaded two rounds ahead:
+    // three buffers take turns (the loop is unrolled by three without
+    // conditions; its last rounds follow it)
     const int rounds = nch + 4;
-    load_o(A, 0);
-    for (int r = 0; r < rounds; r += 2) {
-      load_o(B, r - 3);
+    AdjOutIn C;
+    load_o(A, -4);
+    load_o(B, -3);
+    int r = 0;
+    for (; r + 2 < rounds; r += 3) {
+      load_o(C, r - 2);
       out_round(A, r - 4);
       __syncthreads();
-      if (r + 1 < rounds) {
-        load_o(A, r - 2);
-        out_round(B, r - 3);
-        __syncthreads();
+      load_o(A, r - 1);
+      out_round(B, r - 3);
+      __syncthreads();
+      load_o(B, r);
+      out_round(C, r - 2);
+      __syncthreads();
+    }
+    if (r < rounds) {
+      out_round(A, r - 4);
+      __syncthreads();
+      ++r;
+    }
+    if (r < rounds) {
+      out_round(B, r - 4);
+      __syncthreads();
+    }
+    if (gwr && st.has_first) gwr[wrow] = st.e_first;
+    if (F.lam_part) F.lam_part[y] = st.part;
+  }
+}
+
+// dL/dd and dL/dconf of the generalised upsample from the row adjoint's
+// solution planes v ([x][y]): rhs_x = conf d and rhs_n = conf give
+// dL/dd = conf vx, dL/dconf = d vx + vn (without confidences, rhs_x = d and
+// rhs_n = 1{d > 0}: dL/dd = vx, no dL/dconf). A 64 x 64 tile per block,
+// transposed through LDS so that both the planes and the row-major outputs are
+// read and written along their rows.
+__global__ __launch_bounds__(256) void k_adj_dconf(const FrameDev* __restrict__ frames, BlockMap bm) {
+  __shared__ float tx[64][65], tn[64][65];
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int W = F.W, H = F.H;
+  const int tiles_x = (W + 63) / 64;
+  const int b = blockIdx.x - bm.start[f];
+  const int x0 = (b % tiles_x) * 64, y0 = (b / tiles_x) * 64;
+  const gptr<const float> vx = gp((const float*)F.adj_rowv), vn = vx + (size_t)W * H;
+  const bool have_conf = F.g_conf != nullptr;
+  const gptr<const float> d = gp(F.g_d), conf = gp(F.g_conf);
+  const gptr<float> gd = gp(F.grad_d), gc = gp(F.grad_conf);
+  const int t = threadIdx.x;
+  // whole tiles of images whose sizes are multiples of 4 move as float4 (16
+  // consecutive bytes of a plane column or an image row); others one by one
+  auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool quads = (W & 3) == 0 && (H & 3) == 0 && x0 + 64 <= W && y0 + 64 <= H && a16(F.adj_rowv) &&
+                     a16(F.g_d) && a16(F.g_conf) && a16(F.grad_d) && a16(F.grad_conf);
+  if (quads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = t + 256 * k, c = i >> 4, qy = (i & 15) * 4;
+      const size_t o = (size_t)(x0 + c) * H + y0 + qy;
+      const float4 a = *(const gptr<const float4>)(vx + o), n = *(const gptr<const float4>)(vn + o);
+      tx[c][qy] = a.x; tx[c][qy + 1] = a.y; tx[c][qy + 2] = a.z; tx[c][qy + 3] = a.w;
+      tn[c][qy] = n.x; tn[c][qy + 1] = n.y; tn[c][qy + 2] = n.z; tn[c][qy + 3] = n.w;
+    }
+  } else {
+    for (int c = t >> 6; c < 64; c += 4) {
+      const int x = x0 + c, y = y0 + (t & 63);
+      if (x < W && y < H) {
+        tx[c][t & 63] = vx[(size_t)x * H + y];
+        tn[c][t & 63] = vn[(size_t)x * H + y];
       }
     }
-    if (F.lam_part) F.lam_part[y] = part;
+  }
+  __syncthreads();
+  if (quads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = t + 256 * k, r = i >> 4, qx = (i & 15) * 4;
+      const size_t o = (size_t)(y0 + r) * W + x0 + qx;
+      float v[4], n[4], g0[4], g1[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = tx[qx + e][r];
+        n[e] = tn[qx + e][r];
+      }
+      if (have_conf) {
+        const float4 cf = *(const gptr<const float4>)(conf + o), dv = *(const gptr<const float4>)(d + o);
+        const float cfa[4] = {cf.x, cf.y, cf.z, cf.w}, dva[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          g0[e] = cfa[e] * v[e];
+          g1[e] = dva[e] * v[e] + n[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          g0[e] = v[e];
+          g1[e] = 0.0f;
+        }
+      }
+      if (gd) *(gptr<float4>)(gd + o) = make_float4(g0[0], g0[1], g0[2], g0[3]);
+      if (gc) *(gptr<float4>)(gc + o) = make_float4(g1[0], g1[1], g1[2], g1[3]);
+    }
+  } else {
+    for (int r = t >> 6; r < 64; r += 4) {
+      const int x = x0 + (t & 63), y = y0 + r;
+      if (x < W && y < H) {
+        const size_t i = (size_t)y * W + x;
+        const float v = tx[t & 63][r], n = tn[t & 63][r];
+        if (gd) gd[i] = have_conf ? conf[i] * v : v;
+        if (gc) gc[i] = have_conf ? d[i] * v + n : 0.0f;
+      }
+    }
   }
 }
 
@@ -988,8 +1106,10 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
         b.w[k] = wcol[(size_t)min(c0 + k + 1, H) * W + x];
       }
     };
-    float vxn = 0.0f, vnn = 0.0f, zxn = 0.0f, zyn = 0.0f, carry = 0.0f;
-    double part = 0.0;  // sum over the column's edges of w dL/d(lambda w)
+    AdjEdgeState st;
+    // the weights of the edges that start the two segments (loaded here: a
+    // load inside a round would make the round wait for the next one's inputs)
+    const float w_seg0 = wcol[x], w_seg1 = wcol[(size_t)min(ch.seg, H) * W + x];
     auto out_round = [&](const AdjOutIn& b, int q) __attribute__((always_inline)) {
       if (q < 0 || q >= nch) return;
       const int j = nch - 1 - q;
@@ -1011,51 +1131,53 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
         for (int k = 0; k < kUK; ++k)
           if (k < cnt) adj[c0 + k] = make_float2(XO[k], NO[k]);
       }
+      float ge[kUK];  // edge c0 + k + 1
+      if (cnt == kUK)
+        adj_edges<true>(XO, NO, b, c0, cnt, s0, s1, H, gscale, seg1 ? w_seg1 : w_seg0, st, ge);
+      else
+        adj_edges<false>(XO, NO, b, c0, cnt, s0, s1, H, gscale, seg1 ? w_seg1 : w_seg0, st, ge);
+      // edge c0 + k + 1 of column x: one coalesced store per step
+      if (gw) {
+        if (cnt == kUK) {
 #pragma unroll
-      for (int k = kUK - 1; k >= 0; --k) {
-        if (k < cnt) {
-          const int i = c0 + k;
-          const float vx = XO[k], vn = NO[k], zx = b.zx[k], zy = b.zy[k];
-          if (i + 1 < s1) {
-            const float t = (vx - vxn) * (zx - zxn) + (vn - vnn) * (zy - zyn);
-            if (gw) gw[(size_t)(i + 1) * W + x] = -gscale * t;
-            part += (double)b.w[k] * (double)(-t);
-          } else {
-            const float t = vx * zx + vn * zy;
-            const float e = -gscale * t;
-            part += (double)b.w[k] * (double)(-t);
-            if (gw) gw[(size_t)s1 * W + x] = s1 == H ? e : e + carry;
-          }
-          if (i == s0) {
-            const float t = vx * zx + vn * zy;
-            const float e = -gscale * t;
-            part += (double)wcol[(size_t)s0 * W + x] * (double)(-t);
-            if (s0 == 0) {
-              if (gw) gw[x] = e;
-            } else {
-              carry = e;  // segment 1 is swept first: its share of the split edge
-            }
-          }
-          vxn = vx;
-          vnn = vn;
-          zxn = zx;
-          zyn = zy;
+          for (int k = 0; k < kUK; ++k) gw[(size_t)(c0 + k + 1) * W + x] = ge[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kUK; ++k)
+            if (k < cnt) gw[(size_t)(c0 + k + 1) * W + x] = ge[k];
         }
       }
     };
+    // round r forms chunk r - 4, whose inputs were loaded two rounds ahead:
+    // three buffers take turns (the loop is unrolled by three without
+    // conditions; its last rounds follow it)
     const int rounds = nch + 4;
-    load_o(A, 0);
-    for (int r = 0; r < rounds; r += 2) {
-      load_o(B, r - 3);
+    AdjOutIn C;
+    load_o(A, -4);
+    load_o(B, -3);
+    int r = 0;
+    for (; r + 2 < rounds; r += 3) {
+      load_o(C, r - 2);
       out_round(A, r - 4);
       __syncthreads();
-      if (r + 1 < rounds) {
-        load_o(A, r - 2);
-        out_round(B, r - 3);
-        __syncthreads();
-      }
+      load_o(A, r - 1);
+      out_round(B, r - 3);
+      __syncthreads();
+      load_o(B, r);
+      out_round(C, r - 2);
+      __syncthreads();
     }
-    if (F.lam_part) F.lam_part[(size_t)H + x] = part;
+    if (r < rounds) {
+      out_round(A, r - 4);
+      __syncthreads();
+      ++r;
+    }
+    if (r < rounds) {
+      out_round(B, r - 4);
+      __syncthreads();
+    }
+    if (gw && st.has_first) gw[x] = st.e_first;
+    if (F.lam_part) F.lam_part[(size_t)H + x] = st.part;
   } else {
     // ---------------- out ----------------
     const gptr<float> dense = gp(F.dense);
@@ -1306,6 +1428,17 @@ void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n
     hipLaunchKernelGGL(k_up_row<2>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
   else
     hipLaunchKernelGGL(k_up_row<3>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut);
+}
+
+void launch_adj_dconf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  int counts[kMaxFrames];
+  for (int i = 0; i < n; ++i) {
+    const FrameDev& F = hframes[i];
+    counts[i] = F.adj_rowv && (F.grad_d || F.grad_conf) ? ((F.W + 63) / 64) * ((F.H + 63) / 64) : 0;
+  }
+  const int nb = make_block_map(bm, counts, n);
+  if (nb > 0) hipLaunchKernelGGL(k_adj_dconf, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,

@@ -920,6 +920,11 @@ int qm_upsample_backward_batch(const qm_upsample_problem* probs, int n, void* st
     if ((rc = sc.get(&axn, px * 8))) return rc;
     F.fwd = (float4*)fwd;
     F.adjxn = (float2*)axn;
+    if (q.grad_d || q.grad_conf) {
+      void* rv;  // the row adjoint's solution planes, for dL/dd and dL/dconf
+      if ((rc = sc.get(&rv, px * 8))) return rc;
+      F.adj_rowv = (float*)rv;
+    }
     if (saved) {
       F.rowx = q.state;
       F.colxn = (float2*)(q.state + 2 * px);

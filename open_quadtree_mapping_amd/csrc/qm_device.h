@@ -55,6 +55,7 @@ struct FrameDev {
   // adjoint of the upsample (recomputation buffers and gradients)
   float2* colxn;        // H x W column pass (x, n) or null
   float2* adjxn;        // H x W column-adjoint solution
+  float* adj_rowv;      // the row adjoint's solution: X then N, each a W x H plane [x][y], or null
   const float* g_out;   // dL/d(dense)
   float* grad_d;
   float* grad_conf;

@@ -105,6 +105,8 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
 void launch_upsample_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s,
                               bool recompute = true);
 void launch_fuse_backward(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+// dL/dd and dL/dconf of frames with grad_d / grad_conf from their adj_rowv planes
+void launch_adj_dconf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // pipeline backward: the keyframe upsample's inputs written as explicit arrays
 // into each descriptor's g_d (when want_sparse) / g_wrow / g_wcol
 void launch_compose_inputs(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
