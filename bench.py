@@ -72,6 +72,10 @@ def parse():
                     help="projection arithmetic of the cost volume (qm_params.cost_arith): fused (hoisted "
                          "products, FMA, one exact reciprocal) or the reference's operation order; both are "
                          "bit-identical to the oracle's matching mode")
+    ap.add_argument("--upsample-solver", choices=["scan", "thomas"], default="scan",
+                    help="line solver of the upsample (qm_params.upsample_solver): scan (segmented scans over "
+                         "64 lanes per line) or thomas (the reference's serial recurrences); both are "
+                         "bit-identical to the oracle's matching mode")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -296,7 +300,8 @@ def run_workload(args):
     arith = 1 if args.cost_arith == "fused" else 0
     if wl == "uneven12":
         frames = keyframes(376, 240, 4, 300) + keyframes(752, 480, 4, 310) + keyframes(1280, 720, 4, 320)
-        batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, cost_arith=arith),
+        batch = Batch(frames, default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0, cost_arith=arith,
+                                             upsample_solver=1 if args.upsample_solver == "scan" else 0),
                       outputs=("depth",))
         batch.autotune(reps=args.tune_reps, cache=args.tune_cache or None, refresh=args.retune,
                                seeds=shipped_shapes(args))
@@ -341,6 +346,8 @@ def run_workload(args):
     cfg = {"workload": what}
     if wl != "upsample16":
         cfg["cost_arith"] = args.cost_arith
+        # the batch backward differentiates the Thomas solve
+        cfg["upsample_solver"] = args.upsample_solver if wl == "uneven12" else "thomas"
         cfg["launch_shapes"] = shapes
     print(json.dumps({"workload": wl, "value": round(n / sec, 2), "unit": unit, "ms_per_step": round(sec * 1e3, 4),
                       "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
@@ -394,7 +401,8 @@ def main():
     cam = camera_for(W, H)
     imin, imax = 0.02, 2.0  # 0.5 - 50 m
     arith = 1 if args.cost_arith == "fused" else 0
-    params = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=arith)
+    solver = 1 if args.upsample_solver == "scan" else 0
+    params = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=arith, upsample_solver=solver)
 
     t_setup = time.time()
     nsc = min(args.scenes, B)
@@ -486,7 +494,8 @@ def main():
         valu, vsrc = measured_cost_valu(W, H, N, args.cost_arith)
         if valu is not None:
             c.update({"valu_per_cell_sample": valu, "valu_source": vsrc})
-        other = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=1 - arith)
+        other = default_params(n_hyp=N, idepth_min=imin, idepth_max=imax, cost_arith=1 - arith,
+                               upsample_solver=solver)
         ob = Batch(frames, other, outputs=("depth",))
         ob.set_tuning(**batch.tuning())
         ob.run_timed()
@@ -542,7 +551,7 @@ def main():
         seeds = np.zeros((H, W, 4), np.float32)
         eye = np.eye(3, 4, dtype=np.float32)
         kf = lambda: O.keyframe(imgs[5], [imgs[4], imgs[3], imgs[2], imgs[1]], T, cam, N, imin, imax, seeds, eye,
-                                arith=arith)
+                                arith=arith, solver=solver)
         kf()  # warm-up
         per_stage, ends = [], []
         tc0 = time.perf_counter()
@@ -557,7 +566,7 @@ def main():
         med = np.median(np.array(per_stage), axis=0)
         cpu = {"value": round(len(ends) / tc, 3), "unit": "depth maps/s", "cores": threads, "kind": "port",
                "sample": f"{len(ends)} keyframes of {W}x{H}x{N} after one warm-up, full pipeline, FP32 OpenMP "
-                         f"oracle ({args.cost_arith} cost arithmetic), {tc:.1f}s",
+                         f"oracle ({args.cost_arith} cost arithmetic, {args.upsample_solver} upsample), {tc:.1f}s",
                "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
                "median_ms": round(float(np.median(ends)), 3),
                "stage_median_ms": {k: round(float(v), 3) for k, v in
@@ -582,6 +591,7 @@ def main():
                                "(quadtree, cost volume, HBP, upsample, fusion), "
                                + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
                    "streams": B, "width": W, "height": H, "hypotheses": N, "cost_arith": args.cost_arith,
+                   "upsample_solver": args.upsample_solver,
                    "launch_shapes": batch_tuning,
                    "launch_shapes_source": ("defaults" if tuned is None else
                                             "cache" if all(ms is None for _, ms in tuned.values()) else "autotune")},

@@ -51,6 +51,9 @@ typedef struct qm_params {
   int cost_arith;       /* projection arithmetic of the cost volume, QM_COST_*: both give
                            the same algebra, and both are bit-identical to the oracle's
                            matching mode; default QM_COST_REFERENCE */
+  int upsample_solver;  /* line solver of the global upsample, QM_UPSAMPLE_*: both solve
+                           the same systems, and both are bit-identical to the oracle's
+                           matching mode; default QM_UPSAMPLE_THOMAS */
 } qm_params;
 
 /* Cost-volume projection arithmetic (qm_params.cost_arith).
@@ -67,9 +70,26 @@ typedef struct qm_params {
 #define QM_COST_REFERENCE 0
 #define QM_COST_FUSED 1
 
+/* Line solver of the global upsample (qm_params.upsample_solver), for the
+ * row and column systems (I + lambda L(w)) x = r of src/depth_upsample.cu:94-302.
+ * QM_UPSAMPLE_THOMAS: the reference's Thomas recurrences in its order, one
+ *   serial chain per line.
+ * QM_UPSAMPLE_SCAN: the same recurrences by segmented scans: 64 lanes per
+ *   line each own a run of steps; the maps of the three recurrences (a
+ *   Moebius map for the coefficients, affine maps for the right-hand sides
+ *   and the back substitution) are composed per lane and scanned over the
+ *   lanes in double, and each lane then runs its steps in float from its
+ *   incoming state. As accurate as the Thomas order against FP64; a
+ *   different rounding. Lines longer than 1536 steps use the Thomas solve.
+ *   Applies to the pipeline and qm_upsample; the differentiable entry points
+ *   (qm_upsample_general*, qm_upsample_backward*, qm_batch_backward) use the
+ *   Thomas solve and qm_batch_backward refuses a batch run with the scan. */
+#define QM_UPSAMPLE_THOMAS 0
+#define QM_UPSAMPLE_SCAN 1
+
 /* Version of this ABI; bindings check it at load time so that a stale library
  * fails loudly instead of being called with the wrong argument list. */
-#define QM_ABI_VERSION 12
+#define QM_ABI_VERSION 13
 int qm_abi_version(void);
 const char* qm_last_error(void);
 void qm_default_params(qm_params* p);

@@ -21,6 +21,7 @@ STAGES = ("quadtree", "cost", "pyramid", "bp_sweep", "extract", "upsample_row", 
           "fuse_update")
 TUNE_KNOBS = ("cost_waves", "bp_rows", "extract_rows", "overlap", "bp_pair", "bp_seg", "bp_pair_levels")  # QM_TUNE_* order
 COST_REFERENCE, COST_FUSED = 0, 1  # QM_COST_*: qm_params.cost_arith
+UPSAMPLE_THOMAS, UPSAMPLE_SCAN = 0, 1  # QM_UPSAMPLE_*: qm_params.upsample_solver
 
 
 class QmError(RuntimeError):
@@ -41,6 +42,7 @@ class qm_params(C.Structure):
         ("ref_cos", C.c_float),
         ("ref_baseline", C.c_float),
         ("cost_arith", C.c_int),
+        ("upsample_solver", C.c_int),
     ]
 
 
@@ -88,7 +90,7 @@ class qm_upsample_problem(C.Structure):
     ]
 
 
-ABI_VERSION = 12  # QM_ABI_VERSION of include/qm.h this binding is written against
+ABI_VERSION = 13  # QM_ABI_VERSION of include/qm.h this binding is written against
 
 # name -> (restype, argtypes); mirrors include/qm.h
 SIGNATURES = {

@@ -160,7 +160,8 @@ class Batch:
         {knob: (value, ms)} (group_tuning gives the others').
         cache: a JSON file of earlier choices, keyed by the device, each
         group's size, frame count, hypotheses, cost arithmetic and sweep counts
-        (shape knobs), and the batch's whole composition (overlap knob). When
+        (shape knobs), and the batch's whole composition and upsample solver
+        (overlap knob). When
         it holds every group's and the batch's entries they are applied without
         timing (ms None); else (or with refresh) the batch is tuned and its
         choices are written there. seeds: further read-only JSON files of the
@@ -197,7 +198,10 @@ class Batch:
         sizes = [tuple(f["key"].shape[::-1]) for f in self.frames]
         groups = self.groups()
         per_group = [f"{dev}|{W}x{H}|frames{sizes.count((W, H))}|{common}" for (W, H) in groups]
-        whole = f"{dev}|batch:" + ";".join(f"{W}x{H}" for (W, H) in sizes) + f"|{common}"
+        # the overlap knob also depends on the upsample's line solver (the scan
+        # solver's passes are not latency-bound); Thomas batches keep their keys
+        solver = f"|solver{p.upsample_solver}" if p.upsample_solver else ""
+        whole = f"{dev}|batch:" + ";".join(f"{W}x{H}" for (W, H) in sizes) + f"|{common}{solver}"
         return per_group, whole
 
     def set_tuning(self, **knobs):

@@ -1399,14 +1399,16 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-static int rows_map(BlockMap& bm, const FrameDev* hframes, int n) {
+// rows_map / cols_map: the Thomas kernels' blocks (64 lines each) for the
+// frames with only[i] != 0 (every frame when only is null)
+static int rows_map(BlockMap& bm, const FrameDev* hframes, int n, const int* only = nullptr) {
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].H + 63) / 64;
+  for (int i = 0; i < n; ++i) counts[i] = (!only || only[i]) ? (hframes[i].H + 63) / 64 : 0;
   return make_block_map(bm, counts, n);
 }
-static int cols_map(BlockMap& bm, const FrameDev* hframes, int n) {
+static int cols_map(BlockMap& bm, const FrameDev* hframes, int n, const int* only = nullptr) {
   int counts[kMaxFrames];
-  for (int i = 0; i < n; ++i) counts[i] = (hframes[i].W + 63) / 64;
+  for (int i = 0; i < n; ++i) counts[i] = (!only || only[i]) ? (hframes[i].W + 63) / 64 : 0;
   return make_block_map(bm, counts, n);
 }
 static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
@@ -1416,9 +1418,14 @@ static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
 }
 
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         hipStream_t s) {
+                         hipStream_t s, int solver) {
+  // the segmented-scan solver (modes 0 and 1) takes the frames whose rows fit
+  // its lanes; the Thomas kernels the rest
+  int rest[kMaxFrames];
+  const bool scan = solver == QM_UPSAMPLE_SCAN && mode <= 1;
+  if (scan && launch_upsample_row_scan(dframes, hframes, n, mode, wlut, s, rest) == 0) return;
   BlockMap bm;
-  const int nb = rows_map(bm, hframes, n);
+  const int nb = rows_map(bm, hframes, n, scan ? rest : nullptr);
   if (nb <= 0) return;
   if (mode == 0)
     hipLaunchKernelGGL(k_up_row<0>, dim3(nb), dim3(320), 0, s, dframes, bm, wlut);
@@ -1442,9 +1449,12 @@ void launch_adj_dconf(const FrameDev* dframes, const FrameDev* hframes, int n, h
 }
 
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         bool clear_zbuf, hipStream_t s) {
+                         bool clear_zbuf, hipStream_t s, int solver) {
+  int rest[kMaxFrames];
+  const bool scan = solver == QM_UPSAMPLE_SCAN && mode <= 1;
+  if (scan && launch_upsample_col_scan(dframes, hframes, n, wlut, clear_zbuf, s, rest) == 0) return;
   BlockMap bm;
-  const int nb = cols_map(bm, hframes, n);
+  const int nb = cols_map(bm, hframes, n, scan ? rest : nullptr);
   if (nb <= 0) return;
   if (mode == 2)
     hipLaunchKernelGGL(k_up_col<2>, dim3(nb), dim3(384), 0, s, dframes, bm, wlut, (int)clear_zbuf);
@@ -1473,9 +1483,9 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
 }
 
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                     bool clear_zbuf, hipStream_t s) {
-  launch_upsample_row(dframes, hframes, n, mode, wlut, s);
-  launch_upsample_col(dframes, hframes, n, mode, wlut, clear_zbuf, s);
+                     bool clear_zbuf, hipStream_t s, int solver) {
+  launch_upsample_row(dframes, hframes, n, mode, wlut, s, solver);
+  launch_upsample_col(dframes, hframes, n, mode, wlut, clear_zbuf, s, solver);
 }
 
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s) {

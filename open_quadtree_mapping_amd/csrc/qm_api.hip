@@ -222,6 +222,8 @@ int check_params(const qm_params* p) {
   if (p->n_hyp != 64 && p->n_hyp != 128) return fail(QM_ERR_INVALID, "n_hyp must be 64 or 128");
   if (p->cost_arith != QM_COST_REFERENCE && p->cost_arith != QM_COST_FUSED)
     return fail(QM_ERR_INVALID, "cost_arith must be QM_COST_REFERENCE or QM_COST_FUSED");
+  if (p->upsample_solver != QM_UPSAMPLE_THOMAS && p->upsample_solver != QM_UPSAMPLE_SCAN)
+    return fail(QM_ERR_INVALID, "upsample_solver must be QM_UPSAMPLE_THOMAS or QM_UPSAMPLE_SCAN");
   if (!(p->idepth_max > p->idepth_min) || !(p->idepth_min > 0.0))
     return fail(QM_ERR_INVALID, "need 0 < idepth_min < idepth_max");
   for (int i = 0; i < 4; ++i)
@@ -512,12 +514,12 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
     }
     case QM_STAGE_UPSAMPLE_ROW: {
       Mark m(t, QM_STAGE_UPSAMPLE_ROW);
-      launch_upsample_row(df, hf, n, 0, b->wlut, s);
+      launch_upsample_row(df, hf, n, 0, b->wlut, s, p->upsample_solver);
       break;
     }
     case QM_STAGE_UPSAMPLE_COL: {
       Mark m(t, QM_STAGE_UPSAMPLE_COL);
-      launch_upsample_col(df, hf, n, 0, b->wlut, true, s);
+      launch_upsample_col(df, hf, n, 0, b->wlut, true, s, p->upsample_solver);
       break;
     }
     case QM_STAGE_FUSE_WARP: {
@@ -685,6 +687,7 @@ void qm_default_params(qm_params* p) {
   p->ref_cos = 0.95f;
   p->ref_baseline = 0.03f;
   p->cost_arith = QM_COST_REFERENCE;
+  p->upsample_solver = QM_UPSAMPLE_THOMAS;
 }
 
 void qm_set_alloc_cap(size_t bytes) { g_cap.store(bytes); }
@@ -816,7 +819,7 @@ int qm_upsample(const float* sparse, const uint8_t* img, int W, int H, int pitch
   F.rowx = (float*)rxn;
   QM_HIP(hipMemcpyAsync(df, &F, sizeof(F), hipMemcpyHostToDevice, s));
   QM_HIP(hipMemcpyAsync(dl, lut, sizeof(lut), hipMemcpyHostToDevice, s));
-  launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, false, s);
+  launch_upsample((FrameDev*)df, &F, 1, 1, (const float*)dl, false, s, p->upsample_solver);
   QM_HIP(hipGetLastError());
   QM_HIP(hipStreamSynchronize(s));
   sc.done();
@@ -1726,6 +1729,9 @@ int qm_batch_destroy(qm_batch* b) {
 int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad_lambda, void* stream) {
   if (!b || !grad_lambda) return fail(QM_ERR_INVALID, "NULL pointer");
   if (b->runs == 0) return fail(QM_ERR_STATE, "no forward run to differentiate");
+  if (b->p.upsample_solver != QM_UPSAMPLE_THOMAS)
+    return fail(QM_ERR_INVALID, "the batch backward differentiates the Thomas upsample; create the batch with "
+                                "upsample_solver = QM_UPSAMPLE_THOMAS");
   if (!b->run_state_valid)
     return fail(QM_ERR_STATE,
                 "the batch changed after its last run (set_frame, reset_seeds or autotune); run it again before "

@@ -81,14 +81,23 @@ void launch_copy_messages(const FrameDev* dframes, const FrameDev* hframes, int 
 void launch_cost_to_rows(const float* in, float* out, size_t ncell, int N, hipStream_t s);
 // upsample: mode 0 = compose the sparse map from BP (+ pre-fill), weights from
 // the keyframe; 1 = explicit sparse map, weights from the keyframe;
-// 2 = generalised (d, conf, w_row, w_col given); lambda is FrameDev::lam
+// 2 = generalised (d, conf, w_row, w_col given); lambda is FrameDev::lam.
+// solver QM_UPSAMPLE_SCAN runs modes 0 and 1 by segmented scans (k_scan.hip)
+// for the frames whose lines fit it; everything else runs the Thomas kernels.
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                     bool clear_zbuf, hipStream_t s);
+                     bool clear_zbuf, hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s);
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         hipStream_t s);
+                         hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         bool clear_zbuf, hipStream_t s);
+                         bool clear_zbuf, hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
+// the scan kernels for the frames whose lines fit them (rows of up to 1536
+// pixels, columns likewise); rest[i] = 1 marks a frame left to the Thomas
+// kernels; returns how many there are
+int launch_upsample_row_scan(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
+                             hipStream_t s, int* rest);
+int launch_upsample_col_scan(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
+                             bool clear_zbuf, hipStream_t s, int* rest);
 // the reference's prior-driven alternates; frames without the prior get no work
 void launch_quadtree_prior(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_prior_cost(const FrameDev* dframes, const FrameDev* hframes, int n, int N, bool natural,

@@ -58,15 +58,15 @@ def lib():
         L.qmo_weights_f64.argtypes = [P, I, I, I, P, P]
         L.qmo_weight_of_delta_f32.argtypes = [I]
         L.qmo_weight_of_delta_f32.restype = F
-        L.qmo_upsample_f32.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P]
-        L.qmo_upsample_f64.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P]
+        L.qmo_upsample_f32.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P, I]
+        L.qmo_upsample_f64.argtypes = [I, I, P, P, P, P, D, I, I, P, P, P, I]
         L.qmo_fuse_f32.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
         L.qmo_fuse_f64.argtypes = [I, I, F, F, F, F, P, P, P, P, P, P]
         L.qmo_fuse_update_f32.argtypes = [I, I, P, P, P, P, P]
         L.qmo_fuse_update_f64.argtypes = [I, I, P, P, P, P, P]
         L.qmo_compose_sparse_f32.argtypes = [I, I, P, P, P, P]
         L.qmo_compose_sparse_f64.argtypes = [I, I, P, P, P, P]
-        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P, P, P, I]
+        L.qmo_keyframe_f32.argtypes = [P, P, I, I, P, F, F, F, F, I, D, D, P, P, P, P, P, P, P, P, P, P, I, I]
         L.qmo_sd_gradient.argtypes = [P, I, I, I, P]
         L.qmo_sd_init.argtypes = [P, I, I, P]
         L.qmo_sd_step.argtypes = [P, P, P, P, P, I, I, I, F, F, F, F, P, P, I, P, P]
@@ -121,6 +121,7 @@ def depth_table(N: int, imin: float, imax: float, fp64: bool = False) -> np.ndar
 
 
 COST_REFERENCE, COST_FUSED = 0, 1  # projection arithmetic (the library's QM_COST_*)
+UPSAMPLE_THOMAS, UPSAMPLE_SCAN = 0, 1  # line solver of the upsample (the library's QM_UPSAMPLE_*)
 
 
 def cost_volume(key, refs, T, cam, N, imin, imax, levels, fp64=False, arith=COST_REFERENCE):
@@ -174,7 +175,13 @@ def weight_lut():
     return np.array([lib().qmo_weight_of_delta_f32(i) for i in range(256)], np.float32)
 
 
-def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None, fp64=False, want_rows=False):
+def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None, fp64=False, want_rows=False,
+             solver=UPSAMPLE_THOMAS):
+    """The global upsample of src/depth_upsample.cu:20-302, generalised
+    (confidences, explicit weights). solver UPSAMPLE_THOMAS: the reference's
+    Thomas recurrences and order; UPSAMPLE_SCAN: the segmented-scan solve of the
+    same systems (64 lanes per line, lines up to 64 * 24 steps; longer lines use
+    Thomas)."""
     dt = _dt(fp64)
     d = np.ascontiguousarray(d, dtype=dt)
     H, W = d.shape
@@ -187,7 +194,7 @@ def upsample(d, w_row, w_col, conf=None, lam=10.0, row_seg=None, col_seg=None, f
     rx = np.zeros((H, W), dt)
     rn = np.zeros((H, W), dt)
     fn = lib().qmo_upsample_f64 if fp64 else lib().qmo_upsample_f32
-    fn(W, H, _p(d), _p(conf), _p(w_row), _p(w_col), lam, row_seg, col_seg, _p(out), _p(rx), _p(rn))
+    fn(W, H, _p(d), _p(conf), _p(w_row), _p(w_col), lam, row_seg, col_seg, _p(out), _p(rx), _p(rn), int(solver))
     return (out, rx, rn) if want_rows else out
 
 
@@ -237,7 +244,7 @@ def compose_sparse(W, H, depth_coarse, label, prefill=None, fp64=False):
 
 
 def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=None, prior=None, prior_depth=None,
-             arith=COST_REFERENCE):
+             arith=COST_REFERENCE, solver=UPSAMPLE_THOMAS):
     """Full FP32 keyframe. seeds (HxWx4 float32) is updated in place.
     Returns dict(depth, sparse, levels, label, stage_ms)."""
     key = np.ascontiguousarray(key, dtype=np.uint8)
@@ -256,7 +263,7 @@ def keyframe(key, refs, T, cam, N, imin, imax, seeds, T_last_to_cur, prefill=Non
     ms = np.zeros(5, np.float64)
     lib().qmo_keyframe_f32(_p(key), _ptr_array(refs), W, H, _p(T), cam.fx, cam.fy, cam.cx, cam.cy, N, imin, imax,
                            _p(pf), _p(pr), _p(pd), _p(seeds), _p(Tl), _p(depth), _p(sparse), _p(levels), _p(label),
-                           _p(ms), int(arith))
+                           _p(ms), int(arith), int(solver))
     return dict(depth=depth, sparse=sparse, levels=levels, label=label, stage_ms=ms)
 
 

@@ -38,6 +38,7 @@
 //   * exp(x) for float is (float)exp((double)x); 1/sqrt is 1.0f/sqrtf.
 
 #include <cstdint>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
@@ -508,9 +509,213 @@ void solve_line(int n, const Real* w /*n+1 couplings, stride ws*/, int ws, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same line systems by segmented scans (qm_params.upsample_solver =
+// QM_UPSAMPLE_SCAN): P lanes (32 for lines of up to 512 steps, else 64), lane i
+// running steps [i m, (i + 1) m), m = ceil(n / P); steps past the
+// line's end pad it (a = 0, b = 1, c = 0, zero
+// right-hand sides: their states stay finite and never reach a real step).
+// The three recurrences of the Thomas solve are maps of their previous state,
+// so each lane composes its steps' map, an exclusive scan over the lanes gives
+// every lane its incoming state, and the lane then runs its steps from it:
+//   ch   ch_k = c_k / (b_k - ch_{k-1} a_k): a Moebius map, the 2 x 2 matrix
+//        [[0, c_k], [-a_k, b_k]] on (p, q) with ch = p / q;
+//   x, n dh_k = (r_k - dh_{k-1} a_k) y_k, y_k = 1 / den_k: affine (U, V);
+//   X, N X_k = dh_k - ch_k X_{k+1} backward: affine (P, Q), a suffix scan.
+// A chain start k (0 and seg) uses a_k = 0 in the recurrences (its state is 0
+// before it), a chain end (seg - 1 and n - 1) uses ch_k = 0 in the backward
+// one. The Moebius lane maps and their scan are in double (their products
+// cancel badly in float when lambda is large), rescaled by a power of two
+// (exact) after every 8th step of a lane and at its end; the affine maps are in
+// float. The scans follow one fixed tree
+// (scan_tree). The arithmetic is fixed here and in the HIP kernels
+// (k_scan.hip), which match it bit for bit; it is a different rounding of the
+// same solve as solve_line. Lines with more than 64 kScanMaxSeg steps use
+// solve_line.
+// ---------------------------------------------------------------------------
+constexpr int kScanLanes = 64;
+constexpr int kScanMaxSeg = 24;
+
+template <typename Real>
+struct Mob {
+  Real g11, g12, g21, g22;
+};
+
+// G * 2^-e with e the binary exponent of max |entry| (frexp): exact
+template <typename Real>
+inline void mob_norm(Mob<Real>& G) {
+  Real mx = std::fabs(G.g11), t = std::fabs(G.g12);
+  mx = t > mx ? t : mx;
+  t = std::fabs(G.g21);
+  mx = t > mx ? t : mx;
+  t = std::fabs(G.g22);
+  mx = t > mx ? t : mx;
+  if (!(mx > Real(0)) || !(mx <= std::numeric_limits<Real>::max())) return;
+  int e;
+  (void)std::frexp(mx, &e);
+  G.g11 = std::ldexp(G.g11, -e);
+  G.g12 = std::ldexp(G.g12, -e);
+  G.g21 = std::ldexp(G.g21, -e);
+  G.g22 = std::ldexp(G.g22, -e);
+}
+
+// A B (B applied first); the scan's products are not rescaled (six levels of
+// products of rescaled lane maps stay far inside the double range)
+template <typename Real>
+inline Mob<Real> mob_mul(const Mob<Real>& A, const Mob<Real>& B) {
+  Mob<Real> R;
+  R.g11 = A.g11 * B.g11 + A.g12 * B.g21;
+  R.g12 = A.g11 * B.g12 + A.g12 * B.g22;
+  R.g21 = A.g21 * B.g11 + A.g22 * B.g21;
+  R.g22 = A.g21 * B.g12 + A.g22 * B.g22;
+  return R;
+}
+
+// The scans over the 64 lanes, one fixed tree (the HIP kernels' scan_tree):
+// Hillis-Steele inside each row of 16 lanes (offsets 1, 2, 4, 8), then rows 1
+// and 3 combine with the last lane of rows 0 and 2, then rows 2 and 3 with
+// lane 31; the suffix scan mirrors it (rows 0 and 2 with the first lane of
+// rows 1 and 3, then rows 0 and 1 with lane 32). comb(a, b) combines lane
+// value a with b, the value of lanes before it (prefix) or after it (suffix).
+template <typename T, typename Comb>
+void scan_tree(std::vector<T>& v, bool suffix, Comb comb) {
+  const int P = (int)v.size();  // 32 (rows 0 and 1 of the tree) or 64
+  for (int o = 1; o < 16; o <<= 1) {
+    const std::vector<T> t = v;
+    for (int i = 0; i < P; ++i) {
+      const int j = i & 15;
+      if (suffix ? j + o < 16 : j >= o) v[i] = comb(t[i], t[suffix ? i + o : i - o]);
+    }
+  }
+  {
+    const std::vector<T> t = v;
+    for (int i = 0; i < P; ++i) {
+      const int row = i >> 4;
+      if (suffix ? (row & 1) == 0 : (row & 1) == 1)
+        v[i] = comb(t[i], t[suffix ? (row < 2 ? 16 : 48) : (row < 2 ? 15 : 47)]);
+    }
+  }
+  if (P == 64) {
+    const std::vector<T> t = v;
+    for (int i = 0; i < P; ++i) {
+      const int row = i >> 4;
+      if (suffix ? row < 2 : row >= 2) v[i] = comb(t[i], t[suffix ? 32 : 31]);
+    }
+  }
+}
+
+template <typename Real>
+void solve_line_scan(int P, int n, const Real* w, int ws, const Real* rx, const Real* rn, int rstride, Real* ox,
+                     Real* on, int ostride, int seg, double lam) {
+  using D = double;  // the Moebius lane maps and their scan
+  const int m = (n + P - 1) / P, L = P * m;
+  const int sg = seg < n ? seg : n;
+  // every lane runs m steps; steps k >= n pad the line (a = 0, b = 1, c = 0,
+  // zero right-hand sides) and their results are dropped
+  std::vector<Real> A(L, Real(0)), B(L, Real(1)), Cc(L, Real(0)), R(L, Real(0)), RN(L, Real(0));
+  std::vector<Real> Y(L), CH(L), DH(L), NH(L);
+  for (int k = 0; k < n; ++k) {
+    thomas_coeffs<Real>(w[(size_t)k * ws], w[(size_t)(k + 1) * ws], lam, A[k], B[k], Cc[k]);
+    if (k == 0 || k == sg) A[k] = Real(0);
+    R[k] = rx[(size_t)k * rstride];
+    RN[k] = rn[(size_t)k * rstride];
+  }
+  // ---- ch: lane maps (rescaled after every 8th step and at the end), scan, steps ----
+  std::vector<Mob<D>> G(P);
+  for (int i = 0; i < P; ++i) {
+    Mob<D> g{D(1), D(0), D(0), D(1)};
+    for (int t = 0; t < m; ++t) {
+      const int k = i * m + t;
+      Mob<D> q;
+      q.g11 = (D)Cc[k] * g.g21;
+      q.g12 = (D)Cc[k] * g.g22;
+      q.g21 = (D)B[k] * g.g21 - (D)A[k] * g.g11;
+      q.g22 = (D)B[k] * g.g22 - (D)A[k] * g.g12;
+      if ((t & 7) == 7) mob_norm(q);
+      g = q;
+    }
+    mob_norm(g);
+    G[i] = g;
+  }
+  scan_tree(G, false, [](const Mob<D>& a, const Mob<D>& b) { return mob_mul(a, b); });
+  for (int i = 0; i < P; ++i) {
+    Real st = i == 0 ? Real(0) : (Real)(G[i - 1].g12 / G[i - 1].g22);
+    for (int k = i * m; k < (i + 1) * m; ++k) {
+      const Real den = B[k] - st * A[k];
+      Y[k] = Real(1) / den;
+      st = Cc[k] * Y[k];
+      CH[k] = st;
+    }
+  }
+  // ---- x, n: lane maps (U, V), scan, steps ----
+  struct Aff {
+    Real u, w, v;  // two intercepts sharing one slope
+  };
+  auto aff = [](const Aff& a, const Aff& b) { return Aff{a.u + a.v * b.u, a.w + a.v * b.w, a.v * b.v}; };
+  std::vector<Aff> U(P);
+  for (int i = 0; i < P; ++i) {
+    Real ux = 0, un = 0, v = 1;
+    for (int k = i * m; k < (i + 1) * m; ++k) {
+      ux = (R[k] - ux * A[k]) * Y[k];
+      un = (RN[k] - un * A[k]) * Y[k];
+      v = -(A[k] * Y[k]) * v;
+    }
+    U[i] = Aff{ux, un, v};
+  }
+  scan_tree(U, false, aff);
+  for (int i = 0; i < P; ++i) {
+    Real dx = i == 0 ? Real(0) : U[i - 1].u, dn = i == 0 ? Real(0) : U[i - 1].w;
+    for (int k = i * m; k < (i + 1) * m; ++k) {
+      dx = (R[k] - dx * A[k]) * Y[k];
+      dn = (RN[k] - dn * A[k]) * Y[k];
+      DH[k] = dx;
+      NH[k] = dn;
+    }
+  }
+  // ---- X, N: lane maps (P, Q) backward, suffix scan, steps ----
+  auto cut = [&](int k) { return (k == sg - 1 || k == n - 1) ? Real(0) : CH[k]; };
+  std::vector<Aff> Q(P);
+  for (int i = 0; i < P; ++i) {
+    Real px = 0, pn = 0, q = 1;
+    for (int k = (i + 1) * m - 1; k >= i * m; --k) {
+      const Real c = cut(k);
+      px = DH[k] - c * px;
+      pn = NH[k] - c * pn;
+      q = -c * q;
+    }
+    Q[i] = Aff{px, pn, q};
+  }
+  scan_tree(Q, true, aff);
+  for (int i = 0; i < P; ++i) {
+    Real xv = i == P - 1 ? Real(0) : Q[i + 1].u, nv = i == P - 1 ? Real(0) : Q[i + 1].w;
+    for (int k = (i + 1) * m - 1; k >= i * m; --k) {
+      const Real c = cut(k);
+      xv = DH[k] - c * xv;
+      nv = NH[k] - c * nv;
+      if (k < n) {
+        ox[(size_t)k * ostride] = xv;
+        on[(size_t)k * ostride] = nv;
+      }
+    }
+  }
+}
+
+// one line by the chosen solver: the scan over 32 lanes for lines of up to 512
+// steps, over 64 lanes up to 64 kScanMaxSeg, Thomas beyond
+template <typename Real>
+void solve_line_by(int solver, int n, const Real* w, int ws, const Real* rx, const Real* rn, int rstride, Real* ox,
+                   Real* on, int ostride, int seg, double lam, Real* tmp) {
+  if (solver == 1 && n <= 512)
+    solve_line_scan<Real>(32, n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam);
+  else if (solver == 1 && n <= kScanLanes * kScanMaxSeg)
+    solve_line_scan<Real>(64, n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam);
+  else
+    solve_line<Real>(n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam, tmp);
+}
+
 template <typename Real>
 void upsample_impl(int W, int H, const Real* d, const Real* conf, const Real* w_row, const Real* w_col,
-                   double lam, int row_seg, int col_seg, Real* out, Real* row_x, Real* row_n) {
+                   double lam, int row_seg, int col_seg, Real* out, Real* row_x, Real* row_n, int solver = 0) {
   std::vector<Real> rxv, rnv, rhx, rhn;
   Real* RX = row_x;
   Real* RN = row_n;
@@ -538,8 +743,9 @@ void upsample_impl(int W, int H, const Real* d, const Real* conf, const Real* w_
     std::vector<Real> tmp(3 * (size_t)(W > H ? W : H));
 #pragma omp for schedule(static)
     for (int y = 0; y < H; ++y)
-      solve_line<Real>(W, w_row + (size_t)y * (W + 1), 1, rhx.data() + (size_t)y * W, rhn.data() + (size_t)y * W, 1,
-                       RX + (size_t)y * W, RN + (size_t)y * W, 1, row_seg, lam, tmp.data());
+      solve_line_by<Real>(solver, W, w_row + (size_t)y * (W + 1), 1, rhx.data() + (size_t)y * W,
+                          rhn.data() + (size_t)y * W, 1, RX + (size_t)y * W, RN + (size_t)y * W, 1, row_seg, lam,
+                          tmp.data());
   }
   std::vector<Real> cx((size_t)W * H), cn((size_t)W * H);
 #pragma omp parallel
@@ -547,8 +753,8 @@ void upsample_impl(int W, int H, const Real* d, const Real* conf, const Real* w_
     std::vector<Real> tmp(3 * (size_t)(W > H ? W : H));
 #pragma omp for schedule(static)
     for (int x = 0; x < W; ++x)
-      solve_line<Real>(H, w_col + x, W, RX + x, RN + x, W, cx.data() + x, cn.data() + x, W, col_seg, lam,
-                       tmp.data());
+      solve_line_by<Real>(solver, H, w_col + x, W, RX + x, RN + x, W, cx.data() + x, cn.data() + x, W, col_seg,
+                          lam, tmp.data());
   }
   for (size_t i = 0; i < (size_t)W * H; ++i) out[i] = cx[i] / cn[i];
 }
@@ -821,12 +1027,12 @@ void qmo_weights_f64(const uint8_t* img, int W, int H, int pitch, double* w_row,
 float qmo_weight_of_delta_f32(int delta) { return weight_of_delta<float>(delta); }
 
 void qmo_upsample_f32(int W, int H, const float* d, const float* conf, const float* w_row, const float* w_col,
-                      double lam, int row_seg, int col_seg, float* out, float* row_x, float* row_n) {
-  upsample_impl<float>(W, H, d, conf, w_row, w_col, lam, row_seg, col_seg, out, row_x, row_n);
+                      double lam, int row_seg, int col_seg, float* out, float* row_x, float* row_n, int solver) {
+  upsample_impl<float>(W, H, d, conf, w_row, w_col, lam, row_seg, col_seg, out, row_x, row_n, solver);
 }
 void qmo_upsample_f64(int W, int H, const double* d, const double* conf, const double* w_row, const double* w_col,
-                      double lam, int row_seg, int col_seg, double* out, double* row_x, double* row_n) {
-  upsample_impl<double>(W, H, d, conf, w_row, w_col, lam, row_seg, col_seg, out, row_x, row_n);
+                      double lam, int row_seg, int col_seg, double* out, double* row_x, double* row_n, int solver) {
+  upsample_impl<double>(W, H, d, conf, w_row, w_col, lam, row_seg, col_seg, out, row_x, row_n, solver);
 }
 
 void qmo_fuse_f32(int W, int H, float fx, float fy, float cx, float cy, const float* T, float* seeds,
@@ -876,7 +1082,7 @@ void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int
                       float fy, float cx, float cy, int N, double imin, double imax, const float* prefill,
                       const float* prior, const float* prior_depth, float* seeds, const float* T_last_to_cur,
                       float* depth_out, float* sparse_out, uint8_t* levels_out, int32_t* label_out, double* stage_ms,
-                      int arith) {
+                      int arith, int solver) {
   using clk = std::chrono::steady_clock;
   auto ms = [](clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
@@ -918,7 +1124,7 @@ void qmo_keyframe_f32(const uint8_t* key, const uint8_t* const* refs, int W, int
   auto t3 = clk::now();
   weights_impl<float>(key, W, H, W, wr.data(), wc.data());
   upsample_impl<float>(W, H, sparse, nullptr, wr.data(), wc.data(), 10.0, 16 * (W / 16), 8 * (H / 8), dense.data(),
-                       nullptr, nullptr);
+                       nullptr, nullptr, solver);
   auto t4 = clk::now();
   fuse_impl<float>(W, H, c, T_last_to_cur, seeds, dense.data(), depth_out, seeds_new.data(), nullptr);
   std::memcpy(seeds, seeds_new.data(), sizeof(float) * npx * 4);
