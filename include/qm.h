@@ -80,7 +80,8 @@ typedef struct qm_params {
  *   and the back substitution) are composed per lane and scanned over the
  *   lanes in double, and each lane then runs its steps in float from its
  *   incoming state. As accurate as the Thomas order against FP64; a
- *   different rounding. Lines longer than 1536 steps use the Thomas solve.
+ *   different rounding. Lines longer than 1536 steps, and frames with lambda
+ *   above 2^100, use the Thomas solve.
  *   Applies to the pipeline and qm_upsample; the differentiable entry points
  *   (qm_upsample_general*, qm_upsample_backward*, qm_batch_backward) use the
  *   Thomas solve and qm_batch_backward refuses a batch run with the scan. */

@@ -12,13 +12,16 @@
 // Thomas solve is a map of its previous state:
 //   ch   ch_k = c_k / (b_k - ch_{k-1} a_k), a Moebius map: the 2 x 2 matrix
 //        [[0, c_k], [-a_k, b_k]] on (p, q), ch = p / q;
-//   x, n dh_k = (r_k - dh_{k-1} a_k) y_k with y_k = 1 / den_k: affine;
-//   X, N X_k = dh_k - ch_k X_{k+1}, backward: affine.
+//   x, n dh_k = (r_k - dh_{k-1} a_k) y_k with y_k = 1 / den_k: affine; a lane
+//        keeps each step's value from a zero start and its slope, and forms
+//        dh_k = fma(slope, incoming, value) once the scan has given the
+//        incoming state;
+//   X, N X_k = dh_k - ch_k X_{k+1}, backward: affine, the same way.
 // A lane composes its steps' map (in double), a scan over the 64 lanes
 // (double; DPP moves inside rows of 16 lanes, lane broadcasts across rows)
 // gives each lane the composition of all lanes before it (after it, for the
-// backward map), and the lane then runs its own steps in float from the
-// incoming state.
+// backward map); the lane then runs its steps of the coefficient chain in
+// float from the incoming state and combines the affine ones with it.
 // A chain start (0 and seg) takes a_k = 0 in the recurrences, a chain end
 // (seg - 1 and n - 1) ch_k = 0 in the backward one. Matrices are rescaled by a
 // power of two (exact) every 8 steps of a lane and after every product of the
@@ -142,14 +145,11 @@ struct Aff3 {
 
 // RN(1 / z): the hardware reciprocal refined by one FMA Newton step equals the
 // IEEE quotient for every normal z in [2^-126, 2^126) (checked over all of
-// them on gfx950); the pivots here are >= 1, and the IEEE division runs only
-// in a wave that holds a pivot beyond that range (lambda near 2^125)
+// them on gfx950). The pivots here lie in [1, 1 + 2 lambda]: frames with
+// lambda above 2^100 run the Thomas kernels.
 __device__ __forceinline__ float recip_exact(float z) {
   const float y = __builtin_amdgcn_rcpf(z);
-  float r = fmaf(fmaf(-z, y, 1.0f), y, y);
-  const bool bad = !(z >= 0x1p-126f && z < 0x1p126f);
-  if (__builtin_expect(__builtin_amdgcn_ballot_w64(bad) != 0, 0)) r = bad ? 1.0f / z : r;
-  return r;
+  return fmaf(fmaf(-z, y, 1.0f), y, y);
 }
 
 // One line of n steps, chain split at seg: this lane's m steps s .. s + m - 1
@@ -206,7 +206,9 @@ __device__ __forceinline__ void scan_line(int lane, int s, int m, int n, int seg
       st = am[k + 1] * Y[k];
       CH[k] = st;
     }
-  // ---- x, n: lane map (U, V), scan, steps ----
+  // ---- x, n: each step's value from a zero start (u_k) and slope (V_k) of
+  // the lane's affine map, scan of the lanes' maps, dh_k = fma(V_k, in, u_k) ----
+  float Uu[M], Uw[M], Vs[M];
   Aff3 U{0.0f, 0.0f, 1.0f};
 #pragma unroll
   for (int k = 0; k < M; ++k)
@@ -214,6 +216,9 @@ __device__ __forceinline__ void scan_line(int lane, int s, int m, int n, int seg
       U.u = (rx[k] - U.u * ab[k]) * Y[k];
       U.w = (rn[k] - U.w * ab[k]) * Y[k];
       U.v = -(ab[k] * Y[k]) * U.v;
+      Uu[k] = U.u;
+      Uw[k] = U.w;
+      Vs[k] = U.v;
     }
   auto aff_comb = [](Aff3& g, const Aff3& h) {
     g.u = g.u + g.v * h.u;
@@ -237,12 +242,11 @@ __device__ __forceinline__ void scan_line(int lane, int s, int m, int n, int seg
 #pragma unroll
   for (int k = 0; k < M; ++k)
     if (k < m) {
-      dx = (rx[k] - dx * ab[k]) * Y[k];
-      dn = (rn[k] - dn * ab[k]) * Y[k];
-      DH[k] = dx;
-      NH[k] = dn;
+      DH[k] = fmaf(Vs[k], dx, Uu[k]);
+      NH[k] = fmaf(Vs[k], dn, Uw[k]);
     }
-  // ---- X, N: lane map (P, Q) backward, suffix scan, steps ----
+  // ---- X, N: the same backward from the lane's end, suffix scan,
+  // X_k = fma(Q_k, in, p_k) ----
   float cc[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) cc[k] = (s + k == sg - 1 || s + k == n - 1) ? 0.0f : CH[k];
@@ -253,6 +257,9 @@ __device__ __forceinline__ void scan_line(int lane, int s, int m, int n, int seg
       Pm.u = DH[k] - cc[k] * Pm.u;
       Pm.w = NH[k] - cc[k] * Pm.w;
       Pm.v = -cc[k] * Pm.v;
+      Uu[k] = Pm.u;
+      Uw[k] = Pm.w;
+      Vs[k] = Pm.v;
     }
   scan_tree<P>(
       lane, true, Pm,
@@ -261,50 +268,55 @@ __device__ __forceinline__ void scan_line(int lane, int s, int m, int n, int seg
         return Aff3{dpp_f<0x100 + O>(g.u), dpp_f<0x100 + O>(g.w), dpp_f<0x100 + O>(g.v)};
       },
       aff_comb);
-  float xv, nv;
+  float xe, ne;
   {
     const float nx = __shfl_down(Pm.u, 1u, 64), nn = __shfl_down(Pm.w, 1u, 64);
-    xv = ll == P - 1 ? 0.0f : nx;
-    nv = ll == P - 1 ? 0.0f : nn;
+    xe = ll == P - 1 ? 0.0f : nx;
+    ne = ll == P - 1 ? 0.0f : nn;
   }
 #pragma unroll
-  for (int k = M - 1; k >= 0; --k)
+  for (int k = 0; k < M; ++k)
     if (k < m) {
-      xv = DH[k] - cc[k] * xv;
-      nv = NH[k] - cc[k] * nv;
-      X[k] = xv;
-      Nn[k] = nv;
+      X[k] = fmaf(Vs[k], xe, Uu[k]);
+      Nn[k] = fmaf(Vs[k], ne, Uw[k]);
     }
 }
 
-// Couplings of a lane's m steps from the line's grey levels: pix(j) = I(j) for
-// j in [s - 1, s + m] (clamped to the line), w(0) = 0, w(n) = edge, else the
-// table entry of |I(j) - I(j - 1)|; am = float(-lambda w), and b; past the
-// step n (padding) am = 0 and b = 1.
+// A lane's grey levels px[k] = I(s - 1 + k), k = 0 .. M + 1, clamped to the
+// line, loaded together; the empty asm takes them as operands, so every load
+// is issued up front rather than sunk into a branch of the lanes that use it.
 template <int M, typename PixFn>
+__device__ __forceinline__ void lane_pixels(int s, int n, PixFn pix, int (&px)[M + 2]) {
+#pragma unroll
+  for (int k = 0; k < M + 2; ++k) px[k] = pix(min(max(s - 1 + k, 0), n - 1));
+#pragma unroll
+  for (int k = 0; k < M + 2; ++k) asm volatile("" : "+v"(px[k]));
+}
+
+// Couplings of a lane's m steps from its grey levels: w(0) = 0, w(n) = edge,
+// else the table entry of |I(j) - I(j - 1)|; am = float(-lambda w), and b;
+// past the step n (padding) am = 0 and b = 1.
+template <int M>
 __device__ __forceinline__ void lane_couplings(int s, int n, float edge, double lam, const float* lw,
-                                               const float* lm, PixFn pix, float (&am)[M + 1], float (&bq)[M]) {
+                                               const float* lm, const int (&px)[M + 2], float (&am)[M + 1],
+                                               float (&bq)[M]) {
+  const float a0 = (float)(-lam * 0.0), ae = (float)(-lam * (double)edge);
+  float wt[M + 1], at[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) {
+    const int d = px[k + 1] > px[k] ? px[k + 1] - px[k] : px[k] - px[k + 1];
+    wt[k] = lw[d];
+    at[k] = lm[d];
+  }
+  // every lane reads its table entries (selects below, not branches around the reads)
+#pragma unroll
+  for (int k = 0; k <= M; ++k) asm volatile("" : "+v"(wt[k]), "+v"(at[k]));
   float wv[M + 1];
-  int prev = pix(s > 0 ? (s - 1 < n ? s - 1 : n - 1) : 0);
 #pragma unroll
   for (int k = 0; k <= M; ++k) {
     const int j = s + k;
-    const int cur = pix(j < n ? j : n - 1);
-    if (j == 0) {
-      wv[k] = 0.0f;
-      am[k] = (float)(-lam * 0.0);
-    } else if (j == n) {
-      wv[k] = edge;
-      am[k] = (float)(-lam * (double)edge);
-    } else if (j > n) {
-      wv[k] = 0.0f;
-      am[k] = 0.0f;
-    } else {
-      const int d = cur > prev ? cur - prev : prev - cur;
-      wv[k] = lw[d];
-      am[k] = lm[d];
-    }
-    prev = cur;
+    wv[k] = j == 0 ? 0.0f : j == n ? edge : j > n ? 0.0f : wt[k];
+    am[k] = j == 0 ? a0 : j == n ? ae : j > n ? 0.0f : at[k];
   }
 #pragma unroll
   for (int k = 0; k < M; ++k) bq[k] = s + k < n ? (float)(1.0 + lam * (double)(wv[k] + wv[k + 1])) : 1.0f;
@@ -323,7 +335,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_row_scan(const FrameDev* __re
   constexpr int LINES = WAVES * 64 / P;
   __shared__ float lw[256], lm[256];
   __shared__ float tile[P * M * (LINES + 1)];
-  const int f = find_frame(bm, blockIdx.x);
+  const int f = __builtin_amdgcn_readfirstlane(find_frame(bm, blockIdx.x));  // uniform: scalar loads of F
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
   const double lam = F.lam;
@@ -340,8 +352,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_row_scan(const FrameDev* __re
   const int m = (W + P - 1) / P;     // steps per lane; lane steps at x >= W pad the row
   const int s = (lane & (P - 1)) * m;
   const gptr<const uint8_t> img = gp(F.key + (size_t)y * F.pitch);
+  int px[M + 2];
+  lane_pixels<M>(s, W, [&](int j) { return (int)img[j]; }, px);
   float am[M + 1], bq[M];
-  lane_couplings<M>(s, W, F.edge_r ? 1.0f : 0.0f, lam, lw, lm, [&](int j) { return (int)img[j]; }, am, bq);
+  lane_couplings<M>(s, W, F.edge_r ? 1.0f : 0.0f, lam, lw, lm, px, am, bq);
   float rx[M], rn[M];
   const size_t row = (size_t)y * W;
   if (MODE == 1) {
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
   constexpr int LINES = WAVES * 64 / P;
   __shared__ float lw[256], lm[256];
   __shared__ float tile[P * M * (LINES + 1)];
-  const int f = find_frame(bm, blockIdx.x);
+  const int f = __builtin_amdgcn_readfirstlane(find_frame(bm, blockIdx.x));  // uniform: scalar loads of F
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H;
   const double lam = F.lam;
@@ -429,9 +443,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
   const int s = (lane & (P - 1)) * m;
   const gptr<const uint8_t> key = gp(F.key) + x;
   const int pitch = F.pitch;
+  int pv[M + 2];
+  lane_pixels<M>(s, H, [&](int j) { return (int)key[(size_t)j * pitch]; }, pv);
   float am[M + 1], bq[M];
-  lane_couplings<M>(s, H, F.edge_c ? 1.0f : 0.0f, lam, lw, lm, [&](int j) { return (int)key[(size_t)j * pitch]; },
-                    am, bq);
+  lane_couplings<M>(s, H, F.edge_c ? 1.0f : 0.0f, lam, lw, lm, pv, am, bq);
   float rx[M], rn[M];
   const gptr<const float> px = gp((const float*)F.rowx) + (size_t)x * H;
   const gptr<const float> pn = px + (size_t)W * H;
@@ -475,31 +490,36 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
 // Launches. Lines of up to 512 steps run on 32 lanes, lines of up to 1536 on
 // 64 (the oracle's solve_line_by follows the same rule); a frame's rows
 // (columns) go to the kernel of the smallest shape that holds them: 12 or 16
-// steps per lane on 32 lanes, 12 or 24 on 64. Longer lines are left to the
-// Thomas kernels (rest[i] = 1, the return value counts them).
+// steps per lane on 32 lanes, 12 or 24 on 64. Longer lines, and frames with
+// lambda above 2^100, are left to the Thomas kernels (rest[i] = 1, the return
+// value counts them).
 // ---------------------------------------------------------------------------
+// The waves per workgroup (which do not change the arithmetic) measured on
+// MI355X: the row pass is fastest with many rows per workgroup (its transpose
+// writes 64 B runs of a column plane), the column pass with four waves.
 struct ScanShape {
   int P, M, waves;
 };
-constexpr ScanShape kScanShapes[4] = {{32, 12, 16}, {32, 16, 16}, {64, 12, 16}, {64, 24, 8}};
+constexpr ScanShape kRowShapes[4] = {{32, 12, 16}, {32, 16, 8}, {64, 12, 16}, {64, 24, 8}};
+constexpr ScanShape kColShapes[4] = {{32, 12, 4}, {32, 16, 4}, {64, 12, 4}, {64, 24, 4}};
 
 static int scan_shape_of(int n) {
   const int P = n <= 512 ? 32 : 64;
   for (int i = 0; i < 4; ++i)
-    if (kScanShapes[i].P == P && (n + P - 1) / P <= kScanShapes[i].M) return i;
+    if (kRowShapes[i].P == P && (n + P - 1) / P <= kRowShapes[i].M) return i;
   return -1;
 }
 
 template <int I, int MODE>
 static void row_launch(const FrameDev* dframes, const BlockMap& bm, int nb, const float* wlut, hipStream_t s) {
-  constexpr ScanShape sh = kScanShapes[I];
+  constexpr ScanShape sh = kRowShapes[I];
   hipLaunchKernelGGL((k_up_row_scan<sh.P, sh.M, sh.waves, MODE>), dim3(nb), dim3(64 * sh.waves), 0, s, dframes, bm,
                      wlut);
 }
 template <int I>
 static void col_launch(const FrameDev* dframes, const BlockMap& bm, int nb, const float* wlut, int clear,
                        hipStream_t s) {
-  constexpr ScanShape sh = kScanShapes[I];
+  constexpr ScanShape sh = kColShapes[I];
   hipLaunchKernelGGL((k_up_col_scan<sh.P, sh.M, sh.waves>), dim3(nb), dim3(64 * sh.waves), 0, s, dframes, bm, wlut,
                      clear);
 }
@@ -508,13 +528,14 @@ int launch_upsample_row_scan(const FrameDev* dframes, const FrameDev* hframes, i
                              hipStream_t s, int* rest) {
   int left = 0;
   for (int i = 0; i < n; ++i) {
-    rest[i] = scan_shape_of(hframes[i].W) < 0 ? 1 : 0;
+    rest[i] = scan_shape_of(hframes[i].W) < 0 || !(hframes[i].lam <= 0x1p100) ? 1 : 0;
     left += rest[i];
   }
   for (int sh = 0; sh < 4; ++sh) {
-    const int lines = kScanShapes[sh].waves * 64 / kScanShapes[sh].P;
+    const int lines = kRowShapes[sh].waves * 64 / kRowShapes[sh].P;
     int counts[kMaxFrames];
-    for (int i = 0; i < n; ++i) counts[i] = scan_shape_of(hframes[i].W) == sh ? (hframes[i].H + lines - 1) / lines : 0;
+    for (int i = 0; i < n; ++i)
+      counts[i] = !rest[i] && scan_shape_of(hframes[i].W) == sh ? (hframes[i].H + lines - 1) / lines : 0;
     BlockMap bm;
     const int nb = make_block_map(bm, counts, n);
     if (nb <= 0) continue;
@@ -531,13 +552,14 @@ int launch_upsample_col_scan(const FrameDev* dframes, const FrameDev* hframes, i
                              bool clear_zbuf, hipStream_t s, int* rest) {
   int left = 0;
   for (int i = 0; i < n; ++i) {
-    rest[i] = scan_shape_of(hframes[i].H) < 0 ? 1 : 0;
+    rest[i] = scan_shape_of(hframes[i].H) < 0 || !(hframes[i].lam <= 0x1p100) ? 1 : 0;
     left += rest[i];
   }
   for (int sh = 0; sh < 4; ++sh) {
-    const int lines = kScanShapes[sh].waves * 64 / kScanShapes[sh].P;
+    const int lines = kColShapes[sh].waves * 64 / kColShapes[sh].P;
     int counts[kMaxFrames];
-    for (int i = 0; i < n; ++i) counts[i] = scan_shape_of(hframes[i].H) == sh ? (hframes[i].W + lines - 1) / lines : 0;
+    for (int i = 0; i < n; ++i)
+      counts[i] = !rest[i] && scan_shape_of(hframes[i].H) == sh ? (hframes[i].W + lines - 1) / lines : 0;
     BlockMap bm;
     const int nb = make_block_map(bm, counts, n);
     if (nb <= 0) continue;

@@ -520,8 +520,11 @@ void solve_line(int n, const Real* w /*n+1 couplings, stride ws*/, int ws, const
 // every lane its incoming state, and the lane then runs its steps from it:
 //   ch   ch_k = c_k / (b_k - ch_{k-1} a_k): a Moebius map, the 2 x 2 matrix
 //        [[0, c_k], [-a_k, b_k]] on (p, q) with ch = p / q;
-//   x, n dh_k = (r_k - dh_{k-1} a_k) y_k, y_k = 1 / den_k: affine (U, V);
-//   X, N X_k = dh_k - ch_k X_{k+1} backward: affine (P, Q), a suffix scan.
+//   x, n dh_k = (r_k - dh_{k-1} a_k) y_k, y_k = 1 / den_k: affine; a lane
+//        keeps each step's value from a zero start u_k and slope V_k, and
+//        dh_k = fma(V_k, incoming, u_k);
+//   X, N X_k = dh_k - ch_k X_{k+1} backward: affine, the same with a suffix
+//        scan.
 // A chain start k (0 and seg) uses a_k = 0 in the recurrences (its state is 0
 // before it), a chain end (seg - 1 and n - 1) uses ch_k = 0 in the backward
 // one. The Moebius lane maps and their scan are in double (their products
@@ -647,11 +650,13 @@ void solve_line_scan(int P, int n, const Real* w, int ws, const Real* rx, const 
       CH[k] = st;
     }
   }
-  // ---- x, n: lane maps (U, V), scan, steps ----
+  // ---- x, n: per step the from-zero values and slopes of the lane's affine
+  // map (u_k, V_k), scan of the lanes' maps, dh_k = fma(V_k, incoming, u_k) ----
   struct Aff {
     Real u, w, v;  // two intercepts sharing one slope
   };
   auto aff = [](const Aff& a, const Aff& b) { return Aff{a.u + a.v * b.u, a.w + a.v * b.w, a.v * b.v}; };
+  std::vector<Real> Uu(L), Uw(L), Vs(L);
   std::vector<Aff> U(P);
   for (int i = 0; i < P; ++i) {
     Real ux = 0, un = 0, v = 1;
@@ -659,20 +664,22 @@ void solve_line_scan(int P, int n, const Real* w, int ws, const Real* rx, const 
       ux = (R[k] - ux * A[k]) * Y[k];
       un = (RN[k] - un * A[k]) * Y[k];
       v = -(A[k] * Y[k]) * v;
+      Uu[k] = ux;
+      Uw[k] = un;
+      Vs[k] = v;
     }
     U[i] = Aff{ux, un, v};
   }
   scan_tree(U, false, aff);
   for (int i = 0; i < P; ++i) {
-    Real dx = i == 0 ? Real(0) : U[i - 1].u, dn = i == 0 ? Real(0) : U[i - 1].w;
+    const Real dx = i == 0 ? Real(0) : U[i - 1].u, dn = i == 0 ? Real(0) : U[i - 1].w;
     for (int k = i * m; k < (i + 1) * m; ++k) {
-      dx = (R[k] - dx * A[k]) * Y[k];
-      dn = (RN[k] - dn * A[k]) * Y[k];
-      DH[k] = dx;
-      NH[k] = dn;
+      DH[k] = std::fma(Vs[k], dx, Uu[k]);
+      NH[k] = std::fma(Vs[k], dn, Uw[k]);
     }
   }
-  // ---- X, N: lane maps (P, Q) backward, suffix scan, steps ----
+  // ---- X, N: the same backward: from-zero values and slopes (p_k, Q_k) from
+  // the lane's end, suffix scan, X_k = fma(Q_k, incoming, p_k) ----
   auto cut = [&](int k) { return (k == sg - 1 || k == n - 1) ? Real(0) : CH[k]; };
   std::vector<Aff> Q(P);
   for (int i = 0; i < P; ++i) {
@@ -682,20 +689,18 @@ void solve_line_scan(int P, int n, const Real* w, int ws, const Real* rx, const 
       px = DH[k] - c * px;
       pn = NH[k] - c * pn;
       q = -c * q;
+      Uu[k] = px;
+      Uw[k] = pn;
+      Vs[k] = q;
     }
     Q[i] = Aff{px, pn, q};
   }
   scan_tree(Q, true, aff);
   for (int i = 0; i < P; ++i) {
-    Real xv = i == P - 1 ? Real(0) : Q[i + 1].u, nv = i == P - 1 ? Real(0) : Q[i + 1].w;
-    for (int k = (i + 1) * m - 1; k >= i * m; --k) {
-      const Real c = cut(k);
-      xv = DH[k] - c * xv;
-      nv = NH[k] - c * nv;
-      if (k < n) {
-        ox[(size_t)k * ostride] = xv;
-        on[(size_t)k * ostride] = nv;
-      }
+    const Real xe = i == P - 1 ? Real(0) : Q[i + 1].u, ne = i == P - 1 ? Real(0) : Q[i + 1].w;
+    for (int k = i * m; k < (i + 1) * m && k < n; ++k) {
+      ox[(size_t)k * ostride] = std::fma(Vs[k], xe, Uu[k]);
+      on[(size_t)k * ostride] = std::fma(Vs[k], ne, Uw[k]);
     }
   }
 }
@@ -705,9 +710,10 @@ void solve_line_scan(int P, int n, const Real* w, int ws, const Real* rx, const 
 template <typename Real>
 void solve_line_by(int solver, int n, const Real* w, int ws, const Real* rx, const Real* rn, int rstride, Real* ox,
                    Real* on, int ostride, int seg, double lam, Real* tmp) {
-  if (solver == 1 && n <= 512)
+  // (lambda beyond 2^100 keeps Thomas: the scan's pivots stay below 2^126)
+  if (solver == 1 && n <= 512 && lam <= 0x1p100)
     solve_line_scan<Real>(32, n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam);
-  else if (solver == 1 && n <= kScanLanes * kScanMaxSeg)
+  else if (solver == 1 && n <= kScanLanes * kScanMaxSeg && lam <= 0x1p100)
     solve_line_scan<Real>(64, n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam);
   else
     solve_line<Real>(n, w, ws, rx, rn, rstride, ox, on, ostride, seg, lam, tmp);

@@ -46,10 +46,11 @@ def _random_case(W, H, seed, density=0.05):
 # sizes: the headline rows (64 lanes x 12 steps) and columns (32 x 16), a
 # split row chain (376 -> 368, 32 x 12), rows on 64 lanes x 24 steps (1280),
 # odd sizes with short lanes and a column split (150 x 100 -> 96), rows longer
-# than 1536 (Thomas rows, scan columns) and columns longer than 1536 (scan
-# rows, Thomas columns)
+# than 1536 (Thomas rows, scan columns), columns longer than 1536 (scan rows,
+# Thomas columns) and lambda above 2^100 (Thomas)
 @pytest.mark.parametrize("W,H,lam", [(752, 480, 10.0), (376, 240, 10.0), (1280, 720, 10.0), (150, 100, 10.0),
-                                     (752, 480, 1000.0), (150, 100, 0.5), (1600, 40, 10.0), (40, 1600, 10.0)])
+                                     (752, 480, 1000.0), (150, 100, 0.5), (1600, 40, 10.0), (40, 1600, 10.0),
+                                     (150, 100, 1e31)])
 def test_stage_scan_matches_oracle(W, H, lam):
     img, sparse = _random_case(W, H, W + H)
     print(f"scan upsample {W}x{H} lambda {lam}", flush=True)
