@@ -2,11 +2,12 @@
 
 Each entry plants one plausible mistake (a shifted band, a cascading read, a
 wrong constant or sign, coarser filtering weights, the opposite collision rule)
-in a temporary copy of the oracle's sources, builds it, and runs the pins of
-tests/test_oracle_pins.py that cover that function against the mutated build
-(QM_ORACLE_LIB). Every mutant must make a pin fail.
+in a temporary copy of the oracle's sources, builds it, and runs the pins that
+cover that function (tests/test_oracle_pins.py, or the test file the entry
+names) against the mutated build (QM_ORACLE_LIB). Every mutant must make a pin
+fail.
 
-    python oracle/mutants.py
+    python oracle/mutants.py [test-file substring]
 """
 import os
 import shutil
@@ -30,10 +31,20 @@ MUTS = [
   ("qm_semidense.cpp", "  if (inc.g2(x, y) < kMinGrad * kMinGrad) continue;\n      Seed s = snap", "  if (inc.g2(x, y) < 0.0f) continue;\n      Seed s = snap", "sd_fill"),
   ("qm_semidense.cpp", "      if (k > t) t = k;", "      if (t == 0 || k < t) t = k;", "sd_propagation"),
   ("qm_semidense.cpp", "  np = se3_mul_point(T, kp);\n  new_idepth = 1.0f / np.z;", "  np = se3_mul_point(T, kp);\n  new_idepth = 1.0f / kp.z;", "sd_propagation"),
+  # the upsample's segmented-scan solver, against its FP64 equality with the
+  # Thomas order and the dense FP64 solve (tests/test_upsample_scan.py)
+  ("qm_oracle.cpp", "    if (k == 0 || k == sg) A[k] = Real(0);\n    R[k] =", "    if (k == 0) A[k] = Real(0);\n    R[k] =", "fp64_scan or dense", "tests/test_upsample_scan.py"),
+  ("qm_oracle.cpp", "(row < 2 ? 15 : 47)", "(row < 2 ? 14 : 47)", "fp64_scan or dense", "tests/test_upsample_scan.py"),
+  ("qm_oracle.cpp", "return (k == sg - 1 || k == n - 1) ? Real(0) : CH[k];", "return (k == n - 1) ? Real(0) : CH[k];", "fp64_scan or dense", "tests/test_upsample_scan.py"),
+  ("qm_oracle.cpp", "(Real)(G[i - 1].g12 / G[i - 1].g22)", "(Real)(G[i - 1].g11 / G[i - 1].g21)", "fp64_scan or dense", "tests/test_upsample_scan.py"),
 ]
 def main():
   res = []
-  for i, (f, old, new, k) in enumerate(MUTS):
+  only = sys.argv[1] if len(sys.argv) > 1 else None  # run only the mutants whose test file contains this
+  for i, (f, old, new, k, *tf) in enumerate(MUTS):
+    test_file = tf[0] if tf else 'tests/test_oracle_pins.py'
+    if only and only not in test_file:
+      continue
     tmp = tempfile.mkdtemp(prefix='qm_mutant_')
     for n in ("qm_oracle.cpp", "qm_semidense.cpp", "qm_prior.cpp"):
         shutil.copy(os.path.join(ROOT, 'oracle', n), tmp)
@@ -44,7 +55,7 @@ def main():
     subprocess.check_call(['g++', '-O2', '-std=c++17', '-fPIC', '-fopenmp', '-ffp-contract=off', '-fno-fast-math', '-shared', '-o', so,
                            os.path.join(tmp, 'qm_oracle.cpp'), os.path.join(tmp, 'qm_semidense.cpp'), os.path.join(tmp, 'qm_prior.cpp')])
     env = dict(os.environ, QM_ORACLE_LIB=so)
-    r = subprocess.run([sys.executable, '-m', 'pytest', 'tests/test_oracle_pins.py', '-q', '-k', k], cwd=ROOT, env=env, capture_output=True, text=True)
+    r = subprocess.run([sys.executable, '-m', 'pytest', test_file, '-q', '-k', k], cwd=ROOT, env=env, capture_output=True, text=True)
     last = r.stdout.strip().split('\n')[-1]
     res.append((i, f, k, 'CAUGHT' if r.returncode != 0 else 'MISSED', last))
     print(res[-1], flush=True)
