@@ -335,8 +335,10 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
                                   streams, the second starting with the first, after its cost
                                   volume, or after its BP; 4, 5, 6: three, four or eight
                                   groups, each starting after the previous group's cost volume;
-                                  7: four groups, each after the previous group's BP (results
-                                  identical in every case) */
+                                  7: four groups, each after the previous group's BP; 8, 9, 10:
+                                  two, three or four groups whose cost volumes run one after
+                                  the other on the caller's stream, each group's later stages
+                                  on a stream of its own (results identical in every case) */
 #define QM_TUNE_BP_PAIR 4      /* 0: one launch per BP half-sweep; 8 or 16: the half-sweeps of a
                                   level in fused pairs, one launch per pair, over strips of 14 or
                                   30 cell columns (the first half-sweep's messages pass to the

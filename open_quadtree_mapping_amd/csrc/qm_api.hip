@@ -547,19 +547,31 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
 // serial upsample chains) run. Group g + 1 starts when group g has enqueued
 // the stage the setting names. Settings: 1-3 two groups, starting together,
 // after the cost volume, after BP; 4 three and 5 four groups, each after the
-// previous group's cost volume. Every setting computes the same results.
+// previous group's cost volume; 6 eight such groups; 7 four groups, each after
+// the previous group's BP. Settings 8, 9 and 10 (two, three, four groups) run
+// every group's cost volume on s, one after the other, and fork each group's
+// later stages onto a stream of its own once its cost volume is done (the
+// last group's stay on s): s then runs one long kernel while the other streams
+// start theirs, where a string of short kernels on s (the coarse BP levels)
+// holds back another stream's first kernel. Every setting computes the same
+// results.
 struct OverlapPlan {
   int groups, mark_after;
+  bool costs_first;
 };
+constexpr int kOverlapSettings = 11;
 OverlapPlan overlap_plan(int ov, int n) {
-  OverlapPlan p{1, -1};
-  if (ov == 1) p = {2, -1};
-  if (ov == 2) p = {2, 1};
-  if (ov == 3) p = {2, 3};
-  if (ov == 4) p = {3, 1};
-  if (ov == 5) p = {4, 1};
-  if (ov == 6) p = {8, 1};
-  if (ov == 7) p = {4, 3};
+  OverlapPlan p{1, -1, false};
+  if (ov == 1) p = {2, -1, false};
+  if (ov == 2) p = {2, 1, false};
+  if (ov == 3) p = {2, 3, false};
+  if (ov == 4) p = {3, 1, false};
+  if (ov == 5) p = {4, 1, false};
+  if (ov == 6) p = {8, 1, false};
+  if (ov == 7) p = {4, 3, false};
+  if (ov == 8) p = {2, 1, true};
+  if (ov == 9) p = {3, 1, true};
+  if (ov == 10) p = {4, 1, true};
   if (p.groups > n) p.groups = n;
   return p;
 }
@@ -604,6 +616,29 @@ int run_pipeline(qm_batch* b, int par, hipStream_t s) {
   int active[QM_NUM_STAGES], na = 0;
   for (int u = 0; u < QM_NUM_STAGES; ++u)
     if (u != QM_STAGE_PYRAMID || any_prior(b)) active[na++] = u;
+  if (plan.costs_first) {
+    // every group's head (quadtree, cost volume, pyramid) on s in turn, then
+    // the groups' tails: group g < G - 1 on side stream g after its head, the
+    // last group's on s. Heads are enqueued first, so that the next head is
+    // the first dependent of a cost node and stays on s's queue.
+    auto head_stage = [](int u) { return u <= QM_STAGE_PYRAMID; };
+    for (int g = 0; g < G; ++g) {
+      for (int i = 0; i < na; ++i)
+        if (head_stage(active[i])) enqueue_stage(b, par, s, nullptr, f0[g], nf[g], active[i]);
+      if (g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], s));
+    }
+    for (int g = 0; g < G; ++g) {
+      hipStream_t sg = g + 1 < G ? b->side[g] : s;
+      if (g + 1 < G) QM_HIP(hipStreamWaitEvent(sg, b->ev_mark[g], 0));
+      for (int i = 0; i < na; ++i)
+        if (!head_stage(active[i])) enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[i]);
+    }
+    for (int g = 0; g + 1 < G; ++g) {
+      QM_HIP(hipEventRecord(b->ev_join[g], b->side[g]));
+      QM_HIP(hipStreamWaitEvent(s, b->ev_join[g], 0));
+    }
+    return QM_OK;
+  }
   int mark = -1;  // the stagger point (a stage number) as an index into active
   for (int u = 0; u < na; ++u)
     if (active[u] == plan.mark_after) mark = u;
@@ -1384,7 +1419,7 @@ static void drop_graphs(qm_batch* b) {
 static bool knob_value_ok(int knob, int value) {
   switch (knob) {
     case QM_TUNE_COST_WAVES: return value == 2 || value == 4 || value == 8;
-    case QM_TUNE_OVERLAP: return value >= 0 && value <= 7;
+    case QM_TUNE_OVERLAP: return value >= 0 && value < kOverlapSettings;
     case QM_TUNE_BP_PAIR: return value == 0 || value == 8 || value == 16;
     case QM_TUNE_BP_SEG: return value == 8 || value == 16 || value == 32 || value == 64;
     case QM_TUNE_BP_PAIR_LEVELS: return value >= 1 && value <= 4;
@@ -1643,7 +1678,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
       const std::vector<Tuning> keep_g = b->gtune;
       b->gtune = gbest;
       double best_t = 1e30;
-      for (int v = 0; v <= 7 && !rc; ++v) {
+      for (int v = 0; v < kOverlapSettings && !rc; ++v) {
         Tuning t = keep;
         t.overlap = v;
         double ms = 0.0;
