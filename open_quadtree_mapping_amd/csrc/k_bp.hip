@@ -244,10 +244,11 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
                                                         float p2, int ext, double imin, double istep, int buf,
                                                         int pbuf) {
   constexpr int S = N / 16;
-  const int f = find_frame(bm, blockIdx.x);
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int f = find_frame(bm, lb);
   const FrameDev& F = frames[f];
   const int j = threadIdx.x & 15;
-  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
+  const int item = (lb - bm.start[f]) * ROWS + (threadIdx.x >> 4);
   const int w = F.lw[l], h = F.lh[l];
   int x = 0, y = 0;
   bool upd = false, copy = false, ex = false;
@@ -366,11 +367,13 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
   // per cell of rows y0 - 1 .. y0 + R, columns x0 - 1 .. x0 + TW: bit 0 the
   // quadtree skips it, bit 1 it is a quadtree node (level 0 only)
   __shared__ uint8_t Sk[kPairMaxSeg + 2][TW + 2];
-  const int f = find_frame(bm, blockIdx.x);
+  // neighbouring strips and segments share halo cells: keep them on one XCD's L2
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int f = find_frame(bm, lb);
   const FrameDev& F = frames[f];
   const int w = F.lw[l], h = F.lh[l];
   const int ns = (w + TW - 1) / TW;
-  const int bi = blockIdx.x - bm.start[f];
+  const int bi = lb - bm.start[f];
   const int x0 = (bi % ns) * TW, y0 = (bi / ns) * seg;
   const int R = min(seg, h - y0);
   const int T = 3 * ((R + 5) / 3);  // R + 3 steps, rounded up to a multiple of 3
@@ -578,10 +581,11 @@ template <int N, int ROWS>
 __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restrict__ frames, BlockMap bm, double imin,
                                                        double istep, int colour, int buf) {
   constexpr int S = N / 16;
-  const int f = find_frame(bm, blockIdx.x);
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int f = find_frame(bm, lb);
   const FrameDev& F = frames[f];
   const int lane = threadIdx.x & 63, j = lane & 15;
-  const int item = (blockIdx.x - bm.start[f]) * ROWS + (threadIdx.x >> 4);
+  const int item = (lb - bm.start[f]) * ROWS + (threadIdx.x >> 4);
   const int w = F.lw[0], h = F.lh[0];
   int x, y;
   if (colour < 0) {

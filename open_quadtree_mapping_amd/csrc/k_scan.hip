@@ -425,7 +425,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
   constexpr int LINES = WAVES * 64 / P;
   __shared__ float lw[256], lm[256];
   __shared__ float tile[P * M * (LINES + 1)];
-  const int f = __builtin_amdgcn_readfirstlane(find_frame(bm, blockIdx.x));  // uniform: scalar loads of F
+  // neighbouring workgroups read the same image lines: keep them on one XCD's L2
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int f = __builtin_amdgcn_readfirstlane(find_frame(bm, lb));  // uniform: scalar loads of F
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H;
   const double lam = F.lam;
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int c = (threadIdx.x >> 6) * (64 / P) + lane / P;  // this lane's column in the workgroup
-  const int x0 = (blockIdx.x - bm.start[f]) * LINES;
+  const int x0 = (lb - bm.start[f]) * LINES;
   const int x = min(x0 + c, W - 1);  // columns past the image repeat column W - 1 and are not stored
   const int m = (H + P - 1) / P;     // steps per lane; lane steps at y >= H pad the column
   const int s = (lane & (P - 1)) * m;
