@@ -74,8 +74,9 @@ typedef struct qm_params {
  * row and column systems (I + lambda L(w)) x = r of src/depth_upsample.cu:94-302.
  * QM_UPSAMPLE_THOMAS: the reference's Thomas recurrences in its order, one
  *   serial chain per line.
- * QM_UPSAMPLE_SCAN: the same recurrences by segmented scans: 64 lanes per
- *   line each own a run of steps; the maps of the three recurrences (a
+ * QM_UPSAMPLE_SCAN: the same recurrences by segmented scans: 32 lanes per
+ *   line (lines of up to 512 steps) or 64 lanes (up to 1536 steps) each own
+ *   a run of steps; the maps of the three recurrences (a
  *   Moebius map for the coefficients, affine maps for the right-hand sides
  *   and the back substitution) are composed per lane and scanned over the
  *   lanes in double, and each lane then runs its steps in float from its
