@@ -58,7 +58,7 @@ def parse():
                     help="do not consult the launch shapes measured on MI355X that ship with the package")
     ap.add_argument("--tune-reps", type=int, default=5)
     ap.add_argument("--overlap", type=int, default=None,
-                    help="force the frame-group overlap setting (0-10) after autotuning, e.g. for a profiled run")
+                    help="force the frame-group overlap setting (0-13) after autotuning, e.g. for a profiled run")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the batched launches without hipGraph replay (for counter collection)")
     ap.add_argument("--workload", default="keyframes32",

@@ -339,7 +339,9 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double stage_ms[QM_NUM_STAGES]
                                   7: four groups, each after the previous group's BP; 8, 9, 10:
                                   two, three or four groups whose cost volumes run one after
                                   the other on the caller's stream, each group's later stages
-                                  on a stream of its own (results identical in every case) */
+                                  on a stream of its own; 11, 12, 13: 8, 9, 10 with the fusion
+                                  warp of the whole batch run first, beside the cost volumes
+                                  (results identical in every case) */
 #define QM_TUNE_BP_PAIR 4      /* 0: one launch per BP half-sweep; 8 or 16: the half-sweeps of a
                                   level in fused pairs, one launch per pair, over strips of 14 or
                                   30 cell columns (the first half-sweep's messages pass to the
@@ -366,7 +368,8 @@ int qm_batch_get_group_tuning(qm_batch* b, int group, int knob, int* value);
  * each group on its own frames; per stage, the groups' own shapes are kept
  * when the whole batch runs that stage faster with them (a launch per run of
  * agreeing groups) than with the common shape (one launch). Then the overlap
- * knob over whole runs with those shapes. It first recomputes the quadtree,
+ * knob over whole runs with those shapes, each setting captured once as a
+ * graph and timed over reps replays. It first recomputes the quadtree,
  * cost volume and pyramid; the fusion state is not touched. chosen receives
  * group 0's shapes and the overlap; best_ms the common search's time per
  * launch set for each shape knob and the overlap's per run (nullable both);

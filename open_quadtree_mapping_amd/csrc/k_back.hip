@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(384) void k_up_col(const FrameDev* __restrict__ fra
     const gptr<float> dense = gp(F.dense);
     const gptr<float> dense_out = gp(F.dense_out);
     const gptr<float2> colxn = gp(F.colxn);
-    const gptr<unsigned long long> zbuf = gp(F.zbuf);
+    const gptr<unsigned long long> zbuf = gp(F.zbuf_clear);
     for (int r = 0; r < nch + 2; ++r) {
       if (clear_zbuf && r < nch) {
         int c0, cnt;

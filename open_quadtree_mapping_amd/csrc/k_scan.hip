@@ -475,7 +475,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_col_scan(const FrameDev* __re
   const int cols = min(LINES, W - x0);
   const gptr<float> dense = gp(F.dense) + x0;
   const gptr<float> dense_out = gp(F.dense_out);
-  const gptr<unsigned long long> zbuf = gp(F.zbuf) + x0;
+  const gptr<unsigned long long> zbuf = gp(F.zbuf_clear) + x0;
   for (int i = threadIdx.x; i < H * LINES; i += 64 * WAVES) {
     const int yy = i / LINES, cc = i - yy * LINES;
     if (cc < cols) {

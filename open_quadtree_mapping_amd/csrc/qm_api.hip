@@ -307,7 +307,7 @@ size_t frame_bytes(const FrameDev& F, int N) {
   b += align256(px * 8);                   // rowx (X and N planes)
   b += align256(px * 4);                   // dense
   b += 2 * align256(px * 16);              // seeds x2
-  b += align256(px * 8);                   // zbuf
+  b += 2 * align256(px * 8);               // z-buffers, one per run parity
   b += align256(px * 4);                   // hole-filled warp table
   return b;
 }
@@ -333,6 +333,10 @@ struct qm_batch {
   float* table = nullptr;  // depth samples, then the weight LUT (256)
   float* wlut = nullptr;
   std::vector<float4*> seeds[2];
+  // the fusion z-buffers: a run of parity p scatters into zbuf[p] and its
+  // column pass resets zbuf[p ^ 1] for the next run, so that the warp may
+  // run anywhere before the update
+  std::vector<unsigned long long*> zbuf[2];
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int parity = 0;
@@ -351,7 +355,7 @@ struct qm_batch {
   // extra streams and fork / mark / join events of the frame-group overlap
   static constexpr int kMaxGroups = 8;
   hipStream_t side[kMaxGroups - 1] = {};
-  hipEvent_t ev_fork = nullptr, ev_mark[kMaxGroups - 1] = {}, ev_join[kMaxGroups - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_warp = nullptr, ev_mark[kMaxGroups - 1] = {}, ev_join[kMaxGroups - 1] = {};
   bool side_ready = false;
   std::vector<hipEvent_t> events;
   // backward (per-stream lambda), allocated on the first qm_batch_backward
@@ -553,25 +557,26 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
 // later stages onto a stream of its own once its cost volume is done (the
 // last group's stay on s): s then runs one long kernel while the other streams
 // start theirs, where a string of short kernels on s (the coarse BP levels)
-// holds back another stream's first kernel. Every setting computes the same
-// results.
+// holds back another stream's first kernel. Settings 11, 12 and 13 are 8, 9
+// and 10 with the fusion warp of the whole batch run first, on the first side
+// stream beside the cost volumes (a run's warp scatters into its own parity's
+// z-buffer, which the previous run's column pass reset). Every setting
+// computes the same results.
 struct OverlapPlan {
   int groups, mark_after;
-  bool costs_first;
+  bool costs_first, early_warp;
 };
-constexpr int kOverlapSettings = 11;
+constexpr int kOverlapSettings = 14;
 OverlapPlan overlap_plan(int ov, int n) {
-  OverlapPlan p{1, -1, false};
-  if (ov == 1) p = {2, -1, false};
-  if (ov == 2) p = {2, 1, false};
-  if (ov == 3) p = {2, 3, false};
-  if (ov == 4) p = {3, 1, false};
-  if (ov == 5) p = {4, 1, false};
-  if (ov == 6) p = {8, 1, false};
-  if (ov == 7) p = {4, 3, false};
-  if (ov == 8) p = {2, 1, true};
-  if (ov == 9) p = {3, 1, true};
-  if (ov == 10) p = {4, 1, true};
+  OverlapPlan p{1, -1, false, false};
+  if (ov == 1) p = {2, -1, false, false};
+  if (ov == 2) p = {2, 1, false, false};
+  if (ov == 3) p = {2, 3, false, false};
+  if (ov == 4) p = {3, 1, false, false};
+  if (ov == 5) p = {4, 1, false, false};
+  if (ov == 6) p = {8, 1, false, false};
+  if (ov == 7) p = {4, 3, false, false};
+  if (ov >= 8 && ov <= 13) p = {2 + (ov - 8) % 3, 1, true, ov >= 11};
   if (p.groups > n) p.groups = n;
   return p;
 }
@@ -580,6 +585,7 @@ OverlapPlan overlap_plan(int ov, int n) {
 int ensure_side(qm_batch* b) {
   if (b->side_ready) return QM_OK;
   if (!b->ev_fork) QM_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  if (!b->ev_warp) QM_HIP(hipEventCreateWithFlags(&b->ev_warp, hipEventDisableTiming));
   for (int g = 0; g < qm_batch::kMaxGroups - 1; ++g) {
     if (!b->ev_mark[g]) QM_HIP(hipEventCreateWithFlags(&b->ev_mark[g], hipEventDisableTiming));
     if (!b->ev_join[g]) QM_HIP(hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming));
@@ -617,10 +623,20 @@ int run_pipeline(qm_batch* b, int par, hipStream_t s) {
   for (int u = 0; u < QM_NUM_STAGES; ++u)
     if (u != QM_STAGE_PYRAMID || any_prior(b)) active[na++] = u;
   if (plan.costs_first) {
-    // every group's head (quadtree, cost volume, pyramid) on s in turn, then
-    // the groups' tails: group g < G - 1 on side stream g after its head, the
-    // last group's on s. Heads are enqueued first, so that the next head is
-    // the first dependent of a cost node and stays on s's queue.
+    // with early_warp the fusion warp of the whole batch first, on side
+    // stream 0 beside the cost volumes (it reads only the prior seeds and the
+    // poses, and scatters into this parity's z-buffer, which the previous
+    // run's column pass reset); then every group's head (quadtree, cost
+    // volume, pyramid) on s in turn, and the groups' tails: group g < G - 1 on
+    // side stream g after its head (and, for g > 0, the early warp), the last
+    // group's on s (after the early warp). Heads are enqueued first, so that
+    // the next head is the first dependent of a cost node and stays on s's
+    // queue.
+    if (plan.early_warp) {
+      QM_HIP(hipStreamWaitEvent(b->side[0], b->ev_fork, 0));
+      enqueue_stage(b, par, b->side[0], nullptr, 0, b->n, QM_STAGE_FUSE_WARP);
+      QM_HIP(hipEventRecord(b->ev_warp, b->side[0]));
+    }
     auto head_stage = [](int u) { return u <= QM_STAGE_PYRAMID; };
     for (int g = 0; g < G; ++g) {
       for (int i = 0; i < na; ++i)
@@ -630,8 +646,10 @@ int run_pipeline(qm_batch* b, int par, hipStream_t s) {
     for (int g = 0; g < G; ++g) {
       hipStream_t sg = g + 1 < G ? b->side[g] : s;
       if (g + 1 < G) QM_HIP(hipStreamWaitEvent(sg, b->ev_mark[g], 0));
+      if (plan.early_warp && g > 0) QM_HIP(hipStreamWaitEvent(sg, b->ev_warp, 0));
       for (int i = 0; i < na; ++i)
-        if (!head_stage(active[i])) enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[i]);
+        if (!head_stage(active[i]) && !(plan.early_warp && active[i] == QM_STAGE_FUSE_WARP))
+          enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[i]);
     }
     for (int g = 0; g + 1 < G; ++g) {
       QM_HIP(hipEventRecord(b->ev_join[g], b->side[g]));
@@ -1241,6 +1259,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
   b->wlut = b->table + N;
   b->seeds[0].resize(n);
   b->seeds[1].resize(n);
+  b->zbuf[0].resize(n);
+  b->zbuf[1].resize(n);
   for (int i = 0; i < n; ++i) {
     FrameDev& F = base[i];
     const size_t px = (size_t)F.W * F.H;
@@ -1259,7 +1279,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
     F.dense = cv.take<float>(px * 4);
     b->seeds[0][i] = cv.take<float4>(px * 16);
     b->seeds[1][i] = cv.take<float4>(px * 16);
-    F.zbuf = cv.take<unsigned long long>(px * 8);
+    b->zbuf[0][i] = cv.take<unsigned long long>(px * 8);
+    b->zbuf[1][i] = cv.take<unsigned long long>(px * 8);
     F.table_out = cv.take<int32_t>(px * 4);
     apply_desc(F, frames[i], *p);
   }
@@ -1268,6 +1289,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
     for (int i = 0; i < n; ++i) {
       b->hf[par][i].seeds = b->seeds[par][i];
       b->hf[par][i].seeds_new = b->seeds[par ^ 1][i];
+      b->hf[par][i].zbuf = b->zbuf[par][i];
+      b->hf[par][i].zbuf_clear = b->zbuf[par ^ 1][i];
     }
   }
   std::vector<float> tabs(N + 256);
@@ -1279,6 +1302,8 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
       const size_t px = (size_t)base[i].W * base[i].H;
       e = hipMemset(b->seeds[0][i], 0, px * 16);
       if (e == hipSuccess) e = hipMemset(b->seeds[1][i], 0, px * 16);
+      // both z-buffers start empty (the first run's warp scatters into one)
+      for (int q = 0; q < 2 && e == hipSuccess; ++q) e = hipMemset(b->zbuf[q][i], 0xFF, px * 8);
     }
   }
   if (e != hipSuccess || upload_frames(b) != QM_OK) {
@@ -1653,7 +1678,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     for (int k = 0; k < QM_NUM_TUNE; ++k)
       if (k != QM_TUNE_OVERLAP) best_ms[k] = common_ms[k];
   // the overlap knob, with the groups' chosen shapes: whole pipeline runs
-  // (plain launches), which move the fusion state, so both seed buffers of
+  // (graph replays), which move the fusion state, so both seed buffers of
   // every frame are saved first and restored afterwards (the runs' depth /
   // dense / sparse outputs are not)
   int best_overlap = b->tune.overlap;
@@ -1678,19 +1703,56 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
       const std::vector<Tuning> keep_g = b->gtune;
       b->gtune = gbest;
       double best_t = 1e30;
+      // each setting timed as it runs in qm_batch_run's graph mode: one
+      // captured run, replayed reps times after a warm-up replay
+      hipStream_t cs = nullptr;
+      if (!rc) rc = ensure_side(b);
+      if (!rc) {
+        const hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+      }
       for (int v = 0; v < kOverlapSettings && !rc; ++v) {
-        Tuning t = keep;
-        t.overlap = v;
-        double ms = 0.0;
-        rc = time([&](const Tuning& tt) { b->tune = tt; (void)run_pipeline(b, par, s); }, t, &ms);
-        if (!rc && ms < best_t) {
+        b->tune = keep;
+        b->tune.overlap = v;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          const int prc = run_pipeline(b, par, cs);
+          const hipError_t ee = hipStreamEndCapture(cs, &g);
+          e = prc ? hipErrorUnknown : ee;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(ge, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(ge, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float f = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+          break;
+        }
+        const double ms = (double)f / reps;
+        if (ms < best_t) {
           best_t = ms;
           best_overlap = v;
         }
       }
+      if (cs) (void)hipStreamDestroy(cs);
       b->tune = keep;
       b->gtune = keep_g;
       if (!rc) rc = copy_seeds(false);
+      // the timing runs all used one parity and left its z-buffer filled:
+      // both start empty again
+      for (int i = 0; i < n && !rc; ++i)
+        for (int q = 0; q < 2 && !rc; ++q) {
+          const hipError_t e = hipMemsetAsync(b->zbuf[q][i], 0xFF, (size_t)hf[i].W * hf[i].H * 8, s);
+          if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
       if (!rc) {
         const hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
@@ -1748,6 +1810,7 @@ int qm_batch_destroy(qm_batch* b) {
     if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
   }
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_warp) (void)hipEventDestroy(b->ev_warp);
   for (int par = 0; par < 2; ++par) {
     if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
     if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);

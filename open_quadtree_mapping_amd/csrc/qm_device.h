@@ -43,6 +43,7 @@ struct FrameDev {
   float4* seeds;        // H x W (warped in place)
   float4* seeds_new;    // H x W
   unsigned long long* zbuf;  // H x W
+  unsigned long long* zbuf_clear;  // H x W: the z-buffer the column pass resets (in a batch the next run's)
   float* depth_out;     // H x W or null
   int32_t* table_out;   // H x W or null
   // generalised upsample inputs (differentiable path); null in the pipeline

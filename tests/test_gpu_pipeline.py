@@ -122,7 +122,7 @@ def test_every_launch_shape_gives_identical_results_and_autotune(tmp_path):
 
     b = Batch(frames, p, outputs=("sparse", "dense"))
     want = run(b)
-    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16), "overlap": tuple(range(11)),
+    offered = {"cost_waves": (2, 4, 8), "bp_rows": (4, 8, 16), "extract_rows": (4, 8, 16), "overlap": tuple(range(14)),
                "bp_pair": (0, 8, 16), "bp_seg": (8, 16, 32, 64), "bp_pair_levels": (1, 2, 3, 4)}
     defaults = {"cost_waves": 4, "bp_rows": 16, "extract_rows": 16, "overlap": 0, "bp_pair": 8, "bp_seg": 8,
                 "bp_pair_levels": 2}
@@ -201,7 +201,7 @@ def test_two_group_overlap_matches_one_stream_over_keyframes(graph):
     b = Batch(frames_at(4), p, outputs=("depth",))
     b.set_tuning(overlap=0)
     want = sequence(b)
-    for ov in range(1, 11):
+    for ov in range(1, 14):
         print(f"overlap={ov} graph={graph}", flush=True)
         b.set_tuning(overlap=ov)
         got = sequence(b)
