@@ -368,8 +368,8 @@ int qm_batch_get_group_tuning(qm_batch* b, int group, int knob, int* value);
  * each group on its own frames; per stage, the groups' own shapes are kept
  * when the whole batch runs that stage faster with them (a launch per run of
  * agreeing groups) than with the common shape (one launch). Then the overlap
- * knob over whole runs with those shapes, each setting captured once as a
- * graph and timed over reps replays. It first recomputes the quadtree,
+ * knob over whole runs (plain launches) with those shapes. It first
+ * recomputes the quadtree,
  * cost volume and pyramid; the fusion state is not touched. chosen receives
  * group 0's shapes and the overlap; best_ms the common search's time per
  * launch set for each shape knob and the overlap's per run (nullable both);

@@ -57,11 +57,11 @@ __device__ __forceinline__ float* msg_buf(const FrameDev& F, int l, int b) { ret
 // combines are commutative, so the result is bit-identical.
 template <int S>
 __device__ __forceinline__ float row_tree_min(Row<S> a) {
-#pragma unroll
-  for (int m = S / 2; m >= 1; m >>= 1)
-#pragma unroll
-    for (int s = 0; s < m; ++s) a.v[s] = vmin(a.v[s], a.v[s + m]);
+  // the minimum of a set is one of its values whatever the order: the lane's
+  // S values fold by three-operand minima
   float t = a.v[0];
+#pragma unroll
+  for (int s = 1; s < S; s += 2) t = s + 1 < S ? vmin(t, vmin(a.v[s], a.v[s + 1])) : vmin(t, a.v[s]);
   t = vmin(t, dpp_row_shl<8>(t));
   t = vmin(t, dpp_row_shl<4>(t));
   t = vmin(t, dpp_row_shl<2>(t));
@@ -86,21 +86,28 @@ __device__ __forceinline__ float row_tree_sum(Row<S> a) {
 // Outgoing message for one direction from h (the direction's belief without
 // the target's message): min(h[d], h[d-1]+P1, h[d+1]+P1, min h + P2), minus the
 // tree mean (src/bpextract.cu:272-302). Neighbour d - 1 of sample j + 16 s is
-// lane j - 1 (row_shr:1), or for j = 0 lane 15 of slot s - 1 (row_ror:1);
-// neighbour d + 1 is lane j + 1 (row_shl:1), or for j = 15 lane 0 of slot
-// s + 1 (row_ror:15). Samples 0 and N - 1 lack one neighbour and see their own
-// value there, and h + P1 > h never wins the min.
+// lane j - 1 of slot s, or for j = 0 lane 15 of slot s - 1: one row_ror:1 of a
+// per-lane choice (lane 15 offers slot s - 1, the others slot s), which the
+// compiler folds into the add of P1; neighbour d + 1 likewise by row_ror:15
+// (lane 0 offers slot s + 1). Samples 0 and N - 1 lack one neighbour and see
+// their own value there (row_shr:1 / row_shl:1 keeping the lane's own), and
+// h + P1 > h never wins the min. The minimum is exact, so its operand order
+// does not change the result.
 template <int S>
 __device__ __forceinline__ Row<S> message_from(const Row<S>& h, float p1, float p2, float inv_n) {
   const float c2 = row_tree_min(h) + p2;
+  const int j = __lane_id() & 15;
+  // P1 in a vector register: a VOP2 add with a DPP operand takes its other
+  // operand from a VGPR
+  float p1v = p1;
+  asm volatile("" : "+v"(p1v));
   Row<S> m;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const float lwrap = s > 0 ? dpp_mov<kRowRor1>(h.v[s - 1]) : h.v[s];
-    const float rwrap = s < S - 1 ? dpp_mov<kRowRor15>(h.v[s + 1]) : h.v[s];
-    const float hl = dpp_old<kRowShr1>(lwrap, h.v[s]);
-    const float hr = dpp_old<kRowShl1>(rwrap, h.v[s]);
-    m.v[s] = vmin(vmin(h.v[s], hl + p1), vmin(hr + p1, c2));
+    const float hl = s > 0 ? dpp_mov<kRowRor1>(j == 15 ? h.v[s - 1] : h.v[s]) : dpp_old<kRowShr1>(h.v[0], h.v[0]);
+    const float hr =
+        s < S - 1 ? dpp_mov<kRowRor15>(j == 0 ? h.v[s + 1] : h.v[s]) : dpp_old<kRowShl1>(h.v[s], h.v[s]);
+    m.v[s] = vmin(vmin(hl + p1v, hr + p1v), vmin(h.v[s], c2));
   }
   const float mean = row_tree_sum(m) * inv_n;  // sum / N, exact (N is a power of 2)
 #pragma unroll

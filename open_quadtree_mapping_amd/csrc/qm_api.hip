@@ -1678,7 +1678,7 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
     for (int k = 0; k < QM_NUM_TUNE; ++k)
       if (k != QM_TUNE_OVERLAP) best_ms[k] = common_ms[k];
   // the overlap knob, with the groups' chosen shapes: whole pipeline runs
-  // (graph replays), which move the fusion state, so both seed buffers of
+  // (plain launches), which move the fusion state, so both seed buffers of
   // every frame are saved first and restored afterwards (the runs' depth /
   // dense / sparse outputs are not)
   int best_overlap = b->tune.overlap;
@@ -1703,46 +1703,16 @@ int qm_batch_autotune(qm_batch* b, int reps, void* stream, int* chosen, double* 
       const std::vector<Tuning> keep_g = b->gtune;
       b->gtune = gbest;
       double best_t = 1e30;
-      // each setting timed as it runs in qm_batch_run's graph mode: one
-      // captured run, replayed reps times after a warm-up replay
-      hipStream_t cs = nullptr;
-      if (!rc) rc = ensure_side(b);
-      if (!rc) {
-        const hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-        if (e != hipSuccess) rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-      }
       for (int v = 0; v < kOverlapSettings && !rc; ++v) {
-        b->tune = keep;
-        b->tune.overlap = v;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-          const int prc = run_pipeline(b, par, cs);
-          const hipError_t ee = hipStreamEndCapture(cs, &g);
-          e = prc ? hipErrorUnknown : ee;
-        }
-        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (e == hipSuccess) e = hipGraphLaunch(ge, s);
-        if (e == hipSuccess) e = hipEventRecord(e0, s);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(ge, s);
-        if (e == hipSuccess) e = hipEventRecord(e1, s);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float f = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
-        if (ge) (void)hipGraphExecDestroy(ge);
-        if (g) (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-          rc = fail(QM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-          break;
-        }
-        const double ms = (double)f / reps;
-        if (ms < best_t) {
+        Tuning t = keep;
+        t.overlap = v;
+        double ms = 0.0;
+        rc = time([&](const Tuning& tt) { b->tune = tt; (void)run_pipeline(b, par, s); }, t, &ms);
+        if (!rc && ms < best_t) {
           best_t = ms;
           best_overlap = v;
         }
       }
-      if (cs) (void)hipStreamDestroy(cs);
       b->tune = keep;
       b->gtune = keep_g;
       if (!rc) rc = copy_seeds(false);
