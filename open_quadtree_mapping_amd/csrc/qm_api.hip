@@ -337,8 +337,18 @@ struct qm_batch {
   // column pass resets zbuf[p ^ 1] for the next run, so that the warp may
   // run anywhere before the update
   std::vector<unsigned long long*> zbuf[2];
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  // graph mode: the run's linear segments (see plan_segments), each captured
+  // as a graph of its own, per parity; captured is set once both exist
+  struct Segment {
+    int f0, nf;               // frames [f0, f0 + nf)
+    std::vector<int> stages;  // QM_STAGE_* in order
+    int stream;               // 0: the caller's stream, k: side stream k - 1
+    std::vector<int> waits;   // events (event ids) waited for before the segment
+    int record;               // event id recorded after the segment, or -1
+  };
+  std::vector<Segment> segs;
+  std::vector<hipGraphExec_t> seg_exec[2];
+  bool captured[2] = {false, false};
   int parity = 0;
   long long runs = 0;
   // the state a backward differentiates (the last run's descriptors, warp
@@ -544,12 +554,11 @@ void run_stages(qm_batch* b, int par, hipStream_t s, Timer* t = nullptr) {
   for (int st = 0; st < QM_NUM_STAGES; ++st) enqueue_stage(b, par, s, t, 0, b->n, st);
 }
 
-// One pipeline run. With the overlap setting on (and two frames or more) the
-// batch runs as G frame groups, group 0 on s and the others on their own
-// streams forked from s and joined back into it: the kernels of one group fill
-// the GPU while another group's latency-bound ones (coarse BP levels, the
-// serial upsample chains) run. Group g + 1 starts when group g has enqueued
-// the stage the setting names. Settings: 1-3 two groups, starting together,
+// The overlap setting. With it on (and two frames or more) a run splits the
+// batch into G frame groups, group 0 on s and the others on their own streams
+// forked from s and joined back into it: the kernels of one group fill the GPU
+// while another group's latency-bound ones (coarse BP levels, the upsample)
+// run. Group g + 1 starts when group g has run the stage the setting names. Settings: 1-3 two groups, starting together,
 // after the cost volume, after BP; 4 three and 5 four groups, each after the
 // previous group's cost volume; 6 eight such groups; 7 four groups, each after
 // the previous group's BP. Settings 8, 9 and 10 (two, three, four groups) run
@@ -592,90 +601,6 @@ int ensure_side(qm_batch* b) {
     if (!b->side[g]) QM_HIP(hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking));
   }
   b->side_ready = true;
-  return QM_OK;
-}
-
-int run_pipeline(qm_batch* b, int par, hipStream_t s) {
-  const OverlapPlan plan = overlap_plan(b->tune.overlap, b->n);
-  if (plan.groups < 2) {
-    run_stages(b, par, s);
-    return QM_OK;
-  }
-  int rc = ensure_side(b);
-  if (rc) return rc;
-  const int G = plan.groups;
-  int f0[qm_batch::kMaxGroups], nf[qm_batch::kMaxGroups];
-  for (int g = 0, at = 0; g < G; ++g) {
-    nf[g] = (b->n - at) / (G - g);  // split as evenly as the count allows
-    f0[g] = at;
-    at += nf[g];
-  }
-  auto stream_of = [&](int g) { return g == 0 ? s : b->side[g - 1]; };
-  QM_HIP(hipEventRecord(b->ev_fork, s));
-  // The stages that launch anything (without a prior cost the pyramid is part
-  // of the cost volume). They are enqueued in the order they are expected to
-  // run: active stage u of group g in slot u + g * lag (lag = stages up to the
-  // stagger point), ties by group, so that a group's next stage is always
-  // enqueued before the next group's first one (a captured graph keeps a
-  // node's first dependent on the node's own queue). Group g waits on its own
-  // stream for group g - 1's mark (or the fork) before its first stage.
-  int active[QM_NUM_STAGES], na = 0;
-  for (int u = 0; u < QM_NUM_STAGES; ++u)
-    if (u != QM_STAGE_PYRAMID || any_prior(b)) active[na++] = u;
-  if (plan.costs_first) {
-    // with early_warp the fusion warp of the whole batch first, on side
-    // stream 0 beside the cost volumes (it reads only the prior seeds and the
-    // poses, and scatters into this parity's z-buffer, which the previous
-    // run's column pass reset); then every group's head (quadtree, cost
-    // volume, pyramid) on s in turn, and the groups' tails: group g < G - 1 on
-    // side stream g after its head (and, for g > 0, the early warp), the last
-    // group's on s (after the early warp). Heads are enqueued first, so that
-    // the next head is the first dependent of a cost node and stays on s's
-    // queue.
-    if (plan.early_warp) {
-      QM_HIP(hipStreamWaitEvent(b->side[0], b->ev_fork, 0));
-      enqueue_stage(b, par, b->side[0], nullptr, 0, b->n, QM_STAGE_FUSE_WARP);
-      QM_HIP(hipEventRecord(b->ev_warp, b->side[0]));
-    }
-    auto head_stage = [](int u) { return u <= QM_STAGE_PYRAMID; };
-    for (int g = 0; g < G; ++g) {
-      for (int i = 0; i < na; ++i)
-        if (head_stage(active[i])) enqueue_stage(b, par, s, nullptr, f0[g], nf[g], active[i]);
-      if (g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], s));
-    }
-    for (int g = 0; g < G; ++g) {
-      hipStream_t sg = g + 1 < G ? b->side[g] : s;
-      if (g + 1 < G) QM_HIP(hipStreamWaitEvent(sg, b->ev_mark[g], 0));
-      if (plan.early_warp && g > 0) QM_HIP(hipStreamWaitEvent(sg, b->ev_warp, 0));
-      for (int i = 0; i < na; ++i)
-        if (!head_stage(active[i]) && !(plan.early_warp && active[i] == QM_STAGE_FUSE_WARP))
-          enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[i]);
-    }
-    for (int g = 0; g + 1 < G; ++g) {
-      QM_HIP(hipEventRecord(b->ev_join[g], b->side[g]));
-      QM_HIP(hipStreamWaitEvent(s, b->ev_join[g], 0));
-    }
-    return QM_OK;
-  }
-  int mark = -1;  // the stagger point (a stage number) as an index into active
-  for (int u = 0; u < na; ++u)
-    if (active[u] == plan.mark_after) mark = u;
-  const int lag = mark + 1;
-  const int slots = na + (G - 1) * lag;
-  for (int slot = 0; slot < slots; ++slot) {
-    for (int g = 0; g < G; ++g) {
-      const int u = slot - g * lag;
-      if (u < 0 || u >= na) continue;
-      hipStream_t sg = stream_of(g);
-      if (u == 0 && g > 0) QM_HIP(hipStreamWaitEvent(sg, lag > 0 ? b->ev_mark[g - 1] : b->ev_fork, 0));
-      enqueue_stage(b, par, sg, nullptr, f0[g], nf[g], active[u]);
-      if (lag > 0 && u == mark && g + 1 < G) QM_HIP(hipEventRecord(b->ev_mark[g], sg));
-    }
-  }
-  for (int g = 1; g < G; ++g) {
-    QM_HIP(hipEventRecord(b->ev_join[g - 1], b->side[g - 1]));
-    QM_HIP(hipStreamWaitEvent(s, b->ev_join[g - 1], 0));
-  }
   return QM_OK;
 }
 
@@ -1333,6 +1258,165 @@ int qm_batch_set_frame(qm_batch* b, int i, const qm_frame_desc* d) {
   return upload_frames(b);
 }
 
+// A run as a list of linear segments, each a run of consecutive stages of one
+// frame group on one stream, with events between the streams. Plain runs
+// launch the segments' kernels; graph runs capture each segment as a graph of
+// its own (a linear graph runs on the stream it is launched on) and launch
+// those, which replays faster than one graph holding every branch (a 32-map
+// run 1.775 -> 1.750 ms).
+enum { kEvFork = 0, kEvWarp = 1, kEvMark = 2, kEvJoin = 2 + qm_batch::kMaxGroups - 1 };
+static hipEvent_t event_of(const qm_batch* b, int id) {
+  if (id == kEvFork) return b->ev_fork;
+  if (id == kEvWarp) return b->ev_warp;
+  return id < kEvJoin ? b->ev_mark[id - kEvMark] : b->ev_join[id - kEvJoin];
+}
+
+static std::vector<qm_batch::Segment> plan_segments(const qm_batch* b) {
+  using Seg = qm_batch::Segment;
+  const OverlapPlan plan = overlap_plan(b->tune.overlap, b->n);
+  std::vector<int> active;
+  for (int u = 0; u < QM_NUM_STAGES; ++u)
+    if (u != QM_STAGE_PYRAMID || any_prior(b)) active.push_back(u);
+  std::vector<Seg> out;
+  const int G = plan.groups;
+  if (G < 2) {
+    out.push_back(Seg{0, b->n, active, 0, {}, -1});
+    return out;
+  }
+  int f0[qm_batch::kMaxGroups], nf[qm_batch::kMaxGroups];
+  for (int g = 0, at = 0; g < G; ++g) {
+    nf[g] = (b->n - at) / (G - g);
+    f0[g] = at;
+    at += nf[g];
+  }
+  if (plan.costs_first) {
+    std::vector<int> heads, tails;
+    for (int u : active) {
+      if (u <= QM_STAGE_PYRAMID) heads.push_back(u);
+      else if (!(plan.early_warp && u == QM_STAGE_FUSE_WARP)) tails.push_back(u);
+    }
+    if (plan.early_warp) out.push_back(Seg{0, b->n, {QM_STAGE_FUSE_WARP}, 1, {kEvFork}, kEvWarp});
+    for (int g = 0; g < G; ++g) out.push_back(Seg{f0[g], nf[g], heads, 0, {}, g + 1 < G ? kEvMark + g : -1});
+    for (int g = 0; g < G; ++g) {
+      Seg t{f0[g], nf[g], tails, g + 1 < G ? g + 1 : 0, {}, g + 1 < G ? kEvJoin + g : -1};
+      if (g + 1 < G) t.waits.push_back(kEvMark + g);
+      if (plan.early_warp && g > 0) t.waits.push_back(kEvWarp);
+      out.push_back(t);
+    }
+    return out;
+  }
+  int mark = -1;
+  for (int u = 0; u < (int)active.size(); ++u)
+    if (active[u] == plan.mark_after) mark = u;
+  for (int g = 0; g < G; ++g) {
+    std::vector<int> first_waits;
+    if (g > 0) first_waits.push_back(mark >= 0 ? kEvMark + g - 1 : kEvFork);
+    const int end_rec = g > 0 ? kEvJoin + g - 1 : -1;
+    if (mark >= 0 && g + 1 < G) {
+      out.push_back(Seg{f0[g], nf[g], std::vector<int>(active.begin(), active.begin() + mark + 1), g, first_waits,
+                        kEvMark + g});
+      out.push_back(Seg{f0[g], nf[g], std::vector<int>(active.begin() + mark + 1, active.end()), g, {}, end_rec});
+    } else {
+      out.push_back(Seg{f0[g], nf[g], active, g, first_waits, end_rec});
+    }
+  }
+  return out;
+}
+
+// the segments' graphs of parity par, captured once (set-up calls drop them)
+static int capture_segments(qm_batch* b, int par) {
+  if (b->captured[par]) return QM_OK;
+  // both parities share one plan: planned when neither is captured
+  if (!b->captured[par ^ 1]) b->segs = plan_segments(b);
+  if (b->segs.size() > 1) {
+    const int rc = ensure_side(b);
+    if (rc) return rc;
+  }
+  hipStream_t cs;
+  QM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipSuccess;
+  std::vector<hipGraphExec_t>& ex = b->seg_exec[par];
+  for (const qm_batch::Segment& sg : b->segs) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) break;
+    for (int st : sg.stages) enqueue_stage(b, par, cs, nullptr, sg.f0, sg.nf, st);
+    e = hipStreamEndCapture(cs, &g);
+    size_t nodes = 0;
+    if (e == hipSuccess) e = hipGraphGetNodes(g, nullptr, &nodes);
+    if (e == hipSuccess && nodes > 0) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) break;
+    ex.push_back(ge);  // null for a segment without kernels
+  }
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    for (hipGraphExec_t x : ex)
+      if (x) (void)hipGraphExecDestroy(x);
+    ex.clear();
+    return fail(QM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  b->captured[par] = true;
+  return QM_OK;
+}
+
+// Launches the segments on their streams, as the captured graphs of parity
+// par (graph) or as plain kernel launches: every event is recorded (in host
+// order) before a wait on it is issued, and the side streams' segments go
+// first whenever their events are recorded, so that their queues start at once.
+static int launch_segments(qm_batch* b, int par, hipStream_t s, const std::vector<qm_batch::Segment>& segs,
+                           bool graph) {
+  const int ns = (int)segs.size();
+  if (ns > 1) {
+    const int rc = ensure_side(b);
+    if (rc) return rc;
+  }
+  auto stream_of = [&](int k) { return k == 0 ? s : b->side[k - 1]; };
+  bool recorded[kEvJoin + qm_batch::kMaxGroups] = {};
+  std::vector<bool> done(ns, false);
+  if (ns > 1) {
+    QM_HIP(hipEventRecord(b->ev_fork, s));
+    recorded[kEvFork] = true;
+  }
+  for (int left = ns; left > 0;) {
+    int pick = -1;
+    for (int pass = 0; pass < 2 && pick < 0; ++pass)
+      for (int i = 0; i < ns && pick < 0; ++i) {
+        if (done[i] || (pass == 0 && segs[i].stream == 0)) continue;
+        // a stream's segments keep their list order
+        bool earlier = false;
+        for (int k = 0; k < i; ++k) earlier |= !done[k] && segs[k].stream == segs[i].stream;
+        bool ready = !earlier;
+        for (int w : segs[i].waits) ready &= recorded[w];
+        if (ready) pick = i;
+      }
+    if (pick < 0) return fail(QM_ERR_STATE, "graph segments: no launchable segment");
+    const qm_batch::Segment& sg = segs[pick];
+    hipStream_t st = stream_of(sg.stream);
+    for (int w : sg.waits) QM_HIP(hipStreamWaitEvent(st, event_of(b, w), 0));
+    if (!graph) {
+      for (int u : sg.stages) enqueue_stage(b, par, st, nullptr, sg.f0, sg.nf, u);
+    } else if (b->seg_exec[par][pick]) {
+      QM_HIP(hipGraphLaunch(b->seg_exec[par][pick], st));
+    }
+    if (sg.record >= 0) {
+      QM_HIP(hipEventRecord(event_of(b, sg.record), st));
+      recorded[sg.record] = true;
+    }
+    done[pick] = true;
+    --left;
+  }
+  for (int i = 0; i < ns; ++i)
+    if (segs[i].stream != 0 && segs[i].record >= kEvJoin) QM_HIP(hipStreamWaitEvent(s, event_of(b, segs[i].record), 0));
+  return QM_OK;
+}
+
+// One pipeline run as plain launches (the overlap setting's segments).
+int run_pipeline(qm_batch* b, int par, hipStream_t s) {
+  return launch_segments(b, par, s, plan_segments(b), false);
+}
+
 int qm_batch_run(qm_batch* b, int mode, void* stream) {
   if (!b) return fail(QM_ERR_INVALID, "NULL batch");
   hipStream_t s = (hipStream_t)stream;
@@ -1342,24 +1426,9 @@ int qm_batch_run(qm_batch* b, int mode, void* stream) {
     if (rc) return rc;
     QM_HIP(hipGetLastError());
   } else if (mode == 1) {
-    if (!b->exec[par]) {
-      if (b->tune.overlap != 0) {
-        int rc = ensure_side(b);
-        if (rc) return rc;
-      }
-      hipStream_t cs;
-      QM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        const int rc = run_pipeline(b, par, cs);
-        const hipError_t ee = hipStreamEndCapture(cs, &b->graph[par]);
-        e = rc ? hipErrorUnknown : ee;
-      }
-      if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[par], b->graph[par], nullptr, nullptr, 0);
-      (void)hipStreamDestroy(cs);
-      if (e != hipSuccess) return fail(QM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    }
-    QM_HIP(hipGraphLaunch(b->exec[par], s));
+    int rc = capture_segments(b, par);
+    if (!rc) rc = launch_segments(b, par, s, b->segs, true);
+    if (rc) return rc;
   } else {
     return fail(QM_ERR_INVALID, "mode must be 0 or 1");
   }
@@ -1432,12 +1501,12 @@ static int* tune_slot(Tuning& t, int knob) {
 }
 
 static void drop_graphs(qm_batch* b) {
-  if (b->exec[0] || b->exec[1]) (void)hipDeviceSynchronize();  // no replay may still be running
+  if (b->captured[0] || b->captured[1]) (void)hipDeviceSynchronize();  // no replay may still be running
   for (int par = 0; par < 2; ++par) {
-    if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
-    if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);
-    b->exec[par] = nullptr;
-    b->graph[par] = nullptr;
+    for (hipGraphExec_t e : b->seg_exec[par])
+      if (e) (void)hipGraphExecDestroy(e);
+    b->seg_exec[par].clear();
+    b->captured[par] = false;
   }
 }
 
@@ -1781,11 +1850,8 @@ int qm_batch_destroy(qm_batch* b) {
   }
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_warp) (void)hipEventDestroy(b->ev_warp);
-  for (int par = 0; par < 2; ++par) {
-    if (b->exec[par]) (void)hipGraphExecDestroy(b->exec[par]);
-    if (b->graph[par]) (void)hipGraphDestroy(b->graph[par]);
-    dev_free(b->df[par]);
-  }
+  drop_graphs(b);
+  for (int par = 0; par < 2; ++par) dev_free(b->df[par]);
   dev_free(b->arena);
   dev_free(b->bwd_arena);
   dev_free(b->dfb);
