@@ -185,8 +185,12 @@ def stage_bytes(levels, W, H, N, pairs=True):
         "extract": nodes * (20 * N + 8) + others * 9,
         "upsample_row": px + 8 * C + 8 * px,           # image, labels + depths, row results
         "upsample_col": px + 8 * px + 4 * px + 8 * px,  # image, row results, dense, z-buffer reset
-        "fuse_warp": 40 * px,
-        "fuse_update": 48 * px,
+        # the batch's warp reads the prior seeds and scatters z-buffer keys (the
+        # warped seeds are rebuilt by the update from the winner's key); the
+        # update reads keys, the winner's seed and the dense depth and writes
+        # the new seeds and the depth (the warp table only once a backward asks)
+        "fuse_warp": 16 * px + 8 * px,
+        "fuse_update": 8 * px + 16 * px + 4 * px + 16 * px + 4 * px,
     }, upd
 
 

@@ -392,6 +392,10 @@ int qm_batch_runs(qm_batch* b, long long* runs);
 /* Device pointer of an internal buffer of frame i, for inspection:
  * 0 levels, 1 cost (level 0), 2 depth_coarse, 3 label, 4 dense, 5 seeds (current state),
  * 6 hole-filled warp table of the last run (int32), 7 warped prior seeds of the last run.
+ * A run writes 6 and 7 only once the batch has had a backward; before that
+ * asking for either replays them from the last run's prior seeds and z-buffer
+ * (a kernel on the null stream, then a synchronisation, once per run), which
+ * is possible until the next run or a change of the batch.
  * The internal cost volume keeps each cell's row of N samples in the row layout:
  * sample d = j + 16 s at position (N/16) * j + s (j < 16). */
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr);

@@ -1295,15 +1295,49 @@ __global__ __launch_bounds__(256) void k_fill_zbuf(const FrameDev* __restrict__ 
 // z-buffer entry of a target pixel is the minimum of (float bits of the warped
 // depth, source index) over its sources: nearest wins, ties -> lower index.
 // Source 0 is the reference's empty sentinel and is never stored.
-__global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ frames, BlockMap bm) {
-  const int f = find_frame(bm, blockIdx.x);
-  const FrameDev& F = frames[f];
-  const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+// z-buffer key of a warped seed: (float bits of its depth, NaN as one
+// canonical NaN) above its source index, so that the minimum is the nearest
+// seed and, among equal depths, the lower source
+__device__ __forceinline__ unsigned long long warp_key(float nd, int idx) {
+  uint32_t bits = __float_as_uint(nd);
+  if (nd != nd) bits = 0x7FC00000u;
+  return ((unsigned long long)bits << 32) | (unsigned long long)(uint32_t)idx;
+}
+__device__ __forceinline__ float grown_variance(float s2, float nd) { return (float)((double)s2 + (double)nd * 0.01); }
+
+// The source seed of pixel idx after the warp: its own z-buffer winner, else
+// (hole filling, src/depth_fusion.cu:121-144) the first non-empty neighbour
+// in raster order whose position is in [2, W-2] x [2, H-2]; 0 for none. key
+// receives the z-buffer key the source came from.
+__device__ __forceinline__ int fused_source(const FrameDev& F, int idx, unsigned long long& key) {
   const int W = F.W, H = F.H;
-  if (idx >= W * H) return;
   const int y = idx / W, x = idx - y * W;
-  float4 s = F.seeds[idx];
-  if (is_badpoint(s.x, s.y)) return;
+  const unsigned long long empty = ~0ull;
+  key = F.zbuf[idx];
+  int src = key == empty ? 0 : (int)(uint32_t)(key & 0xFFFFFFFFull);
+  if (src == 0) {
+    for (int sy = y - 1; sy <= y + 1 && src == 0; ++sy)
+      for (int sx = x - 1; sx <= x + 1; ++sx) {
+        if (sx == x && sy == y) continue;
+        if (sx < 2 || sx > W - 2 || sy < 2 || sy > H - 2) continue;
+        const unsigned long long kk = F.zbuf[(size_t)sy * W + sx];
+        if (kk != empty) {
+          src = (int)(uint32_t)(kk & 0xFFFFFFFFull);
+          key = kk;
+          break;
+        }
+      }
+  }
+  return src;
+}
+
+// The warp of seed idx (src/depth_fusion.cu:48-119): false when the seed is
+// bad or leaves the image; else s becomes the warped seed (depth nd, variance
+// grown by 0.01 nd) and (pxi, pyi) its pixel.
+__device__ __forceinline__ bool warp_seed(const FrameDev& F, int idx, float4& s, int& pxi, int& pyi) {
+  const int W = F.W, H = F.H;
+  const int y = idx / W, x = idx - y * W;
+  if (is_badpoint(s.x, s.y)) return false;
   const V3 dir = ray_dir(x, y, F.fx, F.fy, F.cx, F.cy);
   V3 P;
   P.x = dir.x * s.z;
@@ -1311,21 +1345,46 @@ __global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ 
   P.z = dir.z * s.z;
   const V3 q = se3_apply(F.T_fuse, P);
   const float nd = sqrtf((q.x * q.x + q.y * q.y) + q.z * q.z);
-  if ((double)nd <= 0.5) return;
+  if ((double)nd <= 0.5) return false;
   const float pxf = (F.fx * q.x) / q.z + F.cx;
   const float pyf = (F.fy * q.y) / q.z + F.cy;
-  const int pxi = trunc_i32((double)pxf + 0.5);
-  const int pyi = trunc_i32((double)pyf + 0.5);
-  if (pxi >= W || pxi < 0 || pyi >= H || pyi < 0) return;
+  pxi = trunc_i32((double)pxf + 0.5);
+  pyi = trunc_i32((double)pyf + 0.5);
+  if (pxi >= W || pxi < 0 || pyi >= H || pyi < 0) return false;
   s.z = nd;
-  s.w = (float)((double)s.w + (double)nd * 0.01);
-  F.seeds[idx] = s;
-  if (idx != 0) {
-    uint32_t bits = __float_as_uint(nd);
-    if (nd != nd) bits = 0x7FC00000u;
-    const unsigned long long key = ((unsigned long long)bits << 32) | (unsigned long long)(uint32_t)idx;
-    atomicMin(&F.zbuf[(size_t)pyi * W + pxi], key);
-  }
+  s.w = grown_variance(s.w, nd);
+  return true;
+}
+
+// The warp writes the warped seed back in place (the reference's layout)
+// unless warp_skip: then the update rebuilds the winner's warped depth and
+// variance from its z-buffer key and its prior seed, and the seeds stay as
+// they were.
+__global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ frames, BlockMap bm) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+  if (idx >= F.W * F.H) return;
+  float4 s = F.seeds[idx];
+  int pxi, pyi;
+  if (!warp_seed(F, idx, s, pxi, pyi)) return;
+  if (!F.warp_skip) F.seeds[idx] = s;
+  if (idx != 0) atomicMin(&F.zbuf[(size_t)pyi * F.W + pxi], warp_key(s.z, idx));
+}
+
+// Replays, for a batch run that skipped them, the warped seeds (in place) and
+// the hole-filled warp table, from the run's prior seeds and its intact
+// z-buffer: exactly what k_fuse_warp and k_fuse_update would have written.
+__global__ __launch_bounds__(256) void k_fuse_replay(const FrameDev* __restrict__ frames, BlockMap bm) {
+  const int f = find_frame(bm, blockIdx.x);
+  const FrameDev& F = frames[f];
+  const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
+  if (idx >= F.W * F.H) return;
+  unsigned long long key;
+  F.table_out[idx] = fused_source(F, idx, key);
+  float4 s = F.seeds[idx];
+  int pxi, pyi;
+  if (warp_seed(F, idx, s, pxi, pyi)) F.seeds[idx] = s;
 }
 
 // Hole filling from the z-buffer snapshot (an empty cell takes the first
@@ -1338,22 +1397,8 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
   const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
   const int W = F.W, H = F.H;
   if (idx >= W * H) return;
-  const int y = idx / W, x = idx - y * W;
-  const unsigned long long empty = ~0ull;
-  unsigned long long k = F.zbuf[idx];
-  int src = k == empty ? 0 : (int)(uint32_t)(k & 0xFFFFFFFFull);
-  if (src == 0) {
-    for (int sy = y - 1; sy <= y + 1 && src == 0; ++sy)
-      for (int sx = x - 1; sx <= x + 1; ++sx) {
-        if (sx == x && sy == y) continue;
-        if (sx < 2 || sx > W - 2 || sy < 2 || sy > H - 2) continue;
-        const unsigned long long kk = F.zbuf[(size_t)sy * W + sx];
-        if (kk != empty) {
-          src = (int)(uint32_t)(kk & 0xFFFFFFFFull);
-          break;
-        }
-      }
-  }
+  unsigned long long key;
+  const int src = fused_source(F, idx, key);
   if (F.table_out) F.table_out[idx] = src;
   const float de = F.dense[idx];
   float unc = fmaxf(0.5f, (float)((double)de * 0.2));
@@ -1361,7 +1406,12 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
   if (de <= 0.0f) unc = 1e9f;
   float a, b, mu, s2;
   if (src > 0) {
-    const float4 s = F.seeds[src];
+    float4 s = F.seeds[src];
+    if (F.warp_skip) {
+      // the warp left the prior as it was: its depth is the key's, its variance grows by it
+      s.z = __uint_as_float((uint32_t)(key >> 32));
+      s.w = grown_variance(s.w, s.z);
+    }
     a = s.x;
     b = s.y;
     mu = s.z;
@@ -1480,6 +1530,12 @@ void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n,
   BlockMap bm;
   const int nb = pixels_map(bm, hframes, n);
   if (nb > 0) hipLaunchKernelGGL(k_fuse_update, dim3(nb), dim3(256), 0, s, dframes, bm);
+}
+
+void launch_fuse_replay(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s) {
+  BlockMap bm;
+  const int nb = pixels_map(bm, hframes, n);
+  if (nb > 0) hipLaunchKernelGGL(k_fuse_replay, dim3(nb), dim3(256), 0, s, dframes, bm);
 }
 
 void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,

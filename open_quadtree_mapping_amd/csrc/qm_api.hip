@@ -337,6 +337,15 @@ struct qm_batch {
   // column pass resets zbuf[p ^ 1] for the next run, so that the warp may
   // run anywhere before the update
   std::vector<unsigned long long*> zbuf[2];
+  // the hole-filled warp table of each frame. A run writes it, and the warp
+  // writes the warped prior seeds back, only once a backward has been asked
+  // for (keep_fusion); before that the first backward, or a read of those
+  // buffers, replays them from the run's prior seeds and z-buffer (once per
+  // run: replayed_run)
+  std::vector<int32_t*> table_buf;
+  bool keep_fusion = false;
+  long long replayed_run = -1;
+  FrameDev* dfr = nullptr;  // descriptors of the replay
   // graph mode: the run's linear segments (see plan_segments), each captured
   // as a graph of its own, per parity; captured is set once both exist
   struct Segment {
@@ -1186,6 +1195,7 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
   b->seeds[1].resize(n);
   b->zbuf[0].resize(n);
   b->zbuf[1].resize(n);
+  b->table_buf.resize(n);
   for (int i = 0; i < n; ++i) {
     FrameDev& F = base[i];
     const size_t px = (size_t)F.W * F.H;
@@ -1206,7 +1216,9 @@ int qm_batch_create(const qm_frame_desc* frames, int n, const qm_params* p, qm_b
     b->seeds[1][i] = cv.take<float4>(px * 16);
     b->zbuf[0][i] = cv.take<unsigned long long>(px * 8);
     b->zbuf[1][i] = cv.take<unsigned long long>(px * 8);
-    F.table_out = cv.take<int32_t>(px * 4);
+    b->table_buf[i] = cv.take<int32_t>(px * 4);
+    F.table_out = nullptr;
+    F.warp_skip = 1;
     apply_desc(F, frames[i], *p);
   }
   for (int par = 0; par < 2; ++par) {
@@ -1469,9 +1481,37 @@ int qm_batch_run_timed(qm_batch* b, void* stream, double* stage_ms, int* stage_l
   return QM_OK;
 }
 
+// The warp table and the warped prior seeds of the last run, for a run that
+// did not write them (no backward asked for yet): replayed on stream s from
+// its prior seeds and its z-buffer, both intact until the next run. Once per
+// run; nothing to do for a run that wrote them or after the state changed.
+static int replay_fusion(qm_batch* b, hipStream_t s) {
+  if (b->keep_fusion || b->runs == 0 || !b->run_state_valid || b->replayed_run == b->runs) return QM_OK;
+  const int n = b->n;
+  if (!b->dfr) {
+    void* p = nullptr;
+    const int rc = dev_alloc(&p, sizeof(FrameDev) * n);
+    if (rc) return rc;
+    b->dfr = (FrameDev*)p;
+  }
+  std::vector<FrameDev> hr = b->hf[b->parity ^ 1];
+  for (int i = 0; i < n; ++i) hr[i].table_out = b->table_buf[i];
+  QM_HIP(hipMemcpyAsync(b->dfr, hr.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+  launch_fuse_replay(b->dfr, hr.data(), n, s);
+  QM_HIP(hipGetLastError());
+  QM_HIP(hipStreamSynchronize(s));  // hr goes out of scope
+  b->replayed_run = b->runs;
+  return QM_OK;
+}
+
 int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
   if (!b || !ptr) return fail(QM_ERR_INVALID, "NULL pointer");
   if (i < 0 || i >= b->n) return fail(QM_ERR_INVALID, "frame index out of range");
+  if (which == 6 || which == 7) {
+    const int rc = replay_fusion(b, nullptr);
+    if (rc) return rc;
+    QM_HIP(hipStreamSynchronize(nullptr));
+  }
   const FrameDev& F = b->hf[b->parity][i];
   switch (which) {
     case 0: *ptr = F.levels; break;
@@ -1480,7 +1520,7 @@ int qm_batch_buffer(qm_batch* b, int i, int which, void** ptr) {
     case 3: *ptr = F.label; break;
     case 4: *ptr = F.dense; break;
     case 5: *ptr = F.seeds; break;
-    case 6: *ptr = F.table_out; break;
+    case 6: *ptr = b->table_buf[i]; break;
     case 7: *ptr = b->hf[b->parity ^ 1][i].seeds; break;
     default: return fail(QM_ERR_INVALID, "unknown buffer id");
   }
@@ -1852,6 +1892,7 @@ int qm_batch_destroy(qm_batch* b) {
   if (b->ev_warp) (void)hipEventDestroy(b->ev_warp);
   drop_graphs(b);
   for (int par = 0; par < 2; ++par) dev_free(b->df[par]);
+  dev_free(b->dfr);
   dev_free(b->arena);
   dev_free(b->bwd_arena);
   dev_free(b->dfb);
@@ -1908,11 +1949,13 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
       w.lam_part = cv.take<double>(sizeof(double) * (F.W + F.H));
     }
   }
+  // the run's warp table and warped prior seeds, which the fusion adjoint reads
+  if ((rc = replay_fusion(b, s))) return rc;
   std::vector<FrameDev> hb = b->hf[last];
   for (int i = 0; i < n; ++i) {
     FrameDev& F = hb[i];
     const qm_batch::Bwd& w = b->bwd[i];
-    F.table_in = F.table_out;
+    F.table_in = b->table_buf[i];
     F.table_out = nullptr;
     F.depth_out = nullptr;
     F.dense_out = nullptr;
@@ -1949,12 +1992,17 @@ int qm_batch_backward(qm_batch* b, const float* const* g_depth_out, double* grad
   QM_HIP(hipMemcpyAsync(grad_lambda, b->partial, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   QM_HIP(hipStreamSynchronize(s));
   if (b->colxn_since < 0) {
-    // later runs keep their column results for their backward (a few MB of
-    // writes per run)
+    // later runs keep their column results, their warp tables and their
+    // warped prior seeds for their backward (a few MB of writes per run)
     for (int par = 0; par < 2; ++par)
-      for (int i = 0; i < n; ++i) b->hf[par][i].colxn = b->bwd[i].colxn;
+      for (int i = 0; i < n; ++i) {
+        b->hf[par][i].colxn = b->bwd[i].colxn;
+        b->hf[par][i].table_out = b->table_buf[i];
+        b->hf[par][i].warp_skip = 0;
+      }
     if ((rc = upload_frames(b))) return rc;
     b->colxn_since = b->runs;
+    b->keep_fusion = true;
   }
   return QM_OK;
 }

@@ -40,7 +40,8 @@ struct FrameDev {
   float* rowx;          // row pass result: X then N, each a W x H plane [x][y]
   float* dense;         // H x W
   float* dense_out;     // H x W copy for the caller or null
-  float4* seeds;        // H x W (warped in place)
+  float4* seeds;        // H x W (warped in place unless warp_skip)
+  int warp_skip;        // 1: the warp leaves the seeds as they are (the update rebuilds the winner's)
   float4* seeds_new;    // H x W
   unsigned long long* zbuf;  // H x W
   unsigned long long* zbuf_clear;  // H x W: the z-buffer the column pass resets (in a batch the next run's)

@@ -108,6 +108,7 @@ void launch_upsample_local(const float* sparse, const uint8_t* img, int W, int H
 void launch_fill_zbuf(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_warp(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 void launch_fuse_update(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
+void launch_fuse_replay(const FrameDev* dframes, const FrameDev* hframes, int n, hipStream_t s);
 // adjoints: the generalised upsample forward recomputed (recompute; else its
 // rowx planes and colxn are the forward's), then the column and row adjoint
 // solves; fusion update adjoint w.r.t. its dense depth input
