@@ -171,3 +171,45 @@ def test_batch_lambda_gradient_at_small_lambda(lam):
         fd = (f(lam + 1e-6) - f(lam - 1e-6)) / 2e-6
         assert abs(grad[i] - fd) <= 5e-3 * abs(fd) + 1e-6, f"stream {i}: {grad[i]} vs {fd}"
     batch.close()
+
+
+def test_batch_fusion_state_replayed_and_kept():
+    """A batch run writes neither the hole-filled warp table nor the warped
+    prior seeds until a backward has asked for them; reading them replays
+    both from the run's prior seeds and z-buffer (once: a second read returns
+    the same, not a second warp). Replayed, and after a backward written by
+    the run itself, they equal the oracle's warp and table for the run's prior
+    seeds, and the run's depth and seeds equal the oracle's update."""
+    W, H = 376, 240
+    sets = [keyframe_set(W, H, 60 + k) for k in range(2)]
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    c, s = np.cos(0.01), np.sin(0.01)
+    T_last = np.array([[c, 0, s, 0.01], [0, 1, 0, 0.0], [-s, 0, c, 0.005]], np.float32)
+    frames = [dict(key=dev(ks["key"]), refs=[dev(r) for r in ks["refs"]], T=ks["T"], cam=cam_tuple(ks["cam"]),
+                   T_last=T_last) for ks in sets]
+    p = default_params(n_hyp=64, idepth_min=0.02, idepth_max=2.0)
+    batch = Batch(frames, p, outputs=("depth", "dense"))
+
+    def check(kept):
+        prior_in = [batch.read_buffer(i, Batch.BUF_SEEDS) for i in range(2)]
+        batch.run(graph=True)
+        torch.cuda.synchronize()
+        for i, ks in enumerate(sets):
+            dense = batch.outputs[i]["dense"].cpu().numpy()
+            warped, dout, sout, table = O.fuse(prior_in[i], dense, T_last, ks["cam"])
+            got_t = batch.read_buffer(i, Batch.BUF_TABLE)
+            got_p = batch.read_buffer(i, Batch.BUF_PRIOR)
+            assert np.array_equal(got_t, table), f"stream {i} table (kept={kept})"
+            assert np.array_equal(got_p, warped, equal_nan=True), f"stream {i} warped prior (kept={kept})"
+            # a second read neither warps again nor changes the table
+            assert np.array_equal(batch.read_buffer(i, Batch.BUF_PRIOR), got_p, equal_nan=True)
+            assert np.array_equal(batch.read_buffer(i, Batch.BUF_TABLE), got_t)
+            assert np.array_equal(batch.outputs[i]["depth"].cpu().numpy(), dout, equal_nan=True)
+            assert np.array_equal(batch.read_buffer(i, Batch.BUF_SEEDS), sout, equal_nan=True)
+
+    for _ in range(4):
+        batch.run(graph=True)
+    check(kept=False)
+    batch.backward([torch.ones(H, W, device="cuda") for _ in range(2)])
+    check(kept=True)
+    batch.close()
