@@ -463,7 +463,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
               rw[r][j] = (t[r][j][1] == t[r][j][0] + 1u) & (t[r][j][2] == t[r][j][0] + 2u) & (t[r][j][0] <= last4);
-              v[r][j] = img_dword(rs[r], rw[r][j] ? t[r][j][0] : 0u);
+              // unconditional: a dword past the image end (a row starting in
+              // its last three bytes, which also gathers) reads as zero, no
+              // fault, through the buffer's range check
+              v[r][j] = img_dword(rs[r], t[r][j][0]);
             }
 #pragma unroll
             for (int j = 0; j < 3; ++j)
@@ -478,7 +481,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
             // skipped when no lane of the wave gathered that row)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-              if (!rw[r][j]) v[r][j] = t[r][j][0] | (t[r][j][1] << 8) | (t[r][j][2] << 16);
+              if (!rw[r][j]) {
+                // bytes (t0, t1, 0, 0), then (t0, t1, t2, 0): two byte permutes
+                const unsigned lo = __builtin_amdgcn_perm(t[r][j][1], t[r][j][0], 0x0C0C0400u);
+                v[r][j] = __builtin_amdgcn_perm(t[r][j][2], lo, 0x0C040100u);
+              }
             unsigned sad = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) sad = __builtin_amdgcn_sad_u8(v[r][j] & 0xFFFFFFu, prow[j], sad);
