@@ -457,6 +457,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
           // dword. All loads are issued before any SAD.
           unsigned t[4][3][3], v[4][3];
           bool rw[4][3];
+          bool gathered = false;  // this lane gathered some row
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             tap_offsets<FUSED>(pxs[r], pys[r], T[r], pitch, t[r]);
@@ -473,19 +474,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
               if (!rw[r][j]) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) t[r][j][i] = img_byte(rs[r], t[r][j][i]);
+                gathered = true;
+              }
+          }
+          // the gathered rows' three texels packed like a dword row: one
+          // wave-uniform branch, skipped (scalar) when no lane gathered any
+          // row, as with rotation-free poses, where every row is contiguous
+          if (__builtin_amdgcn_ballot_w64(gathered)) {
+            asm volatile("");  // a real branch: not speculated into selects
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) {
+                // bytes (t0, t1, 0, 0), then (t0, t1, t2, 0): two byte permutes
+                const unsigned lo = __builtin_amdgcn_perm(t[r][j][1], t[r][j][0], 0x0C0C0400u);
+                const unsigned g = __builtin_amdgcn_perm(t[r][j][2], lo, 0x0C040100u);
+                v[r][j] = rw[r][j] ? v[r][j] : g;
               }
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            // a gathered row's three texels packed like a dword row (divergent:
-            // skipped when no lane of the wave gathered that row)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-              if (!rw[r][j]) {
-                // bytes (t0, t1, 0, 0), then (t0, t1, t2, 0): two byte permutes
-                const unsigned lo = __builtin_amdgcn_perm(t[r][j][1], t[r][j][0], 0x0C0C0400u);
-                v[r][j] = __builtin_amdgcn_perm(t[r][j][2], lo, 0x0C040100u);
-              }
             unsigned sad = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) sad = __builtin_amdgcn_sad_u8(v[r][j] & 0xFFFFFFu, prow[j], sad);
