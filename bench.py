@@ -76,6 +76,9 @@ def parse():
                     help="line solver of the upsample (qm_params.upsample_solver): scan (segmented scans over "
                          "64 lanes per line) or thomas (the reference's serial recurrences); both are "
                          "bit-identical to the oracle's matching mode")
+    ap.add_argument("--jitter-deg", type=float, default=0.0,
+                    help="rotation jitter of the synthetic camera poses, U(-j, +j) degrees about each axis "
+                         "(default 0: the keyframe-to-reference poses are pure translations)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -412,7 +415,7 @@ def main():
     nsc = min(args.scenes, B)
     scenes = []
     for s in range(nsc):
-        traj = forward_lateral_trajectory(6, seed=100 + s)
+        traj = forward_lateral_trajectory(6, seed=100 + s, jitter_deg=args.jitter_deg)
         sc = make_scene(100 + s)
         imgs = [render(sc, cam, T)[0] for T in traj]
         k = 5
@@ -590,12 +593,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded planar scenes (6 textured rectangles, fractal value noise) rendered to uint8",
+        "data": "synthetic: seeded planar scenes (6 textured rectangles, fractal value noise) rendered to uint8 "
+                "along a camera path of 5 cm per frame"
+                + (f" with U(-{args.jitter_deg}, {args.jitter_deg}) deg rotation jitter" if args.jitter_deg > 0
+                   else " (pure translations)"),
         "config": {"workload": f"{B} streams x {W}x{H} keyframes, {N} inverse-depth hypotheses, full pipeline "
                                "(quadtree, cost volume, HBP, upsample, fusion), "
-                               + ("one hipGraph replay per step" if use_graph else "plain batched launches per step"),
+                               + ("graph replays per step (one graph per stream segment)" if use_graph
+                                  else "plain batched launches per step"),
                    "streams": B, "width": W, "height": H, "hypotheses": N, "cost_arith": args.cost_arith,
-                   "upsample_solver": args.upsample_solver,
+                   "upsample_solver": args.upsample_solver, "pose_jitter_deg": args.jitter_deg,
                    "launch_shapes": batch_tuning,
                    "launch_shapes_source": ("defaults" if tuned is None else
                                             "cache" if all(ms is None for _, ms in tuned.values()) else "autotune")},
