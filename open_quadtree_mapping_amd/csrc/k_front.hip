@@ -463,7 +463,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
             tap_offsets<FUSED>(pxs[r], pys[r], T[r], pitch, t[r]);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-              rw[r][j] = (t[r][j][1] == t[r][j][0] + 1u) & (t[r][j][2] == t[r][j][0] + 2u) & (t[r][j][0] <= last4);
+              // (t1, t2) == (t0 + 1, t0 + 2) as one 64-bit compare
+              const unsigned long long got = ((unsigned long long)t[r][j][2] << 32) | t[r][j][1];
+              const unsigned long long want = ((unsigned long long)(t[r][j][0] + 2u) << 32) | (t[r][j][0] + 1u);
+              rw[r][j] = (got == want) & (t[r][j][0] <= last4);
               // unconditional: a dword past the image end (a row starting in
               // its last three bytes, which also gathers) reads as zero, no
               // fault, through the buffer's range check
