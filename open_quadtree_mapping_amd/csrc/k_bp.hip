@@ -345,9 +345,11 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
 // others ("B" cells), which read only A cells' messages. A block owns a strip
 // of TW = 2 (GA - 1) columns and seg rows and walks down its rows: in step t
 // its A groups update A row y0 - 1 + t (the strip's A cells and the one A cell
-// just outside the strip that borders a B cell of the row) into LDS slot t % 4,
-// while its B groups update B row y0 + t - 3 from the A rows above, at and
-// below it (slots t - 3, t - 2, t - 1). An A cell's messages go to HBM only
+// just outside the strip that borders a B cell of the row) into LDS, while its
+// B groups update B row y0 + t - 3 from the A rows above, at and below it
+// (written in steps t - 3, t - 2, t - 1). Each message direction has a ring
+// only as long as its read trails its write (2, 3 and 4 rows), so a block
+// holds 24 KB of LDS at N = 64 and five blocks fit a CU. An A cell's messages go to HBM only
 // when a later launch reads them (write_first: the level's last pair); the B
 // cells' messages go to the other message buffer (wr), so that no block reads
 // a message another block of the launch rewrites. A cells on the halo (one per
@@ -370,7 +372,13 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
   constexpr bool first = SRC > 0;
   constexpr int S = N / 16, Q = S / 4;  // floats and float4 per lane of one row
   constexpr int TW = 2 * (GA - 1);
-  __shared__ float4 Ls[4][GA][4][16 * Q];        // the A messages of four A rows
+  // the A rows' messages, one ring per direction, each as long as its reads
+  // trail its writes: the down message (dir 0) is read one step after the A
+  // row is updated, the left and right ones (dirs 2, 3) two steps after, the
+  // up message (dir 1) three
+  __shared__ float4 L0[2][GA][16 * Q];
+  __shared__ float4 L23[3][GA][2][16 * Q];
+  __shared__ float4 L1[4][GA][16 * Q];
   // per cell of rows y0 - 1 .. y0 + R, columns x0 - 1 .. x0 + TW: bit 0 the
   // quadtree skips it, bit 1 it is a quadtree node (level 0 only)
   __shared__ uint8_t Sk[kPairMaxSeg + 2][TW + 2];
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
   const int bi = lb - bm.start[f];
   const int x0 = (bi % ns) * TW, y0 = (bi / ns) * seg;
   const int R = min(seg, h - y0);
-  const int T = 3 * ((R + 5) / 3);  // R + 3 steps, rounded up to a multiple of 3
+  const int T = 2 * ((R + 4) / 2);  // R + 3 steps, rounded up to a multiple of 2
   const int grp = threadIdx.x >> 4, j = threadIdx.x & 15;
   for (int i = threadIdx.x; i < (R + 2) * (TW + 2); i += 32 * GA) {
     const int r = i / (TW + 2), c = i - r * (TW + 2);
@@ -468,29 +476,25 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
           store4(M1 + cell_msgs(x, y), out);
         }
       }
-      float4* dst = &Ls[u & 3][g][0][0];
+      float4* dst[4] = {&L0[u & 1][g][0], &L1[u & 3][g][0], &L23[u % 3][g][0][0], &L23[u % 3][g][1][0]};
 #pragma unroll
       for (int dir = 0; dir < 4; ++dir)
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-          dst[dir * 16 * Q + j * Q + q] =
+          dst[dir][j * Q + q] =
               make_float4(out[dir].v[4 * q], out[dir].v[4 * q + 1], out[dir].v[4 * q + 2], out[dir].v[4 * q + 3]);
     };
-    // inputs are loaded two steps ahead into three register sets, which
-    // take turns: the loop is unrolled by three, without conditions, so that
-    // no set is copied (the steps past R + 2 have no cells)
-    In i0, i1, i2;
+    // inputs are loaded a step ahead into two register sets, which take
+    // turns: the loop is unrolled by two, without conditions, so that no set
+    // is copied (the steps past R + 2 have no cells)
+    In i0, i1;
     load_in(0, i0);
-    load_in(1, i1);
-    for (int t = 0; t < T; t += 3) {
-      load_in(t + 2, i2);
+    for (int t = 0; t < T; t += 2) {
+      load_in(t + 1, i1);
       step(t, i0);
       __syncthreads();
-      load_in(t + 3, i0);
+      load_in(t + 2, i0);
       step(t + 1, i1);
-      __syncthreads();
-      load_in(t + 4, i1);
-      step(t + 2, i2);
       __syncthreads();
     }
   } else {
@@ -506,9 +510,9 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
       const bool ok = cell_at(u, x, y);
       r = load_row<S>(cost + (ok ? (size_t)y * w + x : (size_t)0) * N, j);
     };
-    auto lds_row = [&](int slot, int ga, int dir) __attribute__((always_inline)) {
+    auto lds_row = [&](const float4* row) __attribute__((always_inline)) {
       Row<S> r;
-      const float4* src = &Ls[slot][ga][dir][j * Q];
+      const float4* src = row + j * Q;
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const float4 v = src[q];
@@ -550,10 +554,10 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
       zero(fd);
       zero(fl);
       zero(fr);
-      if (has_up) fu = lds_row((u - 3) & 3, gv, 1);
-      if (has_down) fd = lds_row((u - 1) & 3, gv, 0);
-      if (has_left) fl = lds_row((u - 2) & 3, gb, 3);
-      if (has_right) fr = lds_row((u - 2) & 3, gb + 1, 2);
+      if (has_up) fu = lds_row(&L1[(u - 3) & 3][gv][0]);
+      if (has_down) fd = lds_row(&L0[(u - 1) & 1][gv][0]);
+      if (has_left) fl = lds_row(&L23[(u - 2) % 3][gb][1][0]);
+      if (has_right) fr = lds_row(&L23[(u - 2) % 3][gb + 1][0][0]);
       if (ex)
         extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63,
                              belief<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right), imin, istep);
@@ -564,18 +568,14 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
       for (int dir = 0; dir < 4; ++dir) out[dir] = message_from<S>(hv[dir], p1, p2, inv_n);
       store4(Mwr + cell_msgs(x, y), out);
     };
-    Row<S> d0, d1, d2;
+    Row<S> d0, d1;
     load_dv(0, d0);
-    load_dv(1, d1);
-    for (int t = 0; t < T; t += 3) {
-      load_dv(t + 2, d2);
+    for (int t = 0; t < T; t += 2) {
+      load_dv(t + 1, d1);
       step(t, d0);
       __syncthreads();
-      load_dv(t + 3, d0);
+      load_dv(t + 2, d0);
       step(t + 1, d1);
-      __syncthreads();
-      load_dv(t + 4, d1);
-      step(t + 2, d2);
       __syncthreads();
     }
   }
