@@ -184,9 +184,11 @@ __device__ __forceinline__ bool is_node(const FrameDev& F, int x, int y) {
   return ((4 * x) % qs) == 0 && ((4 * y) % qs) == 0;
 }
 
+// Every lane of the row computes the refinement and the depth, without a
+// branch, so that a caller can interleave this chain with other arithmetic.
 template <int N, int S>
-__device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int j, int lane, const Row<S>& b,
-                                               double imin, double istep, bool node = true) {
+__device__ __forceinline__ void winner_of(const Row<S>& b, int j, int lane, double imin, double istep, float& depth,
+                                          int& label) {
   // argmin tree: in-lane stages N/2 .. 16, then row_shl 8 .. 1
   float v[S];
   int id[S];
@@ -217,27 +219,36 @@ __device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int 
   bi = row_bcast0_i(bi);
   // beliefs of the winner's neighbours (clamped to the range as the reference)
   const int dm1 = bi > 0 ? bi - 1 : 0, dp1 = bi < N - 1 ? bi + 1 : N - 1;
+  // bi, dm1 and dp1 are the same in the row's 16 lanes: each lane picks its
+  // sample of dm1's (dp1's) slot, and one permute reads lane dm1 & 15's
   const int rowbase = lane & 48;
-  float pre = 0.0f, post = 0.0f;
+  float am = b.v[0], ap = b.v[0];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const float a = __shfl(b.v[s], rowbase + (dm1 & 15), 64);
-    const float c = __shfl(b.v[s], rowbase + (dp1 & 15), 64);
-    if ((dm1 >> 4) == s) pre = a;
-    if ((dp1 >> 4) == s) post = c;
+  for (int s = 1; s < S; ++s) {
+    am = (dm1 >> 4) == s ? b.v[s] : am;
+    ap = (dp1 >> 4) == s ? b.v[s] : ap;
   }
+  const float pre = __shfl(am, rowbase + (dm1 & 15), 64);
+  const float post = __shfl(ap, rowbase + (dp1 & 15), 64);
+  // parabola refinement, used only for an interior winner with a finite step
+  const float a = (pre - 2.0f * bv) + post;
+  const float bb = (-pre) + post;
+  const float ba = bb / a;
+  const float disp = (bi > 0 && bi < N - 1 && isfinite(ba)) ? (float)bi - ba / 2.0f : (float)bi;
+  depth = (float)(1.0 / (istep * (double)disp + imin));
+  label = bi;
+}
+
+template <int N, int S>
+__device__ __forceinline__ void extract_winner(const FrameDev& F, int cell, int j, int lane, const Row<S>& b,
+                                               double imin, double istep, bool node = true) {
+  float depth;
+  int label;
+  winner_of<N, S>(b, j, lane, imin, istep, depth, label);
   if (j == 0) {
-    float disp = (float)bi;
-    if (bi > 0 && bi < N - 1) {
-      const float a = (pre - 2.0f * bv) + post;
-      const float bb = (-pre) + post;
-      const float ba = bb / a;
-      if (isfinite(ba)) disp = (float)bi - ba / 2.0f;
-    }
     // a cell that is not a node gets 0 and label -1
-    const float depth = (float)(1.0 / (istep * (double)disp + imin));
     gp(F.dcoarse)[cell] = node ? depth : 0.0f;
-    gp(F.label)[cell] = node ? bi : -1;
+    gp(F.label)[cell] = node ? label : -1;
   }
 }
 
@@ -360,15 +371,15 @@ __global__ __launch_bounds__(16 * ROWS) void k_bp_sweep(const FrameDev* __restri
 // first two sweeps: A cells read their neighbours' messages from the parent
 // level (SRC 1, buffer pbuf; SRC 2 at the coarsest level: zeros), and skipped
 // cells of both colours get the parent's messages in both buffers. SRC 0:
-// later sweeps, A cells read buffer rd. ext: the second sweep ends BP (level 0)
-// and also extracts the winners of its B cells.
+// later sweeps, A cells read buffer rd. EXT: the second sweep ends BP (level 0)
+// and also extracts the winners of its B cells, computed beside their messages.
 // ---------------------------------------------------------------------------
 constexpr int kPairMaxSeg = 64;
 
-template <int N, int GA, int SRC>
+template <int N, int GA, int SRC, bool EXT>
 __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict__ frames, BlockMap bm, int l, int ca,
                                                      int rd, int wr, int pbuf, int write_first, int seg, float p1,
-                                                     float p2, int ext, double imin, double istep) {
+                                                     float p2, double imin, double istep) {
   constexpr bool first = SRC > 0;
   constexpr int S = N / 16, Q = S / 4;  // floats and float4 per lane of one row
   constexpr int TW = 2 * (GA - 1);
@@ -538,15 +549,15 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
         store4(M0 + cell_msgs(x, y), m);
         store4(M1 + cell_msgs(x, y), m);
       }
-      bool ex = ext != 0;
-      if (ex && !node(x, y)) {
-        if (j == 0) {
+      // a cell that is not a quadtree node gets depth 0 and label -1
+      const bool ex = EXT && node(x, y);
+      if (sk && !ex) {
+        if (EXT && j == 0) {
           gp(F.dcoarse)[(size_t)y * w + x] = 0.0f;
           gp(F.label)[(size_t)y * w + x] = -1;
         }
-        ex = false;
+        return;
       }
-      if (sk && !ex) return;
       const bool has_up = y > 0, has_down = y < h - 1, has_left = x > 0, has_right = x < w - 1;
       const int gv = gb + ((((x0 + y) & 1) == ca) ? 1 : 0);  // the A group of the cells above and below
       Row<S> fu, fd, fl, fr;
@@ -558,15 +569,22 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
       if (has_down) fd = lds_row(&L0[(u - 1) & 1][gv][0]);
       if (has_left) fl = lds_row(&L23[(u - 2) % 3][gb][1][0]);
       if (has_right) fr = lds_row(&L23[(u - 2) % 3][gb + 1][0][0]);
-      if (ex)
-        extract_winner<N, S>(F, y * w + x, j, threadIdx.x & 63,
-                             belief<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right), imin, istep);
-      if (sk) return;
+      // the messages and the winner in one branch-free stretch (a skipped
+      // node computes messages it does not store), the stores after it
       Row<S> hv[4], out[4];
       directional_h<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right, hv);
 #pragma unroll
       for (int dir = 0; dir < 4; ++dir) out[dir] = message_from<S>(hv[dir], p1, p2, inv_n);
-      store4(Mwr + cell_msgs(x, y), out);
+      float depth = 0.0f;
+      int label = -1;
+      if (EXT)
+        winner_of<N, S>(belief<S>(dv, fu, fd, fl, fr, has_up, has_down, has_left, has_right), j,
+                        threadIdx.x & 63, imin, istep, depth, label);
+      if (!sk) store4(Mwr + cell_msgs(x, y), out);
+      if (EXT && j == 0) {
+        gp(F.dcoarse)[(size_t)y * w + x] = ex ? depth : 0.0f;
+        gp(F.label)[(size_t)y * w + x] = ex ? label : -1;
+      }
     };
     Row<S> d0, d1;
     load_dv(0, d0);
@@ -693,9 +711,12 @@ template <int N, int GA>
 static void bp_pair_shape(int nb, hipStream_t s, const FrameDev* dframes, const BlockMap& bm, int l, bool first,
                           int rd, int wr, int pbuf, bool write_first, int seg, bool ext, const BpLaunchParams& p) {
   const int src = !first ? 0 : l < 3 ? 1 : 2;
-  auto k = src == 0 ? k_bp_pair<N, GA, 0> : src == 1 ? k_bp_pair<N, GA, 1> : k_bp_pair<N, GA, 2>;
+  // extraction ends BP at level 0, so only SRC 0 and 1 have an extracting form
+  auto k = src == 0   ? (ext ? k_bp_pair<N, GA, 0, true> : k_bp_pair<N, GA, 0, false>)
+           : src == 1 ? (ext ? k_bp_pair<N, GA, 1, true> : k_bp_pair<N, GA, 1, false>)
+                      : k_bp_pair<N, GA, 2, false>;
   hipLaunchKernelGGL(k, dim3(nb), dim3(32 * GA), 0, s, dframes, bm, l, 0, rd, wr, pbuf, (int)write_first, seg, p.p1,
-                     p.p2, (int)ext, p.idepth_min, p.idepth_step);
+                     p.p2, p.idepth_min, p.idepth_step);
 }
 
 // sweeps it and it + 1 (it even: the first updates the x + y even cells) of level l
