@@ -1356,8 +1356,8 @@ __device__ __forceinline__ bool warp_seed(const FrameDev& F, int idx, float4& s,
   return true;
 }
 
-// The warp writes the warped seed back in place (the reference's layout)
-// unless warp_skip: then the update rebuilds the winner's warped depth and
+// The warp reads each prior seed once (a streaming load) and writes the
+// warped seed back in place (the reference's layout) unless warp_skip: then the update rebuilds the winner's warped depth and
 // variance from its z-buffer key and its prior seed, and the seeds stay as
 // they were.
 __global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ frames, BlockMap bm) {
@@ -1365,7 +1365,7 @@ __global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ 
   const FrameDev& F = frames[f];
   const int idx = (blockIdx.x - bm.start[f]) * 256 + threadIdx.x;
   if (idx >= F.W * F.H) return;
-  float4 s = F.seeds[idx];
+  float4 s = load4_nt(gp((const float4*)F.seeds) + idx);
   int pxi, pyi;
   if (!warp_seed(F, idx, s, pxi, pyi)) return;
   if (!F.warp_skip) F.seeds[idx] = s;
@@ -1390,7 +1390,8 @@ __global__ __launch_bounds__(256) void k_fuse_replay(const FrameDev* __restrict_
 // Hole filling from the z-buffer snapshot (an empty cell takes the first
 // non-empty source in raster order whose position is in [2, W-2] x [2, H-2],
 // src/depth_fusion.cu:121-144) and the per-pixel update
-// (src/depth_fusion.cu:146-213).
+// (src/depth_fusion.cu:146-213). The dense depth, the source seed and both
+// outputs are touched once per pixel here: streaming accesses.
 __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict__ frames, BlockMap bm) {
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];
@@ -1400,13 +1401,13 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
   unsigned long long key;
   const int src = fused_source(F, idx, key);
   if (F.table_out) F.table_out[idx] = src;
-  const float de = F.dense[idx];
+  const float de = __builtin_nontemporal_load(&gp(F.dense)[idx]);
   float unc = fmaxf(0.5f, (float)((double)de * 0.2));
   unc = unc * unc;
   if (de <= 0.0f) unc = 1e9f;
   float a, b, mu, s2;
   if (src > 0) {
-    float4 s = F.seeds[src];
+    float4 s = load4_nt(gp((const float4*)F.seeds) + src);
     if (F.warp_skip) {
       // the warp left the prior as it was: its depth is the key's, its variance grows by it
       s.z = __uint_as_float((uint32_t)(key >> 32));
@@ -1444,8 +1445,8 @@ __global__ __launch_bounds__(256) void k_fuse_update(const FrameDev* __restrict_
   const float s_p = c1 * (new_sq + new_mu * new_mu) + c2 * (s2 + mu * mu) - mu_p * mu_p;
   const float a_p = (e1 - f1) / (f1 - e1 / f1);
   const float b_p = a_p * (1.0f - f1) / f1;
-  if (F.depth_out) F.depth_out[idx] = is_goodpoint(a, b) ? mu_p : -1.0f;
-  F.seeds_new[idx] = make_float4(a_p, b_p, mu_p, s_p);
+  if (F.depth_out) __builtin_nontemporal_store(is_goodpoint(a, b) ? mu_p : -1.0f, &gp(F.depth_out)[idx]);
+  store4_nt(gp(F.seeds_new) + idx, make_float4(a_p, b_p, mu_p, s_p));
 }
 
 // ---------------------------------------------------------------------------

@@ -45,6 +45,35 @@ __device__ __forceinline__ void store_row(gptr<float> p, int j, const Row<S>& r)
   for (int k = 0; k < S / 4; ++k) q[k] = make_float4(r.v[4 * k], r.v[4 * k + 1], r.v[4 * k + 2], r.v[4 * k + 3]);
 }
 
+// Non-temporal forms, for rows a kernel touches once: the accesses carry the
+// streaming hint (the caches keep such lines the shortest), which lets a
+// gather of scattered 256 B rows run near the streaming rate
+
+template <int S>
+__device__ __forceinline__ Row<S> load_row_nt(gptr<const float> p, int j) {
+  Row<S> r;
+  const gptr<const F4v> q = (gptr<const F4v>)(p + S * j);
+#pragma unroll
+  for (int k = 0; k < S / 4; ++k) {
+    const F4v t = __builtin_nontemporal_load(q + k);
+    r.v[4 * k] = t.x;
+    r.v[4 * k + 1] = t.y;
+    r.v[4 * k + 2] = t.z;
+    r.v[4 * k + 3] = t.w;
+  }
+  return r;
+}
+
+template <int S>
+__device__ __forceinline__ void store_row_nt(gptr<float> p, int j, const Row<S>& r) {
+  const gptr<F4v> q = (gptr<F4v>)(p + S * j);
+#pragma unroll
+  for (int k = 0; k < S / 4; ++k) {
+    F4v t = {r.v[4 * k], r.v[4 * k + 1], r.v[4 * k + 2], r.v[4 * k + 3]};
+    __builtin_nontemporal_store(t, q + k);
+  }
+}
+
 // message buffer b (0: msg, 1: msg2) of level l
 __device__ __forceinline__ float* msg_buf(const FrameDev& F, int l, int b) { return b ? F.msg2[l] : F.msg[l]; }
 
@@ -425,9 +454,11 @@ __global__ __launch_bounds__(32 * GA) void k_bp_pair(const FrameDev* __restrict_
   auto parent_msgs = [&](int x, int y) __attribute__((always_inline)) {
     return ((size_t)(y >> 1) * wp + (x >> 1)) * 4 * N;
   };
+  // a pair's stored messages are read only by a later launch: streaming stores
+  // (its loads stay plain, since neighbouring blocks re-read halo rows)
   auto store4 = [&](gptr<float> dst, const Row<S> (&m)[4]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int dir = 0; dir < 4; ++dir) store_row<S>(dst + dir * N, j, m[dir]);
+    for (int dir = 0; dir < 4; ++dir) store_row_nt<S>(dst + dir * N, j, m[dir]);
   };
   auto zero = [&](Row<S>& r) __attribute__((always_inline)) {
 #pragma unroll
@@ -629,15 +660,15 @@ __global__ __launch_bounds__(16 * ROWS) void k_extract(const FrameDev* __restric
   // the quadtree level and all five rows are loaded at once (a missing
   // neighbour's row is replaced by a valid one and not added; a cell that is
   // not a node runs the extraction too and stores 0 and -1), so their
-  // latencies overlap
+  // latencies overlap; each row is read once, with the streaming hint
   const int qs = 1 << gp((const uint8_t*)F.levels)[(size_t)(4 * y) * F.W + 4 * x];
   const gptr<const float> M = gp((const float*)msg_buf(F, 0, buf));
   const bool hl = x != 0, hr = x != w - 1, hu = y != 0, hd = y != h - 1;
-  Row<S> b = load_row<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
-  const Row<S> tl = load_row<S>(M + (((size_t)(hl ? cell - 1 : cell)) * 4 + 3) * N, j);
-  const Row<S> tr = load_row<S>(M + (((size_t)(hr ? cell + 1 : cell)) * 4 + 2) * N, j);
-  const Row<S> tu = load_row<S>(M + (((size_t)(hu ? cell - w : cell)) * 4 + 1) * N, j);
-  const Row<S> td = load_row<S>(M + (((size_t)(hd ? cell + w : cell)) * 4 + 0) * N, j);
+  Row<S> b = load_row_nt<S>(gp((const float*)F.cost[0]) + (size_t)cell * N, j);
+  const Row<S> tl = load_row_nt<S>(M + (((size_t)(hl ? cell - 1 : cell)) * 4 + 3) * N, j);
+  const Row<S> tr = load_row_nt<S>(M + (((size_t)(hr ? cell + 1 : cell)) * 4 + 2) * N, j);
+  const Row<S> tu = load_row_nt<S>(M + (((size_t)(hu ? cell - w : cell)) * 4 + 1) * N, j);
+  const Row<S> td = load_row_nt<S>(M + (((size_t)(hd ? cell + w : cell)) * 4 + 0) * N, j);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     b.v[s] = hl ? b.v[s] + tl.v[s] : b.v[s];

@@ -524,7 +524,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
   }
   __syncthreads();
   // each tile row's 8 cost rows are contiguous in the volume: written as
-  // float4, in the row layout of the pipeline or, for callers, in sample order
+  // float4, in the row layout of the pipeline or, for callers, in sample
+  // order, with streaming stores (BP reads them in later launches)
   constexpr int kQ = N / 4;  // float4 per row
 #pragma unroll
   for (int k = 0; k < (16 * N) / (64 * WAVES); ++k) {
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_cost(const FrameDev* __restrict_
         v = make_float4(src[row_sample<N>(4 * q)], src[row_sample<N>(4 * q + 1)], src[row_sample<N>(4 * q + 2)],
                         src[row_sample<N>(4 * q + 3)]);
       }
-      ((gptr<float4>)(gp(F.cost[0]) + ((size_t)oy * Wc + ox) * N))[q] = v;
+      store4_nt((gptr<float4>)(gp(F.cost[0]) + ((size_t)oy * Wc + ox) * N) + q, v);
     }
   }
   if (pyramid) cost_pyramid_tile<N, 64 * WAVES>(F, tile, kRow, tx, ty, natural);

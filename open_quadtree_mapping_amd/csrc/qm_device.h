@@ -94,6 +94,19 @@ __device__ __forceinline__ gptr<const T> gp(const T* p) {
   return (gptr<const T>)p;
 }
 
+// Streaming (non-temporal) accesses, for data a kernel touches once: the
+// caches keep such lines the shortest, so they do not push out data that
+// other kernels of the run read again
+typedef float F4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load4_nt(gptr<const float4> p) {
+  const F4v t = __builtin_nontemporal_load((gptr<const F4v>)p);
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ void store4_nt(gptr<float4> p, float4 v) {
+  const F4v t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, (gptr<F4v>)p);
+}
+
 // blockIdx.x -> frame: frame f owns blocks [start[f], start[f+1]).
 struct BlockMap {
   int n;
