@@ -1357,9 +1357,9 @@ __device__ __forceinline__ bool warp_seed(const FrameDev& F, int idx, float4& s,
 }
 
 // The warp reads each prior seed once (a streaming load) and writes the
-// warped seed back in place (the reference's layout) unless warp_skip: then the update rebuilds the winner's warped depth and
-// variance from its z-buffer key and its prior seed, and the seeds stay as
-// they were.
+// warped seed back in place (the reference's layout) unless warp_skip: then
+// the update rebuilds the winner's warped depth and variance from its
+// z-buffer key and its prior seed, and the seeds stay as they were.
 __global__ __launch_bounds__(256) void k_fuse_warp(const FrameDev* __restrict__ frames, BlockMap bm) {
   const int f = find_frame(bm, blockIdx.x);
   const FrameDev& F = frames[f];

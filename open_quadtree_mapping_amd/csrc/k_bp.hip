@@ -48,7 +48,6 @@ __device__ __forceinline__ void store_row(gptr<float> p, int j, const Row<S>& r)
 // Non-temporal forms, for rows a kernel touches once: the accesses carry the
 // streaming hint (the caches keep such lines the shortest), which lets a
 // gather of scattered 256 B rows run near the streaming rate
-
 template <int S>
 __device__ __forceinline__ Row<S> load_row_nt(gptr<const float> p, int j) {
   Row<S> r;
