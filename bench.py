@@ -160,7 +160,7 @@ def bp_launches(iters=(4, 10, 10, 10), pairs=True):
     return sum(1 if it == 0 else (it // 2 + it % 2 if pairs and it >= 2 else it) for it in iters)
 
 
-def stage_bytes(levels, W, H, N, pairs=True):
+def stage_bytes(levels, W, H, N, pairs=True, reset_in_rows=True):
     import numpy as np
     C = (W // 4) * (H // 4)
     lw, lh = [W // 4], [H // 4]
@@ -186,8 +186,10 @@ def stage_bytes(levels, W, H, N, pairs=True):
         "pyramid": 4 * N * sum(a * b for a, b in zip(lw, lh)),
         "bp_sweep": bp,
         "extract": nodes * (20 * N + 8) + others * 9,
-        "upsample_row": px + 8 * C + 8 * px,           # image, labels + depths, row results
-        "upsample_col": px + 8 * px + 4 * px + 8 * px,  # image, row results, dense, z-buffer reset
+        # image, labels + depths, row results; the next run's z-buffer reset
+        # (8 bytes per pixel) in the row scan, else in the column pass
+        "upsample_row": px + 8 * C + 8 * px + (8 * px if reset_in_rows else 0),
+        "upsample_col": px + 8 * px + 4 * px + (0 if reset_in_rows else 8 * px),  # image, row results, dense
         # the batch's warp reads the prior seeds and scatters z-buffer keys (the
         # warped seeds are rebuilt by the update from the winner's key); the
         # update reads keys, the winner's seed and the dense depth and writes
@@ -475,7 +477,7 @@ def main():
     stage_alg = {}
     bp_updates = 0
     for lv in levels_per_stream:
-        sb, upd = stage_bytes(lv, W, H, N, pairs=pairs)
+        sb, upd = stage_bytes(lv, W, H, N, pairs=pairs, reset_in_rows=solver == 1 and W <= 1536)
         bp_updates += upd
         for k, v in sb.items():
             stage_alg[k] = stage_alg.get(k, 0) + v

@@ -1469,12 +1469,13 @@ static int pixels_map(BlockMap& bm, const FrameDev* hframes, int n) {
 }
 
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         hipStream_t s, int solver) {
+                         hipStream_t s, int solver, bool clear_zbuf) {
   // the segmented-scan solver (modes 0 and 1) takes the frames whose rows fit
-  // its lanes; the Thomas kernels the rest
+  // its lanes; the Thomas kernels the rest. clear_zbuf: the scan resets the
+  // next run's fusion z-buffer when it takes every frame (row_scan_covers)
   int rest[kMaxFrames];
   const bool scan = solver == QM_UPSAMPLE_SCAN && mode <= 1;
-  if (scan && launch_upsample_row_scan(dframes, hframes, n, mode, wlut, s, rest) == 0) return;
+  if (scan && launch_upsample_row_scan(dframes, hframes, n, mode, wlut, s, rest, clear_zbuf) == 0) return;
   BlockMap bm;
   const int nb = rows_map(bm, hframes, n, scan ? rest : nullptr);
   if (nb <= 0) return;

@@ -331,7 +331,7 @@ __device__ __forceinline__ void lane_couplings(int s, int n, float edge, double 
 // ---------------------------------------------------------------------------
 template <int P, int M, int WAVES, int MODE>
 __global__ __launch_bounds__(64 * WAVES) void k_up_row_scan(const FrameDev* __restrict__ frames, BlockMap bm,
-                                                            const float* __restrict__ wlut) {
+                                                            const float* __restrict__ wlut, int clear_zbuf) {
   constexpr int LINES = WAVES * 64 / P;
   __shared__ float lw[256], lm[256];
   __shared__ float tile[P * M * (LINES + 1)];
@@ -339,6 +339,25 @@ __global__ __launch_bounds__(64 * WAVES) void k_up_row_scan(const FrameDev* __re
   const FrameDev& F = frames[f];
   const int W = F.W, H = F.H, Wc = F.Wc, Hc = F.Hc;
   const double lam = F.lam;
+  if (clear_zbuf) {
+    // the next run's fusion z-buffer over this workgroup's rows, reset to
+    // empty as whole lines: 16-byte streaming stores (an odd first or last
+    // entry on its own)
+    typedef unsigned U4v __attribute__((ext_vector_type(4)));
+    const gptr<unsigned long long> z = gp(F.zbuf_clear);
+    const int ya = (blockIdx.x - bm.start[f]) * LINES;
+    size_t a = (size_t)ya * W;
+    size_t e = (size_t)min(ya + LINES, H) * W;
+    if ((a & 1) && a < e) {
+      if (threadIdx.x == 0) z[a] = ~0ull;
+      ++a;
+    }
+    if (((e - a) & 1) && threadIdx.x == 0) z[e - 1] = ~0ull;
+    const size_t pairs = (e - a) >> 1;
+    const gptr<U4v> z2 = (gptr<U4v>)(z + a);
+    const U4v ones = {~0u, ~0u, ~0u, ~0u};
+    for (size_t k = threadIdx.x; k < pairs; k += 64 * WAVES) __builtin_nontemporal_store(ones, z2 + k);
+  }
   for (int t = threadIdx.x; t < 256; t += 64 * WAVES) {
     const float w = wlut[t];
     lw[t] = w;
@@ -513,10 +532,11 @@ static int scan_shape_of(int n) {
 }
 
 template <int I, int MODE>
-static void row_launch(const FrameDev* dframes, const BlockMap& bm, int nb, const float* wlut, hipStream_t s) {
+static void row_launch(const FrameDev* dframes, const BlockMap& bm, int nb, const float* wlut, int clear,
+                       hipStream_t s) {
   constexpr ScanShape sh = kRowShapes[I];
   hipLaunchKernelGGL((k_up_row_scan<sh.P, sh.M, sh.waves, MODE>), dim3(nb), dim3(64 * sh.waves), 0, s, dframes, bm,
-                     wlut);
+                     wlut, clear);
 }
 template <int I>
 static void col_launch(const FrameDev* dframes, const BlockMap& bm, int nb, const float* wlut, int clear,
@@ -526,13 +546,24 @@ static void col_launch(const FrameDev* dframes, const BlockMap& bm, int nb, cons
                      clear);
 }
 
+// The row scan takes a frame whose rows fit its lanes and whose lambda is in
+// its range; row_scan_covers: it takes every frame of the launch.
+static bool row_scan_takes(const FrameDev& h) { return scan_shape_of(h.W) >= 0 && h.lam <= 0x1p100; }
+bool row_scan_covers(const FrameDev* hframes, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!row_scan_takes(hframes[i])) return false;
+  return true;
+}
+
 int launch_upsample_row_scan(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                             hipStream_t s, int* rest) {
+                             hipStream_t s, int* rest, bool clear_zbuf) {
   int left = 0;
   for (int i = 0; i < n; ++i) {
-    rest[i] = scan_shape_of(hframes[i].W) < 0 || !(hframes[i].lam <= 0x1p100) ? 1 : 0;
+    rest[i] = row_scan_takes(hframes[i]) ? 0 : 1;
     left += rest[i];
   }
+  // the z-buffer reset moves here only when every frame is the scan's
+  const int cz = clear_zbuf && left == 0 ? 1 : 0;
   for (int sh = 0; sh < 4; ++sh) {
     const int lines = kRowShapes[sh].waves * 64 / kRowShapes[sh].P;
     int counts[kMaxFrames];
@@ -542,10 +573,10 @@ int launch_upsample_row_scan(const FrameDev* dframes, const FrameDev* hframes, i
     const int nb = make_block_map(bm, counts, n);
     if (nb <= 0) continue;
     const bool m1 = mode == 1;
-    if (sh == 0) m1 ? row_launch<0, 1>(dframes, bm, nb, wlut, s) : row_launch<0, 0>(dframes, bm, nb, wlut, s);
-    if (sh == 1) m1 ? row_launch<1, 1>(dframes, bm, nb, wlut, s) : row_launch<1, 0>(dframes, bm, nb, wlut, s);
-    if (sh == 2) m1 ? row_launch<2, 1>(dframes, bm, nb, wlut, s) : row_launch<2, 0>(dframes, bm, nb, wlut, s);
-    if (sh == 3) m1 ? row_launch<3, 1>(dframes, bm, nb, wlut, s) : row_launch<3, 0>(dframes, bm, nb, wlut, s);
+    if (sh == 0) m1 ? row_launch<0, 1>(dframes, bm, nb, wlut, cz, s) : row_launch<0, 0>(dframes, bm, nb, wlut, cz, s);
+    if (sh == 1) m1 ? row_launch<1, 1>(dframes, bm, nb, wlut, cz, s) : row_launch<1, 0>(dframes, bm, nb, wlut, cz, s);
+    if (sh == 2) m1 ? row_launch<2, 1>(dframes, bm, nb, wlut, cz, s) : row_launch<2, 0>(dframes, bm, nb, wlut, cz, s);
+    if (sh == 3) m1 ? row_launch<3, 1>(dframes, bm, nb, wlut, cz, s) : row_launch<3, 0>(dframes, bm, nb, wlut, cz, s);
   }
   return left;
 }

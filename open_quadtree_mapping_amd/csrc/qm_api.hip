@@ -537,12 +537,15 @@ void enqueue_stage(qm_batch* b, int par, hipStream_t s, Timer* t, int f0, int n,
     }
     case QM_STAGE_UPSAMPLE_ROW: {
       Mark m(t, QM_STAGE_UPSAMPLE_ROW);
-      launch_upsample_row(df, hf, n, 0, b->wlut, s, p->upsample_solver);
+      // the row scan resets the next run's z-buffer in whole lines when it
+      // takes every frame of the launch; else the column pass does
+      launch_upsample_row(df, hf, n, 0, b->wlut, s, p->upsample_solver, true);
       break;
     }
     case QM_STAGE_UPSAMPLE_COL: {
       Mark m(t, QM_STAGE_UPSAMPLE_COL);
-      launch_upsample_col(df, hf, n, 0, b->wlut, true, s, p->upsample_solver);
+      const bool rows_reset = p->upsample_solver == QM_UPSAMPLE_SCAN && row_scan_covers(hf, n);
+      launch_upsample_col(df, hf, n, 0, b->wlut, !rows_reset, s, p->upsample_solver);
       break;
     }
     case QM_STAGE_FUSE_WARP: {

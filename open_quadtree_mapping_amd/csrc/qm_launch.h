@@ -88,14 +88,17 @@ void launch_upsample(const FrameDev* dframes, const FrameDev* hframes, int n, in
                      bool clear_zbuf, hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
 void launch_fuse(const FrameDev* dframes, const FrameDev* hframes, int n, bool clear_zbuf, hipStream_t s);
 void launch_upsample_row(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                         hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
+                         hipStream_t s, int solver = QM_UPSAMPLE_THOMAS, bool clear_zbuf = false);
 void launch_upsample_col(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
                          bool clear_zbuf, hipStream_t s, int solver = QM_UPSAMPLE_THOMAS);
 // the scan kernels for the frames whose lines fit them (rows of up to 1536
 // pixels, columns likewise); rest[i] = 1 marks a frame left to the Thomas
 // kernels; returns how many there are
 int launch_upsample_row_scan(const FrameDev* dframes, const FrameDev* hframes, int n, int mode, const float* wlut,
-                             hipStream_t s, int* rest);
+                             hipStream_t s, int* rest, bool clear_zbuf = false);
+// true when the row scan takes every frame of the launch (then a row pass asked
+// to reset the fusion z-buffer does it, and the column pass need not)
+bool row_scan_covers(const FrameDev* hframes, int n);
 int launch_upsample_col_scan(const FrameDev* dframes, const FrameDev* hframes, int n, const float* wlut,
                              bool clear_zbuf, hipStream_t s, int* rest);
 // the reference's prior-driven alternates; frames without the prior get no work
